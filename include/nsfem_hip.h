@@ -1,0 +1,247 @@
+/*
+ * nsfem_hip.h - C ABI of the MI355X (gfx950) Chorin-projection library.
+ *
+ * Every entry point returns an int status (NSFEM_OK on success) and records a
+ * thread-local message readable with nsfem_last_error().  Status codes follow
+ * the reference's exception split: std::invalid_argument -> NSFEM_EINVAL,
+ * std::runtime_error -> NSFEM_ERUNTIME, failed HIP calls -> NSFEM_EDEVICE, a
+ * non-finite solver residual -> NSFEM_ENONFINITE.
+ *
+ * Device pointers are plain HIP device addresses (hipMalloc'd by
+ * nsfem_malloc or owned by a handle).  Blocking behaviour matches the
+ * reference: every call that the reference runs synchronously (cudaMemcpy,
+ * host-pointer-mode cuBLAS reductions, solver calls) has completed on return.
+ * Vectors with several channels are planar: channel c of a vector with n rows
+ * per channel starts at element c * n.
+ *
+ * Reference citations are relative to kosyonikolov/NsFemOptim.
+ */
+#ifndef NSFEM_HIP_H
+#define NSFEM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSFEM_OK 0
+#define NSFEM_EINVAL 1
+#define NSFEM_ERUNTIME 2
+#define NSFEM_EDEVICE 3
+#define NSFEM_ENONFINITE 4
+
+/* Value types; ids match the linalg .bin dtypeId (cpp/libs/LinAlg/src/io.cpp:11-21). */
+#define NSFEM_F32 0
+#define NSFEM_F64 1
+
+const char * nsfem_last_error(void);
+int nsfem_version(int * major, int * minor);
+int nsfem_synchronize(void);
+
+/* ------------------------------------------------------------------------
+ * Device buffers.  Replace cu::vec<T> (cpp/libs/CudaUtils/include/cu/vec.h:16-387):
+ * allocation (:73-86, size 0 rejected), upload/download (:248-274),
+ * copyTo (:167-178), memsetZero (:180-191).
+ * ---------------------------------------------------------------------- */
+int nsfem_malloc(size_t bytes, void ** dptr);
+int nsfem_free(void * dptr);
+int nsfem_upload(void * dst, const void * hostSrc, size_t bytes);
+int nsfem_download(void * hostDst, const void * src, size_t bytes);
+int nsfem_copy(void * dst, const void * src, size_t bytes);
+int nsfem_memset_zero(void * dptr, size_t bytes);
+
+/* ------------------------------------------------------------------------
+ * Device CSR.  Replaces cu::csrF (cpp/libs/CudaUtils/src/csrF.cu:5-40):
+ * int32, 0-based, deep copy of host arrays, values mutable in place.
+ * ---------------------------------------------------------------------- */
+typedef struct nsfem_csr nsfem_csr;
+int nsfem_csr_create(int rows, int cols, int nnz, const int * rowStart, const int * column, const void * values, int dtype,
+                     nsfem_csr ** out);
+int nsfem_csr_destroy(nsfem_csr * m);
+int nsfem_csr_values(nsfem_csr * m, void ** dvalues);
+int nsfem_csr_shape(const nsfem_csr * m, int * rows, int * cols, int * nnz, int * dtype);
+
+/* y = A x.  Replaces cu::spmv::compute (cpp/libs/CudaUtils/src/spmv.cu:45-63). */
+int nsfem_spmv(nsfem_csr * m, const void * x, void * y);
+/* Y = A X, X and Y column-major with ld = cols / rows.
+ * Replaces cu::spmm::compute (cpp/libs/CudaUtils/src/spmm.cu:45-70). */
+int nsfem_spmm(nsfem_csr * m, int numCh, const void * x, void * y);
+/* General form: y = alpha A x + beta y, explicit leading dimensions, lanes per
+ * row (0 = automatic; 4/8/16/32/64). */
+int nsfem_csr_mul(nsfem_csr * m, int numCh, const void * x, long ldx, void * y, long ldy, double alpha, double beta, int lanes);
+
+/* ------------------------------------------------------------------------
+ * Vector operations.  Replace cu::saxpy / cu::scale (cpp/libs/CudaUtils/src/blas.cu:51-67),
+ * cublasSgeam used as a vector add (execs/NavierStokes/src/chorinCuda.cu:246-250),
+ * cublasSdot (src/conjGradF.cu:57), cublasSnrm2 (src/gaussSeidel.cu:278),
+ * cusparseScatter / cusparseGather (chorinCuda.cu:73, :134, :146).
+ * Reductions accumulate in FP64 and are deterministic.
+ * ---------------------------------------------------------------------- */
+int nsfem_axpy(int dtype, long n, double alpha, const void * src, void * dst); /* dst += alpha src */
+int nsfem_scale(int dtype, long n, double alpha, void * dst);
+int nsfem_geam(int dtype, long n, double alpha, const void * a, double beta, const void * b, void * c);
+int nsfem_dot(int dtype, long n, const void * x, const void * y, double * out);
+int nsfem_nrm2(int dtype, long n, const void * x, double * out);
+int nsfem_scatter(int dtype, int nnz, const int * idx, const void * values, void * dense);
+int nsfem_gather(int dtype, int nnz, const void * dense, const int * idx, void * values);
+int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * dst);
+
+/* ------------------------------------------------------------------------
+ * Linear solvers.  Replace cu::createSolver (cpp/libs/CudaUtils/src/solvers/solverFactory.cu:209-237)
+ * and cu::AbstractSolver (include/cu/solvers/abstractSolver.h:8-32).
+ *   name "gs"        : multicolour Gauss-Seidel with the reference's semantics
+ *                      (gaussSeidel.cu:121-404): RMS residual |b-Ax|/sqrt(n)
+ *                      checked every mseCheckInterval sweeps, stop below targetMse.
+ *   name "pcg"       : CG preconditioned by a symmetric multicolour GS sweep;
+ *                      targetMse is a relative residual |b-Ax| / |b|.
+ *   name "pcg_mixed" : FP64 iterative refinement around FP32 inner PCG
+ *                      (values must be NSFEM_F64).
+ *   name "dss"       : rejected with NSFEM_EINVAL (no direct solver here).
+ * The compute precision is valueDtype ("gs"/"pcg").  Colouring is the
+ * reference's smallest-last greedy colouring (graphs.cpp:64-176).
+ * rhs/sol are device buffers with stable addresses, numCh * rows elements of
+ * the io dtype (NSFEM_F32 for an F32 solver, NSFEM_F64 otherwise).
+ * ---------------------------------------------------------------------- */
+typedef struct nsfem_solver nsfem_solver;
+int nsfem_solver_create(const char * name, int numCh, int rows, int nnz, const int * rowStart, const int * column, const void * values,
+                        int valueDtype, int maxIters, double targetMse, int mseCheckInterval, nsfem_solver ** out);
+/* Block-diagonal batch of independent systems, each coloured on its own,
+ * solved together with per-system convergence (values FP64 host arrays). */
+int nsfem_solver_create_batch(const char * name, int numCh, int numSystems, const int * rows, const int * nnz,
+                              const int * const * rowStart, const int * const * column, const double * const * values, int dtype,
+                              int maxIters, double targetMse, int mseCheckInterval, nsfem_solver ** out);
+int nsfem_solver_destroy(nsfem_solver * s);
+int nsfem_solver_num_ch(const nsfem_solver * s, int * out);
+int nsfem_solver_rows(const nsfem_solver * s, int * out);
+int nsfem_solver_io_dtype(const nsfem_solver * s, int * out);
+int nsfem_solver_rhs(nsfem_solver * s, void ** dptr);
+int nsfem_solver_sol(nsfem_solver * s, void ** dptr);
+int nsfem_solver_solve(nsfem_solver * s);
+/* ch = -1: combined value, sqrt(m0^2 + m1^2) for two channels (gaussSeidel.cu:397). */
+int nsfem_solver_last_mse(const nsfem_solver * s, int ch, double * out);
+int nsfem_solver_last_relres(const nsfem_solver * s, int system, int ch, double * out);
+int nsfem_solver_last_iterations(const nsfem_solver * s, int * out);
+int nsfem_solver_last_iterations_system(const nsfem_solver * s, int system, int * out);
+int nsfem_solver_set_max_iters(nsfem_solver * s, int n);
+int nsfem_solver_set_target_mse(nsfem_solver * s, double target);
+int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
+/* Tunables: "innerTol" (pcg_mixed), "checkChunk" (iterations between host polls). */
+int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
+int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
+int nsfem_solver_coloring(const nsfem_solver * s, int system, int * coloring, int * partitionStart);
+
+/* Smallest-last greedy colouring of a square CSR pattern, host only
+ * (buildSmallestLastOrdering + partitionGraphGreedy + per-colour sort,
+ * graphs.cpp:64-176, gaussSeidel.cu:140-159).  partitionStart needs rows + 1 slots. */
+int nsfem_mcgs_coloring(int rows, const int * rowStart, const int * column, int * coloring, int * partitionStart, int * numColors);
+
+/* ------------------------------------------------------------------------
+ * Meshes.  Replace mesh::parseTriangleGmsh (cpp/libs/Mesh/src/io.cpp:65-214)
+ * and mesh::createMesh (src/concreteMesh.cpp:78-293).
+ * ---------------------------------------------------------------------- */
+typedef struct nsfem_mesh nsfem_mesh;
+int nsfem_mesh_load_gmsh(const char * path, nsfem_mesh ** out);
+/* borderEdges: numBorderEdges x (from, to, group), oriented like gmsh line elements. */
+int nsfem_mesh_from_arrays(int numNodes, const double * xy, int numTriangles, const int * triangles, int numBorderEdges,
+                           const int * borderEdges, int numGroups, const char * const * groupNames, nsfem_mesh ** out);
+int nsfem_mesh_destroy(nsfem_mesh * m);
+int nsfem_mesh_info(const nsfem_mesh * m, int * numNodes, int * numTriangles, int * numBorderElements, int * numGroups);
+/* xy: 2*numNodes, triangles: 3*numTriangles, border: 3*numBorderElements (triangle, side, group). */
+int nsfem_mesh_arrays(const nsfem_mesh * m, double * xy, int * triangles, int * border);
+int nsfem_mesh_group_name(const nsfem_mesh * m, int group, char * buf, int bufSize);
+
+/* ------------------------------------------------------------------------
+ * Chorin context: Dirichlet lists and all FP64 matrices of one mesh.
+ * Replaces buildChorinContext (cpp/execs/NavierStokes/src/buildContext.cpp:17-142)
+ * with fem::buildChorinCsrMatrices (cpp/libs/Fem/src/chorinCsr.cpp:437-502).
+ * ---------------------------------------------------------------------- */
+typedef struct nsfem_dfg_conditions
+{
+    double viscosity;
+    double peakVelocity;
+    double channelHeight; /* 0.41 in the reference (dfgCondtions.h:9-12) */
+    double profileScale;  /* 1 in the reference; inflow = scale*peak*y(H-y)/H^2 */
+} nsfem_dfg_conditions;
+
+typedef struct nsfem_context nsfem_context;
+int nsfem_context_build(const nsfem_mesh * mesh, const nsfem_dfg_conditions * cond, int numThreads, nsfem_context ** out);
+int nsfem_context_destroy(nsfem_context * c);
+int nsfem_context_sizes(const nsfem_context * c, int * numVelocityNodes, int * numPressureNodes, int * numDirichletVelocity,
+                        int * numInternalPressure, int * numElements);
+
+#define NSFEM_MAT_VELOCITY_MASS 0
+#define NSFEM_MAT_VELOCITY_STIFFNESS 1
+#define NSFEM_MAT_PRESSURE_STIFFNESS 2
+#define NSFEM_MAT_PRESSURE_STIFFNESS_INTERNAL 3
+#define NSFEM_MAT_VELOCITY_PRESSURE_DIV 4
+#define NSFEM_MAT_PRESSURE_VELOCITY_DIV 5
+/* Borrowed host pointers, valid while the context lives. */
+int nsfem_context_matrix(const nsfem_context * c, int which, int * rows, int * cols, int * nnz, const int ** rowStart,
+                         const int ** column, const double ** values);
+/* which: 0 = velocity x, 1 = velocity y, 2 = pressure; arrays of the matching count. */
+int nsfem_context_dirichlet(const nsfem_context * c, int which, int * count, int * ids, double * values);
+int nsfem_context_internal_pressure(const nsfem_context * c, int * ids);
+/* P2 (velocity) or P1 (pressure) numbering: xy 2*nodes, elements 6*E or 3*E. */
+int nsfem_context_velocity_mesh(const nsfem_context * c, double * xy, int * elements);
+int nsfem_context_pressure_mesh(const nsfem_context * c, double * xy, int * elements);
+/* FastConvection integration map (cpp/libs/Fem/src/fastConvection.cpp:9-100),
+ * FP64, rows = nnz of the velocity pattern, cols = 2N.  Arrays are malloc'd;
+ * release with nsfem_host_free. */
+int nsfem_fast_convection_map(const nsfem_context * c, int numThreads, int * rows, int * cols, int * nnz, int ** rowStart, int ** column,
+                              double ** values);
+void nsfem_host_free(void * p);
+
+/* ------------------------------------------------------------------------
+ * Chorin projection time stepping on the GPU.  Replaces the loop of
+ * solveNsChorinCuda (cpp/execs/NavierStokes/src/chorinCuda.cu:154-379) and
+ * its config (include/NavierStokes/chorinCudaConfig.h:6-12).
+ * ---------------------------------------------------------------------- */
+typedef struct nsfem_solver_config
+{
+    const char * method; /* "gs", "pcg", "pcg_mixed" */
+    int maxIterations;
+    double targetMse;
+    int mseCheckInterval;
+} nsfem_solver_config;
+
+typedef struct nsfem_chorin_config
+{
+    nsfem_solver_config velocitySolver;
+    nsfem_solver_config pressureSolver;
+    int dtype;       /* NSFEM_F64 (plain / mixed) or NSFEM_F32 (reference precision) */
+    double innerTol; /* pcg_mixed inner relative tolerance */
+    int lanesVelocity;
+    int lanesPressure;
+} nsfem_chorin_config;
+
+typedef struct nsfem_step_stats
+{
+    double mseTentX, mseTentY, msePressure, mseFinalX, mseFinalY;
+    double relTent, relPressure, relFinal;
+    double msConvection, msTentative, msPressure, msFinal, msTotal;
+    int itersTent, itersPressure, itersFinal;
+    int outerTent, outerPressure, outerFinal;
+} nsfem_step_stats;
+
+typedef struct nsfem_chorin nsfem_chorin;
+int nsfem_chorin_create(const nsfem_context * c, const nsfem_chorin_config * cfg, nsfem_chorin ** out);
+int nsfem_chorin_destroy(nsfem_chorin * ch);
+int nsfem_chorin_set_time_step(nsfem_chorin * ch, double tau);
+/* One projection step; stats may be NULL.  NSFEM_ENONFINITE on a bad
+ * velocity solve ("Tentative/Final acceleration is bad", chorinCuda.cu:270-273, 349-352). */
+int nsfem_chorin_step(nsfem_chorin * ch, nsfem_step_stats * stats);
+/* Device state: velocity 2N ([ux; uy]), pressure V, in the chorin dtype. */
+int nsfem_chorin_state(nsfem_chorin * ch, void ** velocity, void ** pressure);
+int nsfem_chorin_download(nsfem_chorin * ch, void * velocity, void * pressure);
+int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
+/* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
+int nsfem_chorin_reset(nsfem_chorin * ch);
+/* A = nu K + C(u) in the velocity pattern (C optional): device arrays. */
+int nsfem_chorin_assemble_convection(nsfem_chorin * ch, const void * velocity, void * aValues, void * cValues);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSFEM_HIP_H */

@@ -1,0 +1,312 @@
+#include "chorin.hpp"
+
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace nsfem
+{
+    // Reference-element convection tensors (see ConvectionTensors).
+    __constant__ double gConvTx[216];
+    __constant__ double gConvTy[216];
+
+    // Local 6x6 convection matrix of every P2 element from the current
+    // velocity, stored structure-of-arrays: elemC[(r*6 + c) * E + e].
+    template <typename T>
+    __global__ void __launch_bounds__(256) convectionElementKernel(int E, const int * __restrict__ nodes, const double * __restrict__ geom,
+                                                                   const T * __restrict__ u, int N, T * __restrict__ elemC)
+    {
+        const int e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= E)
+        {
+            return;
+        }
+        const double * g = geom + static_cast<size_t>(e) * 5;
+        const double b00 = g[0], b01 = g[1], b10 = g[2], b11 = g[3], sgn = g[4];
+        double a[6], c[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+        {
+            const int id = nodes[static_cast<size_t>(e) * 6 + k];
+            const double ux = static_cast<double>(u[id]);
+            const double uy = static_cast<double>(u[id + N]);
+            a[k] = b00 * ux + b10 * uy; // reference-frame flow, x component
+            c[k] = b01 * ux + b11 * uy; // y component
+        }
+#pragma unroll
+        for (int rc = 0; rc < 36; rc++)
+        {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+            {
+                s = fma(a[k], gConvTx[rc * 6 + k], s);
+                s = fma(c[k], gConvTy[rc * 6 + k], s);
+            }
+            elemC[static_cast<size_t>(rc) * E + e] = static_cast<T>(sgn * s);
+        }
+    }
+
+    // A[j] = nuK[j] + sum of the element contributions that land on entry j.
+    template <typename T>
+    __global__ void __launch_bounds__(256) convectionGatherKernel(int nnz, const int * __restrict__ cs, const int * __restrict__ ci,
+                                                                  const T * __restrict__ elemC, const T * __restrict__ nuK,
+                                                                  T * __restrict__ A, T * __restrict__ C)
+    {
+        const int j = blockIdx.x * blockDim.x + threadIdx.x;
+        if (j >= nnz)
+        {
+            return;
+        }
+        double s = 0;
+        for (int t = cs[j]; t < cs[j + 1]; t++)
+        {
+            s += static_cast<double>(elemC[ci[t]]);
+        }
+        if (C)
+        {
+            C[j] = static_cast<T>(s);
+        }
+        A[j] = static_cast<T>(static_cast<double>(nuK[j]) + s);
+    }
+
+    void ChorinGpu::upload(DevCsr & d, const CsrD & m, const std::vector<double> * scaled)
+    {
+        d.rows = m.rows;
+        d.cols = m.cols;
+        d.nnz = m.nnz();
+        d.rowStart.upload(m.rowStart);
+        d.column.upload(m.column);
+        const std::vector<double> & v = scaled ? *scaled : m.values;
+        if (cfg_.dtype == F64)
+        {
+            d.values.upload(reinterpret_cast<const char *>(v.data()), v.size() * 8);
+        }
+        else
+        {
+            std::vector<float> f(v.begin(), v.end());
+            d.values.upload(reinterpret_cast<const char *>(f.data()), f.size() * 4);
+        }
+    }
+
+    ChorinGpu::ChorinGpu(const ChorinContext & ctx, const ChorinConfig & cfg) : cfg_(cfg)
+    {
+        checkDtype(cfg_.dtype);
+        tsize_ = dtypeSize(cfg_.dtype);
+        N_ = ctx.numVelocityNodes;
+        V_ = ctx.numPressureNodes;
+        E_ = ctx.convection.numElements;
+        nu_ = ctx.viscosity;
+        const hipStream_t s = libraryStream();
+
+        std::vector<double> nuK(ctx.velocityStiffness.values);
+        for (auto & v : nuK)
+        {
+            v *= nu_;
+        }
+        upload(nuK_, ctx.velocityStiffness, &nuK);
+        upload(A_, ctx.velocityStiffness, &nuK);
+        upload(vpd_, ctx.velocityPressureDiv);
+        upload(pvd_, ctx.pressureVelocityDiv);
+
+        // Element-wise convection data; contributions re-indexed for the
+        // structure-of-arrays element buffer.
+        const auto & cd = ctx.convection;
+        elemNodes_.upload(cd.elementNodes);
+        elemGeom_.upload(cd.geometry);
+        contribStart_.upload(cd.contribStart);
+        {
+            std::vector<int> soa(cd.contribIndex.size());
+            for (size_t t = 0; t < soa.size(); t++)
+            {
+                const int k = cd.contribIndex[t];
+                soa[t] = (k % 36) * E_ + k / 36;
+            }
+            contribIndex_.upload(soa);
+        }
+        elemC_.resize(static_cast<size_t>(E_) * 36 * tsize_);
+        const auto & T = convectionTensors();
+        NSFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gConvTx), &T.tx[0][0][0], sizeof(double) * 216));
+        NSFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gConvTy), &T.ty[0][0][0], sizeof(double) * 216));
+
+        // Dirichlet velocity: X ids, then Y ids offset by N (chorinCuda.cu:41-54)
+        nDir_ = static_cast<int>(ctx.dirichletVx.size() + ctx.dirichletVy.size());
+        std::vector<int> ids;
+        std::vector<double> vals;
+        for (const auto & d : ctx.dirichletVx)
+        {
+            ids.push_back(d.id);
+            vals.push_back(d.value);
+        }
+        for (const auto & d : ctx.dirichletVy)
+        {
+            ids.push_back(d.id + N_);
+            vals.push_back(d.value);
+        }
+        dirIds_.upload(ids.empty() ? std::vector<int>{0} : ids);
+        if (cfg_.dtype == F64)
+        {
+            dirValues_.upload(reinterpret_cast<const char *>(vals.data()), std::max<size_t>(1, vals.size()) * 8);
+        }
+        else
+        {
+            std::vector<float> f(vals.begin(), vals.end());
+            f.resize(std::max<size_t>(1, f.size()));
+            dirValues_.upload(reinterpret_cast<const char *>(f.data()), f.size() * 4);
+        }
+        nIntP_ = static_cast<int>(ctx.internalPressureNodes.size());
+        internalP_.upload(ctx.internalPressureNodes);
+
+        u_.resize(static_cast<size_t>(2) * N_ * tsize_);
+        p_.resize(static_cast<size_t>(V_) * tsize_);
+
+        auto solverOpts = [&](const ChorinSolverConfig & sc, int nch, int lanes)
+        {
+            SolverOptions o = parseSolverName(sc.method);
+            if (o.precision == SolverPrecision::Mixed)
+            {
+                if (cfg_.dtype != F64)
+                {
+                    throw std::invalid_argument("pcg_mixed needs an FP64 state");
+                }
+            }
+            else
+            {
+                o.precision = cfg_.dtype == F64 ? SolverPrecision::F64 : SolverPrecision::F32;
+            }
+            o.numCh = nch;
+            o.maxIters = sc.maxIterations;
+            o.target = sc.target;
+            o.mseCheckInterval = sc.checkInterval;
+            o.innerTol = cfg_.innerTol;
+            o.lanes = lanes;
+            return o;
+        };
+        vSolver_ = std::make_unique<McgsSolver>(std::vector<const CsrD *>{&ctx.velocityMass}, solverOpts(cfg_.velocity, 2, cfg_.lanesVelocity));
+        pSolver_ = std::make_unique<McgsSolver>(std::vector<const CsrD *>{&ctx.pressureStiffnessInternal},
+                                                solverOpts(cfg_.pressure, 1, cfg_.lanesPressure));
+        for (auto & e : ev_)
+        {
+            NSFEM_HIP(hipEventCreate(&e));
+        }
+        resetState();
+        (void)s;
+    }
+
+    ChorinGpu::~ChorinGpu()
+    {
+        for (auto & e : ev_)
+        {
+            (void)hipEventDestroy(e);
+        }
+    }
+
+    void ChorinGpu::imposeDirichlet(hipStream_t s) { scatter(cfg_.dtype, nDir_, dirIds_.get(), dirValues_.get(), u_.get(), s); }
+
+    void ChorinGpu::resetState()
+    {
+        const hipStream_t s = libraryStream();
+        fill(cfg_.dtype, 2L * N_, 0.0, u_.get(), s);
+        fill(cfg_.dtype, V_, 0.0, p_.get(), s);
+        imposeDirichlet(s);
+        fill(pSolver_->ioDtype(), static_cast<long>(pSolver_->n()), 0.0, pSolver_->ioSol(), s);
+        syncLibraryStream();
+    }
+
+    void ChorinGpu::assembleConvection(const void * u, void * aOut, void * cOut, hipStream_t s)
+    {
+        const int gE = ceilDiv(E_, 256);
+        const int gN = ceilDiv(A_.nnz, 256);
+        if (cfg_.dtype == F64)
+        {
+            convectionElementKernel<double><<<gE, 256, 0, s>>>(E_, elemNodes_.get(), elemGeom_.get(), static_cast<const double *>(u), N_,
+                                                               reinterpret_cast<double *>(elemC_.get()));
+            convectionGatherKernel<double><<<gN, 256, 0, s>>>(A_.nnz, contribStart_.get(), contribIndex_.get(),
+                                                              reinterpret_cast<const double *>(elemC_.get()),
+                                                              reinterpret_cast<const double *>(nuK_.values.get()), static_cast<double *>(aOut),
+                                                              static_cast<double *>(cOut));
+        }
+        else
+        {
+            convectionElementKernel<float><<<gE, 256, 0, s>>>(E_, elemNodes_.get(), elemGeom_.get(), static_cast<const float *>(u), N_,
+                                                              reinterpret_cast<float *>(elemC_.get()));
+            convectionGatherKernel<float><<<gN, 256, 0, s>>>(A_.nnz, contribStart_.get(), contribIndex_.get(),
+                                                             reinterpret_cast<const float *>(elemC_.get()),
+                                                             reinterpret_cast<const float *>(nuK_.values.get()), static_cast<float *>(aOut),
+                                                             static_cast<float *>(cOut));
+        }
+        NSFEM_HIP(hipGetLastError());
+    }
+
+    void ChorinGpu::step(ChorinStepStats * stats)
+    {
+        const hipStream_t s = libraryStream();
+        const int dt = cfg_.dtype;
+        const long twoN = 2L * N_;
+        ChorinStepStats st{};
+        NSFEM_HIP(hipEventRecord(ev_[0], s));
+
+        // Convection: A = nu K + C(u)
+        assembleConvection(u_.get(), A_.values.get(), nullptr, s);
+        NSFEM_HIP(hipEventRecord(ev_[1], s));
+
+        // Tentative velocity: M a = A u;  u -= tau a
+        csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.lanesVelocity, s);
+        fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
+        vSolver_->solve();
+        st.mseTentX = vSolver_->lastMse(0, 0);
+        st.mseTentY = vSolver_->lastMse(0, 1);
+        st.relTent = std::max(vSolver_->lastRelRes(0, 0), vSolver_->lastRelRes(0, 1));
+        st.itersTent = vSolver_->lastIterations(0);
+        st.outerTent = vSolver_->lastOuterIterations();
+        if (!std::isfinite(st.mseTentX) || !std::isfinite(st.mseTentY) || vSolver_->lastNonFinite())
+        {
+            throw NonFiniteError("Tentative acceleration is bad");
+        }
+        axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+        imposeDirichlet(s);
+        NSFEM_HIP(hipEventRecord(ev_[2], s));
+
+        // Pressure: Kp_int p = -(1/tau) (VPD u)[internal], warm start
+        csrMul(vpd_.view(dt), 1, u_.get(), 0, pSolver_->ioRhs(), 0, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
+        pSolver_->solve();
+        st.msePressure = pSolver_->lastMse(0, 0);
+        st.relPressure = pSolver_->lastRelRes(0, 0);
+        st.itersPressure = pSolver_->lastIterations(0);
+        st.outerPressure = pSolver_->lastOuterIterations();
+        scatter(dt, nIntP_, internalP_.get(), pSolver_->ioSol(), p_.get(), s);
+        NSFEM_HIP(hipEventRecord(ev_[3], s));
+
+        // Projection: M a = PVD p;  u -= tau a
+        csrMul(pvd_.view(dt), 1, p_.get(), 0, vSolver_->ioRhs(), 0, 1.0, 0.0, nullptr, 2 * N_, 0, s);
+        fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
+        vSolver_->solve();
+        st.mseFinalX = vSolver_->lastMse(0, 0);
+        st.mseFinalY = vSolver_->lastMse(0, 1);
+        st.relFinal = std::max(vSolver_->lastRelRes(0, 0), vSolver_->lastRelRes(0, 1));
+        st.itersFinal = vSolver_->lastIterations(0);
+        st.outerFinal = vSolver_->lastOuterIterations();
+        if (!std::isfinite(st.mseFinalX) || !std::isfinite(st.mseFinalY) || vSolver_->lastNonFinite())
+        {
+            throw NonFiniteError("Final acceleration is bad");
+        }
+        axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+        imposeDirichlet(s);
+        NSFEM_HIP(hipEventRecord(ev_[4], s));
+        NSFEM_HIP(hipEventSynchronize(ev_[4]));
+        float ms[4];
+        for (int i = 0; i < 4; i++)
+        {
+            NSFEM_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
+        }
+        st.msConvection = ms[0];
+        st.msTentative = ms[1];
+        st.msPressure = ms[2];
+        st.msFinal = ms[3];
+        st.msTotal = ms[0] + ms[1] + ms[2] + ms[3];
+        if (stats)
+        {
+            *stats = st;
+        }
+    }
+} // namespace nsfem

@@ -1,0 +1,957 @@
+// Device kernels for the sparse hot path on gfx950 (wave64).
+//
+// Layout conventions
+//   * CSR: int32 rowStart/column, values of type T.  Rows are handled by a
+//     group of LANES consecutive lanes of one wavefront (LANES | 64): the
+//     lanes stride over the row, so one wavefront reads 64/LANES adjacent row
+//     segments as one coalesced stream, and the row sum is finished with
+//     xor-shuffles inside the lane group.
+//   * Multi-channel vectors are planar: channel c of vector v is v[c*ld + i]
+//     (the reference's [ux; uy] layout, chorinCuda.cu:189-196).
+//   * Reductions are deterministic: each block writes its partial, the last
+//     block to arrive (device-scope ticket) sums the partials in block order.
+//   * Solver kernels walk a block map: every block owns one row range that
+//     lies inside one (colour, system) chunk of the colour-major permuted
+//     system, so colour sweeps launch only their colour's blocks and every
+//     block knows which system's scalars it updates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nsfem
+{
+    constexpr int kBlock = 256;
+    constexpr int kWaves = kBlock / 64;
+
+    template <typename T>
+    __device__ __forceinline__ T fmaT(T a, T b, T c);
+    template <>
+    __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+    template <>
+    __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+    template <int LANES, typename T>
+    __device__ __forceinline__ T laneGroupSum(T v)
+    {
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1)
+        {
+            v += __shfl_xor(v, off, 64);
+        }
+        return v;
+    }
+
+    // Sum of v over the block (256 threads); result valid in thread 0.
+    template <int K>
+    __device__ __forceinline__ void blockSum(double (&v)[K], double * sm /* K * kWaves */)
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+            {
+                v[k] += __shfl_xor(v[k], off, 64);
+            }
+        }
+        const int wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                sm[k * kWaves + wave] = v[k];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                double s = 0;
+                for (int w = 0; w < kWaves; w++)
+                {
+                    s += sm[k * kWaves + w];
+                }
+                v[k] = s;
+            }
+        }
+        __syncthreads();
+    }
+
+    // Block partial -> partials[block*K + k]; returns true in the block that
+    // arrived last (every thread of it), after an acquire fence.
+    template <int K>
+    __device__ __forceinline__ bool arriveLast(const double (&v)[K], double * partials, unsigned * counter)
+    {
+        __shared__ bool last;
+        if (threadIdx.x == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                partials[static_cast<size_t>(blockIdx.x) * K + k] = v[k];
+            }
+            __threadfence();
+            const unsigned ticket = atomicAdd(counter, 1u);
+            last = ticket == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (last)
+        {
+            __threadfence();
+        }
+        return last;
+    }
+
+    // ------------------------------------------------------------------
+    // Plain CSR products: y = alpha * A x (+ beta y), NCH planar channels.
+    // rowIds != nullptr: output row i is A's row rowIds[i] (fused gather).
+    // ------------------------------------------------------------------
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) csrMulKernel(int rows, const int * __restrict__ rowStart,
+                                                            const int * __restrict__ column, const T * __restrict__ values,
+                                                            const T * __restrict__ x, long ldx, T * __restrict__ y, long ldy,
+                                                            T alpha, T beta, const int * __restrict__ rowIds)
+    {
+        const long gid = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+        const int row = static_cast<int>(gid / LANES);
+        const int lane = threadIdx.x & (LANES - 1);
+        T acc[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            acc[c] = T(0);
+        }
+        if (row < rows)
+        {
+            const int src = rowIds ? rowIds[row] : row;
+            const int j1 = rowStart[src + 1];
+            for (int j = rowStart[src] + lane; j < j1; j += LANES)
+            {
+                const T v = values[j];
+                const int col = column[j];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    acc[c] = fmaT(v, x[col + c * ldx], acc[c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            acc[c] = laneGroupSum<LANES>(acc[c]);
+        }
+        if (row < rows && lane == 0)
+        {
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                T out = alpha * acc[c];
+                if (beta != T(0))
+                {
+                    out = fmaT(beta, y[row + c * ldy], out);
+                }
+                y[row + c * ldy] = out;
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // BLAS-1 style kernels (grid-stride, n up to 2^31 * channels)
+    // ------------------------------------------------------------------
+    template <typename T>
+    __global__ void axpyKernel(long n, T alpha, const T * __restrict__ src, T * __restrict__ dst)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            dst[i] = fmaT(alpha, src[i], dst[i]);
+        }
+    }
+
+    template <typename T>
+    __global__ void scaleKernel(long n, T alpha, T * __restrict__ dst)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            dst[i] *= alpha;
+        }
+    }
+
+    template <typename T>
+    __global__ void geamKernel(long n, T alpha, const T * __restrict__ a, T beta, const T * __restrict__ b, T * __restrict__ c)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            c[i] = alpha * a[i] + beta * b[i];
+        }
+    }
+
+    template <typename T>
+    __global__ void fillKernel(long n, T value, T * __restrict__ dst)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            dst[i] = value;
+        }
+    }
+
+    template <typename S, typename D>
+    __global__ void convertKernel(long n, const S * __restrict__ src, D * __restrict__ dst)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            dst[i] = static_cast<D>(src[i]);
+        }
+    }
+
+    template <typename T>
+    __global__ void scatterKernel(int nnz, const int * __restrict__ idx, const T * __restrict__ vals, T * __restrict__ dense)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < nnz)
+        {
+            dense[idx[i]] = vals[i];
+        }
+    }
+
+    template <typename T>
+    __global__ void gatherKernel(int nnz, const T * __restrict__ dense, const int * __restrict__ idx, T * __restrict__ vals)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < nnz)
+        {
+            vals[i] = dense[idx[i]];
+        }
+    }
+
+    // Two-level dot products: K pairs (x_k, y_k) of length n each.
+    template <typename T, int K>
+    __global__ void __launch_bounds__(kBlock) dotKernel(long n, const T * __restrict__ x, long ldx, const T * __restrict__ y,
+                                                         long ldy, double * partials, unsigned * counter, double * out)
+    {
+        __shared__ double sm[K * kWaves];
+        double acc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+        {
+            acc[k] = 0;
+        }
+        for (long i = blockIdx.x * static_cast<long>(kBlock) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * kBlock)
+        {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                acc[k] = __builtin_fma(static_cast<double>(x[i + k * ldx]), static_cast<double>(y[i + k * ldy]), acc[k]);
+            }
+        }
+        blockSum<K>(acc, sm);
+        if (arriveLast<K>(acc, partials, counter))
+        {
+            double tot[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                tot[k] = 0;
+            }
+            for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+            {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    tot[k] += partials[static_cast<size_t>(b) * K + k];
+                }
+            }
+            blockSum<K>(tot, sm);
+            if (threadIdx.x == 0)
+            {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    out[k] = tot[k];
+                }
+                *counter = 0;
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // Colour permutation of planar vectors.
+    // fwd: dst[c*n + i] = src[c*n + perm[i]];   inv: dst[c*n + perm[i]] = src[c*n + i]
+    // ------------------------------------------------------------------
+    template <typename S, typename D>
+    __global__ void permuteKernel(int n, int nch, const int * __restrict__ perm, const S * __restrict__ src,
+                                  D * __restrict__ dst, int inverse)
+    {
+        const long total = static_cast<long>(n) * nch;
+        for (long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; t < total; t += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            const int c = static_cast<int>(t / n);
+            const int i = static_cast<int>(t - static_cast<long>(c) * n);
+            const long p = static_cast<long>(c) * n + perm[i];
+            if (inverse)
+            {
+                dst[p] = static_cast<D>(src[t]);
+            }
+            else
+            {
+                dst[t] = static_cast<D>(src[p]);
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // Multicolour Gauss-Seidel / PCG engine
+    // ------------------------------------------------------------------
+    template <typename T>
+    struct McgsMatrix
+    {
+        int n;
+        const int * rowStart; // stripped (off-diagonal) CSR
+        const int * rowSplit; // end of the lower-colour part of each row
+        const int * column;
+        const T * values;
+        const T * diag;
+        const T * invDiag;
+    };
+
+    struct BlockMap
+    {
+        const int2 * blocks;       // [rowBegin, rowEnd) per block, colour-major
+        const int * blockSystem;   // system of each block
+        const int * sysBlockStart; // numSystems + 1
+        const int * sysBlocks;     // blocks grouped by system
+        const int * systemRows;    // rows per system (for RMS norms)
+        int numSystems;
+    };
+
+    // Per (system, channel) scalars, index s*NCH + c, plus global flags.
+    struct SolveState
+    {
+        double * rho;
+        double * alpha;
+        double * beta;
+        double * rr;    // |r|^2 after the last update
+        double * bb;    // |b|^2
+        double * ref;   // reference |.|^2 the tolerance is relative to
+        double * mse;   // RMS residual (gs mode)
+        int * active;   // 1 while the channel iterates
+        int * iters;    // iterations done / iteration index at convergence
+        int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen
+        unsigned * counter;
+        double * partials; // blocks * 2 * NCH
+        double tol2;       // relative tolerance squared (pcg) or target RMS (gs)
+    };
+
+
+    // Finishes a per-system reduction inside the last block: out[s*K + k] is
+    // the sum over the blocks of system s of partials[b*K + k].  The summation
+    // order depends only on the block map, so results are reproducible.
+    template <int K>
+    __device__ __forceinline__ void systemTotals(const BlockMap & bm, const double * partials, double * out, double * sm)
+    {
+        if (bm.numSystems == 1)
+        {
+            double acc[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                acc[k] = 0;
+            }
+            const int nb = bm.sysBlockStart[1];
+#pragma unroll 4
+            for (int i = threadIdx.x; i < nb; i += kBlock)
+            {
+                const int b = bm.sysBlocks[i];
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    acc[k] += partials[static_cast<size_t>(b) * K + k];
+                }
+            }
+            blockSum<K>(acc, sm);
+            if (threadIdx.x == 0)
+            {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    out[k] = acc[k];
+                }
+            }
+            __syncthreads();
+            return;
+        }
+        // One wavefront per system, lanes stride the system's blocks.
+        const int wave = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        for (int s = wave; s < bm.numSystems; s += kWaves)
+        {
+            double acc[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+                acc[k] = 0;
+            }
+            for (int i = bm.sysBlockStart[s] + lane; i < bm.sysBlockStart[s + 1]; i += 64)
+            {
+                const int b = bm.sysBlocks[i];
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    acc[k] += partials[static_cast<size_t>(b) * K + k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++)
+            {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1)
+                {
+                    acc[k] += __shfl_xor(acc[k], off, 64);
+                }
+            }
+            if (lane == 0)
+            {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                {
+                    out[s * K + k] = acc[k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    template <int NCH>
+    __device__ __forceinline__ bool systemActive(const SolveState & st, int s)
+    {
+        int a = 0;
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            a |= st.active[s * NCH + c];
+        }
+        return a != 0;
+    }
+
+    // Lane-group sum over j in [j0, j1) of val[j] * v[col[j] + c*ld].
+    template <typename T, int LANES, int NCH>
+    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const T * __restrict__ v, long ld,
+                                           T (&acc)[NCH])
+    {
+        const int lane = threadIdx.x & (LANES - 1);
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            acc[c] = T(0);
+        }
+        for (int j = j0 + lane; j < j1; j += LANES)
+        {
+            const T a = A.values[j];
+            const int col = A.column[j];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                acc[c] = fmaT(a, v[col + c * ld], acc[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            acc[c] = laneGroupSum<LANES>(acc[c]);
+        }
+    }
+
+    template <int LANES>
+    __device__ __forceinline__ bool groupLeader()
+    {
+        return (threadIdx.x & (LANES - 1)) == 0;
+    }
+
+    // Gauss-Seidel update of one colour.  lowerOnly = 0: full row (plain GS
+    // sweep, backward half of symmetric GS); lowerOnly = 1: lower-colour part
+    // only (forward half of symmetric GS started from zero).
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st,
+                                                              const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
+    {
+        if (st.flags[0])
+        {
+            return;
+        }
+        const int blk = blockOffset + blockIdx.x;
+        if (!systemActive<NCH>(st, bm.blockSystem[blk]))
+        {
+            return;
+        }
+        const int2 range = bm.blocks[blk];
+        for (int base = range.x; base < range.y; base += kBlock / LANES)
+        {
+            const int row = base + static_cast<int>(threadIdx.x) / LANES;
+            const bool valid = row < range.y;
+            const int j0 = valid ? A.rowStart[row] : 0;
+            const int j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
+            T acc[NCH];
+            rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+            if (valid && groupLeader<LANES>())
+            {
+                const T inv = A.invDiag[row];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = row + static_cast<long>(c) * A.n;
+                    x[i] = (b[i] - acc[c]) * inv;
+                }
+            }
+        }
+    }
+
+    // Residual check of the reference GS solver: mse = |b - A x| / sqrt(rows)
+    // per system and channel; a system stops when all its channels are below
+    // the target (gaussSeidel.cu:267-290, 354-380).  st.tol2 holds the target.
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) gsResidualKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
+                                                               const T * __restrict__ x, int iter, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return; // the whole grid skips, nobody touches the counter
+        }
+        double part[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            part[c] = 0;
+        }
+        if (systemActive<NCH>(st, bm.blockSystem[blockIdx.x]))
+        {
+            const int2 range = bm.blocks[blockIdx.x];
+            for (int base = range.x; base < range.y; base += kBlock / LANES)
+            {
+                const int row = base + static_cast<int>(threadIdx.x) / LANES;
+                const bool valid = row < range.y;
+                const int j0 = valid ? A.rowStart[row] : 0;
+                const int j1 = valid ? A.rowStart[row + 1] : 0;
+                T acc[NCH];
+                rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+                if (valid && groupLeader<LANES>())
+                {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const long i = row + static_cast<long>(c) * A.n;
+                        const T ax = fmaT(A.diag[row], x[i], acc[c]);
+                        const double r = static_cast<double>(ax - b[i]);
+                        part[c] = fma(r, r, part[c]);
+                    }
+                }
+            }
+        }
+        blockSum<NCH>(part, sm);
+        if (arriveLast<NCH>(part, st.partials, st.counter))
+        {
+            systemTotals<NCH>(bm, st.partials, totals, sm);
+            if (threadIdx.x == 0)
+            {
+                int anyActive = 0;
+                for (int sys = 0; sys < bm.numSystems; sys++)
+                {
+                    if (!systemActive<NCH>(st, sys))
+                    {
+                        continue;
+                    }
+                    bool all = true;
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const double m = sqrt(totals[sys * NCH + c] / bm.systemRows[sys]);
+                        st.mse[sys * NCH + c] = m;
+                        if (!(m < st.tol2))
+                        {
+                            all = false;
+                        }
+                        if (!(m == m) || m > 1e300)
+                        {
+                            st.flags[2] = 1;
+                        }
+                    }
+                    if (all)
+                    {
+                        for (int c = 0; c < NCH; c++)
+                        {
+                            st.active[sys * NCH + c] = 0;
+                            st.iters[sys * NCH + c] = iter;
+                        }
+                    }
+                    else
+                    {
+                        anyActive = 1;
+                    }
+                }
+                if (!anyActive)
+                {
+                    st.flags[0] = 1;
+                }
+                *st.counter = 0;
+            }
+        }
+    }
+
+    // PCG set-up: r = b - A x (or r = b with x = 0), z = r / d on colour 0,
+    // per-channel |b|^2 and |r|^2, convergence test.  gate (optional) switches
+    // channels off whose outer refinement loop has already converged.
+    // r64 (optional): the residual is also stored in double (outer refinement).
+    template <typename T, typename TR, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgInitKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
+                                                            T * __restrict__ x, TR * __restrict__ r, T * __restrict__ z,
+                                                            int color0End, int zeroGuess, const int * __restrict__ gate,
+                                                            double * totals)
+    {
+        __shared__ double sm[2 * NCH * kWaves];
+        double part[2 * NCH];
+#pragma unroll
+        for (int k = 0; k < 2 * NCH; k++)
+        {
+            part[k] = 0;
+        }
+        const int2 range = bm.blocks[blockIdx.x];
+        for (int base = range.x; base < range.y; base += kBlock / LANES)
+        {
+            const int row = base + static_cast<int>(threadIdx.x) / LANES;
+            const bool valid = row < range.y;
+            T acc[NCH];
+            if (!zeroGuess)
+            {
+                const int j0 = valid ? A.rowStart[row] : 0;
+                const int j1 = valid ? A.rowStart[row + 1] : 0;
+                rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+            }
+            if (valid && groupLeader<LANES>())
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = row + static_cast<long>(c) * A.n;
+                    const T bi = b[i];
+                    T ri;
+                    if (zeroGuess)
+                    {
+                        x[i] = T(0);
+                        ri = bi;
+                    }
+                    else
+                    {
+                        ri = bi - fmaT(A.diag[row], x[i], acc[c]);
+                    }
+                    r[i] = static_cast<TR>(ri);
+                    if (row < color0End)
+                    {
+                        z[i] = ri * A.invDiag[row];
+                    }
+                    part[c] = fma(static_cast<double>(bi), static_cast<double>(bi), part[c]);
+                    part[NCH + c] = fma(static_cast<double>(ri), static_cast<double>(ri), part[NCH + c]);
+                }
+            }
+        }
+        blockSum<2 * NCH>(part, sm);
+        if (arriveLast<2 * NCH>(part, st.partials, st.counter))
+        {
+            systemTotals<2 * NCH>(bm, st.partials, totals, sm);
+            if (threadIdx.x == 0)
+            {
+                int anyActive = 0;
+                for (int s = 0; s < bm.numSystems; s++)
+                {
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const int k = s * NCH + c;
+                        const double bb = totals[s * 2 * NCH + c];
+                        const double rr = totals[s * 2 * NCH + NCH + c];
+                        st.bb[k] = bb;
+                        st.rr[k] = rr;
+                        st.ref[k] = bb > 0 ? bb : rr;
+                        st.iters[k] = 0;
+                        st.alpha[k] = 0;
+                        st.beta[k] = 0;
+                        st.rho[k] = 0;
+                        if (!(rr == rr) || rr > 1e300)
+                        {
+                            st.flags[2] = 1;
+                        }
+                        int act = rr > st.tol2 * st.ref[k] ? 1 : 0;
+                        if (gate && !gate[k])
+                        {
+                            act = 0;
+                        }
+                        st.active[k] = act;
+                        anyActive |= act;
+                    }
+                }
+                st.flags[0] = anyActive ? 0 : 1;
+                st.flags[1] = 1;
+                *st.counter = 0;
+            }
+        }
+    }
+
+    // Backward colour 0 of symmetric GS fused with rho = r.z and beta.
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ r,
+                                                           T * __restrict__ z, int color0End, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return;
+        }
+        double part[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            part[c] = 0;
+        }
+        if (systemActive<NCH>(st, bm.blockSystem[blockIdx.x]))
+        {
+            const int2 range = bm.blocks[blockIdx.x];
+            const bool c0 = range.x < color0End; // blocks never straddle colours
+            for (int base = range.x; base < range.y; base += kBlock / LANES)
+            {
+                const int row = base + static_cast<int>(threadIdx.x) / LANES;
+                const bool valid = row < range.y;
+                T acc[NCH];
+                if (c0)
+                {
+                    const int j0 = valid ? A.rowStart[row] : 0;
+                    const int j1 = valid ? A.rowStart[row + 1] : 0;
+                    rowDot<T, LANES, NCH>(A, j0, j1, z, A.n, acc);
+                }
+                if (valid && groupLeader<LANES>())
+                {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const long i = row + static_cast<long>(c) * A.n;
+                        const T ri = r[i];
+                        T zi;
+                        if (c0)
+                        {
+                            zi = (ri - acc[c]) * A.invDiag[row];
+                            z[i] = zi;
+                        }
+                        else
+                        {
+                            zi = z[i];
+                        }
+                        part[c] = fma(static_cast<double>(ri), static_cast<double>(zi), part[c]);
+                    }
+                }
+            }
+        }
+        blockSum<NCH>(part, sm);
+        if (arriveLast<NCH>(part, st.partials, st.counter))
+        {
+            systemTotals<NCH>(bm, st.partials, totals, sm);
+            if (threadIdx.x == 0)
+            {
+                const bool first = st.flags[1] != 0;
+                for (int k = 0; k < bm.numSystems * NCH; k++)
+                {
+                    if (!st.active[k])
+                    {
+                        continue;
+                    }
+                    const double rhoNew = totals[k];
+                    st.beta[k] = (first || st.rho[k] == 0) ? 0.0 : rhoNew / st.rho[k];
+                    st.rho[k] = rhoNew;
+                    if (!(rhoNew == rhoNew))
+                    {
+                        st.flags[2] = 1;
+                    }
+                }
+                *st.counter = 0;
+            }
+        }
+    }
+
+    // p = z + beta p;  q = A z + beta q  (= A p);  alpha = rho / (p.q)
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ z,
+                                                                 T * __restrict__ p, T * __restrict__ q, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return;
+        }
+        double part[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            part[c] = 0;
+        }
+        const int s = bm.blockSystem[blockIdx.x];
+        if (systemActive<NCH>(st, s))
+        {
+            const bool first = st.flags[1] != 0;
+            T beta[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                beta[c] = static_cast<T>(st.beta[s * NCH + c]);
+            }
+            const int2 range = bm.blocks[blockIdx.x];
+            for (int base = range.x; base < range.y; base += kBlock / LANES)
+            {
+                const int row = base + static_cast<int>(threadIdx.x) / LANES;
+                const bool valid = row < range.y;
+                const int j0 = valid ? A.rowStart[row] : 0;
+                const int j1 = valid ? A.rowStart[row + 1] : 0;
+                T acc[NCH];
+                rowDot<T, LANES, NCH>(A, j0, j1, z, A.n, acc);
+                if (valid && groupLeader<LANES>())
+                {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const long i = row + static_cast<long>(c) * A.n;
+                        const T zi = z[i];
+                        const T az = fmaT(A.diag[row], zi, acc[c]);
+                        T pi, qi;
+                        if (first)
+                        {
+                            pi = zi;
+                            qi = az;
+                        }
+                        else
+                        {
+                            pi = fmaT(beta[c], p[i], zi);
+                            qi = fmaT(beta[c], q[i], az);
+                        }
+                        p[i] = pi;
+                        q[i] = qi;
+                        part[c] = fma(static_cast<double>(pi), static_cast<double>(qi), part[c]);
+                    }
+                }
+            }
+        }
+        blockSum<NCH>(part, sm);
+        if (arriveLast<NCH>(part, st.partials, st.counter))
+        {
+            systemTotals<NCH>(bm, st.partials, totals, sm);
+            if (threadIdx.x == 0)
+            {
+                for (int k = 0; k < bm.numSystems * NCH; k++)
+                {
+                    const double pq = totals[k];
+                    st.alpha[k] = (st.active[k] && pq != 0) ? st.rho[k] / pq : 0.0;
+                }
+                st.flags[1] = 0;
+                *st.counter = 0;
+            }
+        }
+    }
+
+    // x += alpha p;  r -= alpha q;  z = r / d on colour 0;  convergence test.
+    template <typename T, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ p,
+                                                              const T * __restrict__ q, T * __restrict__ x, T * __restrict__ r,
+                                                              T * __restrict__ z, int color0End, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return;
+        }
+        double part[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            part[c] = 0;
+        }
+        const int s = bm.blockSystem[blockIdx.x];
+        if (systemActive<NCH>(st, s))
+        {
+            T alpha[NCH];
+            int act[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                alpha[c] = static_cast<T>(st.alpha[s * NCH + c]);
+                act[c] = st.active[s * NCH + c];
+            }
+            const int2 range = bm.blocks[blockIdx.x];
+            for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = row + static_cast<long>(c) * A.n;
+                    T ri = r[i];
+                    if (act[c])
+                    {
+                        x[i] = fmaT(alpha[c], p[i], x[i]);
+                        ri = fmaT(-alpha[c], q[i], ri);
+                        r[i] = ri;
+                    }
+                    if (row < color0End)
+                    {
+                        z[i] = ri * A.invDiag[row];
+                    }
+                    part[c] = fma(static_cast<double>(ri), static_cast<double>(ri), part[c]);
+                }
+            }
+        }
+        blockSum<NCH>(part, sm);
+        if (arriveLast<NCH>(part, st.partials, st.counter))
+        {
+            systemTotals<NCH>(bm, st.partials, totals, sm);
+            if (threadIdx.x == 0)
+            {
+                int anyActive = 0;
+                for (int k = 0; k < bm.numSystems * NCH; k++)
+                {
+                    if (!st.active[k])
+                    {
+                        continue;
+                    }
+                    const double rr = totals[k];
+                    st.rr[k] = rr;
+                    st.iters[k] += 1;
+                    if (!(rr == rr) || rr > 1e300)
+                    {
+                        st.flags[2] = 1;
+                        st.active[k] = 0;
+                        continue;
+                    }
+                    if (rr <= st.tol2 * st.ref[k])
+                    {
+                        st.active[k] = 0;
+                    }
+                    else
+                    {
+                        anyActive = 1;
+                    }
+                }
+                if (!anyActive)
+                {
+                    st.flags[0] = 1;
+                }
+                *st.counter = 0;
+            }
+        }
+    }
+
+    // x64 += (double) e  over all rows and channels (outer refinement update).
+    template <typename S>
+    __global__ void addConvertKernel(long n, const S * __restrict__ e, double * __restrict__ x)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            x[i] += static_cast<double>(e[i]);
+        }
+    }
+} // namespace nsfem

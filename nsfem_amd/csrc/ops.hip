@@ -1,0 +1,349 @@
+#include "ops.hpp"
+
+#include <algorithm>
+#include <mutex>
+
+#include "kernels.hpp"
+
+namespace nsfem
+{
+    hipStream_t libraryStream()
+    {
+        static hipStream_t stream = []
+        {
+            hipStream_t s = nullptr;
+            NSFEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            return s;
+        }();
+        return stream;
+    }
+
+    void syncLibraryStream() { NSFEM_HIP(hipStreamSynchronize(libraryStream())); }
+
+    void checkDtype(int dt)
+    {
+        if (dt != F32 && dt != F64)
+        {
+            throw std::invalid_argument("Unknown dtype " + std::to_string(dt) + " (0 = f32, 1 = f64)");
+        }
+    }
+
+    int lanesForMeanRow(double m)
+    {
+        if (m < 5)
+        {
+            return 4;
+        }
+        if (m < 12)
+        {
+            return 8;
+        }
+        if (m < 24)
+        {
+            return 16;
+        }
+        if (m < 48)
+        {
+            return 32;
+        }
+        return 64;
+    }
+
+    int elementwiseGrid(long n)
+    {
+        const long blocks = (n + 255) / 256;
+        return static_cast<int>(std::max(1L, std::min(blocks, 8192L)));
+    }
+
+    namespace
+    {
+        template <typename T>
+        void csrMulT(const CsrView & A, int numCh, const void * x, long ldx, void * y, long ldy, double alpha, double beta,
+                     const int * rowIds, int outRows, int lanes, hipStream_t s)
+        {
+            const T * vals = static_cast<const T *>(A.values);
+            auto launch = [&]<int L, int C>(const T * xs, T * ys)
+            {
+                const long threads = static_cast<long>(outRows) * L;
+                const int grid = ceilDiv(threads, kBlock);
+                csrMulKernel<T, L, C><<<grid, kBlock, 0, s>>>(outRows, A.rowStart, A.column, vals, xs, ldx, ys, ldy,
+                                                              static_cast<T>(alpha), static_cast<T>(beta), rowIds);
+            };
+            auto byLanes = [&]<int C>(const T * xs, T * ys)
+            {
+                switch (lanes)
+                {
+                case 4: launch.template operator()<4, C>(xs, ys); break;
+                case 8: launch.template operator()<8, C>(xs, ys); break;
+                case 16: launch.template operator()<16, C>(xs, ys); break;
+                case 32: launch.template operator()<32, C>(xs, ys); break;
+                case 64: launch.template operator()<64, C>(xs, ys); break;
+                default: throw std::invalid_argument("lanes must be 4, 8, 16, 32 or 64");
+                }
+            };
+            // Channels are processed two at a time (the matrix is read once per pair).
+            for (int c0 = 0; c0 < numCh; c0 += 2)
+            {
+                const T * xs = static_cast<const T *>(x) + static_cast<long>(c0) * ldx;
+                T * ys = static_cast<T *>(y) + static_cast<long>(c0) * ldy;
+                if (numCh - c0 >= 2)
+                {
+                    byLanes.template operator()<2>(xs, ys);
+                }
+                else
+                {
+                    byLanes.template operator()<1>(xs, ys);
+                }
+            }
+            NSFEM_HIP(hipGetLastError());
+        }
+    } // namespace
+
+    void csrMul(const CsrView & A, int numCh, const void * x, long ldx, void * y, long ldy, double alpha, double beta,
+                const int * rowIds, int outRows, int lanes, hipStream_t s)
+    {
+        if (numCh < 1)
+        {
+            throw std::invalid_argument("SPMM channels must be at least 1");
+        }
+        if (outRows == 0)
+        {
+            return;
+        }
+        if (lanes <= 0)
+        {
+            lanes = lanesForMeanRow(A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 1.0);
+        }
+        if (A.dtype == F64)
+        {
+            csrMulT<double>(A, numCh, x, ldx, y, ldy, alpha, beta, rowIds, outRows, lanes, s);
+        }
+        else
+        {
+            csrMulT<float>(A, numCh, x, ldx, y, ldy, alpha, beta, rowIds, outRows, lanes, s);
+        }
+    }
+
+#define NSFEM_TYPED(dt, CALL_F32, CALL_F64) \
+    do                                      \
+    {                                       \
+        checkDtype(dt);                     \
+        if ((dt) == F64)                    \
+        {                                   \
+            CALL_F64;                       \
+        }                                   \
+        else                                \
+        {                                   \
+            CALL_F32;                       \
+        }                                   \
+        NSFEM_HIP(hipGetLastError());       \
+    } while (0)
+
+    void axpy(int dt, long n, double alpha, const void * src, void * dst, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        NSFEM_TYPED(dt,
+                    (axpyKernel<float><<<g, 256, 0, s>>>(n, static_cast<float>(alpha), static_cast<const float *>(src), static_cast<float *>(dst))),
+                    (axpyKernel<double><<<g, 256, 0, s>>>(n, alpha, static_cast<const double *>(src), static_cast<double *>(dst))));
+    }
+
+    void scale(int dt, long n, double alpha, void * dst, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        NSFEM_TYPED(dt, (scaleKernel<float><<<g, 256, 0, s>>>(n, static_cast<float>(alpha), static_cast<float *>(dst))),
+                    (scaleKernel<double><<<g, 256, 0, s>>>(n, alpha, static_cast<double *>(dst))));
+    }
+
+    void geam(int dt, long n, double alpha, const void * a, double beta, const void * b, void * c, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        NSFEM_TYPED(dt,
+                    (geamKernel<float><<<g, 256, 0, s>>>(n, static_cast<float>(alpha), static_cast<const float *>(a), static_cast<float>(beta),
+                                                         static_cast<const float *>(b), static_cast<float *>(c))),
+                    (geamKernel<double><<<g, 256, 0, s>>>(n, alpha, static_cast<const double *>(a), beta, static_cast<const double *>(b),
+                                                          static_cast<double *>(c))));
+    }
+
+    void fill(int dt, long n, double value, void * dst, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        NSFEM_TYPED(dt, (fillKernel<float><<<g, 256, 0, s>>>(n, static_cast<float>(value), static_cast<float *>(dst))),
+                    (fillKernel<double><<<g, 256, 0, s>>>(n, value, static_cast<double *>(dst))));
+    }
+
+    void fillInt(long n, int value, int * dst, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        fillKernel<int><<<elementwiseGrid(n), 256, 0, s>>>(n, value, dst);
+        NSFEM_HIP(hipGetLastError());
+    }
+
+    void convert(int srcDt, int dstDt, long n, const void * src, void * dst, hipStream_t s)
+    {
+        checkDtype(srcDt);
+        checkDtype(dstDt);
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        if (srcDt == F64 && dstDt == F32)
+        {
+            convertKernel<double, float><<<g, 256, 0, s>>>(n, static_cast<const double *>(src), static_cast<float *>(dst));
+        }
+        else if (srcDt == F32 && dstDt == F64)
+        {
+            convertKernel<float, double><<<g, 256, 0, s>>>(n, static_cast<const float *>(src), static_cast<double *>(dst));
+        }
+        else
+        {
+            NSFEM_HIP(hipMemcpyAsync(dst, src, n * dtypeSize(srcDt), hipMemcpyDeviceToDevice, s));
+        }
+        NSFEM_HIP(hipGetLastError());
+    }
+
+    void scatter(int dt, int nnz, const int * idx, const void * vals, void * dense, hipStream_t s)
+    {
+        if (nnz <= 0)
+        {
+            return;
+        }
+        const int g = ceilDiv(nnz, 256);
+        NSFEM_TYPED(dt, (scatterKernel<float><<<g, 256, 0, s>>>(nnz, idx, static_cast<const float *>(vals), static_cast<float *>(dense))),
+                    (scatterKernel<double><<<g, 256, 0, s>>>(nnz, idx, static_cast<const double *>(vals), static_cast<double *>(dense))));
+    }
+
+    void gather(int dt, int nnz, const void * dense, const int * idx, void * vals, hipStream_t s)
+    {
+        if (nnz <= 0)
+        {
+            return;
+        }
+        const int g = ceilDiv(nnz, 256);
+        NSFEM_TYPED(dt, (gatherKernel<float><<<g, 256, 0, s>>>(nnz, static_cast<const float *>(dense), idx, static_cast<float *>(vals))),
+                    (gatherKernel<double><<<g, 256, 0, s>>>(nnz, static_cast<const double *>(dense), idx, static_cast<double *>(vals))));
+    }
+
+    void permute(int srcDt, int dstDt, int n, int nch, const int * perm, const void * src, void * dst, bool inverse, hipStream_t s)
+    {
+        checkDtype(srcDt);
+        checkDtype(dstDt);
+        const long total = static_cast<long>(n) * nch;
+        if (total <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(total);
+        const int inv = inverse ? 1 : 0;
+        if (srcDt == F64 && dstDt == F64)
+        {
+            permuteKernel<double, double><<<g, 256, 0, s>>>(n, nch, perm, static_cast<const double *>(src), static_cast<double *>(dst), inv);
+        }
+        else if (srcDt == F32 && dstDt == F32)
+        {
+            permuteKernel<float, float><<<g, 256, 0, s>>>(n, nch, perm, static_cast<const float *>(src), static_cast<float *>(dst), inv);
+        }
+        else if (srcDt == F64 && dstDt == F32)
+        {
+            permuteKernel<double, float><<<g, 256, 0, s>>>(n, nch, perm, static_cast<const double *>(src), static_cast<float *>(dst), inv);
+        }
+        else
+        {
+            permuteKernel<float, double><<<g, 256, 0, s>>>(n, nch, perm, static_cast<const float *>(src), static_cast<double *>(dst), inv);
+        }
+        NSFEM_HIP(hipGetLastError());
+    }
+
+    namespace
+    {
+        struct DotScratch
+        {
+            DeviceArray<double> partials;
+            DeviceArray<unsigned> counter;
+            DeviceArray<double> out;
+            std::mutex mutex;
+        };
+        DotScratch & dotScratch()
+        {
+            static DotScratch * d = []
+            {
+                auto * p = new DotScratch;
+                p->partials.resize(2048 * 2);
+                p->counter.resize(1);
+                p->out.resize(2);
+                NSFEM_HIP(hipMemsetAsync(p->counter.get(), 0, sizeof(unsigned), libraryStream()));
+                syncLibraryStream();
+                return p;
+            }();
+            return *d;
+        }
+    } // namespace
+
+    void dot(int dt, long n, int K, const void * x, long ldx, const void * y, long ldy, double * out, hipStream_t s)
+    {
+        checkDtype(dt);
+        if (K < 1 || K > 2)
+        {
+            throw std::invalid_argument("dot: K must be 1 or 2");
+        }
+        if (n <= 0)
+        {
+            for (int k = 0; k < K; k++)
+            {
+                out[k] = 0;
+            }
+            return;
+        }
+        auto & d = dotScratch();
+        std::lock_guard<std::mutex> lock(d.mutex);
+        const int g = static_cast<int>(std::max(1L, std::min((n + kBlock - 1) / kBlock, 2048L)));
+        if (dt == F64)
+        {
+            if (K == 1)
+            {
+                dotKernel<double, 1><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
+                                                         d.partials.get(), d.counter.get(), d.out.get());
+            }
+            else
+            {
+                dotKernel<double, 2><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
+                                                         d.partials.get(), d.counter.get(), d.out.get());
+            }
+        }
+        else
+        {
+            if (K == 1)
+            {
+                dotKernel<float, 1><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
+                                                        d.partials.get(), d.counter.get(), d.out.get());
+            }
+            else
+            {
+                dotKernel<float, 2><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
+                                                        d.partials.get(), d.counter.get(), d.out.get());
+            }
+        }
+        NSFEM_HIP(hipGetLastError());
+        NSFEM_HIP(hipMemcpyAsync(out, d.out.get(), K * sizeof(double), hipMemcpyDeviceToHost, s));
+        NSFEM_HIP(hipStreamSynchronize(s));
+    }
+} // namespace nsfem

@@ -1,0 +1,52 @@
+// Host launchers for the generic sparse / vector kernels.  Everything is
+// enqueued on the given stream; nothing here synchronises except dot().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace nsfem
+{
+    enum DType : int
+    {
+        F32 = 0, // ids match the linalg .bin format (LinAlg/src/io.cpp:11-21)
+        F64 = 1
+    };
+
+    inline size_t dtypeSize(int dt) { return dt == F64 ? 8 : 4; }
+    void checkDtype(int dt);
+
+    // Lanes per row for a CSR product, from the mean row length.
+    int lanesForMeanRow(double meanNnzPerRow);
+
+    struct CsrView
+    {
+        int rows, cols, nnz;
+        const int * rowStart;
+        const int * column;
+        const void * values;
+        int dtype;
+    };
+
+    // y[c*ldy + i] = alpha * sum_j A[i,j] x[c*ldx + j] + beta * y[...]
+    // rowIds (optional, device): output row i is A's row rowIds[i]; outRows rows.
+    void csrMul(const CsrView & A, int numCh, const void * x, long ldx, void * y, long ldy, double alpha, double beta,
+                const int * rowIds, int outRows, int lanes, hipStream_t s);
+
+    void axpy(int dt, long n, double alpha, const void * src, void * dst, hipStream_t s);
+    void scale(int dt, long n, double alpha, void * dst, hipStream_t s);
+    void geam(int dt, long n, double alpha, const void * a, double beta, const void * b, void * c, hipStream_t s);
+    void fill(int dt, long n, double value, void * dst, hipStream_t s);
+    void fillInt(long n, int value, int * dst, hipStream_t s);
+    void convert(int srcDt, int dstDt, long n, const void * src, void * dst, hipStream_t s);
+    void scatter(int dt, int nnz, const int * idx, const void * vals, void * dense, hipStream_t s);
+    void gather(int dt, int nnz, const void * dense, const int * idx, void * vals, hipStream_t s);
+    void permute(int srcDt, int dstDt, int n, int nch, const int * perm, const void * src, void * dst, bool inverse, hipStream_t s);
+
+    // Blocking FP64-accumulated dot product(s): out[k] = x_k . y_k, k < K (1 or 2).
+    void dot(int dt, long n, int K, const void * x, long ldx, const void * y, long ldy, double * out, hipStream_t s);
+
+    // Grid size for grid-stride element-wise kernels.
+    int elementwiseGrid(long n);
+} // namespace nsfem

@@ -1,0 +1,305 @@
+"""Meshes, the Chorin context and the GPU time loop (Python side of the C ABI).
+
+  mesh::parseTriangleGmsh / createMesh   cpp/libs/Mesh/src/io.cpp:65-214, concreteMesh.cpp:78-293
+  buildChorinContext                     cpp/execs/NavierStokes/src/buildContext.cpp:17-142
+  solveNsChorinCuda                      cpp/execs/NavierStokes/src/chorinCuda.cu:154-379
+  ChorinCudaConfig / SolverConfig        include/NavierStokes/chorinCudaConfig.h:6-17
+  DfgConditions                          include/NavierStokes/dfgCondtions.h:4-13
+"""
+
+import ctypes
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import F32, F64, call
+
+_NP = {F32: np.float32, F64: np.float64}
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class TriangleMesh:
+    """Corner-only triangle mesh with border elements and named groups."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def load_gmsh(cls, path):
+        h = ctypes.c_void_p()
+        call("nsfem_mesh_load_gmsh", str(path).encode(), ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, xy, triangles, border_edges, groups):
+        xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+        tri = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+        be = np.ascontiguousarray(border_edges, np.int32).reshape(-1, 3)
+        names = (ctypes.c_char_p * len(groups))(*[g.encode() for g in groups])
+        h = ctypes.c_void_p()
+        call("nsfem_mesh_from_arrays", xy.shape[0], _ptr(xy), tri.shape[0], _ptr(tri), be.shape[0], _ptr(be), len(groups),
+             ctypes.cast(names, ctypes.c_void_p), ctypes.byref(h))
+        return cls(h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n, t, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        call("nsfem_mesh_info", self._h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(b), ctypes.byref(g))
+        return n.value, t.value, b.value, g.value
+
+    def arrays(self):
+        n, t, b, g = self.info()
+        xy = np.empty((n, 2), np.float64)
+        tri = np.empty((t, 3), np.int32)
+        border = np.empty((b, 3), np.int32)
+        call("nsfem_mesh_arrays", self._h, _ptr(xy), _ptr(tri), _ptr(border))
+        names = []
+        buf = ctypes.create_string_buffer(256)
+        for k in range(g):
+            call("nsfem_mesh_group_name", self._h, k, buf, 256)
+            names.append(buf.value.decode())
+        return xy, tri, border, names
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("nsfem_mesh_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+@dataclasses.dataclass
+class DfgConditions:
+    viscosity: float = 1e-3
+    peakVelocity: float = 1.0
+    channelHeight: float = 0.41
+    profileScale: float = 1.0
+
+    def left_velocity(self, y):
+        h = self.channelHeight
+        return self.profileScale * self.peakVelocity * y * (h - y) / (h * h)
+
+
+MATRICES = {
+    "velocityMass": _lib.MAT_VELOCITY_MASS,
+    "velocityStiffness": _lib.MAT_VELOCITY_STIFFNESS,
+    "pressureStiffness": _lib.MAT_PRESSURE_STIFFNESS,
+    "pressureStiffnessInternal": _lib.MAT_PRESSURE_STIFFNESS_INTERNAL,
+    "velocityPressureDiv": _lib.MAT_VELOCITY_PRESSURE_DIV,
+    "pressureVelocityDiv": _lib.MAT_PRESSURE_VELOCITY_DIV,
+}
+
+
+class ChorinContext:
+    """Host-side set-up of one mesh: P2/P1 numbering, Dirichlet lists, FP64 matrices."""
+
+    def __init__(self, mesh: TriangleMesh, cond: DfgConditions, threads=0):
+        self.cond = cond
+        c = _lib.DfgConditions(cond.viscosity, cond.peakVelocity, cond.channelHeight, cond.profileScale)
+        self._h = ctypes.c_void_p()
+        call("nsfem_context_build", mesh.handle, ctypes.byref(c), threads, ctypes.byref(self._h))
+        N, V, nd, ni, E = (ctypes.c_int() for _ in range(5))
+        call("nsfem_context_sizes", self._h, ctypes.byref(N), ctypes.byref(V), ctypes.byref(nd), ctypes.byref(ni), ctypes.byref(E))
+        self.numVelocityNodes = N.value
+        self.numPressureNodes = V.value
+        self.numDirichletVelocity = nd.value
+        self.numInternalPressure = ni.value
+        self.numElements = E.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def matrix(self, name):
+        """(rows, cols, rowStart, column, values) copies of a context matrix."""
+        rows, cols, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rs, col, val = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        call("nsfem_context_matrix", self._h, MATRICES[name], ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz),
+             ctypes.byref(rs), ctypes.byref(col), ctypes.byref(val))
+        r, n = rows.value, nnz.value
+        rowStart = np.ctypeslib.as_array(ctypes.cast(rs, ctypes.POINTER(ctypes.c_int)), (r + 1,)).copy()
+        column = np.ctypeslib.as_array(ctypes.cast(col, ctypes.POINTER(ctypes.c_int)), (max(n, 1),))[:n].copy()
+        values = np.ctypeslib.as_array(ctypes.cast(val, ctypes.POINTER(ctypes.c_double)), (max(n, 1),))[:n].copy()
+        return r, cols.value, rowStart, column, values
+
+    def scipy(self, name):
+        import scipy.sparse as sp
+        r, c, rs, col, val = self.matrix(name)
+        return sp.csr_matrix((val, col, rs), shape=(r, c))
+
+    def dirichlet(self, which):
+        """which: 'vx', 'vy' or 'p' -> (ids, values)."""
+        k = {"vx": 0, "vy": 1, "p": 2}[which]
+        cnt = ctypes.c_int()
+        call("nsfem_context_dirichlet", self._h, k, ctypes.byref(cnt), None, None)
+        ids = np.empty(cnt.value, np.int32)
+        vals = np.empty(cnt.value, np.float64)
+        call("nsfem_context_dirichlet", self._h, k, ctypes.byref(cnt), _ptr(ids), _ptr(vals))
+        return ids, vals
+
+    def internal_pressure(self):
+        ids = np.empty(self.numInternalPressure, np.int32)
+        call("nsfem_context_internal_pressure", self._h, _ptr(ids))
+        return ids
+
+    def velocity_mesh(self):
+        xy = np.empty((self.numVelocityNodes, 2), np.float64)
+        el = np.empty((self.numElements, 6), np.int32)
+        call("nsfem_context_velocity_mesh", self._h, _ptr(xy), _ptr(el))
+        return xy, el
+
+    def pressure_mesh(self):
+        xy = np.empty((self.numPressureNodes, 2), np.float64)
+        el = np.empty((self.numElements, 3), np.int32)
+        call("nsfem_context_pressure_mesh", self._h, _ptr(xy), _ptr(el))
+        return xy, el
+
+    def fast_convection_map(self, threads=0):
+        """The FastConvection integration map as a scipy CSR matrix (nnz_C x 2N)."""
+        import scipy.sparse as sp
+        rows, cols, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rs, col, val = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        call("nsfem_fast_convection_map", self._h, threads, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz),
+             ctypes.byref(rs), ctypes.byref(col), ctypes.byref(val))
+        try:
+            r, n = rows.value, nnz.value
+            a = np.ctypeslib.as_array(ctypes.cast(rs, ctypes.POINTER(ctypes.c_int)), (r + 1,)).copy()
+            b = np.ctypeslib.as_array(ctypes.cast(col, ctypes.POINTER(ctypes.c_int)), (max(n, 1),))[:n].copy()
+            v = np.ctypeslib.as_array(ctypes.cast(val, ctypes.POINTER(ctypes.c_double)), (max(n, 1),))[:n].copy()
+        finally:
+            for p in (rs, col, val):
+                _lib.load().nsfem_host_free(p)
+        return sp.csr_matrix((v, b, a), shape=(r, cols.value))
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("nsfem_context_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+@dataclasses.dataclass
+class SolverConfig:
+    method: str = "gs"
+    maxIterations: int = 200
+    targetMse: float = 1e-6
+    mseCheckInterval: int = 1
+
+
+@dataclasses.dataclass
+class ChorinConfig:
+    velocitySolver: SolverConfig = dataclasses.field(default_factory=SolverConfig)
+    pressureSolver: SolverConfig = dataclasses.field(default_factory=SolverConfig)
+    dtype: int = F64
+    innerTol: float = 1e-4
+    lanesVelocity: int = 0
+    lanesPressure: int = 0
+
+
+class Chorin:
+    """GPU Chorin projection stepping over a ChorinContext."""
+
+    def __init__(self, ctx: ChorinContext, cfg: ChorinConfig):
+        self.ctx = ctx
+        self.cfg = cfg
+        self._m1 = cfg.velocitySolver.method.encode()
+        self._m2 = cfg.pressureSolver.method.encode()
+        v, p = cfg.velocitySolver, cfg.pressureSolver
+        c = _lib.ChorinConfigC(_lib.SolverConfig(self._m1, v.maxIterations, v.targetMse, v.mseCheckInterval),
+                               _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval),
+                               cfg.dtype, cfg.innerTol, cfg.lanesVelocity, cfg.lanesPressure)
+        self._h = ctypes.c_void_p()
+        call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
+        self.dtype = _NP[cfg.dtype]
+        self.N = ctx.numVelocityNodes
+        self.V = ctx.numPressureNodes
+
+    def set_time_step(self, tau):
+        call("nsfem_chorin_set_time_step", self._h, float(tau))
+
+    def step(self):
+        st = _lib.StepStats()
+        call("nsfem_chorin_step", self._h, ctypes.byref(st))
+        return st.as_dict()
+
+    def download(self):
+        u = np.empty(2 * self.N, self.dtype)
+        p = np.empty(self.V, self.dtype)
+        call("nsfem_chorin_download", self._h, _ptr(u), _ptr(p))
+        return u, p
+
+    def upload_velocity(self, u):
+        u = np.ascontiguousarray(u, self.dtype)
+        if u.size != 2 * self.N:
+            raise ValueError("velocity must have 2N entries")
+        call("nsfem_chorin_upload_velocity", self._h, _ptr(u))
+
+    def reset(self):
+        call("nsfem_chorin_reset", self._h)
+
+    def state_pointers(self):
+        u, p = ctypes.c_void_p(), ctypes.c_void_p()
+        call("nsfem_chorin_state", self._h, ctypes.byref(u), ctypes.byref(p))
+        return u.value, p.value
+
+    def assemble_convection(self, u_host):
+        """Returns (A, C) values in the velocity pattern for a host velocity."""
+        from .cu import vec
+        nnz = self.ctx.matrix("velocityStiffness")[3].size
+        u = vec(np.ascontiguousarray(u_host, self.dtype))
+        a = vec(nnz, self.dtype)
+        c = vec(nnz, self.dtype)
+        call("nsfem_chorin_assemble_convection", self._h, u.ptr, a.ptr, c.ptr)
+        return a.download(), c.download()
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("nsfem_chorin_destroy", self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+@dataclasses.dataclass
+class TimeStepSolution:
+    time: float
+    velocity: np.ndarray
+    pressure: np.ndarray
+
+
+def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t: float, cfg: ChorinConfig,
+                    frame_step: int = 1, verbose: bool = False):
+    """solveNsChorinCuda: numTimeSteps = ceil(maxT / tau0), tau = maxT / numTimeSteps,
+    numTimeSteps + 1 steps from u = Dirichlet values; every frame_step-th state is kept."""
+    ctx = ChorinContext(mesh, cond)
+    ch = Chorin(ctx, cfg)
+    n_steps = int(math.ceil(max_t / tau0))
+    tau = max_t / n_steps
+    ch.set_time_step(tau)
+    out = []
+    stats = []
+    for it in range(n_steps + 1):
+        t0 = time.perf_counter()
+        st = ch.step()
+        st["wall_ms"] = 1e3 * (time.perf_counter() - t0)
+        stats.append(st)
+        if it % frame_step == 0:
+            u, p = ch.download()
+            out.append(TimeStepSolution((it + 1) * tau, u, p))
+        if verbose:
+            print(f"{it} / {n_steps}: {st['wall_ms']:.3f} ms  tent iters {st['itersTent']} pressure iters "
+                  f"{st['itersPressure']} final iters {st['itersFinal']}", flush=True)
+    return out, stats
