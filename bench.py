@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Chorin projection steps/s on one MI355X: plain FP64 path vs the mixed
+FP32/FP64 path, at 500k P1 pressure DOFs (about 2M P2 velocity DOFs per
+component), every linear solve to a 1e-10 relative residual.
+
+Workload: seeded channel 5.4 x 1 with a hole (r = 0.12 at (0.5, 0.49)),
+Re = 100 with a parabolic inflow of peak 1.5, convection reassembled every
+step into the fixed CSR pattern, a seeded 1e-3 perturbation of the
+inflow-initialised field.  The requested tau = 5e-4 violates the explicit
+scheme's diffusion bound (tau nu lambda_max(M^-1 K) <= 2) at this mesh size,
+so tau is clamped to 1 / (nu lambda_max); both values are reported.
+
+Prints one JSON line.  Progress goes to stderr."""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(nodes, seed):
+    import numpy as np
+    from nsfem_amd import fem, mesher
+    t0 = time.perf_counter()
+    m = mesher.channel_mesh(nodes, seed=seed)
+    t1 = time.perf_counter()
+    peak, re = 1.5, 100.0
+    diameter = 2 * m["radius"]
+    nu = (2.0 / 3.0) * peak * diameter / re
+    cond = fem.DfgConditions(viscosity=nu, peakVelocity=peak, channelHeight=m["height"], profileScale=4.0)
+    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), cond)
+    t2 = time.perf_counter()
+    log(f"mesh {t1 - t0:.1f}s (V={len(m['xy'])}, E={len(m['triangles'])}), context {t2 - t1:.1f}s "
+        f"(N={ctx.numVelocityNodes})")
+    return m, cond, ctx
+
+
+def initial_velocity(ctx, cond, seed):
+    """Dirichlet values plus a seeded uniform(-1, 1) * 1e-3 * peak
+    perturbation on the interior velocity nodes (both components)."""
+    import numpy as np
+    N = ctx.numVelocityNodes
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-1, 1, 2 * N) * 1e-3 * cond.peakVelocity
+    ids, vals = ctx.dirichlet("vx")
+    idy, valy = ctx.dirichlet("vy")
+    u[ids] = vals
+    u[idy + N] = valy
+    return u
+
+
+def run_path(name, ctx, cfg, tau, u0, warmup, steps):
+    import numpy as np
+    from nsfem_amd import _lib, fem
+    ch = fem.Chorin(ctx, cfg)
+    ch.set_time_step(tau)
+    ch.upload_velocity(u0)
+    first = ch.step()
+    u1, p1 = ch.download()
+    for _ in range(warmup):
+        ch.step()
+    _lib.call("nsfem_synchronize")
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ch.step())
+    _lib.call("nsfem_synchronize")
+    dt = time.perf_counter() - t0
+    u, p = ch.download()
+    if not (np.isfinite(u).all() and np.isfinite(p).all()):
+        raise RuntimeError(f"{name}: non-finite state")
+    agg = {k: float(np.mean([s[k] for s in stats])) for k in stats[0]}
+    res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+           "iters": {"tentative": agg["itersTent"], "pressure": agg["itersPressure"], "final": agg["itersFinal"]},
+           "outer": {"tentative": agg["outerTent"], "pressure": agg["outerPressure"], "final": agg["outerFinal"]},
+           "max_relres": max(max(s["relTent"], s["relPressure"], s["relFinal"]) for s in stats),
+           "ms": {"convection": agg["msConvection"], "tentative": agg["msTentative"], "pressure": agg["msPressure"],
+                  "final": agg["msFinal"]}}
+    log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
+        f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
+    return ch, res, (u1, p1), (u, p)
+
+
+def kernel_table(ch, res):
+    """Per-kernel average launch time (HIP events), algorithmic bytes, GB/s,
+    and an estimate of each kernel's share of one step."""
+    it = res["iters"]
+    vel_iters = it["tentative"] + it["final"]
+    launches = {"velocity_direction": vel_iters, "velocity_sweep": 1.5 * vel_iters,
+                "pressure_direction": it["pressure"], "pressure_sweep": 1.5 * it["pressure"],
+                "velocity_update": vel_iters, "pressure_update": it["pressure"], "convection": 1, "spmm": 1}
+    out = {}
+    for k, n in launches.items():
+        ms, b = ch.time_kernel(k, reps=30)
+        out[k] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "per_step": n, "ms_per_step": ms * n}
+    return out
+
+
+def cpu_baseline(ctx, cond, tau, u0):
+    """The FP64 CPU oracle timed on one step of the same workload (single
+    thread, numpy/scipy), from the same initial state."""
+    import numpy as np
+    from oracle import nsfem_oracle as O
+    t0 = time.perf_counter()
+    xyp, ep = ctx.pressure_mesh()
+    _, ev = ctx.velocity_mesh()
+    B, det = O.geometry(xyp, ep)
+    octx = {"N": ctx.numVelocityNodes, "V": ctx.numPressureNodes, "M": ctx.scipy("velocityMass"),
+            "K": ctx.scipy("velocityStiffness"), "Kp_int": ctx.scipy("pressureStiffnessInternal"),
+            "VPD": ctx.scipy("velocityPressureDiv"), "PVD": ctx.scipy("pressureVelocityDiv"),
+            "vm": {"elements": ev.astype(np.int64)}, "B": B, "det": det, "dvx": ctx.dirichlet("vx"),
+            "dvy": ctx.dirichlet("vy"), "int_p": ctx.internal_pressure(), "viscosity": cond.viscosity}
+    orc = O.ChorinOracle(octx, tau, vel=("pcg", 2000, 1e-10, 1), pres=("pcg", 20000, 1e-10, 1))
+    orc.u = np.array(u0, dtype=np.float64)
+    t1 = time.perf_counter()
+    u, p = orc.step()
+    t2 = time.perf_counter()
+    log(f"cpu oracle: set-up {t1 - t0:.1f}s, one step {t2 - t1:.1f}s")
+    return 1.0 / (t2 - t1), (u.copy(), p.copy())
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nodes", type=int, default=500_000, help="P1 pressure DOFs")
+    ap.add_argument("--tau", type=float, default=5e-4)
+    ap.add_argument("--seed", type=int, default=20241103)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        log("bench.py: only --gpus 1 is supported (single-GPU build)")
+        return 2
+
+    import numpy as np
+    from nsfem_amd import F64, fem
+
+    t_setup = time.perf_counter()
+    m, cond, ctx = build_workload(args.nodes, args.seed)
+    lam = fem.estimate_lambda_max(ctx)
+    tau_stable = 1.0 / (cond.viscosity * lam)
+    tau = min(args.tau, tau_stable)
+    log(f"lambda_max(M^-1 K) ~ {lam:.4g}; tau requested {args.tau:g}, used {tau:.4g} "
+        f"(tau nu lambda = {tau * cond.viscosity * lam:.3f})")
+    u0 = initial_velocity(ctx, cond, 1)
+    setup_s = time.perf_counter() - t_setup
+
+    target = 1e-10
+    cfg64 = fem.ChorinConfig(fem.SolverConfig("pcg", 5000, target, 1), fem.SolverConfig("pcg", 50000, target, 1), dtype=F64)
+    cfgmx = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 5000, target, 1), fem.SolverConfig("pcg_mixed", 50000, target, 1),
+                             dtype=F64, innerTol=args.inner_tol)
+    ch64, r64, first64, last64 = run_path("fp64", ctx, cfg64, tau, u0, args.warmup, args.steps)
+    kern64 = kernel_table(ch64, r64)
+    del ch64
+    chmx, rmx, firstmx, lastmx = run_path("mixed", ctx, cfgmx, tau, u0, args.warmup, args.steps)
+    kernmx = kernel_table(chmx, rmx)
+    del chmx
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    errors = {"mixed_vs_fp64_after_steps": {"velocity": rel(lastmx[0], last64[0]), "pressure": rel(lastmx[1], last64[1])}}
+    cpu = None
+    if not args.no_cpu:
+        cpu_rate, (uo, po) = cpu_baseline(ctx, cond, tau, u0)
+        errors["fp64_vs_oracle_step1"] = {"velocity": rel(first64[0], uo), "pressure": rel(first64[1], po)}
+        errors["mixed_vs_oracle_step1"] = {"velocity": rel(firstmx[0], uo), "pressure": rel(firstmx[1], po)}
+        cpu = {"value": cpu_rate, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": "1 Chorin step of the same 500k-DOF workload from the same initial state: FP64 numpy/scipy "
+                         "oracle, PCG-MCGS to 1e-10 for all three solves, single thread"}
+    log(f"errors: {errors}")
+
+    # Dominant kernel of the headline (mixed) path by estimated time per step.
+    dom_name = max(kernmx, key=lambda k: kernmx[k]["ms_per_step"])
+    dom = kernmx[dom_name]
+    peak = 8000.0
+    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": peak, "unit": "GB/s",
+                "frac": dom["GBps"] / peak, "traffic": None, "bytes_per_launch": dom["bytes"],
+                "avg_launch_ms": dom["avg_ms"]}
+    out = {
+        "metric": "Chorin steps/s @500k p-DOFs, mixed vs FP64 GPU, same residual; SpMV % HBM peak",
+        "value": rmx["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": rmx["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic: seeded channel mesh and seeded initial perturbation",
+        "config": {"workload": "chorin_step_channel_500k", "pressure_dofs": ctx.numPressureNodes,
+                   "velocity_dofs_per_component": ctx.numVelocityNodes, "elements": ctx.numElements,
+                   "reynolds": 100, "peak_inflow": cond.peakVelocity, "viscosity": cond.viscosity,
+                   "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
+                   "inner_tol": args.inner_tol, "seed": args.seed},
+        "fp64": r64,
+        "mixed": rmx,
+        "mixed_over_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
+        "errors": errors,
+        "roofline": roofline,
+        "kernels": {"fp64": kern64, "mixed": kernmx},
+        "cpu_baseline": cpu,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -128,6 +128,11 @@ int nsfem_solver_set_target_mse(nsfem_solver * s, double target);
 int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
 /* Tunables: "innerTol" (pcg_mixed), "checkChunk" (iterations between host polls). */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
+/* Kernel timing (HIP events on the library stream): average ms per launch of
+ * "direction" (A z fused with the PCG direction update), "sweep" (one
+ * full-row pass over all colours), "update"; bytes = algorithmic bytes per
+ * launch.  pcg solvers only; overwrites the solver's internal vectors. */
+int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes);
 int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
 int nsfem_solver_coloring(const nsfem_solver * s, int system, int * coloring, int * partitionStart);
 
@@ -237,6 +242,9 @@ int nsfem_chorin_download(nsfem_chorin * ch, void * velocity, void * pressure);
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);
+/* Kernel timing inside a Chorin context: "convection", "spmm",
+ * "velocity_<k>", "pressure_<k>" (k as in nsfem_solver_time_kernel). */
+int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes);
 /* A = nu K + C(u) in the velocity pattern (C optional): device arrays. */
 int nsfem_chorin_assemble_convection(nsfem_chorin * ch, const void * velocity, void * aValues, void * cValues);
 

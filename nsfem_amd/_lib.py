@@ -120,6 +120,7 @@ _SIGNATURES = {
     "nsfem_solver_set_target_mse": [_vp, ctypes.c_double],
     "nsfem_solver_set_mse_check_interval": [_vp, ctypes.c_int],
     "nsfem_solver_set_option": [_vp, ctypes.c_char_p, ctypes.c_double],
+    "nsfem_solver_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p],
     "nsfem_solver_num_colors": [_vp, ctypes.c_int, _c_int_p],
     "nsfem_solver_coloring": [_vp, ctypes.c_int, _vp, _vp],
     "nsfem_mcgs_coloring": [ctypes.c_int, _vp, _vp, _vp, _vp, _c_int_p],
@@ -150,6 +151,7 @@ _SIGNATURES = {
     "nsfem_chorin_upload_velocity": [_vp, _vp],
     "nsfem_chorin_reset": [_vp],
     "nsfem_chorin_assemble_convection": [_vp, _vp, _vp, _vp],
+    "nsfem_chorin_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p],
 }
 _RESTYPES = {"nsfem_last_error": ctypes.c_char_p, "nsfem_host_free": None}
 

@@ -1,5 +1,6 @@
 """Builds libnsfem_hip.so in-tree: HIP sources with hipcc for gfx950, host
-set-up sources with g++, linked by hipcc.  Run as ``python -m nsfem_amd.build``."""
+set-up sources with g++, linked by hipcc.  Run as ``python nsfem_amd/build.py``
+(a plain script: importing the package would load the library first)."""
 
 import concurrent.futures
 import os
@@ -14,6 +15,7 @@ LIB = os.path.join(HERE, "libnsfem_hip.so")
 
 HIP_SOURCES = ["ops.hip", "solver.hip", "chorin.hip", "capi.hip"]
 CPP_SOURCES = ["host/mesh.cpp", "host/csr.cpp", "host/fem.cpp", "host/context.cpp"]
+APPS = ["navier_stokes", "solver_test"]
 HEADERS = ["common.hpp", "kernels.hpp", "ops.hpp", "solver.hpp", "chorin.hpp", "host/mesh.hpp", "host/csr.hpp",
            "host/fem.hpp", "host/context.hpp"]
 
@@ -67,6 +69,12 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    # C++ host tools over the C ABI (the reference's host language)
+    for app in APPS:
+        out = os.path.join(HERE, app)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", out + ".tmp", os.path.join(CSRC, "app", app + ".cpp"),
+              "-L" + HERE, "-lnsfem_hip", "-Wl,-rpath,$ORIGIN"], verbose)
+        os.replace(out + ".tmp", out)
     return LIB
 
 

@@ -613,6 +613,16 @@ extern "C"
             } });
     }
 
+    int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            need(which, "which");
+            need(avgMs, "avgMs");
+            *avgMs = s->core->timeKernel(which, reps, bytes); });
+    }
+
     int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out)
     {
         return guarded([&]
@@ -1012,6 +1022,16 @@ extern "C"
                        {
             need(ch, "chorin");
             ch->core->resetState(); });
+    }
+
+    int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            need(which, "which");
+            need(avgMs, "avgMs");
+            *avgMs = ch->core->timeKernel(which, reps, bytes); });
     }
 
     int nsfem_chorin_assemble_convection(nsfem_chorin * ch, const void * velocity, void * aValues, void * cValues)

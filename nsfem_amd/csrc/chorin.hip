@@ -1,6 +1,7 @@
 #include "chorin.hpp"
 
 #include <cmath>
+#include <functional>
 
 #include "kernels.hpp"
 
@@ -236,6 +237,62 @@ namespace nsfem
                                                              static_cast<float *>(cOut));
         }
         NSFEM_HIP(hipGetLastError());
+    }
+
+    double ChorinGpu::timeKernel(const std::string & which, int reps, double * bytesPerLaunch)
+    {
+        if (which.rfind("velocity_", 0) == 0)
+        {
+            return vSolver_->timeKernel(which.substr(9), reps, bytesPerLaunch);
+        }
+        if (which.rfind("pressure_", 0) == 0)
+        {
+            return pSolver_->timeKernel(which.substr(9), reps, bytesPerLaunch);
+        }
+        if (reps < 1)
+        {
+            throw std::invalid_argument("reps must be positive");
+        }
+        const hipStream_t s = libraryStream();
+        const double ts = static_cast<double>(tsize_);
+        const double nnz = A_.nnz, E = E_, N = N_;
+        double bytes = 0;
+        std::function<void()> launch;
+        if (which == "convection")
+        {
+            bytes = E * (6 * 4 + 5 * 8 + 12 * ts + 36 * ts) + nnz * (4 + 2 * ts) + 36 * E * (4 + ts);
+            launch = [&]() { assembleConvection(u_.get(), A_.values.get(), nullptr, s); };
+        }
+        else if (which == "spmm")
+        {
+            bytes = nnz * (ts + 4) + (N + 1) * 4 + 4 * N * ts;
+            launch = [&]()
+            { csrMul(A_.view(cfg_.dtype), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.lanesVelocity, s); };
+        }
+        else
+        {
+            throw std::invalid_argument("unknown kernel '" + which + "'");
+        }
+        launch();
+        hipEvent_t e0, e1;
+        NSFEM_HIP(hipEventCreate(&e0));
+        NSFEM_HIP(hipEventCreate(&e1));
+        NSFEM_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; i++)
+        {
+            launch();
+        }
+        NSFEM_HIP(hipEventRecord(e1, s));
+        NSFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        NSFEM_HIP(hipEventDestroy(e0));
+        NSFEM_HIP(hipEventDestroy(e1));
+        if (bytesPerLaunch)
+        {
+            *bytesPerLaunch = bytes;
+        }
+        return static_cast<double>(ms) / reps;
     }
 
     void ChorinGpu::step(ChorinStepStats * stats)

@@ -73,6 +73,11 @@ namespace nsfem
         void resetState();
         void imposeDirichlet(hipStream_t s);
 
+        // Average launch duration (ms) and algorithmic bytes per launch of a
+        // step kernel: "convection" (element + gather kernels), "spmm" (A u),
+        // "velocity_<k>" / "pressure_<k>" with k a solver kernel name.
+        double timeKernel(const std::string & which, int reps, double * bytesPerLaunch);
+
         McgsSolver & velocitySolver() { return *vSolver_; }
         McgsSolver & pressureSolver() { return *pSolver_; }
 

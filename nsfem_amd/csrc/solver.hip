@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <stdexcept>
 
 namespace nsfem
@@ -333,6 +334,13 @@ namespace nsfem
     McgsSolver::~McgsSolver()
     {
         (void)hipStreamSynchronize(stream_);
+        for (ChunkGraph * g : {&graph64_, &graph32_})
+        {
+            if (g->exec)
+            {
+                (void)hipGraphExecDestroy(g->exec);
+            }
+        }
         if (hostFlags_)
         {
             (void)hipHostFree(hostFlags_);
@@ -488,42 +496,79 @@ namespace nsfem
     }
 
     template <typename T>
+    void McgsSolver::pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q)
+    {
+        const McgsMatrix<T> Av = A.view(n_);
+        const int nch = opt_.numCh;
+        double * totals = inner_.totals.get();
+        dispatchLC(lanes_, nch, [&]<int L, int C>()
+                   { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q, totals); });
+        if (nch == 1)
+        {
+            pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+        }
+        else
+        {
+            pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+        }
+        sgsSweeps<T>(s, A, st, r, z);
+        dispatchLC(lanes_, nch, [&]<int L, int C>()
+                   { pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_, totals); });
+    }
+
+    template <typename T>
     int McgsSolver::runPcg(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * b, T * x, T * r, T * z, T * p, T * q,
                            bool zeroGuess, const int * gate, int maxIters, bool sync)
     {
         const int nch = opt_.numCh;
-        const int K = numSystems_ * nch;
         DeviceState & ds = inner_;
         const McgsMatrix<T> Av = A.view(n_);
         double * totals = ds.totals.get();
         fillInt(4, 0, st.flags, s);
-        (void)K;
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    { pcgInitKernel<T, T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x, r, z, color0End_, zeroGuess ? 1 : 0, gate, totals); });
-        auto precondition = [&]()
+        sgsSweeps<T>(s, A, st, r, z);
+        dispatchLC(lanes_, nch, [&]<int L, int C>()
+                   { pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_, totals); });
+        NSFEM_HIP(hipGetLastError());
+
+        // Iterations run in chunks of checkChunk; a full chunk is one launch
+        // of a captured graph.  Kernels after convergence exit at once.
+        ChunkGraph & g = std::is_same_v<T, double> ? graph64_ : graph32_;
+        const int chunk = std::max(1, opt_.checkChunk);
+        if (useGraphs_ && (g.exec == nullptr || g.k != chunk || g.tol2 != st.tol2))
         {
-            sgsSweeps<T>(s, A, st, r, z);
-            dispatchLC(lanes_, nch, [&]<int L, int C>()
-                       { pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_, totals); });
-        };
-        precondition();
+            if (g.exec)
+            {
+                NSFEM_HIP(hipGraphExecDestroy(g.exec));
+                g.exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < chunk; i++)
+            {
+                pcgIteration<T>(s, A, st, x, r, z, p, q);
+            }
+            NSFEM_HIP(hipStreamEndCapture(s, &graph));
+            NSFEM_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+            NSFEM_HIP(hipGraphDestroy(graph));
+            g.k = chunk;
+            g.tol2 = st.tol2;
+        }
         int launched = 0;
         while (launched < maxIters)
         {
-            const int k = std::min(opt_.checkChunk, maxIters - launched);
-            for (int i = 0; i < k; i++)
+            const int k = std::min(chunk, maxIters - launched);
+            if (useGraphs_ && k == chunk)
             {
-                dispatchLC(lanes_, nch, [&]<int L, int C>()
-                           { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q, totals); });
-                if (nch == 1)
+                NSFEM_HIP(hipGraphLaunch(g.exec, s));
+            }
+            else
+            {
+                for (int i = 0; i < k; i++)
                 {
-                    pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+                    pcgIteration<T>(s, A, st, x, r, z, p, q);
                 }
-                else
-                {
-                    pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
-                }
-                precondition();
             }
             NSFEM_HIP(hipGetLastError());
             launched += k;
@@ -634,6 +679,125 @@ namespace nsfem
     }
 
     void McgsSolver::solveAsync(hipStream_t) { solve(); }
+
+    double McgsSolver::timeKernel(const std::string & which, int reps, double * bytesPerLaunch)
+    {
+        if (reps < 1)
+        {
+            throw std::invalid_argument("reps must be positive");
+        }
+        if (opt_.method != SolverMethod::Pcg)
+        {
+            throw std::invalid_argument("kernel timing is available for pcg solvers");
+        }
+        hipStream_t s = stream_;
+        const int nch = opt_.numCh;
+        const int K = numSystems_ * nch;
+        const bool f64 = opt_.precision == SolverPrecision::F64;
+        const double ts = f64 ? 8.0 : 4.0;
+        SolveState st = inner_.view(K, 0.0);
+        double * totals = inner_.totals.get();
+        long nnzOff = 0;
+        {
+            int last = 0;
+            NSFEM_HIP(hipMemcpy(&last, (f64 ? A64_.rowStart.get() : A32_.rowStart.get()) + n_, sizeof(int), hipMemcpyDeviceToHost));
+            nnzOff = last;
+        }
+        const double n = n_;
+        const double matrixBytes = nnzOff * (ts + 4.0) + (n + 1) * 4.0 + n * ts;
+        double bytes = 0;
+        std::function<void()> launch;
+        auto run = [&](auto tag)
+        {
+            using T = decltype(tag);
+            const DeviceMcgs<T> & A = [&]() -> const DeviceMcgs<T> &
+            {
+                if constexpr (std::is_same_v<T, double>) return A64_;
+                else return A32_;
+            }();
+            T * x = f64 ? reinterpret_cast<T *>(x64_.get()) : reinterpret_cast<T *>(x32_.get());
+            T * r = f64 ? reinterpret_cast<T *>(r64_.get()) : reinterpret_cast<T *>(r32_.get());
+            T * z = f64 ? reinterpret_cast<T *>(z64_.get()) : reinterpret_cast<T *>(z32_.get());
+            T * p = f64 ? reinterpret_cast<T *>(p64_.get()) : reinterpret_cast<T *>(p32_.get());
+            T * q = f64 ? reinterpret_cast<T *>(q64_.get()) : reinterpret_cast<T *>(q32_.get());
+            const McgsMatrix<T> Av = A.view(n_);
+            if (which == "direction")
+            {
+                // z read once, p and q read and written
+                bytes = matrixBytes + 5.0 * nch * n * ts;
+                launch = [=, this]()
+                {
+                    dispatchLC(lanes_, nch, [&]<int L, int C>()
+                               { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q, totals); });
+                };
+            }
+            else if (which == "sweep")
+            {
+                // one full-row GS pass over every colour (numColors launches)
+                bytes = matrixBytes + 3.0 * nch * n * ts;
+                launch = [=, this]()
+                {
+                    for (int c = 0; c < numColors_; c++)
+                    {
+                        const int cnt = colorBlockStart_[c + 1] - colorBlockStart_[c];
+                        if (cnt == 0) continue;
+                        dispatchLC(lanes_, nch, [&]<int L, int C>()
+                                   { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, bm_, colorBlockStart_[c], st, r, z, 0); });
+                    }
+                };
+            }
+            else if (which == "update")
+            {
+                bytes = (6.0 * nch * n + n) * ts;
+                launch = [=, this]()
+                {
+                    if (nch == 1)
+                        pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+                    else
+                        pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+                };
+            }
+            else
+            {
+                throw std::invalid_argument("unknown kernel '" + which + "' (direction, sweep, update)");
+            }
+        };
+        if (f64)
+        {
+            run(double{});
+        }
+        else
+        {
+            run(float{});
+        }
+        // A state in which every kernel does its full work: all channels
+        // active, not the first iteration, zero step lengths.
+        fillInt(K, 1, st.active, s);
+        fillInt(4, 0, st.flags, s);
+        fill(F64, 3L * K, 0.0, st.rho, s); // rho, alpha, beta
+        fill(F64, 1L * K, 1.0, st.ref, s);
+        launch();
+        NSFEM_HIP(hipGetLastError());
+        hipEvent_t e0, e1;
+        NSFEM_HIP(hipEventCreate(&e0));
+        NSFEM_HIP(hipEventCreate(&e1));
+        NSFEM_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; i++)
+        {
+            launch();
+        }
+        NSFEM_HIP(hipEventRecord(e1, s));
+        NSFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        NSFEM_HIP(hipEventDestroy(e0));
+        NSFEM_HIP(hipEventDestroy(e1));
+        if (bytesPerLaunch)
+        {
+            *bytesPerLaunch = bytes;
+        }
+        return static_cast<double>(ms) / reps;
+    }
 
     template void McgsSolver::runGs<double>(hipStream_t);
     template void McgsSolver::runGs<float>(hipStream_t);

@@ -116,8 +116,16 @@ namespace nsfem
         void setInnerTol(double t) { opt_.innerTol = t; }
         const SolverOptions & options() const { return opt_; }
 
-        // Pre-allocated device views for fused callers.
         hipStream_t stream() const { return stream_; }
+        void setUseGraphs(bool on) { useGraphs_ = on; }
+
+        // Average duration (ms) of one launch of a solver kernel over reps
+        // launches, timed with HIP events on the solver's stream, and the
+        // algorithmic bytes one launch moves.  which: "direction" (A z fused
+        // with the p/q updates), "sweep" (one full-row GS pass over all
+        // colours, per colour launch averaged), "update".  Leaves the io
+        // vectors untouched; internal work vectors are overwritten.
+        double timeKernel(const std::string & which, int reps, double * bytesPerLaunch);
 
     private:
         template <typename T>
@@ -128,6 +136,8 @@ namespace nsfem
         void runMixed(hipStream_t s);
         template <typename T>
         void sgsSweeps(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * r, T * z);
+        template <typename T>
+        void pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
         bool pollDone(hipStream_t s, DeviceState & st);
 
@@ -165,5 +175,15 @@ namespace nsfem
         int lastOuter_ = 0;
         bool lastNonFinite_ = false;
         hipStream_t stream_;
+
+        // One captured graph of checkChunk PCG iterations per precision path.
+        struct ChunkGraph
+        {
+            hipGraphExec_t exec = nullptr;
+            int k = 0;
+            double tol2 = -1;
+        };
+        ChunkGraph graph64_, graph32_;
+        bool useGraphs_ = true;
     };
 } // namespace nsfem

@@ -265,6 +265,11 @@ class AbstractSolver:
     def setOption(self, key, value):
         call("nsfem_solver_set_option", self._h, key.encode(), float(value))
 
+    def timeKernel(self, which, reps=50):
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        call("nsfem_solver_time_kernel", self._h, which.encode(), int(reps), ctypes.byref(ms), ctypes.byref(b))
+        return ms.value, b.value
+
     def numColors(self, system=0):
         out = ctypes.c_int()
         call("nsfem_solver_num_colors", self._h, system, ctypes.byref(out))

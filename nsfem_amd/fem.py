@@ -254,6 +254,12 @@ class Chorin:
         call("nsfem_chorin_state", self._h, ctypes.byref(u), ctypes.byref(p))
         return u.value, p.value
 
+    def time_kernel(self, which, reps=50):
+        """(average ms per launch, algorithmic bytes per launch) of a step kernel."""
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        call("nsfem_chorin_time_kernel", self._h, which.encode(), int(reps), ctypes.byref(ms), ctypes.byref(b))
+        return ms.value, b.value
+
     def assemble_convection(self, u_host):
         """Returns (A, C) values in the velocity pattern for a host velocity."""
         from .cu import vec
@@ -303,3 +309,29 @@ def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t:
             print(f"{it} / {n_steps}: {st['wall_ms']:.3f} ms  tent iters {st['itersTent']} pressure iters "
                   f"{st['itersPressure']} final iters {st['itersFinal']}", flush=True)
     return out, stats
+
+
+def estimate_lambda_max(ctx: ChorinContext, iters=40, seed=1):
+    """Largest eigenvalue of M^-1 K (velocity mass / stiffness) by power
+    iteration on the GPU; bounds the forward-Euler diffusion step,
+    tau * nu * lambda_max <= 2."""
+    from . import cu
+    rng = np.random.default_rng(seed)
+    K = cu.csrF(*ctx.matrix("velocityStiffness")[:2], *ctx.matrix("velocityStiffness")[2:])
+    M = cu.csrF(*ctx.matrix("velocityMass")[:2], *ctx.matrix("velocityMass")[2:])
+    solver = cu.createSolver("pcg", 1, ctx.scipy("velocityMass"), 500, 1e-10, 1)
+    n = ctx.numVelocityNodes
+    x = cu.vec(rng.standard_normal(n))
+    y = cu.vec(n, np.float64)
+    mx = cu.vec(n, np.float64)
+    lam = 0.0
+    for _ in range(iters):
+        K.mul(x, solver.getRhs())
+        solver.getSol().memsetZero()
+        solver.solve()
+        solver.getSol().copyTo(x)
+        K.mul(x, y)
+        M.mul(x, mx)
+        lam = cu.dot(n, x, y) / cu.dot(n, x, mx)
+        cu.scale(n, x, 1.0 / cu.nrm2(n, x))
+    return lam

@@ -106,9 +106,12 @@ def channel_mesh(num_nodes, length=5.4, height=1.0, center=(0.5, 0.49), radius=0
     bsides = sides[idx[cnt == 1]]
     a, b = xy[bsides[:, 0]], xy[bsides[:, 1]]
     eps = 1e-6 * h
+    # walls at their float32-rounded positions (the coordinates are rounded)
+    lr = float(np.float32(length))
+    hr = float(np.float32(height))
     group = np.full(len(bsides), CIRCLE, np.int64)
-    group[(np.abs(a[:, 0] - length) < eps) & (np.abs(b[:, 0] - length) < eps)] = RIGHT
-    group[(np.abs(a[:, 1] - height) < eps) & (np.abs(b[:, 1] - height) < eps)] = TOP
+    group[(np.abs(a[:, 0] - lr) < eps) & (np.abs(b[:, 0] - lr) < eps)] = RIGHT
+    group[(np.abs(a[:, 1] - hr) < eps) & (np.abs(b[:, 1] - hr) < eps)] = TOP
     group[(np.abs(a[:, 0]) < eps) & (np.abs(b[:, 0]) < eps)] = LEFT
     group[(np.abs(a[:, 1]) < eps) & (np.abs(b[:, 1]) < eps)] = BOTTOM
     mid = 0.5 * (a + b)

@@ -18,8 +18,11 @@ def pytest_configure(config):
     # library is used as is.
     lib = os.path.join(ROOT, "nsfem_amd", "libnsfem_hip.so")
     if not os.path.exists(lib):
-        from nsfem_amd.build import build
-        build()
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("nsfem_build", os.path.join(ROOT, "nsfem_amd", "build.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod.build()
     if not os.path.exists(os.path.join(ROOT, "oracle", "_build", "liboracle.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 

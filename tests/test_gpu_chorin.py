@@ -1,0 +1,137 @@
+"""The GPU Chorin step against the FP64 CPU oracle.
+
+Tolerances: the plain FP64 path (both solves 'pcg' to a 1e-13 relative
+residual, or the reference's 'gs' solver in FP64) must agree with the oracle
+to 1e-10 relative (max-norm over the field) after several steps; the FP32
+reference-precision path and the mixed path report their error and are held
+to their own bounds."""
+
+import numpy as np
+import pytest
+
+from conftest import mesh_path
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(name="c3", visc=1e-3, peak=1.0):
+    from nsfem_amd import fem
+    return fem.ChorinContext(fem.TriangleMesh.load_gmsh(mesh_path(name)), fem.DfgConditions(visc, peak))
+
+
+def _cfg(vel, pres, dtype=1, inner=1e-4):
+    from nsfem_amd import fem
+    return fem.ChorinConfig(fem.SolverConfig(*vel), fem.SolverConfig(*pres), dtype=dtype, innerTol=inner)
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def test_convection_assembly_matches_integration_map(oracle, octx_c3):
+    """C(u) from the element kernel equals Int u (FastConvection) and the
+    oracle's element form; A = nu K + C."""
+    from nsfem_amd import fem
+    ctx = _ctx()
+    ch = fem.Chorin(ctx, _cfg(("pcg", 100, 1e-10, 1), ("pcg", 100, 1e-10, 1)))
+    rng = np.random.default_rng(41)
+    u = rng.standard_normal(2 * ctx.numVelocityNodes)
+    A, C = ch.assemble_convection(u)
+    Int = ctx.fast_convection_map()
+    ref = Int @ u
+    assert _rel(C, ref) < 1e-13
+    assert _rel(C, oracle.convection_values(octx_c3, u)) < 1e-13
+    K = ctx.matrix("velocityStiffness")[4]
+    assert _rel(A, 1e-3 * K + ref) < 1e-13
+
+
+@pytest.mark.parametrize("mesh", ["c3", "h_0.025"])
+def test_fp64_pcg_steps_match_oracle(oracle, mesh):
+    from nsfem_amd import fem
+    tau = 1e-3
+    octx = oracle.build_context(oracle.parse_gmsh(mesh_path(mesh)), 1e-3, 1.0)
+    orc = oracle.ChorinOracle(octx, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    ch = fem.Chorin(_ctx(mesh), _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1)))
+    ch.set_time_step(tau)
+    for k in range(4):
+        st = ch.step()
+        uo, po = orc.step()
+        u, p = ch.download()
+        assert st["relTent"] <= 1e-13 and st["relPressure"] <= 1e-13 and st["relFinal"] <= 1e-13
+        assert _rel(u, uo) <= 1e-10, k
+        assert _rel(p, po) <= 1e-10, k
+
+
+def test_fp64_reference_gs_steps_match_oracle(oracle, octx_c3):
+    """Reference solver semantics (gs, RMS target, iteration cap) in FP64."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    orc = oracle.ChorinOracle(octx_c3, tau, vel=("gs", 200, 1e-9, 1), pres=("gs", 200, 1e-9, 1))
+    ch = fem.Chorin(_ctx(), _cfg(("gs", 200, 1e-9, 1), ("gs", 200, 1e-9, 1)))
+    ch.set_time_step(tau)
+    for k in range(3):
+        st = ch.step()
+        uo, po = orc.step()
+        u, p = ch.download()
+        assert _rel(u, uo) <= 1e-10, k
+        assert _rel(p, po) <= 1e-10, k
+    assert 1 <= st["itersPressure"] <= 200
+
+
+def test_fp32_reference_precision_path(oracle, octx_c3):
+    """The reference's own precision (FP32 state, gs solvers): error against
+    the FP64 oracle is reported and stays at single-precision level."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    orc = oracle.ChorinOracle(octx_c3, tau, vel=("gs", 200, 1e-6, 1), pres=("gs", 200, 1e-6, 1))
+    ch = fem.Chorin(_ctx(), _cfg(("gs", 200, 1e-6, 1), ("gs", 200, 1e-6, 1), dtype=0))
+    ch.set_time_step(tau)
+    for _ in range(3):
+        ch.step()
+        uo, po = orc.step()
+    u, p = ch.download()
+    assert u.dtype == np.float32
+    eu, ep = _rel(u, uo), _rel(p, po)
+    print(f"fp32 path: velocity {eu:.2e}, pressure {ep:.2e}")
+    assert eu < 1e-4 and ep < 1e-2
+
+
+def test_mixed_path_matches_oracle(oracle, octx_c3):
+    from nsfem_amd import fem
+    tau = 1e-3
+    orc = oracle.ChorinOracle(octx_c3, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    ch = fem.Chorin(_ctx(), _cfg(("pcg_mixed", 2000, 1e-10, 1), ("pcg_mixed", 8000, 1e-10, 1)))
+    ch.set_time_step(tau)
+    for _ in range(3):
+        st = ch.step()
+        uo, po = orc.step()
+        assert st["relTent"] <= 1e-10 and st["relPressure"] <= 1e-10 and st["relFinal"] <= 1e-10
+    u, p = ch.download()
+    eu, ep = _rel(u, uo), _rel(p, po)
+    print(f"mixed path: velocity {eu:.2e}, pressure {ep:.2e}")
+    assert eu < 1e-8 and ep < 1e-7
+
+
+def test_non_finite_velocity_is_reported():
+    """A NaN in the state makes the tentative solve fail like the reference
+    ("Tentative acceleration is bad", chorinCuda.cu:270-273)."""
+    from nsfem_amd import _lib, fem
+    ctx = _ctx()
+    ch = fem.Chorin(ctx, _cfg(("gs", 20, 1e-6, 1), ("gs", 20, 1e-6, 1)))
+    u = np.zeros(2 * ctx.numVelocityNodes)
+    u[123] = np.nan
+    ch.upload_velocity(u)
+    with pytest.raises(_lib.NsfemNonFinite, match="Tentative acceleration is bad"):
+        ch.step()
+    ch.reset()
+    ch.step()  # recovers after a reset
+
+
+def test_solve_ns_chorin_time_loop():
+    """numTimeSteps = ceil(maxT / tau0), tau = maxT / numTimeSteps, numTimeSteps + 1 steps."""
+    from nsfem_amd import fem
+    mesh = fem.TriangleMesh.load_gmsh(mesh_path("c3"))
+    frames, stats = fem.solve_ns_chorin(mesh, fem.DfgConditions(1e-3, 1.0), 0.0009, 0.003,
+                                        _cfg(("pcg", 500, 1e-10, 1), ("pcg", 2000, 1e-10, 1)))
+    assert len(stats) == 5 and len(frames) == 5
+    assert all(np.isfinite(f.velocity).all() for f in frames)

@@ -1,0 +1,201 @@
+"""GPU solvers against the oracle: the reference GS solver (iterations, RMS,
+solution), symmetric-GS PCG (relative residual, dense solution), mixed
+precision refinement and the block-diagonal batch."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(name, m, b, nch=1, max_iters=200, target=1e-6, interval=1, x0=None, dtype=np.float64, **opts):
+    from nsfem_amd import cu
+    s = cu.createSolver(name, nch, m, max_iters, target, interval, dtype=dtype)
+    for k, v in opts.items():
+        s.setOption(k, v)
+    s.getRhs().upload(b)
+    if x0 is None:
+        s.getSol().memsetZero()
+    else:
+        s.getSol().upload(x0)
+    s.solve()
+    return s, s.getSol().download()
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+def test_gs_matches_oracle_one_channel(oracle, octx_h025, interval):
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(21)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x = _solve("gs", m, b, max_iters=60, target=1e-4, interval=interval)
+    xo, mse, it = oracle.gs_solve(m, b, np.zeros_like(b), 60, 1e-4, interval)
+    assert s.getLastIterations() == it
+    assert abs(s.getLastMse() - mse) <= 1e-10 * mse
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+def test_gs_converges_and_reports_iteration_of_break(oracle, octx_c3):
+    m = octx_c3["M"]
+    rng = np.random.default_rng(22)
+    b = rng.uniform(-1, 1, m.shape[0]) * 1e-3
+    s, x = _solve("gs", m, b, max_iters=200, target=1e-9)
+    xo, mse, it = oracle.gs_solve(m, b, np.zeros_like(b), 200, 1e-9, 1)
+    assert it < 200 and s.getLastIterations() == it
+    assert s.getLastMse() < 1e-9
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+def test_gs_two_channels_stop_together(oracle, octx_c3):
+    m = octx_c3["M"]
+    n = m.shape[0]
+    rng = np.random.default_rng(23)
+    b = np.concatenate([rng.uniform(-1, 1, n) * 1e-3, rng.uniform(-1, 1, n) * 1e-6])
+    s, x = _solve("gs", m, b, nch=2, max_iters=100, target=1e-10)
+    L = oracle.Mcgs(m)
+    # channel-wise oracle with a shared stop: both below target
+    it_ref = None
+    xs = [np.zeros(n), np.zeros(n)]
+    for it in range(100):
+        xs = [oracle.gs_solve(m, b[c * n:(c + 1) * n], xs[c], 1, 0.0, 1, L)[0] for c in range(2)]
+        r = [np.sqrt(np.sum((m @ xs[c] - b[c * n:(c + 1) * n]) ** 2) / n) for c in range(2)]
+        if r[0] < 1e-10 and r[1] < 1e-10:
+            it_ref = it
+            break
+    assert s.getLastIterations() == it_ref
+    assert abs(s.getLastMse(-1) - np.hypot(s.getLastMse(0), s.getLastMse(1))) < 1e-20
+    assert np.max(np.abs(x - np.concatenate(xs))) <= 1e-12 * np.max(np.abs(x))
+
+
+def test_gs_float32_matches_float64_to_single_precision(octx_c3):
+    m = octx_c3["Kp_int"]
+    rng = np.random.default_rng(24)
+    b = rng.uniform(-1, 1, m.shape[0])
+    _, x64 = _solve("gs", m, b, max_iters=40, target=0.0)
+    _, x32 = _solve("gs", m, b.astype(np.float32), max_iters=40, target=0.0, dtype=np.float32)
+    assert x32.dtype == np.float32
+    assert np.max(np.abs(x32 - x64)) <= 1e-4 * np.max(np.abs(x64))
+
+
+@pytest.mark.parametrize("which", ["Kp_int", "M"])
+def test_pcg_matches_oracle_and_direct(oracle, octx_h025, which):
+    m = octx_h025[which]
+    rng = np.random.default_rng(25)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x = _solve("pcg", m, b, max_iters=3000, target=1e-12)
+    xo, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-12, 3000)
+    assert abs(s.getLastIterations() - it) <= 1
+    assert s.getLastRelRes() <= 1e-12
+    true_rel = np.linalg.norm(b - m @ x) / np.linalg.norm(b)
+    assert true_rel <= 1e-11
+    xd = spla.spsolve(m.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+    assert np.max(np.abs(x - xo)) <= 1e-9 * np.max(np.abs(xo))
+
+
+def test_pcg_two_channels_and_warm_start(octx_h025):
+    m = octx_h025["M"]
+    n = m.shape[0]
+    rng = np.random.default_rng(26)
+    b = rng.uniform(-1, 1, 2 * n)
+    s, x = _solve("pcg", m, b, nch=2, max_iters=500, target=1e-10)
+    for c in range(2):
+        bc = b[c * n:(c + 1) * n]
+        assert np.linalg.norm(bc - m @ x[c * n:(c + 1) * n]) <= 1.0001e-10 * np.linalg.norm(bc)
+    it_cold = s.getLastIterations()
+    # warm start from the solution: converged at set-up, zero iterations
+    s.getSol().upload(x)
+    s.solve()
+    assert s.getLastIterations() <= 1
+    assert it_cold > 3
+
+
+def test_pcg_zero_rhs_gives_zero(octx_c3):
+    m = octx_c3["Kp_int"]
+    s, x = _solve("pcg", m, np.zeros(m.shape[0]), max_iters=100, target=1e-10)
+    assert not x.any() and s.getLastIterations() == 0
+
+
+def test_pcg_float32(octx_h025):
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(27)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x = _solve("pcg", m, b.astype(np.float32), max_iters=2000, target=1e-5, dtype=np.float32)
+    # the recursive FP32 residual meets the target; the true FP64 residual of
+    # an FP32 solution is bounded by the rounding of the matrix (~kappa * 6e-8)
+    assert s.getLastRelRes() <= 1e-5
+    assert np.linalg.norm(b - m @ x.astype(np.float64)) <= 1e-3 * np.linalg.norm(b)
+    xd = spla.spsolve(m.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-4 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("which,nch", [("Kp_int", 1), ("M", 2)])
+def test_pcg_mixed_reaches_fp64_tolerance(octx_h025, which, nch):
+    m = octx_h025[which]
+    n = m.shape[0]
+    rng = np.random.default_rng(28)
+    b = rng.uniform(-1, 1, nch * n)
+    s, x = _solve("pcg_mixed", m, b, nch=nch, max_iters=3000, target=1e-10)
+    for c in range(nch):
+        bc = b[c * n:(c + 1) * n]
+        rel = np.linalg.norm(bc - m @ x[c * n:(c + 1) * n]) / np.linalg.norm(bc)
+        assert rel <= 1e-10
+        assert s.getLastRelRes(c) <= 1e-10
+    xd = spla.spsolve(m.tocsc(), b[:n])
+    assert np.linalg.norm(x[:n] - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def test_batch_of_uneven_systems_matches_individual_solves(oracle):
+    from nsfem_amd import cu, fem, mesher
+    mats = []
+    for k, n in enumerate([600, 2500, 1300]):
+        m = mesher.channel_mesh(n, seed=100 + k)
+        ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), fem.DfgConditions(0.01, 1.0, 1.0, 4.0))
+        mats.append(ctx.scipy("pressureStiffnessInternal"))
+    s = cu.createBatchSolver("pcg", 1, mats, 3000, 1e-11, 1)
+    rng = np.random.default_rng(29)
+    bs = [rng.uniform(-1, 1, m.shape[0]) for m in mats]
+    s.getRhs().upload(np.concatenate(bs))
+    s.getSol().memsetZero()
+    s.solve()
+    x = s.getSol().download()
+    for k, (m, b) in enumerate(zip(mats, bs)):
+        o = s.system_offsets[k]
+        xk = x[o:o + m.shape[0]]
+        assert np.linalg.norm(b - m @ xk) <= 1.0001e-11 * np.linalg.norm(b)
+        xo, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-11, 3000)
+        assert abs(s.getLastIterations(k) - it) <= 1
+        assert s.getLastRelRes(0, k) <= 1e-11
+    # colourings of each system equal the single-system colouring
+    for k, m in enumerate(mats):
+        col, ps = s.coloring(k, m.shape[0])
+        oc, ops, _ = oracle.mcgs_layout(m)
+        assert np.array_equal(col, oc) and np.array_equal(ps, ops)
+
+
+def test_batch_gs_per_system_iterations(oracle):
+    from nsfem_amd import cu
+    rng = np.random.default_rng(30)
+    mats = []
+    for n in (50, 400):
+        d = sp.diags([-1, 4.0, -1], [-1, 0, 1], shape=(n, n)).tocsr()
+        mats.append(d)
+    s = cu.createBatchSolver("gs", 1, mats, 100, 1e-8, 1)
+    bs = [rng.uniform(-1, 1, m.shape[0]) for m in mats]
+    s.getRhs().upload(np.concatenate(bs))
+    s.getSol().memsetZero()
+    s.solve()
+    x = s.getSol().download()
+    for k, (m, b) in enumerate(zip(mats, bs)):
+        xo, mse, it = oracle.gs_solve(m, b, np.zeros_like(b), 100, 1e-8, 1)
+        assert s.getLastIterations(k) == it
+        o = s.system_offsets[k]
+        assert np.max(np.abs(x[o:o + m.shape[0]] - xo)) <= 1e-12 * np.max(np.abs(xo))
+
+
+def test_zero_diagonal_rejected():
+    from nsfem_amd import _lib, cu
+    m = sp.csr_matrix(np.array([[0.0, 1.0], [1.0, 2.0]]))
+    with pytest.raises(_lib.NsfemInvalidArgument, match="diagonal"):
+        cu.createSolver("gs", 1, m, 10, 1e-6, 1)
