@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20241103)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run 3 device copies of this size (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -145,7 +148,16 @@ def main():
         return 2
 
     import numpy as np
-    from nsfem_amd import F64, fem
+    from nsfem_amd import F64, cu, fem
+
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        a, b = cu.vec(n, np.float64), cu.vec(n, np.float64)
+        a.memsetZero()
+        for _ in range(3):
+            a.copyTo(b)
+        log(f"calibration: 3 device copies of {args.copy_calibration_mib} MiB")
+        del a, b
 
     t_setup = time.perf_counter()
     m, cond, ctx = build_workload(args.nodes, args.seed)
@@ -161,28 +173,36 @@ def main():
     cfg64 = fem.ChorinConfig(fem.SolverConfig("pcg", 5000, target, 1), fem.SolverConfig("pcg", 50000, target, 1), dtype=F64)
     cfgmx = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 5000, target, 1), fem.SolverConfig("pcg_mixed", 50000, target, 1),
                              dtype=F64, innerTol=args.inner_tol)
-    ch64, r64, first64, last64 = run_path("fp64", ctx, cfg64, tau, u0, args.warmup, args.steps)
-    kern64 = kernel_table(ch64, r64)
-    del ch64
-    chmx, rmx, firstmx, lastmx = run_path("mixed", ctx, cfgmx, tau, u0, args.warmup, args.steps)
-    kernmx = kernel_table(chmx, rmx)
-    del chmx
+    paths = [p.strip() for p in args.paths.split(",") if p.strip()]
+    results, kernels, firsts, lasts = {}, {}, {}, {}
+    for name in paths:
+        cfg = {"fp64": cfg64, "mixed": cfgmx}[name]
+        ch, r, first, last = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps)
+        kernels[name] = kernel_table(ch, r)
+        results[name], firsts[name], lasts[name] = r, first, last
+        del ch
 
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
-    errors = {"mixed_vs_fp64_after_steps": {"velocity": rel(lastmx[0], last64[0]), "pressure": rel(lastmx[1], last64[1])}}
+    errors = {}
+    if "fp64" in results and "mixed" in results:
+        errors["mixed_vs_fp64_after_steps"] = {"velocity": rel(lasts["mixed"][0], lasts["fp64"][0]),
+                                               "pressure": rel(lasts["mixed"][1], lasts["fp64"][1])}
     cpu = None
     if not args.no_cpu:
         cpu_rate, (uo, po) = cpu_baseline(ctx, cond, tau, u0)
-        errors["fp64_vs_oracle_step1"] = {"velocity": rel(first64[0], uo), "pressure": rel(first64[1], po)}
-        errors["mixed_vs_oracle_step1"] = {"velocity": rel(firstmx[0], uo), "pressure": rel(firstmx[1], po)}
+        for name in results:
+            errors[f"{name}_vs_oracle_step1"] = {"velocity": rel(firsts[name][0], uo), "pressure": rel(firsts[name][1], po)}
         cpu = {"value": cpu_rate, "unit": "steps/s", "cores": 1, "kind": "port",
                "sample": "1 Chorin step of the same 500k-DOF workload from the same initial state: FP64 numpy/scipy "
                          "oracle, PCG-MCGS to 1e-10 for all three solves, single thread"}
     log(f"errors: {errors}")
+    head = "mixed" if "mixed" in results else paths[0]
+    rmx, kernmx = results[head], kernels[head]
+    r64 = results.get("fp64")
 
-    # Dominant kernel of the headline (mixed) path by estimated time per step.
+    # Dominant kernel of the headline path by estimated time per step.
     dom_name = max(kernmx, key=lambda k: kernmx[k]["ms_per_step"])
     dom = kernmx[dom_name]
     peak = 8000.0
@@ -200,19 +220,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic: seeded channel mesh and seeded initial perturbation",
         "config": {"workload": "chorin_step_channel_500k", "pressure_dofs": ctx.numPressureNodes,
                    "velocity_dofs_per_component": ctx.numVelocityNodes, "elements": ctx.numElements,
                    "reynolds": 100, "peak_inflow": cond.peakVelocity, "viscosity": cond.viscosity,
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
                    "inner_tol": args.inner_tol, "seed": args.seed},
+        "path": head,
         "fp64": r64,
-        "mixed": rmx,
-        "mixed_over_fp64": rmx["steps_per_s"] / r64["steps_per_s"],
+        "mixed": results.get("mixed"),
+        "mixed_over_fp64": (results["mixed"]["steps_per_s"] / r64["steps_per_s"]) if (r64 and "mixed" in results) else None,
         "errors": errors,
         "roofline": roofline,
-        "kernels": {"fp64": kern64, "mixed": kernmx},
+        "kernels": kernels,
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }

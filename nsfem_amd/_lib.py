@@ -131,6 +131,9 @@ _SIGNATURES = {
     "nsfem_mesh_arrays": [_vp, _vp, _vp, _vp],
     "nsfem_mesh_group_name": [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
     "nsfem_context_build": [_vp, ctypes.POINTER(DfgConditions), ctypes.c_int, ctypes.POINTER(_vp)],
+    "nsfem_context_build_batch": [_vp, ctypes.POINTER(DfgConditions), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+    "nsfem_context_num_systems": [_vp, _c_int_p],
+    "nsfem_context_system_offsets": [_vp, _vp, _vp],
     "nsfem_context_destroy": [_vp],
     "nsfem_context_sizes": [_vp, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p],
     "nsfem_context_matrix": [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, ctypes.POINTER(_vp), ctypes.POINTER(_vp),

@@ -776,6 +776,55 @@ extern "C"
             *out = h.release(); });
     }
 
+    int nsfem_context_build_batch(const nsfem_mesh * const * meshes, const nsfem_dfg_conditions * conds, int count, int numThreads,
+                                  nsfem_context ** out)
+    {
+        return guarded([&]
+                       {
+            need(meshes, "meshes");
+            need(conds, "conds");
+            need(out, "out");
+            if (count < 1) throw std::invalid_argument("count must be positive");
+            std::vector<ChorinContext> parts;
+            parts.reserve(count);
+            const int nt = numThreads > 0 ? numThreads : defaultThreads();
+            for (int k = 0; k < count; k++)
+            {
+                need(meshes[k], "mesh");
+                if (!(conds[k].channelHeight > 0)) throw std::invalid_argument("channelHeight must be positive");
+                DfgConditions c{conds[k].viscosity, conds[k].peakVelocity, conds[k].channelHeight, conds[k].profileScale};
+                ConcreteMesh vm = createMesh(meshes[k]->tri, ElementType::P2);
+                ConcreteMesh pm = createMesh(meshes[k]->tri, ElementType::P1);
+                parts.push_back(buildChorinContext(vm, pm, c, nt));
+            }
+            auto h = std::make_unique<nsfem_context>();
+            h->ctx = concatenateContexts(parts);
+            h->velocityMesh.numElements = h->ctx.convection.numElements;
+            *out = h.release(); });
+    }
+
+    int nsfem_context_num_systems(const nsfem_context * c, int * out)
+    {
+        return guarded([&]
+                       {
+            need(c, "context");
+            need(out, "out");
+            *out = c->ctx.numSystems(); });
+    }
+
+    int nsfem_context_system_offsets(const nsfem_context * c, int * velocityOffsets, int * pressureOffsets)
+    {
+        return guarded([&]
+                       {
+            need(c, "context");
+            const int S = c->ctx.numSystems();
+            for (int k = 0; k < S; k++)
+            {
+                if (velocityOffsets) velocityOffsets[k] = c->ctx.velocityOffsets.empty() ? 0 : c->ctx.velocityOffsets[k];
+                if (pressureOffsets) pressureOffsets[k] = c->ctx.pressureOffsets.empty() ? 0 : c->ctx.pressureOffsets[k];
+            } });
+    }
+
     int nsfem_context_destroy(nsfem_context * c)
     {
         return guarded([&]
@@ -867,6 +916,7 @@ extern "C"
         return guarded([&]
                        {
             need(c, "context");
+            if (c->ctx.numSystems() > 1) throw std::invalid_argument("batch contexts have no single mesh");
             meshOut(c->velocityMesh, xy, elements); });
     }
 
@@ -875,6 +925,7 @@ extern "C"
         return guarded([&]
                        {
             need(c, "context");
+            if (c->ctx.numSystems() > 1) throw std::invalid_argument("batch contexts have no single mesh");
             meshOut(c->pressureMesh, xy, elements); });
     }
 
@@ -890,6 +941,7 @@ extern "C"
             need(rowStart, "rowStart");
             need(column, "column");
             need(values, "values");
+            if (c->ctx.numSystems() > 1) throw std::invalid_argument("batch contexts have no single mesh");
             CsrD m = buildFastConvectionMap(c->velocityMesh, c->ctx.velocityStiffness, numThreads > 0 ? numThreads : defaultThreads());
             *rows = m.rows;
             *cols = m.cols;

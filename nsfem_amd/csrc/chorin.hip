@@ -1,5 +1,6 @@
 #include "chorin.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <functional>
 
@@ -183,9 +184,21 @@ namespace nsfem
             o.lanes = lanes;
             return o;
         };
-        vSolver_ = std::make_unique<McgsSolver>(std::vector<const CsrD *>{&ctx.velocityMass}, solverOpts(cfg_.velocity, 2, cfg_.lanesVelocity));
-        pSolver_ = std::make_unique<McgsSolver>(std::vector<const CsrD *>{&ctx.pressureStiffnessInternal},
-                                                solverOpts(cfg_.pressure, 1, cfg_.lanesPressure));
+        // One solver system per instance (per-instance colouring and
+        // convergence) for batches; the whole matrix otherwise.
+        std::vector<const CsrD *> vsys, psys;
+        if (ctx.massBlocks.empty())
+        {
+            vsys.push_back(&ctx.velocityMass);
+            psys.push_back(&ctx.pressureStiffnessInternal);
+        }
+        else
+        {
+            for (const auto & m : ctx.massBlocks) vsys.push_back(&m);
+            for (const auto & m : ctx.pressureBlocks) psys.push_back(&m);
+        }
+        vSolver_ = std::make_unique<McgsSolver>(vsys, solverOpts(cfg_.velocity, 2, cfg_.lanesVelocity));
+        pSolver_ = std::make_unique<McgsSolver>(psys, solverOpts(cfg_.pressure, 1, cfg_.lanesPressure));
         for (auto & e : ev_)
         {
             NSFEM_HIP(hipEventCreate(&e));
@@ -295,6 +308,42 @@ namespace nsfem
         return static_cast<double>(ms) / reps;
     }
 
+    namespace
+    {
+        // Worst instance of a (possibly batched) solve.
+        double maxMse(const McgsSolver & s, int c)
+        {
+            double m = s.lastMse(0, c);
+            for (int k = 1; k < s.numSystems(); k++)
+            {
+                const double v = s.lastMse(k, c);
+                m = (v != v || v > m) ? v : m;
+            }
+            return m;
+        }
+        double maxRel(const McgsSolver & s)
+        {
+            double m = -1;
+            for (int k = 0; k < s.numSystems(); k++)
+            {
+                for (int c = 0; c < s.numCh(); c++)
+                {
+                    m = std::max(m, s.lastRelRes(k, c));
+                }
+            }
+            return m;
+        }
+        int maxIters(const McgsSolver & s)
+        {
+            int m = 0;
+            for (int k = 0; k < s.numSystems(); k++)
+            {
+                m = std::max(m, s.lastIterations(k));
+            }
+            return m;
+        }
+    } // namespace
+
     void ChorinGpu::step(ChorinStepStats * stats)
     {
         const hipStream_t s = libraryStream();
@@ -311,10 +360,10 @@ namespace nsfem
         csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.lanesVelocity, s);
         fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
         vSolver_->solve();
-        st.mseTentX = vSolver_->lastMse(0, 0);
-        st.mseTentY = vSolver_->lastMse(0, 1);
-        st.relTent = std::max(vSolver_->lastRelRes(0, 0), vSolver_->lastRelRes(0, 1));
-        st.itersTent = vSolver_->lastIterations(0);
+        st.mseTentX = maxMse(*vSolver_, 0);
+        st.mseTentY = maxMse(*vSolver_, 1);
+        st.relTent = maxRel(*vSolver_);
+        st.itersTent = maxIters(*vSolver_);
         st.outerTent = vSolver_->lastOuterIterations();
         if (!std::isfinite(st.mseTentX) || !std::isfinite(st.mseTentY) || vSolver_->lastNonFinite())
         {
@@ -327,9 +376,9 @@ namespace nsfem
         // Pressure: Kp_int p = -(1/tau) (VPD u)[internal], warm start
         csrMul(vpd_.view(dt), 1, u_.get(), 0, pSolver_->ioRhs(), 0, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
         pSolver_->solve();
-        st.msePressure = pSolver_->lastMse(0, 0);
-        st.relPressure = pSolver_->lastRelRes(0, 0);
-        st.itersPressure = pSolver_->lastIterations(0);
+        st.msePressure = maxMse(*pSolver_, 0);
+        st.relPressure = maxRel(*pSolver_);
+        st.itersPressure = maxIters(*pSolver_);
         st.outerPressure = pSolver_->lastOuterIterations();
         scatter(dt, nIntP_, internalP_.get(), pSolver_->ioSol(), p_.get(), s);
         NSFEM_HIP(hipEventRecord(ev_[3], s));
@@ -338,10 +387,10 @@ namespace nsfem
         csrMul(pvd_.view(dt), 1, p_.get(), 0, vSolver_->ioRhs(), 0, 1.0, 0.0, nullptr, 2 * N_, 0, s);
         fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
         vSolver_->solve();
-        st.mseFinalX = vSolver_->lastMse(0, 0);
-        st.mseFinalY = vSolver_->lastMse(0, 1);
-        st.relFinal = std::max(vSolver_->lastRelRes(0, 0), vSolver_->lastRelRes(0, 1));
-        st.itersFinal = vSolver_->lastIterations(0);
+        st.mseFinalX = maxMse(*vSolver_, 0);
+        st.mseFinalY = maxMse(*vSolver_, 1);
+        st.relFinal = maxRel(*vSolver_);
+        st.itersFinal = maxIters(*vSolver_);
         st.outerFinal = vSolver_->lastOuterIterations();
         if (!std::isfinite(st.mseFinalX) || !std::isfinite(st.mseFinalY) || vSolver_->lastNonFinite())
         {

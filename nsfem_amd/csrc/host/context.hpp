@@ -54,6 +54,14 @@ namespace nsfem
 
         ConvectionElementData convection;
         double viscosity = 0;
+
+        // Block-diagonal batches of independent instances: the solvers are
+        // built per instance from these blocks (empty for one instance).
+        std::vector<CsrD> massBlocks;
+        std::vector<CsrD> pressureBlocks;
+        std::vector<int> velocityOffsets; // per instance, into one velocity component
+        std::vector<int> pressureOffsets;
+        int numSystems() const { return massBlocks.empty() ? 1 : static_cast<int>(massBlocks.size()); }
     };
 
     std::vector<DirichletNode> extractDirichletNodes(const ConcreteMesh & mesh, const std::vector<int> & groupIds,
@@ -63,4 +71,10 @@ namespace nsfem
 
     ChorinContext buildChorinContext(const ConcreteMesh & velocityMesh, const ConcreteMesh & pressureMesh,
                                      const DfgConditions & cond, int nThreads);
+
+    // Concatenates independent instances (different meshes and conditions)
+    // into one block-diagonal context: velocity planar over the whole batch
+    // ([ux of all instances; uy of all instances]), K pre-scaled by each
+    // instance's viscosity (the batch viscosity is 1).
+    ChorinContext concatenateContexts(const std::vector<ChorinContext> & parts);
 } // namespace nsfem

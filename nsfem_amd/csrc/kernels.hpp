@@ -43,6 +43,16 @@ namespace nsfem
         return v;
     }
 
+    // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+    // L2).  Map the dispatch index to a logical block so that every XCD works
+    // on one contiguous range of logical blocks (neighbouring rows share
+    // their x entries); a bijection on [0, G) for any G.
+    __device__ __forceinline__ int logicalBlock(int p, int G)
+    {
+        const int q = G >> 3, r = G & 7, x = p & 7, i = p >> 3;
+        return x * q + (x < r ? x : r) + i;
+    }
+
     // Sum of v over the block (256 threads); result valid in thread 0.
     template <int K>
     __device__ __forceinline__ void blockSum(double (&v)[K], double * sm /* K * kWaves */)
@@ -85,7 +95,7 @@ namespace nsfem
     // Block partial -> partials[block*K + k]; returns true in the block that
     // arrived last (every thread of it), after an acquire fence.
     template <int K>
-    __device__ __forceinline__ bool arriveLast(const double (&v)[K], double * partials, unsigned * counter)
+    __device__ __forceinline__ bool arriveLast(const double (&v)[K], double * partials, unsigned * counter, int slot)
     {
         __shared__ bool last;
         if (threadIdx.x == 0)
@@ -93,7 +103,7 @@ namespace nsfem
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
-                partials[static_cast<size_t>(blockIdx.x) * K + k] = v[k];
+                partials[static_cast<size_t>(slot) * K + k] = v[k];
             }
             __threadfence();
             const unsigned ticket = atomicAdd(counter, 1u);
@@ -117,7 +127,7 @@ namespace nsfem
                                                             const T * __restrict__ x, long ldx, T * __restrict__ y, long ldy,
                                                             T alpha, T beta, const int * __restrict__ rowIds)
     {
-        const long gid = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x;
+        const long gid = static_cast<long>(logicalBlock(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
         const int row = static_cast<int>(gid / LANES);
         const int lane = threadIdx.x & (LANES - 1);
         T acc[NCH];
@@ -250,7 +260,7 @@ namespace nsfem
             }
         }
         blockSum<K>(acc, sm);
-        if (arriveLast<K>(acc, partials, counter))
+        if (arriveLast<K>(acc, partials, counter, blockIdx.x))
         {
             double tot[K];
 #pragma unroll
@@ -483,7 +493,7 @@ namespace nsfem
         {
             return;
         }
-        const int blk = blockOffset + blockIdx.x;
+        const int blk = blockOffset + logicalBlock(blockIdx.x, gridDim.x);
         if (!systemActive<NCH>(st, bm.blockSystem[blk]))
         {
             return;
@@ -518,6 +528,7 @@ namespace nsfem
                                                                const T * __restrict__ x, int iter, double * totals)
     {
         __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
         if (st.flags[0])
         {
             return; // the whole grid skips, nobody touches the counter
@@ -528,9 +539,9 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        if (systemActive<NCH>(st, bm.blockSystem[blockIdx.x]))
+        if (systemActive<NCH>(st, bm.blockSystem[lb]))
         {
-            const int2 range = bm.blocks[blockIdx.x];
+            const int2 range = bm.blocks[lb];
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
                 const int row = base + static_cast<int>(threadIdx.x) / LANES;
@@ -553,7 +564,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter))
+        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
         {
             systemTotals<NCH>(bm, st.partials, totals, sm);
             if (threadIdx.x == 0)
@@ -612,13 +623,14 @@ namespace nsfem
                                                             double * totals)
     {
         __shared__ double sm[2 * NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
         double part[2 * NCH];
 #pragma unroll
         for (int k = 0; k < 2 * NCH; k++)
         {
             part[k] = 0;
         }
-        const int2 range = bm.blocks[blockIdx.x];
+        const int2 range = bm.blocks[lb];
         for (int base = range.x; base < range.y; base += kBlock / LANES)
         {
             const int row = base + static_cast<int>(threadIdx.x) / LANES;
@@ -658,7 +670,7 @@ namespace nsfem
             }
         }
         blockSum<2 * NCH>(part, sm);
-        if (arriveLast<2 * NCH>(part, st.partials, st.counter))
+        if (arriveLast<2 * NCH>(part, st.partials, st.counter, lb))
         {
             systemTotals<2 * NCH>(bm, st.partials, totals, sm);
             if (threadIdx.x == 0)
@@ -704,6 +716,7 @@ namespace nsfem
                                                            T * __restrict__ z, int color0End, double * totals)
     {
         __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
         if (st.flags[0])
         {
             return;
@@ -714,9 +727,9 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        if (systemActive<NCH>(st, bm.blockSystem[blockIdx.x]))
+        if (systemActive<NCH>(st, bm.blockSystem[lb]))
         {
-            const int2 range = bm.blocks[blockIdx.x];
+            const int2 range = bm.blocks[lb];
             const bool c0 = range.x < color0End; // blocks never straddle colours
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
@@ -752,7 +765,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter))
+        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
         {
             systemTotals<NCH>(bm, st.partials, totals, sm);
             if (threadIdx.x == 0)
@@ -783,6 +796,7 @@ namespace nsfem
                                                                  T * __restrict__ p, T * __restrict__ q, double * totals)
     {
         __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
         if (st.flags[0])
         {
             return;
@@ -793,7 +807,7 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int s = bm.blockSystem[blockIdx.x];
+        const int s = bm.blockSystem[lb];
         if (systemActive<NCH>(st, s))
         {
             const bool first = st.flags[1] != 0;
@@ -803,7 +817,7 @@ namespace nsfem
             {
                 beta[c] = static_cast<T>(st.beta[s * NCH + c]);
             }
-            const int2 range = bm.blocks[blockIdx.x];
+            const int2 range = bm.blocks[lb];
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
                 const int row = base + static_cast<int>(threadIdx.x) / LANES;
@@ -839,7 +853,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter))
+        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
         {
             systemTotals<NCH>(bm, st.partials, totals, sm);
             if (threadIdx.x == 0)
@@ -862,6 +876,7 @@ namespace nsfem
                                                               T * __restrict__ z, int color0End, double * totals)
     {
         __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
         if (st.flags[0])
         {
             return;
@@ -872,7 +887,7 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int s = bm.blockSystem[blockIdx.x];
+        const int s = bm.blockSystem[lb];
         if (systemActive<NCH>(st, s))
         {
             T alpha[NCH];
@@ -883,7 +898,7 @@ namespace nsfem
                 alpha[c] = static_cast<T>(st.alpha[s * NCH + c]);
                 act[c] = st.active[s * NCH + c];
             }
-            const int2 range = bm.blocks[blockIdx.x];
+            const int2 range = bm.blocks[lb];
             for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
             {
 #pragma unroll
@@ -906,7 +921,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter))
+        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
         {
             systemTotals<NCH>(bm, st.partials, totals, sm);
             if (threadIdx.x == 0)

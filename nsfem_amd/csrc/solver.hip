@@ -518,7 +518,7 @@ namespace nsfem
 
     template <typename T>
     int McgsSolver::runPcg(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * b, T * x, T * r, T * z, T * p, T * q,
-                           bool zeroGuess, const int * gate, int maxIters, bool sync)
+                           bool zeroGuess, const int * gate, int maxIters, bool sync, bool usePrediction)
     {
         const int nch = opt_.numCh;
         DeviceState & ds = inner_;
@@ -555,6 +555,10 @@ namespace nsfem
             g.k = chunk;
             g.tol2 = st.tol2;
         }
+        // The first host poll comes one chunk before the iteration count of
+        // the previous solve on this path (warm-started solves repeat it).
+        int & predicted = std::is_same_v<T, double> ? predicted64_ : predicted32_;
+        const int firstPoll = usePrediction ? std::max(chunk, predicted - chunk) : chunk;
         int launched = 0;
         while (launched < maxIters)
         {
@@ -572,10 +576,14 @@ namespace nsfem
             }
             NSFEM_HIP(hipGetLastError());
             launched += k;
-            if (sync && pollDone(s, ds))
+            if (sync && launched >= firstPoll && pollDone(s, ds))
             {
                 break;
             }
+        }
+        if (usePrediction)
+        {
+            predicted = launched;
         }
         return launched;
     }
@@ -605,7 +613,7 @@ namespace nsfem
             }
             convert(F64, F32, nv, r64_.get(), b32_.get(), s);
             runPcg<float>(s, A32_, inner, b32_.get(), x32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(), true,
-                          outer.active, opt_.maxIters, true);
+                          outer.active, opt_.maxIters, true, false);
             // accumulate inner iteration counts
             NSFEM_HIP(hipMemcpyAsync(hostInts_, inner_.ints.get(), (2 * K + 4) * sizeof(int), hipMemcpyDeviceToHost, s));
             addConvertKernel<float><<<elementwiseGrid(nv), 256, 0, s>>>(nv, x32_.get(), x64_.get());
@@ -661,7 +669,7 @@ namespace nsfem
             permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
             SolveState st = inner_.view(K, tol2);
             runPcg<double>(s, A64_, st, b64_.get(), x64_.get(), r64_.get(), z64_.get(), p64_.get(), q64_.get(), false, nullptr,
-                           opt_.maxIters, true);
+                           opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
             permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
         }
@@ -671,7 +679,7 @@ namespace nsfem
             permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
             SolveState st = inner_.view(K, tol2);
             runPcg<float>(s, A32_, st, b32_.get(), x32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(), false, nullptr,
-                          opt_.maxIters, true);
+                          opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
             permute(F32, F32, n_, nch, perm_.get(), x32_.get(), ioSol_.get(), true, s);
         }

@@ -132,7 +132,7 @@ namespace nsfem
         void runGs(hipStream_t s);
         template <typename T>
         int runPcg(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * b, T * x, T * r, T * z, T * p, T * q,
-                   bool zeroGuess, const int * gate, int maxIters, bool sync);
+                   bool zeroGuess, const int * gate, int maxIters, bool sync, bool usePrediction);
         void runMixed(hipStream_t s);
         template <typename T>
         void sgsSweeps(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * r, T * z);
@@ -185,5 +185,6 @@ namespace nsfem
         };
         ChunkGraph graph64_, graph32_;
         bool useGraphs_ = true;
+        int predicted64_ = 0, predicted32_ = 0;
     };
 } // namespace nsfem

@@ -103,11 +103,42 @@ MATRICES = {
 class ChorinContext:
     """Host-side set-up of one mesh: P2/P1 numbering, Dirichlet lists, FP64 matrices."""
 
-    def __init__(self, mesh: TriangleMesh, cond: DfgConditions, threads=0):
+    def __init__(self, mesh: TriangleMesh, cond: DfgConditions, threads=0, _handle=None):
         self.cond = cond
-        c = _lib.DfgConditions(cond.viscosity, cond.peakVelocity, cond.channelHeight, cond.profileScale)
-        self._h = ctypes.c_void_p()
-        call("nsfem_context_build", mesh.handle, ctypes.byref(c), threads, ctypes.byref(self._h))
+        if _handle is None:
+            c = _lib.DfgConditions(cond.viscosity, cond.peakVelocity, cond.channelHeight, cond.profileScale)
+            self._h = ctypes.c_void_p()
+            call("nsfem_context_build", mesh.handle, ctypes.byref(c), threads, ctypes.byref(self._h))
+        else:
+            self._h = _handle
+        self._init_sizes()
+
+    @classmethod
+    def batch(cls, meshes, conds, threads=0):
+        """Independent instances as one block-diagonal context (velocity
+        [ux of all; uy of all]); the Chorin solvers keep one system per instance."""
+        if len(meshes) != len(conds) or not meshes:
+            raise ValueError("need one condition per mesh")
+        arr = (_lib.DfgConditions * len(conds))(*[
+            _lib.DfgConditions(c.viscosity, c.peakVelocity, c.channelHeight, c.profileScale) for c in conds])
+        hs = (ctypes.c_void_p * len(meshes))(*[m.handle.value for m in meshes])
+        h = ctypes.c_void_p()
+        call("nsfem_context_build_batch", ctypes.cast(hs, ctypes.c_void_p), arr, len(meshes), threads, ctypes.byref(h))
+        return cls(None, None, _handle=h)
+
+    def num_systems(self):
+        out = ctypes.c_int()
+        call("nsfem_context_num_systems", self._h, ctypes.byref(out))
+        return out.value
+
+    def system_offsets(self):
+        k = self.num_systems()
+        vo = np.empty(k, np.int32)
+        po = np.empty(k, np.int32)
+        call("nsfem_context_system_offsets", self._h, _ptr(vo), _ptr(po))
+        return vo, po
+
+    def _init_sizes(self):
         N, V, nd, ni, E = (ctypes.c_int() for _ in range(5))
         call("nsfem_context_sizes", self._h, ctypes.byref(N), ctypes.byref(V), ctypes.byref(nd), ctypes.byref(ni), ctypes.byref(E))
         self.numVelocityNodes = N.value
