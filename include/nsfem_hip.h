@@ -126,7 +126,9 @@ int nsfem_solver_last_iterations_system(const nsfem_solver * s, int system, int 
 int nsfem_solver_set_max_iters(nsfem_solver * s, int n);
 int nsfem_solver_set_target_mse(nsfem_solver * s, double target);
 int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
-/* Tunables: "innerTol" (pcg_mixed), "checkChunk" (iterations between host polls). */
+/* Options: "innerTol" (pcg_mixed inner relative tolerance), "projectMean" (1:
+ * pure-Neumann system with a zero-mean right-hand side; the solution is
+ * returned with zero mean and FP32 inner right-hand sides are projected). */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream): average ms per launch of
  * "direction" (A z fused with the PCG direction update), "sweep" (one

@@ -607,6 +607,10 @@ extern "C"
                 if (!(value > 0 && value < 1)) throw std::invalid_argument("innerTol must be in (0, 1)");
                 s->core->setInnerTol(value);
             }
+            else if (k == "projectMean")
+            {
+                s->core->setProjectMean(value != 0);
+            }
             else
             {
                 throw std::invalid_argument("Unknown solver option: " + k);

@@ -969,4 +969,62 @@ namespace nsfem
             x[i] += static_cast<double>(e[i]);
         }
     }
+
+    // Mean of each (system, channel) over its rows: mean[s*NCH + c] (FP64).
+    template <typename T, int NCH>
+    __global__ void __launch_bounds__(kBlock) meanKernel(int n, BlockMap bm, const T * __restrict__ v, double * partials,
+                                                         unsigned * counter, double * totals, double * mean)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        double part[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            part[c] = 0;
+        }
+        const int2 range = bm.blocks[lb];
+        for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
+        {
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                part[c] += static_cast<double>(v[row + static_cast<long>(c) * n]);
+            }
+        }
+        blockSum<NCH>(part, sm);
+        if (arriveLast<NCH>(part, partials, counter, lb))
+        {
+            systemTotals<NCH>(bm, partials, totals, sm);
+            if (threadIdx.x == 0)
+            {
+                for (int s = 0; s < bm.numSystems; s++)
+                {
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        mean[s * NCH + c] = totals[s * NCH + c] / bm.systemRows[s];
+                    }
+                }
+                *counter = 0;
+            }
+        }
+    }
+
+    // v -= mean of its (system, channel).
+    template <typename T, int NCH>
+    __global__ void __launch_bounds__(kBlock) subtractMeanKernel(int n, BlockMap bm, T * __restrict__ v, const double * __restrict__ mean)
+    {
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        const int s = bm.blockSystem[lb];
+        const int2 range = bm.blocks[lb];
+        for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
+        {
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                const long i = row + static_cast<long>(c) * n;
+                v[i] = static_cast<T>(static_cast<double>(v[i]) - mean[s * NCH + c]);
+            }
+        }
+    }
 } // namespace nsfem

@@ -320,6 +320,7 @@ namespace nsfem
             outer_.init(numBlocks_, numSystems_, opt_.numCh);
         }
         inner_.init(numBlocks_, numSystems_, opt_.numCh);
+        means_.resize(static_cast<size_t>(numSystems_) * opt_.numCh);
 
         const int K = numSystems_ * opt_.numCh;
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostFlags_), 4 * sizeof(int), hipHostMallocDefault));
@@ -612,6 +613,10 @@ namespace nsfem
                 break;
             }
             convert(F64, F32, nv, r64_.get(), b32_.get(), s);
+            if (opt_.projectMean)
+            {
+                projectMean<float>(s, b32_.get());
+            }
             runPcg<float>(s, A32_, inner, b32_.get(), x32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(), true,
                           outer.active, opt_.maxIters, true, false);
             // accumulate inner iteration counts
@@ -635,6 +640,10 @@ namespace nsfem
             lastIters_[sys] = totalIters[sys];
         }
         lastOuter_ = outerIt;
+        if (opt_.projectMean)
+        {
+            projectMean<double>(s, x64_.get());
+        }
         permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
         NSFEM_HIP(hipStreamSynchronize(s));
     }
@@ -671,6 +680,10 @@ namespace nsfem
             runPcg<double>(s, A64_, st, b64_.get(), x64_.get(), r64_.get(), z64_.get(), p64_.get(), q64_.get(), false, nullptr,
                            opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
+            if (opt_.projectMean)
+            {
+                projectMean<double>(s, x64_.get());
+            }
             permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
         }
         else
@@ -681,9 +694,32 @@ namespace nsfem
             runPcg<float>(s, A32_, st, b32_.get(), x32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(), false, nullptr,
                           opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
+            if (opt_.projectMean)
+            {
+                projectMean<float>(s, x32_.get());
+            }
             permute(F32, F32, n_, nch, perm_.get(), x32_.get(), ioSol_.get(), true, s);
         }
         NSFEM_HIP(hipStreamSynchronize(s));
+    }
+
+    template <typename T>
+    void McgsSolver::projectMean(hipStream_t s, T * v)
+    {
+        // reuse the inner state's reduction scratch (no solve is in flight)
+        if (opt_.numCh == 1)
+        {
+            meanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get(), inner_.counter.get(), inner_.totals.get(),
+                                                           means_.get());
+            subtractMeanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
+        }
+        else
+        {
+            meanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get(), inner_.counter.get(), inner_.totals.get(),
+                                                           means_.get());
+            subtractMeanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
+        }
+        NSFEM_HIP(hipGetLastError());
     }
 
     void McgsSolver::solveAsync(hipStream_t) { solve(); }

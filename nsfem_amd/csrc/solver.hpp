@@ -53,6 +53,10 @@ namespace nsfem
         int checkChunk = 16;     // iterations between host convergence polls
         double innerTol = 1e-4;  // mixed: relative tolerance of each inner solve
         int maxOuter = 40;       // mixed: refinement steps
+        // Pure-Neumann (singular, zero row sum) systems with zero-mean right
+        // hand sides: the solution is returned with zero mean, and the FP32
+        // inner right-hand sides of the mixed path are projected to zero mean.
+        bool projectMean = false;
     };
 
     SolverOptions parseSolverName(const std::string & name);
@@ -114,6 +118,7 @@ namespace nsfem
         void setTarget(double t) { opt_.target = t; }
         void setMseCheckInterval(int k);
         void setInnerTol(double t) { opt_.innerTol = t; }
+        void setProjectMean(bool on) { opt_.projectMean = on; }
         const SolverOptions & options() const { return opt_; }
 
         hipStream_t stream() const { return stream_; }
@@ -139,6 +144,8 @@ namespace nsfem
         template <typename T>
         void pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
+        template <typename T>
+        void projectMean(hipStream_t s, T * v);
         bool pollDone(hipStream_t s, DeviceState & st);
 
         SolverOptions opt_;
@@ -166,6 +173,7 @@ namespace nsfem
         DeviceArray<float> b32_, x32_, r32_, z32_, p32_, q32_;
 
         DeviceState inner_, outer_;
+        DeviceArray<double> means_;
         int * hostFlags_ = nullptr; // pinned
         double * hostScal_ = nullptr;
         int * hostInts_ = nullptr;

@@ -199,3 +199,28 @@ def test_zero_diagonal_rejected():
     m = sp.csr_matrix(np.array([[0.0, 1.0], [1.0, 2.0]]))
     with pytest.raises(_lib.NsfemInvalidArgument, match="diagonal"):
         cu.createSolver("gs", 1, m, 10, 1e-6, 1)
+
+
+@pytest.mark.parametrize("name", ["pcg", "pcg_mixed"])
+def test_pure_neumann_laplacian_with_mean_projection(oracle, octx_h025, name):
+    """Singular P1 Laplacian (all-Neumann), zero-mean RHS: the solution is the
+    zero-mean one (pseudo-inverse), to 1e-10 relative residual."""
+    from nsfem_amd import cu
+    m = octx_h025["Kp"]
+    n = m.shape[0]
+    rng = np.random.default_rng(31)
+    b = rng.uniform(-1, 1, n)
+    b -= b.mean()
+    s = cu.createSolver(name, 1, m, 5000, 1e-10, 1)
+    s.setOption("projectMean", 1)
+    s.getRhs().upload(b)
+    s.getSol().memsetZero()
+    s.solve()
+    x = s.getSol().download()
+    assert abs(x.mean()) <= 1e-12 * np.max(np.abs(x))
+    assert np.linalg.norm(b - m @ x) <= 1.01e-10 * np.linalg.norm(b)
+    xp = np.linalg.lstsq(m.toarray(), b, rcond=None)[0]
+    xp -= xp.mean()
+    assert np.linalg.norm(x - xp) <= 1e-7 * np.linalg.norm(xp)
+    xo, rel, it = oracle.pcg_solve(m, b, np.zeros(n), 1e-10, 5000)
+    assert np.linalg.norm((xo - xo.mean()) - xp) <= 1e-7 * np.linalg.norm(xp)

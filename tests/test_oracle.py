@@ -293,3 +293,16 @@ def test_chorin_zero_inflow_stays_at_rest(oracle, octx_c3):
     orc = oracle.ChorinOracle(ctx, 1e-3)
     u, p = orc.step()
     assert np.all(u == 0) and np.all(p == 0)
+
+
+def test_pcg_on_consistent_singular_system(oracle, h025):
+    """All-Neumann P1 Laplacian with a zero-mean RHS: SGS-PCG converges; the
+    zero-mean part of the iterate is the pseudo-inverse solution."""
+    m = h025["Kp"]
+    rng = np.random.default_rng(8)
+    b = rng.uniform(-1, 1, m.shape[0])
+    b -= b.mean()
+    x, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-10, 5000)
+    assert rel <= 1e-10
+    xp = np.linalg.lstsq(m.toarray(), b, rcond=None)[0]
+    assert np.linalg.norm((x - x.mean()) - (xp - xp.mean())) <= 1e-7 * np.linalg.norm(xp)
