@@ -8,8 +8,8 @@
 //     xor-shuffles inside the lane group.
 //   * Multi-channel vectors are planar: channel c of vector v is v[c*ld + i]
 //     (the reference's [ux; uy] layout, chorinCuda.cu:189-196).
-//   * Reductions are deterministic: each block writes its partial, the last
-//     block to arrive (device-scope ticket) sums the partials in block order.
+//   * Reductions are deterministic and fence-free: each block writes its
+//     partial, a one-block finalize kernel sums them in block-map order.
 //   * Solver kernels walk a block map: every block owns one row range that
 //     lies inside one (colour, system) chunk of the colour-major permuted
 //     system, so colour sweeps launch only their colour's blocks and every
@@ -92,12 +92,14 @@ namespace nsfem
         __syncthreads();
     }
 
-    // Block partial -> partials[block*K + k]; returns true in the block that
-    // arrived last (every thread of it), after an acquire fence.
+    // Block partial -> partials[slot*K + k] (thread 0, after blockSum).  The
+    // partials are summed by a one-block finalize kernel launched next on the
+    // same stream: the kernel boundary makes them visible across XCDs, which
+    // a device-scope fence inside the producer would do per block at the
+    // price of an L2 write-back and invalidate on every block.
     template <int K>
-    __device__ __forceinline__ bool arriveLast(const double (&v)[K], double * partials, unsigned * counter, int slot)
+    __device__ __forceinline__ void storePartial(const double (&v)[K], double * partials, int slot)
     {
-        __shared__ bool last;
         if (threadIdx.x == 0)
         {
 #pragma unroll
@@ -105,16 +107,7 @@ namespace nsfem
             {
                 partials[static_cast<size_t>(slot) * K + k] = v[k];
             }
-            __threadfence();
-            const unsigned ticket = atomicAdd(counter, 1u);
-            last = ticket == gridDim.x - 1;
         }
-        __syncthreads();
-        if (last)
-        {
-            __threadfence();
-        }
-        return last;
     }
 
     // ------------------------------------------------------------------
@@ -242,7 +235,7 @@ namespace nsfem
     // Two-level dot products: K pairs (x_k, y_k) of length n each.
     template <typename T, int K>
     __global__ void __launch_bounds__(kBlock) dotKernel(long n, const T * __restrict__ x, long ldx, const T * __restrict__ y,
-                                                         long ldy, double * partials, unsigned * counter, double * out)
+                                                         long ldy, double * partials)
     {
         __shared__ double sm[K * kWaves];
         double acc[K];
@@ -260,31 +253,35 @@ namespace nsfem
             }
         }
         blockSum<K>(acc, sm);
-        if (arriveLast<K>(acc, partials, counter, blockIdx.x))
+        storePartial<K>(acc, partials, blockIdx.x);
+    }
+
+    // One block: out[k] = sum over the producer's blocks, in block order.
+    template <int K>
+    __global__ void __launch_bounds__(kBlock) dotFinalizeKernel(int blocks, const double * __restrict__ partials, double * out)
+    {
+        __shared__ double sm[K * kWaves];
+        double tot[K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
         {
-            double tot[K];
+            tot[k] = 0;
+        }
+        for (int b = threadIdx.x; b < blocks; b += kBlock)
+        {
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
-                tot[k] = 0;
+                tot[k] += partials[static_cast<size_t>(b) * K + k];
             }
-            for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-            {
+        }
+        blockSum<K>(tot, sm);
+        if (threadIdx.x == 0)
+        {
 #pragma unroll
-                for (int k = 0; k < K; k++)
-                {
-                    tot[k] += partials[static_cast<size_t>(b) * K + k];
-                }
-            }
-            blockSum<K>(tot, sm);
-            if (threadIdx.x == 0)
+            for (int k = 0; k < K; k++)
             {
-#pragma unroll
-                for (int k = 0; k < K; k++)
-                {
-                    out[k] = tot[k];
-                }
-                *counter = 0;
+                out[k] = tot[k];
             }
         }
     }
@@ -352,7 +349,6 @@ namespace nsfem
         int * active;   // 1 while the channel iterates
         int * iters;    // iterations done / iteration index at convergence
         int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen
-        unsigned * counter;
         double * partials; // blocks * 2 * NCH
         double tol2;       // relative tolerance squared (pcg) or target RMS (gs)
     };
@@ -525,13 +521,13 @@ namespace nsfem
     // the target (gaussSeidel.cu:267-290, 354-380).  st.tol2 holds the target.
     template <typename T, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) gsResidualKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
-                                                               const T * __restrict__ x, int iter, double * totals)
+                                                               const T * __restrict__ x)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         if (st.flags[0])
         {
-            return; // the whole grid skips, nobody touches the counter
+            return; // the whole grid and its finalize skip
         }
         double part[NCH];
 #pragma unroll
@@ -564,52 +560,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
-        {
-            systemTotals<NCH>(bm, st.partials, totals, sm);
-            if (threadIdx.x == 0)
-            {
-                int anyActive = 0;
-                for (int sys = 0; sys < bm.numSystems; sys++)
-                {
-                    if (!systemActive<NCH>(st, sys))
-                    {
-                        continue;
-                    }
-                    bool all = true;
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        const double m = sqrt(totals[sys * NCH + c] / bm.systemRows[sys]);
-                        st.mse[sys * NCH + c] = m;
-                        if (!(m < st.tol2))
-                        {
-                            all = false;
-                        }
-                        if (!(m == m) || m > 1e300)
-                        {
-                            st.flags[2] = 1;
-                        }
-                    }
-                    if (all)
-                    {
-                        for (int c = 0; c < NCH; c++)
-                        {
-                            st.active[sys * NCH + c] = 0;
-                            st.iters[sys * NCH + c] = iter;
-                        }
-                    }
-                    else
-                    {
-                        anyActive = 1;
-                    }
-                }
-                if (!anyActive)
-                {
-                    st.flags[0] = 1;
-                }
-                *st.counter = 0;
-            }
-        }
+        storePartial<NCH>(part, st.partials, lb);
     }
 
     // PCG set-up: r = b - A x (or r = b with x = 0), z = r / d on colour 0,
@@ -619,8 +570,7 @@ namespace nsfem
     template <typename T, typename TR, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgInitKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
                                                             T * __restrict__ x, TR * __restrict__ r, T * __restrict__ z,
-                                                            int color0End, int zeroGuess, const int * __restrict__ gate,
-                                                            double * totals)
+                                                            int color0End, int zeroGuess)
     {
         __shared__ double sm[2 * NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -670,50 +620,13 @@ namespace nsfem
             }
         }
         blockSum<2 * NCH>(part, sm);
-        if (arriveLast<2 * NCH>(part, st.partials, st.counter, lb))
-        {
-            systemTotals<2 * NCH>(bm, st.partials, totals, sm);
-            if (threadIdx.x == 0)
-            {
-                int anyActive = 0;
-                for (int s = 0; s < bm.numSystems; s++)
-                {
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        const int k = s * NCH + c;
-                        const double bb = totals[s * 2 * NCH + c];
-                        const double rr = totals[s * 2 * NCH + NCH + c];
-                        st.bb[k] = bb;
-                        st.rr[k] = rr;
-                        st.ref[k] = bb > 0 ? bb : rr;
-                        st.iters[k] = 0;
-                        st.alpha[k] = 0;
-                        st.beta[k] = 0;
-                        st.rho[k] = 0;
-                        if (!(rr == rr) || rr > 1e300)
-                        {
-                            st.flags[2] = 1;
-                        }
-                        int act = rr > st.tol2 * st.ref[k] ? 1 : 0;
-                        if (gate && !gate[k])
-                        {
-                            act = 0;
-                        }
-                        st.active[k] = act;
-                        anyActive |= act;
-                    }
-                }
-                st.flags[0] = anyActive ? 0 : 1;
-                st.flags[1] = 1;
-                *st.counter = 0;
-            }
-        }
+        storePartial<2 * NCH>(part, st.partials, lb);
     }
 
     // Backward colour 0 of symmetric GS fused with rho = r.z and beta.
     template <typename T, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ r,
-                                                           T * __restrict__ z, int color0End, double * totals)
+                                                           T * __restrict__ z, int color0End)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -765,35 +678,13 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
-        {
-            systemTotals<NCH>(bm, st.partials, totals, sm);
-            if (threadIdx.x == 0)
-            {
-                const bool first = st.flags[1] != 0;
-                for (int k = 0; k < bm.numSystems * NCH; k++)
-                {
-                    if (!st.active[k])
-                    {
-                        continue;
-                    }
-                    const double rhoNew = totals[k];
-                    st.beta[k] = (first || st.rho[k] == 0) ? 0.0 : rhoNew / st.rho[k];
-                    st.rho[k] = rhoNew;
-                    if (!(rhoNew == rhoNew))
-                    {
-                        st.flags[2] = 1;
-                    }
-                }
-                *st.counter = 0;
-            }
-        }
+        storePartial<NCH>(part, st.partials, lb);
     }
 
     // p = z + beta p;  q = A z + beta q  (= A p);  alpha = rho / (p.q)
     template <typename T, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ z,
-                                                                 T * __restrict__ p, T * __restrict__ q, double * totals)
+                                                                 T * __restrict__ p, T * __restrict__ q)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -853,27 +744,14 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
-        {
-            systemTotals<NCH>(bm, st.partials, totals, sm);
-            if (threadIdx.x == 0)
-            {
-                for (int k = 0; k < bm.numSystems * NCH; k++)
-                {
-                    const double pq = totals[k];
-                    st.alpha[k] = (st.active[k] && pq != 0) ? st.rho[k] / pq : 0.0;
-                }
-                st.flags[1] = 0;
-                *st.counter = 0;
-            }
-        }
+        storePartial<NCH>(part, st.partials, lb);
     }
 
     // x += alpha p;  r -= alpha q;  z = r / d on colour 0;  convergence test.
     template <typename T, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ p,
                                                               const T * __restrict__ q, T * __restrict__ x, T * __restrict__ r,
-                                                              T * __restrict__ z, int color0End, double * totals)
+                                                              T * __restrict__ z, int color0End)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -921,41 +799,195 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, st.partials, st.counter, lb))
+        storePartial<NCH>(part, st.partials, lb);
+    }
+
+    // Per-system RMS residuals of gsResidualKernel, stop test and flags.
+    template <int NCH>
+    __global__ void __launch_bounds__(kBlock) gsResidualFinalizeKernel(BlockMap bm, SolveState st, int iter, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
         {
-            systemTotals<NCH>(bm, st.partials, totals, sm);
-            if (threadIdx.x == 0)
+            return; // the producer skipped as well
+        }
+        systemTotals<NCH>(bm, st.partials, totals, sm);
+        if (threadIdx.x == 0)
+        {
+            int anyActive = 0;
+            for (int sys = 0; sys < bm.numSystems; sys++)
             {
-                int anyActive = 0;
-                for (int k = 0; k < bm.numSystems * NCH; k++)
+                if (!systemActive<NCH>(st, sys))
                 {
-                    if (!st.active[k])
+                    continue;
+                }
+                bool all = true;
+                for (int c = 0; c < NCH; c++)
+                {
+                    const double m = sqrt(totals[sys * NCH + c] / bm.systemRows[sys]);
+                    st.mse[sys * NCH + c] = m;
+                    if (!(m < st.tol2))
                     {
-                        continue;
+                        all = false;
                     }
-                    const double rr = totals[k];
+                    if (!(m == m) || m > 1e300)
+                    {
+                        st.flags[2] = 1;
+                    }
+                }
+                if (all)
+                {
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        st.active[sys * NCH + c] = 0;
+                        st.iters[sys * NCH + c] = iter;
+                    }
+                }
+                else
+                {
+                    anyActive = 1;
+                }
+            }
+            if (!anyActive)
+            {
+                st.flags[0] = 1;
+            }
+        }
+    }
+
+    // |b|^2, |r|^2 of pcgInitKernel; resets the iteration scalars.  gate
+    // (optional) switches off channels whose outer refinement has converged.
+    template <int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgInitFinalizeKernel(BlockMap bm, SolveState st, const int * __restrict__ gate, double * totals)
+    {
+        __shared__ double sm[2 * NCH * kWaves];
+        systemTotals<2 * NCH>(bm, st.partials, totals, sm);
+        if (threadIdx.x == 0)
+        {
+            int anyActive = 0;
+            for (int s = 0; s < bm.numSystems; s++)
+            {
+                for (int c = 0; c < NCH; c++)
+                {
+                    const int k = s * NCH + c;
+                    const double bb = totals[s * 2 * NCH + c];
+                    const double rr = totals[s * 2 * NCH + NCH + c];
+                    st.bb[k] = bb;
                     st.rr[k] = rr;
-                    st.iters[k] += 1;
+                    st.ref[k] = bb > 0 ? bb : rr;
+                    st.iters[k] = 0;
+                    st.alpha[k] = 0;
+                    st.beta[k] = 0;
+                    st.rho[k] = 0;
                     if (!(rr == rr) || rr > 1e300)
                     {
                         st.flags[2] = 1;
-                        st.active[k] = 0;
-                        continue;
                     }
-                    if (rr <= st.tol2 * st.ref[k])
+                    int act = rr > st.tol2 * st.ref[k] ? 1 : 0;
+                    if (gate && !gate[k])
                     {
-                        st.active[k] = 0;
+                        act = 0;
                     }
-                    else
-                    {
-                        anyActive = 1;
-                    }
+                    st.active[k] = act;
+                    anyActive |= act;
                 }
-                if (!anyActive)
+            }
+            st.flags[0] = anyActive ? 0 : 1;
+            st.flags[1] = 1;
+        }
+    }
+
+    // rho = r.z and beta = rho / rho_old.
+    template <int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgRhoFinalizeKernel(BlockMap bm, SolveState st, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return; // the producer skipped as well
+        }
+        systemTotals<NCH>(bm, st.partials, totals, sm);
+        if (threadIdx.x == 0)
+        {
+            const bool first = st.flags[1] != 0;
+            for (int k = 0; k < bm.numSystems * NCH; k++)
+            {
+                if (!st.active[k])
                 {
-                    st.flags[0] = 1;
+                    continue;
                 }
-                *st.counter = 0;
+                const double rhoNew = totals[k];
+                st.beta[k] = (first || st.rho[k] == 0) ? 0.0 : rhoNew / st.rho[k];
+                st.rho[k] = rhoNew;
+                if (!(rhoNew == rhoNew))
+                {
+                    st.flags[2] = 1;
+                }
+            }
+        }
+    }
+
+    // alpha = rho / (p.q).
+    template <int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgDirectionFinalizeKernel(BlockMap bm, SolveState st, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return; // the producer skipped as well
+        }
+        systemTotals<NCH>(bm, st.partials, totals, sm);
+        if (threadIdx.x == 0)
+        {
+            for (int k = 0; k < bm.numSystems * NCH; k++)
+            {
+                const double pq = totals[k];
+                st.alpha[k] = (st.active[k] && pq != 0) ? st.rho[k] / pq : 0.0;
+            }
+            st.flags[1] = 0;
+        }
+    }
+
+    // |r|^2 after the update, iteration count and convergence test.
+    template <int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgUpdateFinalizeKernel(BlockMap bm, SolveState st, double * totals)
+    {
+        __shared__ double sm[NCH * kWaves];
+        if (st.flags[0])
+        {
+            return; // the producer skipped as well
+        }
+        systemTotals<NCH>(bm, st.partials, totals, sm);
+        if (threadIdx.x == 0)
+        {
+            int anyActive = 0;
+            for (int k = 0; k < bm.numSystems * NCH; k++)
+            {
+                if (!st.active[k])
+                {
+                    continue;
+                }
+                const double rr = totals[k];
+                st.rr[k] = rr;
+                st.iters[k] += 1;
+                if (!(rr == rr) || rr > 1e300)
+                {
+                    st.flags[2] = 1;
+                    st.active[k] = 0;
+                    continue;
+                }
+                if (rr <= st.tol2 * st.ref[k])
+                {
+                    st.active[k] = 0;
+                }
+                else
+                {
+                    anyActive = 1;
+                }
+            }
+            if (!anyActive)
+            {
+                st.flags[0] = 1;
             }
         }
     }
@@ -972,8 +1004,7 @@ namespace nsfem
 
     // Mean of each (system, channel) over its rows: mean[s*NCH + c] (FP64).
     template <typename T, int NCH>
-    __global__ void __launch_bounds__(kBlock) meanKernel(int n, BlockMap bm, const T * __restrict__ v, double * partials,
-                                                         unsigned * counter, double * totals, double * mean)
+    __global__ void __launch_bounds__(kBlock) meanKernel(int n, BlockMap bm, const T * __restrict__ v, double * partials)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -993,19 +1024,23 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        if (arriveLast<NCH>(part, partials, counter, lb))
+        storePartial<NCH>(part, partials, lb);
+    }
+
+    // mean[s*NCH + c] from the partial sums of meanKernel.
+    template <int NCH>
+    __global__ void __launch_bounds__(kBlock) meanFinalizeKernel(BlockMap bm, const double * __restrict__ partials, double * totals, double * mean)
+    {
+        __shared__ double sm[NCH * kWaves];
+        systemTotals<NCH>(bm, partials, totals, sm);
+        if (threadIdx.x == 0)
         {
-            systemTotals<NCH>(bm, partials, totals, sm);
-            if (threadIdx.x == 0)
+            for (int s = 0; s < bm.numSystems; s++)
             {
-                for (int s = 0; s < bm.numSystems; s++)
+                for (int c = 0; c < NCH; c++)
                 {
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        mean[s * NCH + c] = totals[s * NCH + c] / bm.systemRows[s];
-                    }
+                    mean[s * NCH + c] = totals[s * NCH + c] / bm.systemRows[s];
                 }
-                *counter = 0;
             }
         }
     }

@@ -278,7 +278,6 @@ namespace nsfem
         struct DotScratch
         {
             DeviceArray<double> partials;
-            DeviceArray<unsigned> counter;
             DeviceArray<double> out;
             std::mutex mutex;
         };
@@ -288,10 +287,7 @@ namespace nsfem
             {
                 auto * p = new DotScratch;
                 p->partials.resize(2048 * 2);
-                p->counter.resize(1);
                 p->out.resize(2);
-                NSFEM_HIP(hipMemsetAsync(p->counter.get(), 0, sizeof(unsigned), libraryStream()));
-                syncLibraryStream();
                 return p;
             }();
             return *d;
@@ -321,12 +317,12 @@ namespace nsfem
             if (K == 1)
             {
                 dotKernel<double, 1><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
-                                                         d.partials.get(), d.counter.get(), d.out.get());
+                                                         d.partials.get());
             }
             else
             {
                 dotKernel<double, 2><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
-                                                         d.partials.get(), d.counter.get(), d.out.get());
+                                                         d.partials.get());
             }
         }
         else
@@ -334,13 +330,21 @@ namespace nsfem
             if (K == 1)
             {
                 dotKernel<float, 1><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
-                                                        d.partials.get(), d.counter.get(), d.out.get());
+                                                        d.partials.get());
             }
             else
             {
                 dotKernel<float, 2><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
-                                                        d.partials.get(), d.counter.get(), d.out.get());
+                                                        d.partials.get());
             }
+        }
+        if (K == 1)
+        {
+            dotFinalizeKernel<1><<<1, kBlock, 0, s>>>(g, d.partials.get(), d.out.get());
+        }
+        else
+        {
+            dotFinalizeKernel<2><<<1, kBlock, 0, s>>>(g, d.partials.get(), d.out.get());
         }
         NSFEM_HIP(hipGetLastError());
         NSFEM_HIP(hipMemcpyAsync(out, d.out.get(), K * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -102,12 +102,10 @@ namespace nsfem
         const int K = numSystems * numCh;
         scal.resize(static_cast<size_t>(7) * K);
         ints.resize(static_cast<size_t>(2) * K + 4);
-        counter.resize(1);
         partials.resize(static_cast<size_t>(std::max(1, numBlocks)) * 2 * numCh);
         totals.resize(static_cast<size_t>(numSystems) * 2 * numCh);
         scal.zero();
         ints.zero();
-        counter.zero();
     }
 
     SolveState DeviceState::view(int K, double tol2)
@@ -124,7 +122,6 @@ namespace nsfem
         s.active = ints.get();
         s.iters = ints.get() + K;
         s.flags = ints.get() + 2 * K;
-        s.counter = counter.get();
         s.partials = partials.get();
         s.tol2 = tol2;
         return s;
@@ -453,7 +450,10 @@ namespace nsfem
             if (iter % opt_.mseCheckInterval == 0)
             {
                 dispatchLC(lanes_, nch, [&]<int L, int C>()
-                           { gsResidualKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x, iter, inner_.totals.get()); });
+                           {
+                               gsResidualKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x);
+                               gsResidualFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, iter, inner_.totals.get());
+                           });
             }
             NSFEM_HIP(hipGetLastError());
             if ((iter + 1) % opt_.checkChunk == 0 && pollDone(s, inner_))
@@ -497,24 +497,38 @@ namespace nsfem
     }
 
     template <typename T>
+    void McgsSolver::launchRho(hipStream_t s, const McgsMatrix<T> & Av, SolveState st, const T * r, T * z, double * totals)
+    {
+        dispatchLC(lanes_, opt_.numCh, [&]<int L, int C>()
+                   {
+                       pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_);
+                       pcgRhoFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                   });
+    }
+
+    template <typename T>
     void McgsSolver::pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q)
     {
         const McgsMatrix<T> Av = A.view(n_);
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
         dispatchLC(lanes_, nch, [&]<int L, int C>()
-                   { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q, totals); });
+                   {
+                       pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q);
+                       pcgDirectionFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                   });
         if (nch == 1)
         {
-            pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+            pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
+            pcgUpdateFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, st, totals);
         }
         else
         {
-            pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+            pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
+            pcgUpdateFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, st, totals);
         }
         sgsSweeps<T>(s, A, st, r, z);
-        dispatchLC(lanes_, nch, [&]<int L, int C>()
-                   { pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_, totals); });
+        launchRho<T>(s, Av, st, r, z, totals);
     }
 
     template <typename T>
@@ -527,10 +541,12 @@ namespace nsfem
         double * totals = ds.totals.get();
         fillInt(4, 0, st.flags, s);
         dispatchLC(lanes_, nch, [&]<int L, int C>()
-                   { pcgInitKernel<T, T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x, r, z, color0End_, zeroGuess ? 1 : 0, gate, totals); });
+                   {
+                       pcgInitKernel<T, T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x, r, z, color0End_, zeroGuess ? 1 : 0);
+                       pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, gate, totals);
+                   });
         sgsSweeps<T>(s, A, st, r, z);
-        dispatchLC(lanes_, nch, [&]<int L, int C>()
-                   { pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_, totals); });
+        launchRho<T>(s, Av, st, r, z, totals);
         NSFEM_HIP(hipGetLastError());
 
         // Iterations run in chunks of checkChunk; a full chunk is one launch
@@ -605,8 +621,11 @@ namespace nsfem
         {
             fillInt(4, 0, outer.flags, s);
             dispatchLC(lanes_, nch, [&]<int L, int C>()
-                       { pcgInitKernel<double, double, L, C><<<numBlocks_, kBlock, 0, s>>>(A64, bm_, outer, b64_.get(), x64_.get(), r64_.get(),
-                                                                                               z64_.get(), color0End_, 0, nullptr, outer_.totals.get()); });
+                       {
+                           pcgInitKernel<double, double, L, C><<<numBlocks_, kBlock, 0, s>>>(A64, bm_, outer, b64_.get(), x64_.get(), r64_.get(),
+                                                                                             z64_.get(), color0End_, 0);
+                           pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, outer, nullptr, outer_.totals.get());
+                       });
             NSFEM_HIP(hipGetLastError());
             if (pollDone(s, outer_) || outerIt >= opt_.maxOuter)
             {
@@ -709,14 +728,14 @@ namespace nsfem
         // reuse the inner state's reduction scratch (no solve is in flight)
         if (opt_.numCh == 1)
         {
-            meanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get(), inner_.counter.get(), inner_.totals.get(),
-                                                           means_.get());
+            meanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
+            meanFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
             subtractMeanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
         }
         else
         {
-            meanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get(), inner_.counter.get(), inner_.totals.get(),
-                                                           means_.get());
+            meanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
+            meanFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
             subtractMeanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
         }
         NSFEM_HIP(hipGetLastError());
@@ -772,7 +791,7 @@ namespace nsfem
                 launch = [=, this]()
                 {
                     dispatchLC(lanes_, nch, [&]<int L, int C>()
-                               { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q, totals); });
+                               { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q); });
                 };
             }
             else if (which == "sweep")
@@ -796,9 +815,9 @@ namespace nsfem
                 launch = [=, this]()
                 {
                     if (nch == 1)
-                        pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+                        pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
                     else
-                        pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_, totals);
+                        pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
                 };
             }
             else

@@ -76,7 +76,6 @@ namespace nsfem
     {
         DeviceArray<double> scal;    // 8 arrays of S*NCH doubles
         DeviceArray<int> ints;       // active, iters (S*NCH each), flags[4]
-        DeviceArray<unsigned> counter;
         DeviceArray<double> partials;
         DeviceArray<double> totals;
         SolveState view(int k, double tol2);
@@ -141,6 +140,8 @@ namespace nsfem
         void runMixed(hipStream_t s);
         template <typename T>
         void sgsSweeps(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * r, T * z);
+        template <typename T>
+        void launchRho(hipStream_t s, const McgsMatrix<T> & Av, SolveState st, const T * r, T * z, double * totals);
         template <typename T>
         void pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
