@@ -58,11 +58,14 @@ def initial_velocity(ctx, cond, seed):
     return u
 
 
-def run_path(name, ctx, cfg, tau, u0, warmup, steps):
+def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune):
     import numpy as np
     from nsfem_amd import _lib, fem
     ch = fem.Chorin(ctx, cfg)
     ch.set_time_step(tau)
+    t0 = time.perf_counter()
+    shape = ch.autotune(10) if autotune else ch.shape()
+    tune_s = time.perf_counter() - t0 if autotune else 0.0
     ch.upload_velocity(u0)
     first = ch.step()
     u1, p1 = ch.download()
@@ -84,24 +87,30 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps):
            "outer": {"tentative": agg["outerTent"], "pressure": agg["outerPressure"], "final": agg["outerFinal"]},
            "max_relres": max(max(s["relTent"], s["relPressure"], s["relFinal"]) for s in stats),
            "ms": {"convection": agg["msConvection"], "tentative": agg["msTentative"], "pressure": agg["msPressure"],
-                  "final": agg["msFinal"]}}
+                  "final": agg["msFinal"]},
+           "shape": {"velocity_lanes": shape[0], "velocity_rows_per_block": shape[1], "pressure_lanes": shape[2],
+                     "pressure_rows_per_block": shape[3], "autotuned": bool(autotune), "autotune_s": tune_s}}
     log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
         f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
     return ch, res, (u1, p1), (u, p)
 
 
 def kernel_table(ch, res):
-    """Per-kernel average launch time (HIP events), algorithmic bytes, GB/s,
-    and an estimate of each kernel's share of one step."""
+    """Per-kernel average launch time (HIP events on the library stream,
+    replaying the launches one solver iteration makes), algorithmic bytes per
+    launch, GB/s, and each kernel's estimated share of one step: launches per
+    iteration times the measured iterations per step."""
     it = res["iters"]
     vel_iters = it["tentative"] + it["final"]
-    launches = {"velocity_direction": vel_iters, "velocity_sweep": 1.5 * vel_iters,
-                "pressure_direction": it["pressure"], "pressure_sweep": 1.5 * it["pressure"],
-                "velocity_update": vel_iters, "pressure_update": it["pressure"], "convection": 1, "spmm": 1}
     out = {}
-    for k, n in launches.items():
-        ms, b = ch.time_kernel(k, reps=30)
-        out[k] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "per_step": n, "ms_per_step": ms * n}
+    for solver, iters in (("velocity", vel_iters), ("pressure", it["pressure"])):
+        for k in ("direction", "sweep", "rho", "update", "finalize"):
+            ms, b, per_it = ch.time_kernel(f"{solver}_{k}", reps=30)
+            out[f"{solver}_{k}"] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9,
+                                    "per_step": per_it * iters, "ms_per_step": ms * per_it * iters}
+    for k in ("convection", "spmm"):
+        ms, b, _ = ch.time_kernel(k, reps=30)
+        out[k] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "per_step": 1, "ms_per_step": ms}
     return out
 
 
@@ -137,10 +146,13 @@ def main():
     ap.add_argument("--tau", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=20241103)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--shape", default="0,0,0,0",
+                    help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default)")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="also run 3 device copies of this size (scale for HBM byte counters)")
+                    help="also run 3 in-place scalings of an array this size (known traffic, to scale HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -148,16 +160,19 @@ def main():
         return 2
 
     import numpy as np
-    from nsfem_amd import F64, cu, fem
+    from nsfem_amd import F64, _lib, cu, fem
 
     if args.copy_calibration_mib > 0:
+        # a plain streaming kernel of known traffic: each launch reads and
+        # writes the whole array once (scaleKernel<double>)
         n = args.copy_calibration_mib * (1 << 20) // 8
-        a, b = cu.vec(n, np.float64), cu.vec(n, np.float64)
+        a = cu.vec(n, np.float64)
         a.memsetZero()
         for _ in range(3):
-            a.copyTo(b)
-        log(f"calibration: 3 device copies of {args.copy_calibration_mib} MiB")
-        del a, b
+            cu.scale(n, a, 1.0)
+        _lib.call("nsfem_synchronize")
+        log(f"calibration: 3 in-place scalings of {args.copy_calibration_mib} MiB (read + write)")
+        del a
 
     t_setup = time.perf_counter()
     m, cond, ctx = build_workload(args.nodes, args.seed)
@@ -170,15 +185,19 @@ def main():
     setup_s = time.perf_counter() - t_setup
 
     target = 1e-10
-    cfg64 = fem.ChorinConfig(fem.SolverConfig("pcg", 5000, target, 1), fem.SolverConfig("pcg", 50000, target, 1), dtype=F64)
-    cfgmx = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 5000, target, 1), fem.SolverConfig("pcg_mixed", 50000, target, 1),
-                             dtype=F64, innerTol=args.inner_tol)
+    vl, vr, pl, pr = (int(v) for v in args.shape.split(","))
+
+    def solvers(method):
+        return (fem.SolverConfig(method, 5000, target, 1, vl, vr), fem.SolverConfig(method, 50000, target, 1, pl, pr))
+
+    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64)
+    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64)
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     results, kernels, firsts, lasts = {}, {}, {}, {}
     for name in paths:
         cfg = {"fp64": cfg64, "mixed": cfgmx}[name]
-        ch, r, first, last = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps)
-        kernels[name] = kernel_table(ch, r)
+        ch, r, first, last = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps, not args.no_autotune)
+        kernels[name] = {} if args.no_kernels else kernel_table(ch, r)
         results[name], firsts[name], lasts[name] = r, first, last
         del ch
 
@@ -203,12 +222,14 @@ def main():
     r64 = results.get("fp64")
 
     # Dominant kernel of the headline path by estimated time per step.
-    dom_name = max(kernmx, key=lambda k: kernmx[k]["ms_per_step"])
-    dom = kernmx[dom_name]
-    peak = 8000.0
-    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": peak, "unit": "GB/s",
-                "frac": dom["GBps"] / peak, "traffic": None, "bytes_per_launch": dom["bytes"],
-                "avg_launch_ms": dom["avg_ms"]}
+    roofline = None
+    if kernmx:
+        dom_name = max(kernmx, key=lambda k: kernmx[k]["ms_per_step"])
+        dom = kernmx[dom_name]
+        peak = 8000.0
+        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": peak, "unit": "GB/s",
+                    "frac": dom["GBps"] / peak, "traffic": None, "bytes_per_launch": dom["bytes"],
+                    "avg_launch_ms": dom["avg_ms"]}
     out = {
         "metric": "Chorin steps/s @500k p-DOFs, mixed vs FP64 GPU, same residual; SpMV % HBM peak",
         "value": rmx["steps_per_s"],
@@ -226,7 +247,8 @@ def main():
                    "velocity_dofs_per_component": ctx.numVelocityNodes, "elements": ctx.numElements,
                    "reynolds": 100, "peak_inflow": cond.peakVelocity, "viscosity": cond.viscosity,
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
-                   "inner_tol": args.inner_tol, "seed": args.seed},
+                   "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 symmetric GS preconditioner",
+                   "seed": args.seed},
         "path": head,
         "fp64": r64,
         "mixed": results.get("mixed"),

@@ -123,19 +123,44 @@ int nsfem_solver_last_mse(const nsfem_solver * s, int ch, double * out);
 int nsfem_solver_last_relres(const nsfem_solver * s, int system, int ch, double * out);
 int nsfem_solver_last_iterations(const nsfem_solver * s, int * out);
 int nsfem_solver_last_iterations_system(const nsfem_solver * s, int system, int * out);
+/* pcg_mixed with mixedRestart: refinement steps of the last solve; 0
+ * otherwise.  No reference counterpart. */
+int nsfem_solver_last_outer_iterations(const nsfem_solver * s, int * out);
 int nsfem_solver_set_max_iters(nsfem_solver * s, int n);
 int nsfem_solver_set_target_mse(nsfem_solver * s, double target);
 int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
-/* Options: "innerTol" (pcg_mixed inner relative tolerance), "projectMean" (1:
- * pure-Neumann system with a zero-mean right-hand side; the solution is
- * returned with zero mean and FP32 inner right-hand sides are projected). */
+/* Options (no reference counterpart; the reference solvers have none):
+ *   "projectMean"  1: pure-Neumann system with a zero-mean right-hand side;
+ *                  the solution is returned with zero mean (and the FP32
+ *                  inner right-hand sides of mixedRestart are projected).
+ *   "mixedRestart" 1: pcg_mixed as iterative refinement (FP32 PCG inner
+ *                  solves from zero on the FP64 residual) instead of the
+ *                  default FP64 PCG with an FP32 symmetric GS preconditioner.
+ *   "innerTol"     pcg_mixed with mixedRestart: inner relative tolerance. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
-/* Kernel timing (HIP events on the library stream): average ms per launch of
- * "direction" (A z fused with the PCG direction update), "sweep" (one
- * full-row pass over all colours), "update"; bytes = algorithmic bytes per
- * launch.  pcg solvers only; overwrites the solver's internal vectors. */
-int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes);
+/* Kernel timing (HIP events on the library stream), replaying the launches
+ * of one PCG iteration: "direction" (A z fused with the direction update and
+ * p.q), "sweep" (the colour sweeps of one preconditioner application:
+ * forward colours 1..C-1 on the lower part, backward C-2..1), "rho"
+ * (backward colour 0 fused with r.z), "update" (x, r, colour-0 z, r.r),
+ * "finalize" (a one-block reduction finalize).  avgMs and bytes
+ * (algorithmic, x read once) are per kernel launch, averaged over the
+ * launches; launchesPerIteration (optional) is how many launches of that
+ * kind one iteration makes.  pcg solvers only; overwrites the solver's
+ * internal vectors.  No reference counterpart (measurement hook). */
+int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes,
+                             double * launchesPerIteration);
 int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
+/* Launch shape of the solver kernels: lanes per matrix row (4, 8, 16; 0
+ * picks from the mean row length) and rows per workgroup (0: default).
+ * reshape rebuilds the workgroup map only (colouring and matrices kept).
+ * autotune times one PCG iteration's launches (pcg solvers, reps graph
+ * replays per kernel) for lanes {4, 8, 16} x {1, 2, 4, 8} row slabs and keeps
+ * the fastest; it overwrites the internal work vectors, not rhs/sol.  No
+ * reference counterpart (the reference's launch shapes are fixed). */
+int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock);
+int nsfem_solver_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock);
+int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock);
 int nsfem_solver_coloring(const nsfem_solver * s, int system, int * coloring, int * partitionStart);
 
 /* Smallest-last greedy colouring of a square CSR pattern, host only
@@ -220,6 +245,8 @@ typedef struct nsfem_solver_config
     int maxIterations;
     double targetMse;
     int mseCheckInterval;
+    int lanes;        /* lanes per matrix row: 4, 8, 16 (0: from the mean row length) */
+    int rowsPerBlock; /* rows per workgroup of the solver kernels (0: default) */
 } nsfem_solver_config;
 
 typedef struct nsfem_chorin_config
@@ -228,8 +255,6 @@ typedef struct nsfem_chorin_config
     nsfem_solver_config pressureSolver;
     int dtype;       /* NSFEM_F64 (plain / mixed) or NSFEM_F32 (reference precision) */
     double innerTol; /* pcg_mixed inner relative tolerance */
-    int lanesVelocity;
-    int lanesPressure;
 } nsfem_chorin_config;
 
 typedef struct nsfem_step_stats
@@ -254,9 +279,15 @@ int nsfem_chorin_download(nsfem_chorin * ch, void * velocity, void * pressure);
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);
+/* Launch shapes of the step's solvers, shape[4] = {velocity lanes, velocity
+ * rows per workgroup, pressure lanes, pressure rows per workgroup}; autotune
+ * as nsfem_solver_autotune for both solvers (the state u, p is kept). */
+int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
+int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
 /* Kernel timing inside a Chorin context: "convection", "spmm",
  * "velocity_<k>", "pressure_<k>" (k as in nsfem_solver_time_kernel). */
-int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes);
+int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes,
+                             double * launchesPerIteration);
 /* A = nu K + C(u) in the velocity pattern (C optional): device arrays. */
 int nsfem_chorin_assemble_convection(nsfem_chorin * ch, const void * velocity, void * aValues, void * cValues);
 

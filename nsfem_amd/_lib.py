@@ -57,12 +57,13 @@ class DfgConditions(ctypes.Structure):
 
 class SolverConfig(ctypes.Structure):
     _fields_ = [("method", ctypes.c_char_p), ("maxIterations", ctypes.c_int),
-                ("targetMse", ctypes.c_double), ("mseCheckInterval", ctypes.c_int)]
+                ("targetMse", ctypes.c_double), ("mseCheckInterval", ctypes.c_int),
+                ("lanes", ctypes.c_int), ("rowsPerBlock", ctypes.c_int)]
 
 
 class ChorinConfigC(ctypes.Structure):
     _fields_ = [("velocitySolver", SolverConfig), ("pressureSolver", SolverConfig), ("dtype", ctypes.c_int),
-                ("innerTol", ctypes.c_double), ("lanesVelocity", ctypes.c_int), ("lanesPressure", ctypes.c_int)]
+                ("innerTol", ctypes.c_double)]
 
 
 class StepStats(ctypes.Structure):
@@ -116,12 +117,16 @@ _SIGNATURES = {
     "nsfem_solver_last_relres": [_vp, ctypes.c_int, ctypes.c_int, _c_double_p],
     "nsfem_solver_last_iterations": [_vp, _c_int_p],
     "nsfem_solver_last_iterations_system": [_vp, ctypes.c_int, _c_int_p],
+    "nsfem_solver_last_outer_iterations": [_vp, _c_int_p],
     "nsfem_solver_set_max_iters": [_vp, ctypes.c_int],
     "nsfem_solver_set_target_mse": [_vp, ctypes.c_double],
     "nsfem_solver_set_mse_check_interval": [_vp, ctypes.c_int],
     "nsfem_solver_set_option": [_vp, ctypes.c_char_p, ctypes.c_double],
-    "nsfem_solver_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p],
+    "nsfem_solver_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p],
     "nsfem_solver_num_colors": [_vp, ctypes.c_int, _c_int_p],
+    "nsfem_solver_reshape": [_vp, ctypes.c_int, ctypes.c_int],
+    "nsfem_solver_shape": [_vp, _c_int_p, _c_int_p],
+    "nsfem_solver_autotune": [_vp, ctypes.c_int, _c_int_p, _c_int_p],
     "nsfem_solver_coloring": [_vp, ctypes.c_int, _vp, _vp],
     "nsfem_mcgs_coloring": [ctypes.c_int, _vp, _vp, _vp, _vp, _c_int_p],
     "nsfem_mesh_load_gmsh": [ctypes.c_char_p, ctypes.POINTER(_vp)],
@@ -154,7 +159,9 @@ _SIGNATURES = {
     "nsfem_chorin_upload_velocity": [_vp, _vp],
     "nsfem_chorin_reset": [_vp],
     "nsfem_chorin_assemble_convection": [_vp, _vp, _vp, _vp],
-    "nsfem_chorin_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p],
+    "nsfem_chorin_shape": [_vp, _vp],
+    "nsfem_chorin_autotune": [_vp, ctypes.c_int, _vp],
+    "nsfem_chorin_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p],
 }
 _RESTYPES = {"nsfem_last_error": ctypes.c_char_p, "nsfem_host_free": None}
 

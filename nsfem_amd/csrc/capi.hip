@@ -570,6 +570,15 @@ extern "C"
             *out = s->core->lastIterations(system); });
     }
 
+    int nsfem_solver_last_outer_iterations(const nsfem_solver * s, int * out)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            need(out, "out");
+            *out = s->core->lastOuterIterations(); });
+    }
+
     int nsfem_solver_set_max_iters(nsfem_solver * s, int n)
     {
         return guarded([&]
@@ -611,20 +620,53 @@ extern "C"
             {
                 s->core->setProjectMean(value != 0);
             }
+            else if (k == "mixedRestart")
+            {
+                s->core->setMixedRestart(value != 0);
+            }
             else
             {
                 throw std::invalid_argument("Unknown solver option: " + k);
             } });
     }
 
-    int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes)
+    int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes,
+                                 double * launchesPerIteration)
     {
         return guarded([&]
                        {
             need(s, "solver");
             need(which, "which");
             need(avgMs, "avgMs");
-            *avgMs = s->core->timeKernel(which, reps, bytes); });
+            *avgMs = s->core->timeKernel(which, reps, bytes, launchesPerIteration); });
+    }
+
+    int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            s->core->reshape(lanes, rowsPerBlock); });
+    }
+
+    int nsfem_solver_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            if (lanes) *lanes = s->core->lanes();
+            if (rowsPerBlock) *rowsPerBlock = s->core->rowsPerBlock(); });
+    }
+
+    int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            if (reps < 1) throw std::invalid_argument("reps must be positive");
+            const auto shape = s->core->autotune(reps);
+            if (lanes) *lanes = shape.first;
+            if (rowsPerBlock) *rowsPerBlock = shape.second; });
     }
 
     int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out)
@@ -976,14 +1018,15 @@ extern "C"
                 d.maxIterations = s.maxIterations;
                 d.target = s.targetMse;
                 d.checkInterval = s.mseCheckInterval;
+                d.lanes = s.lanes;
+                d.rowsPerBlock = s.rowsPerBlock;
+                if (d.rowsPerBlock < 0) throw std::invalid_argument("rowsPerBlock must be >= 0");
                 if (d.checkInterval < 1) throw std::invalid_argument("Invalid MSE check interval for solver: " + std::to_string(d.checkInterval));
             };
             conv(cfg->velocitySolver, cc.velocity);
             conv(cfg->pressureSolver, cc.pressure);
             cc.dtype = cfg->dtype;
             cc.innerTol = cfg->innerTol > 0 ? cfg->innerTol : 1e-4;
-            cc.lanesVelocity = cfg->lanesVelocity;
-            cc.lanesPressure = cfg->lanesPressure;
             auto h = std::make_unique<nsfem_chorin>();
             h->core = std::make_unique<ChorinGpu>(c->ctx, cc);
             *out = h.release(); });
@@ -1080,14 +1123,48 @@ extern "C"
             ch->core->resetState(); });
     }
 
-    int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes)
+    int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            need(shape, "shape");
+            McgsSolver & v = ch->core->velocitySolver();
+            McgsSolver & p = ch->core->pressureSolver();
+            shape[0] = v.lanes();
+            shape[1] = v.rowsPerBlock();
+            shape[2] = p.lanes();
+            shape[3] = p.rowsPerBlock(); });
+    }
+
+    int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            if (reps < 1) throw std::invalid_argument("reps must be positive");
+            ch->core->velocitySolver().autotune(reps);
+            ch->core->pressureSolver().autotune(reps);
+            if (shape)
+            {
+                McgsSolver & v = ch->core->velocitySolver();
+                McgsSolver & p = ch->core->pressureSolver();
+                shape[0] = v.lanes();
+                shape[1] = v.rowsPerBlock();
+                shape[2] = p.lanes();
+                shape[3] = p.rowsPerBlock();
+            } });
+    }
+
+    int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes,
+                                 double * launchesPerIteration)
     {
         return guarded([&]
                        {
             need(ch, "chorin");
             need(which, "which");
             need(avgMs, "avgMs");
-            *avgMs = ch->core->timeKernel(which, reps, bytes); });
+            *avgMs = ch->core->timeKernel(which, reps, bytes, launchesPerIteration); });
     }
 
     int nsfem_chorin_assemble_convection(nsfem_chorin * ch, const void * velocity, void * aValues, void * cValues)

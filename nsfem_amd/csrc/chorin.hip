@@ -162,7 +162,7 @@ namespace nsfem
         u_.resize(static_cast<size_t>(2) * N_ * tsize_);
         p_.resize(static_cast<size_t>(V_) * tsize_);
 
-        auto solverOpts = [&](const ChorinSolverConfig & sc, int nch, int lanes)
+        auto solverOpts = [&](const ChorinSolverConfig & sc, int nch)
         {
             SolverOptions o = parseSolverName(sc.method);
             if (o.precision == SolverPrecision::Mixed)
@@ -181,7 +181,8 @@ namespace nsfem
             o.target = sc.target;
             o.mseCheckInterval = sc.checkInterval;
             o.innerTol = cfg_.innerTol;
-            o.lanes = lanes;
+            o.lanes = sc.lanes;
+            o.rowsPerBlock = sc.rowsPerBlock;
             return o;
         };
         // One solver system per instance (per-instance colouring and
@@ -197,8 +198,8 @@ namespace nsfem
             for (const auto & m : ctx.massBlocks) vsys.push_back(&m);
             for (const auto & m : ctx.pressureBlocks) psys.push_back(&m);
         }
-        vSolver_ = std::make_unique<McgsSolver>(vsys, solverOpts(cfg_.velocity, 2, cfg_.lanesVelocity));
-        pSolver_ = std::make_unique<McgsSolver>(psys, solverOpts(cfg_.pressure, 1, cfg_.lanesPressure));
+        vSolver_ = std::make_unique<McgsSolver>(vsys, solverOpts(cfg_.velocity, 2));
+        pSolver_ = std::make_unique<McgsSolver>(psys, solverOpts(cfg_.pressure, 1));
         for (auto & e : ev_)
         {
             NSFEM_HIP(hipEventCreate(&e));
@@ -252,15 +253,15 @@ namespace nsfem
         NSFEM_HIP(hipGetLastError());
     }
 
-    double ChorinGpu::timeKernel(const std::string & which, int reps, double * bytesPerLaunch)
+    double ChorinGpu::timeKernel(const std::string & which, int reps, double * bytesPerLaunch, double * launchesPerIteration)
     {
         if (which.rfind("velocity_", 0) == 0)
         {
-            return vSolver_->timeKernel(which.substr(9), reps, bytesPerLaunch);
+            return vSolver_->timeKernel(which.substr(9), reps, bytesPerLaunch, launchesPerIteration);
         }
         if (which.rfind("pressure_", 0) == 0)
         {
-            return pSolver_->timeKernel(which.substr(9), reps, bytesPerLaunch);
+            return pSolver_->timeKernel(which.substr(9), reps, bytesPerLaunch, launchesPerIteration);
         }
         if (reps < 1)
         {
@@ -280,7 +281,7 @@ namespace nsfem
         {
             bytes = nnz * (ts + 4) + (N + 1) * 4 + 4 * N * ts;
             launch = [&]()
-            { csrMul(A_.view(cfg_.dtype), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.lanesVelocity, s); };
+            { csrMul(A_.view(cfg_.dtype), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.velocity.lanes, s); };
         }
         else
         {
@@ -304,6 +305,10 @@ namespace nsfem
         if (bytesPerLaunch)
         {
             *bytesPerLaunch = bytes;
+        }
+        if (launchesPerIteration)
+        {
+            *launchesPerIteration = 1; // once per step
         }
         return static_cast<double>(ms) / reps;
     }
@@ -357,7 +362,7 @@ namespace nsfem
         NSFEM_HIP(hipEventRecord(ev_[1], s));
 
         // Tentative velocity: M a = A u;  u -= tau a
-        csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.lanesVelocity, s);
+        csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.velocity.lanes, s);
         fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
         vSolver_->solve();
         st.mseTentX = maxMse(*vSolver_, 0);

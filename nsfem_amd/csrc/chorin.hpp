@@ -31,6 +31,8 @@ namespace nsfem
         int maxIterations = 200;
         double target = 1e-6;
         int checkInterval = 1;
+        int lanes = 0;        // 0: from the mean row length
+        int rowsPerBlock = 0; // 0: solver default
     };
 
     struct ChorinConfig
@@ -39,8 +41,6 @@ namespace nsfem
         ChorinSolverConfig pressure;
         int dtype = F64;        // state and matrix precision (F32 = the reference's precision)
         double innerTol = 1e-4; // pcg_mixed inner tolerance
-        int lanesVelocity = 0;
-        int lanesPressure = 0;
     };
 
     struct ChorinStepStats
@@ -76,7 +76,7 @@ namespace nsfem
         // Average launch duration (ms) and algorithmic bytes per launch of a
         // step kernel: "convection" (element + gather kernels), "spmm" (A u),
         // "velocity_<k>" / "pressure_<k>" with k a solver kernel name.
-        double timeKernel(const std::string & which, int reps, double * bytesPerLaunch);
+        double timeKernel(const std::string & which, int reps, double * bytesPerLaunch, double * launchesPerIteration);
 
         McgsSolver & velocitySolver() { return *vSolver_; }
         McgsSolver & pressureSolver() { return *pSolver_; }

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace nsfem
 {
@@ -444,9 +445,11 @@ namespace nsfem
         return a != 0;
     }
 
-    // Lane-group sum over j in [j0, j1) of val[j] * v[col[j] + c*ld].
-    template <typename T, int LANES, int NCH>
-    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const T * __restrict__ v, long ld,
+    // Lane-group sum over j in [j0, j1) of val[j] * v[col[j] + c*ld], in the
+    // matrix type T (v may be stored narrower, e.g. an FP32 preconditioned
+    // vector under an FP64 matrix).
+    template <typename T, int LANES, int NCH, typename V>
+    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const V * __restrict__ v, long ld,
                                            T (&acc)[NCH])
     {
         const int lane = threadIdx.x & (LANES - 1);
@@ -462,7 +465,7 @@ namespace nsfem
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                acc[c] = fmaT(a, v[col + c * ld], acc[c]);
+                acc[c] = fmaT(a, static_cast<T>(v[col + c * ld]), acc[c]);
             }
         }
 #pragma unroll
@@ -564,13 +567,14 @@ namespace nsfem
     }
 
     // PCG set-up: r = b - A x (or r = b with x = 0), z = r / d on colour 0,
-    // per-channel |b|^2 and |r|^2, convergence test.  gate (optional) switches
-    // channels off whose outer refinement loop has already converged.
-    // r64 (optional): the residual is also stored in double (outer refinement).
-    template <typename T, typename TR, int LANES, int NCH>
+    // per-channel |b|^2 and |r|^2 (then pcgInitFinalizeKernel).
+    // Two precisions: T for the Krylov vectors and the matrix products, P for
+    // the preconditioner (the symmetric GS sweeps).  P narrower than T: the
+    // residual is also stored as rP = (P) r for the sweeps; P == T: rP == r.
+    template <typename T, typename P, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgInitKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
-                                                            T * __restrict__ x, TR * __restrict__ r, T * __restrict__ z,
-                                                            int color0End, int zeroGuess)
+                                                            T * __restrict__ x, T * r, P * rP, P * __restrict__ z,
+                                                            const P * __restrict__ invDiagP, int color0End, int zeroGuess)
     {
         __shared__ double sm[2 * NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -609,10 +613,15 @@ namespace nsfem
                     {
                         ri = bi - fmaT(A.diag[row], x[i], acc[c]);
                     }
-                    r[i] = static_cast<TR>(ri);
+                    r[i] = ri;
+                    const P rp = static_cast<P>(ri);
+                    if constexpr (!std::is_same_v<T, P>)
+                    {
+                        rP[i] = rp;
+                    }
                     if (row < color0End)
                     {
-                        z[i] = ri * A.invDiag[row];
+                        z[i] = rp * invDiagP[row];
                     }
                     part[c] = fma(static_cast<double>(bi), static_cast<double>(bi), part[c]);
                     part[NCH + c] = fma(static_cast<double>(ri), static_cast<double>(ri), part[NCH + c]);
@@ -623,10 +632,11 @@ namespace nsfem
         storePartial<2 * NCH>(part, st.partials, lb);
     }
 
-    // Backward colour 0 of symmetric GS fused with rho = r.z and beta.
-    template <typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ r,
-                                                           T * __restrict__ z, int color0End)
+    // Backward colour 0 of symmetric GS (on the preconditioner matrix M, type
+    // P) fused with the partial sums of rho = r.z (r of the Krylov type T).
+    template <typename P, typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP,
+                                                           const T * r, P * __restrict__ z, int color0End)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -648,31 +658,30 @@ namespace nsfem
             {
                 const int row = base + static_cast<int>(threadIdx.x) / LANES;
                 const bool valid = row < range.y;
-                T acc[NCH];
+                P acc[NCH];
                 if (c0)
                 {
-                    const int j0 = valid ? A.rowStart[row] : 0;
-                    const int j1 = valid ? A.rowStart[row + 1] : 0;
-                    rowDot<T, LANES, NCH>(A, j0, j1, z, A.n, acc);
+                    const int j0 = valid ? M.rowStart[row] : 0;
+                    const int j1 = valid ? M.rowStart[row + 1] : 0;
+                    rowDot<P, LANES, NCH>(M, j0, j1, z, M.n, acc);
                 }
                 if (valid && groupLeader<LANES>())
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
-                        const long i = row + static_cast<long>(c) * A.n;
-                        const T ri = r[i];
-                        T zi;
+                        const long i = row + static_cast<long>(c) * M.n;
+                        P zi;
                         if (c0)
                         {
-                            zi = (ri - acc[c]) * A.invDiag[row];
+                            zi = (rP[i] - acc[c]) * M.invDiag[row];
                             z[i] = zi;
                         }
                         else
                         {
                             zi = z[i];
                         }
-                        part[c] = fma(static_cast<double>(ri), static_cast<double>(zi), part[c]);
+                        part[c] = fma(static_cast<double>(r[i]), static_cast<double>(zi), part[c]);
                     }
                 }
             }
@@ -681,9 +690,10 @@ namespace nsfem
         storePartial<NCH>(part, st.partials, lb);
     }
 
-    // p = z + beta p;  q = A z + beta q  (= A p);  alpha = rho / (p.q)
-    template <typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ z,
+    // p = z + beta p;  q = A z + beta q  (= A p);  partial sums of p.q.
+    // z is the preconditioned vector (type P), p and q are of the Krylov type T.
+    template <typename T, typename P, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
                                                                  T * __restrict__ p, T * __restrict__ q)
     {
         __shared__ double sm[NCH * kWaves];
@@ -723,7 +733,7 @@ namespace nsfem
                     for (int c = 0; c < NCH; c++)
                     {
                         const long i = row + static_cast<long>(c) * A.n;
-                        const T zi = z[i];
+                        const T zi = static_cast<T>(z[i]);
                         const T az = fmaT(A.diag[row], zi, acc[c]);
                         T pi, qi;
                         if (first)
@@ -747,11 +757,12 @@ namespace nsfem
         storePartial<NCH>(part, st.partials, lb);
     }
 
-    // x += alpha p;  r -= alpha q;  z = r / d on colour 0;  convergence test.
-    template <typename T, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ p,
-                                                              const T * __restrict__ q, T * __restrict__ x, T * __restrict__ r,
-                                                              T * __restrict__ z, int color0End)
+    // x += alpha p;  r -= alpha q;  z = r / d on colour 0 (preconditioner
+    // type P; rP = (P) r when P is narrower than T); partial sums of |r|^2.
+    template <typename T, typename P, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(int n, BlockMap bm, SolveState st, const T * __restrict__ p,
+                                                              const T * __restrict__ q, T * __restrict__ x, T * r, P * rP,
+                                                              P * __restrict__ z, const P * __restrict__ invDiagP, int color0End)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -782,7 +793,7 @@ namespace nsfem
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = row + static_cast<long>(c) * A.n;
+                    const long i = row + static_cast<long>(c) * n;
                     T ri = r[i];
                     if (act[c])
                     {
@@ -790,9 +801,17 @@ namespace nsfem
                         ri = fmaT(-alpha[c], q[i], ri);
                         r[i] = ri;
                     }
+                    const P rp = static_cast<P>(ri);
+                    if constexpr (!std::is_same_v<T, P>)
+                    {
+                        if (act[c])
+                        {
+                            rP[i] = rp;
+                        }
+                    }
                     if (row < color0End)
                     {
-                        z[i] = ri * A.invDiag[row];
+                        z[i] = rp * invDiagP[row];
                     }
                     part[c] = fma(static_cast<double>(ri), static_cast<double>(ri), part[c]);
                 }

@@ -162,30 +162,25 @@ namespace nsfem
             colorings_.push_back(layouts.back().coloring);
             partitions_.push_back(layouts.back().partitionStart);
         }
-        lanes_ = opt_.lanes > 0 ? opt_.lanes : std::min(16, lanesForMeanRow(nnzOff / n_));
-        if (lanes_ != 4 && lanes_ != 8 && lanes_ != 16)
+        meanRow_ = n_ > 0 ? nnzOff / n_ : 0.0;
+        if (opt_.lanes != 0 && opt_.lanes != 4 && opt_.lanes != 8 && opt_.lanes != 16)
         {
             throw std::invalid_argument("solver lanes must be 4, 8 or 16");
         }
-        const int slab = kBlock / lanes_;
-        int rowsPerBlock = opt_.rowsPerBlock > 0 ? opt_.rowsPerBlock : 4 * slab;
-        rowsPerBlock = std::max(slab, rowsPerBlock / slab * slab);
 
         // Colour-major global order: colour c, then system s, then the
-        // system's own colour order.
+        // system's own colour order.  Each (colour, system) range of rows is
+        // one chunk; workgroups never straddle chunks.
         std::vector<int> perm(n_);
         std::vector<std::vector<int>> gpos(numSystems_);
         for (int s = 0; s < numSystems_; s++)
         {
             gpos[s].resize(layouts[s].n);
         }
-        std::vector<int2> blocks;
-        std::vector<int> blockSystem;
-        colorBlockStart_.assign(numColors_ + 1, 0);
+        chunks_.clear();
         int g = 0;
         for (int c = 0; c < numColors_; c++)
         {
-            colorBlockStart_[c] = static_cast<int>(blocks.size());
             for (int s = 0; s < numSystems_; s++)
             {
                 const auto & L = layouts[s];
@@ -201,10 +196,9 @@ namespace nsfem
                     perm[g] = sysOffset_[s] + L.coloring[l];
                     g++;
                 }
-                for (int b0 = g0; b0 < g; b0 += rowsPerBlock)
+                if (g > g0)
                 {
-                    blocks.push_back(int2{b0, std::min(g, b0 + rowsPerBlock)});
-                    blockSystem.push_back(s);
+                    chunks_.push_back(RowChunk{c, s, g0, g});
                 }
             }
             if (c == 0)
@@ -212,8 +206,6 @@ namespace nsfem
                 color0End_ = g;
             }
         }
-        colorBlockStart_[numColors_] = static_cast<int>(blocks.size());
-        numBlocks_ = static_cast<int>(blocks.size());
 
         // Global stripped matrix in colour-major order.
         std::vector<int> rowStart(n_ + 1, 0), rowSplit(n_), column;
@@ -246,6 +238,43 @@ namespace nsfem
             rowStart[n_] = static_cast<int>(column.size());
         }
 
+        // Per-colour sizes for algorithmic byte counts: rows, off-diagonal
+        // entries of the lower-colour part and of the full row, and the
+        // number of distinct columns each part gathers (x read once).
+        colorStats_.assign(numColors_, ColorStats{});
+        {
+            std::vector<int> colorOfRow(n_, 0), seenLower(n_, -1), seenFull(n_, -1);
+            for (const RowChunk & ch : chunks_)
+            {
+                for (int r = ch.begin; r < ch.end; r++)
+                {
+                    colorOfRow[r] = ch.color;
+                }
+            }
+            for (int r = 0; r < n_; r++)
+            {
+                const int c = colorOfRow[r];
+                ColorStats & cs = colorStats_[c];
+                cs.rows += 1;
+                cs.lowerNnz += rowSplit[r] - rowStart[r];
+                cs.fullNnz += rowStart[r + 1] - rowStart[r];
+                for (int j = rowStart[r]; j < rowStart[r + 1]; j++)
+                {
+                    const int col = column[j];
+                    if (j < rowSplit[r] && seenLower[col] != c)
+                    {
+                        seenLower[col] = c;
+                        cs.lowerCols += 1;
+                    }
+                    if (seenFull[col] != c)
+                    {
+                        seenFull[col] = c;
+                        cs.fullCols += 1;
+                    }
+                }
+            }
+        }
+
         if (opt_.precision != SolverPrecision::F32)
         {
             uploadMatrix(A64_, rowStart, rowSplit, column, values, diag, invDiag);
@@ -255,30 +284,8 @@ namespace nsfem
             uploadMatrix(A32_, rowStart, rowSplit, column, values, diag, invDiag);
         }
 
-        // Block map
-        std::vector<int> sysBlockStart(numSystems_ + 1, 0), sysBlocks(numBlocks_);
-        for (int b = 0; b < numBlocks_; b++)
-        {
-            sysBlockStart[blockSystem[b] + 1]++;
-        }
-        for (int s = 0; s < numSystems_; s++)
-        {
-            sysBlockStart[s + 1] += sysBlockStart[s];
-        }
-        {
-            std::vector<int> fillPos(sysBlockStart.begin(), sysBlockStart.end() - 1);
-            for (int b = 0; b < numBlocks_; b++)
-            {
-                sysBlocks[fillPos[blockSystem[b]]++] = b;
-            }
-        }
         perm_.upload(perm);
-        blocks_.upload(blocks);
-        blockSystem_.upload(blockSystem);
-        sysBlockStart_.upload(sysBlockStart);
-        sysBlocks_.upload(sysBlocks);
         systemRows_.upload(sysRows_);
-        bm_ = BlockMap{blocks_.get(), blockSystem_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_};
 
         const size_t nv = static_cast<size_t>(n_) * opt_.numCh;
         const size_t ioBytes = nv * (ioDtype() == F64 ? 8 : 4);
@@ -304,7 +311,7 @@ namespace nsfem
         }
         else
         {
-            for (auto * a : {&b64_, &x64_, &r64_, &z64_})
+            for (auto * a : {&b64_, &x64_, &r64_, &z64_, &p64_, &q64_})
             {
                 a->resize(nv);
                 a->zero();
@@ -314,10 +321,9 @@ namespace nsfem
                 a->resize(nv);
                 a->zero();
             }
-            outer_.init(numBlocks_, numSystems_, opt_.numCh);
         }
-        inner_.init(numBlocks_, numSystems_, opt_.numCh);
         means_.resize(static_cast<size_t>(numSystems_) * opt_.numCh);
+        reshape(opt_.lanes, opt_.rowsPerBlock);
 
         const int K = numSystems_ * opt_.numCh;
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostFlags_), 4 * sizeof(int), hipHostMallocDefault));
@@ -332,7 +338,7 @@ namespace nsfem
     McgsSolver::~McgsSolver()
     {
         (void)hipStreamSynchronize(stream_);
-        for (ChunkGraph * g : {&graph64_, &graph32_})
+        for (ChunkGraph * g : {&graph64_, &graph32_, &graphMixed_})
         {
             if (g->exec)
             {
@@ -351,6 +357,121 @@ namespace nsfem
         {
             (void)hipHostFree(hostInts_);
         }
+    }
+
+    void McgsSolver::reshape(int lanes, int rowsPerBlock)
+    {
+        if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16)
+        {
+            throw std::invalid_argument("solver lanes must be 4, 8 or 16");
+        }
+        if (rowsPerBlock < 0)
+        {
+            throw std::invalid_argument("rowsPerBlock must be >= 0");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        lanes_ = lanes > 0 ? lanes : std::min(16, lanesForMeanRow(meanRow_));
+        const int slab = kBlock / lanes_;
+        int rpb = rowsPerBlock > 0 ? rowsPerBlock : 4 * slab;
+        rpb = std::max(slab, rpb / slab * slab);
+        rowsPerBlock_ = rpb;
+
+        std::vector<int2> blocks;
+        std::vector<int> blockSystem;
+        colorBlockStart_.assign(numColors_ + 1, 0);
+        size_t next = 0;
+        for (int c = 0; c < numColors_; c++)
+        {
+            colorBlockStart_[c] = static_cast<int>(blocks.size());
+            for (; next < chunks_.size() && chunks_[next].color == c; next++)
+            {
+                const RowChunk & ch = chunks_[next];
+                for (int b0 = ch.begin; b0 < ch.end; b0 += rpb)
+                {
+                    blocks.push_back(int2{b0, std::min(ch.end, b0 + rpb)});
+                    blockSystem.push_back(ch.system);
+                }
+            }
+        }
+        colorBlockStart_[numColors_] = static_cast<int>(blocks.size());
+        numBlocks_ = static_cast<int>(blocks.size());
+
+        std::vector<int> sysBlockStart(numSystems_ + 1, 0), sysBlocks(numBlocks_);
+        for (int b = 0; b < numBlocks_; b++)
+        {
+            sysBlockStart[blockSystem[b] + 1]++;
+        }
+        for (int s = 0; s < numSystems_; s++)
+        {
+            sysBlockStart[s + 1] += sysBlockStart[s];
+        }
+        {
+            std::vector<int> fillPos(sysBlockStart.begin(), sysBlockStart.end() - 1);
+            for (int b = 0; b < numBlocks_; b++)
+            {
+                sysBlocks[fillPos[blockSystem[b]]++] = b;
+            }
+        }
+        blocks_.upload(blocks.empty() ? std::vector<int2>{int2{0, 0}} : blocks);
+        blockSystem_.upload(blockSystem.empty() ? std::vector<int>{0} : blockSystem);
+        sysBlockStart_.upload(sysBlockStart);
+        sysBlocks_.upload(sysBlocks.empty() ? std::vector<int>{0} : sysBlocks);
+        bm_ = BlockMap{blocks_.get(), blockSystem_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_};
+
+        // reduction scratch sized by the block count; captured graphs hold
+        // the old launch shapes and buffers
+        inner_.init(numBlocks_, numSystems_, opt_.numCh);
+        if (opt_.precision == SolverPrecision::Mixed)
+        {
+            outer_.init(numBlocks_, numSystems_, opt_.numCh);
+        }
+        for (ChunkGraph * g : {&graph64_, &graph32_, &graphMixed_})
+        {
+            if (g->exec)
+            {
+                NSFEM_HIP(hipGraphExecDestroy(g->exec));
+                g->exec = nullptr;
+            }
+        }
+        syncLibraryStream();
+    }
+
+    double McgsSolver::iterationTime(int reps)
+    {
+        double total = 0;
+        for (const char * k : {"direction", "sweep", "rho", "update", "finalize"})
+        {
+            double bytes = 0, launches = 0;
+            const double ms = timeKernel(k, reps, &bytes, &launches);
+            total += ms * launches;
+        }
+        return total;
+    }
+
+    std::pair<int, int> McgsSolver::autotune(int reps)
+    {
+        if (opt_.method != SolverMethod::Pcg)
+        {
+            throw std::invalid_argument("autotuning is available for pcg solvers");
+        }
+        double best = 0;
+        int bestLanes = lanes_, bestRows = rowsPerBlock_;
+        for (int lanes : {4, 8, 16})
+        {
+            for (int slabs : {1, 2, 4, 8})
+            {
+                reshape(lanes, slabs * (kBlock / lanes));
+                const double t = iterationTime(reps);
+                if (best == 0 || t < best)
+                {
+                    best = t;
+                    bestLanes = lanes;
+                    bestRows = rowsPerBlock_;
+                }
+            }
+        }
+        reshape(bestLanes, bestRows);
+        return {bestLanes, bestRows};
     }
 
     void McgsSolver::setMseCheckInterval(int k)
@@ -467,10 +588,53 @@ namespace nsfem
         NSFEM_HIP(hipStreamSynchronize(s));
     }
 
-    template <typename T>
-    void McgsSolver::sgsSweeps(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * r, T * z)
+    template <>
+    PcgOperands<double, double> McgsSolver::operands<double, double>()
     {
-        const McgsMatrix<T> Av = A.view(n_);
+        return {A64_.view(n_), A64_.view(n_), b64_.get(), x64_.get(), r64_.get(), r64_.get(), z64_.get(), p64_.get(), q64_.get()};
+    }
+
+    template <>
+    PcgOperands<float, float> McgsSolver::operands<float, float>()
+    {
+        return {A32_.view(n_), A32_.view(n_), b32_.get(), x32_.get(), r32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get()};
+    }
+
+    template <>
+    PcgOperands<double, float> McgsSolver::operands<double, float>()
+    {
+        return {A64_.view(n_), A32_.view(n_), b64_.get(), x64_.get(), r64_.get(), r32_.get(), z32_.get(), p64_.get(), q64_.get()};
+    }
+
+    template <typename T, typename P>
+    McgsSolver::ChunkGraph & McgsSolver::chunkGraph()
+    {
+        if constexpr (std::is_same_v<T, P>)
+        {
+            return std::is_same_v<T, double> ? graph64_ : graph32_;
+        }
+        else
+        {
+            return graphMixed_;
+        }
+    }
+
+    template <typename T, typename P>
+    int & McgsSolver::predicted()
+    {
+        if constexpr (std::is_same_v<T, P>)
+        {
+            return std::is_same_v<T, double> ? predicted64_ : predicted32_;
+        }
+        else
+        {
+            return predictedMixed_;
+        }
+    }
+
+    template <typename P>
+    void McgsSolver::sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z)
+    {
         const int nch = opt_.numCh;
         // forward colours 1 .. C-1 on the lower part (colour 0 was set to r/d)
         for (int c = 1; c < numColors_; c++)
@@ -481,7 +645,7 @@ namespace nsfem
                 continue;
             }
             dispatchLC(lanes_, nch, [&]<int L, int C>()
-                       { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, bm_, colorBlockStart_[c], st, r, z, 1); });
+                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, bm_, colorBlockStart_[c], st, r, z, 1); });
         }
         // backward colours C-2 .. 1 on full rows (colour C-1 is unchanged)
         for (int c = numColors_ - 2; c >= 1; c--)
@@ -492,67 +656,82 @@ namespace nsfem
                 continue;
             }
             dispatchLC(lanes_, nch, [&]<int L, int C>()
-                       { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, bm_, colorBlockStart_[c], st, r, z, 0); });
+                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, bm_, colorBlockStart_[c], st, r, z, 0); });
         }
     }
 
-    template <typename T>
-    void McgsSolver::launchRho(hipStream_t s, const McgsMatrix<T> & Av, SolveState st, const T * r, T * z, double * totals)
+    template <typename T, typename P>
+    void McgsSolver::launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals)
     {
         dispatchLC(lanes_, opt_.numCh, [&]<int L, int C>()
                    {
-                       pcgRhoKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, r, z, color0End_);
+                       pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_);
                        pcgRhoFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals);
                    });
     }
 
-    template <typename T>
-    void McgsSolver::pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q)
+    template <typename T, typename P>
+    void McgsSolver::pcgIteration(hipStream_t s, const PcgOperands<T, P> & v, SolveState st)
     {
-        const McgsMatrix<T> Av = A.view(n_);
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
-                       pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q);
+                       pcgDirectionKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.z, v.p, v.q);
                        pcgDirectionFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals);
                    });
         if (nch == 1)
         {
-            pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
+            pcgUpdateKernel<T, P, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag, color0End_);
             pcgUpdateFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, st, totals);
         }
         else
         {
-            pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
+            pcgUpdateKernel<T, P, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag, color0End_);
             pcgUpdateFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, st, totals);
         }
-        sgsSweeps<T>(s, A, st, r, z);
-        launchRho<T>(s, Av, st, r, z, totals);
+        sgsSweeps<P>(s, v.M, st, v.rP, v.z);
+        launchRho<T, P>(s, v, st, totals);
     }
 
-    template <typename T>
-    int McgsSolver::runPcg(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * b, T * x, T * r, T * z, T * p, T * q,
-                           bool zeroGuess, const int * gate, int maxIters, bool sync, bool usePrediction)
+    template <typename T, typename P>
+    int McgsSolver::runPcg(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, bool zeroGuess, const int * gate, int maxIters,
+                           bool sync, bool usePrediction)
     {
         const int nch = opt_.numCh;
-        DeviceState & ds = inner_;
-        const McgsMatrix<T> Av = A.view(n_);
-        double * totals = ds.totals.get();
+        double * totals = inner_.totals.get();
         fillInt(4, 0, st.flags, s);
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
-                       pcgInitKernel<T, T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x, r, z, color0End_, zeroGuess ? 1 : 0);
+                       pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                              color0End_, zeroGuess ? 1 : 0);
                        pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, gate, totals);
                    });
-        sgsSweeps<T>(s, A, st, r, z);
-        launchRho<T>(s, Av, st, r, z, totals);
+        sgsSweeps<P>(s, v.M, st, v.rP, v.z);
+        launchRho<T, P>(s, v, st, totals);
         NSFEM_HIP(hipGetLastError());
+        return iterate<T, P>(s, v, st, maxIters, sync, usePrediction);
+    }
 
-        // Iterations run in chunks of checkChunk; a full chunk is one launch
-        // of a captured graph.  Kernels after convergence exit at once.
-        ChunkGraph & g = std::is_same_v<T, double> ? graph64_ : graph32_;
-        const int chunk = std::max(1, opt_.checkChunk);
+    template <typename T, typename P>
+    int McgsSolver::iterate(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, int maxIters, bool sync, bool usePrediction)
+    {
+        DeviceState & ds = inner_;
+        // Iterations run in chunks; a full chunk is one launch of a captured
+        // graph.  Kernels after convergence exit at once.
+        ChunkGraph & g = chunkGraph<T, P>();
+        int & predictedIters = predicted<T, P>();
+        // Chunks of up to checkChunk iterations, shorter for solves expected
+        // to take few iterations (at least 8 chunks per solve, at least 4
+        // iterations per chunk), so the host polls keep pace.
+        int chunk = std::max(1, opt_.checkChunk);
+        if (usePrediction)
+        {
+            while (chunk > 4 && 8 * chunk > std::max(predictedIters, 1))
+            {
+                chunk /= 2;
+            }
+        }
         if (useGraphs_ && (g.exec == nullptr || g.k != chunk || g.tol2 != st.tol2))
         {
             if (g.exec)
@@ -564,7 +743,7 @@ namespace nsfem
             NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < chunk; i++)
             {
-                pcgIteration<T>(s, A, st, x, r, z, p, q);
+                pcgIteration<T, P>(s, v, st);
             }
             NSFEM_HIP(hipStreamEndCapture(s, &graph));
             NSFEM_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
@@ -574,8 +753,7 @@ namespace nsfem
         }
         // The first host poll comes one chunk before the iteration count of
         // the previous solve on this path (warm-started solves repeat it).
-        int & predicted = std::is_same_v<T, double> ? predicted64_ : predicted32_;
-        const int firstPoll = usePrediction ? std::max(chunk, predicted - chunk) : chunk;
+        const int firstPoll = usePrediction ? std::max(chunk, predictedIters - chunk) : chunk;
         int launched = 0;
         while (launched < maxIters)
         {
@@ -588,7 +766,7 @@ namespace nsfem
             {
                 for (int i = 0; i < k; i++)
                 {
-                    pcgIteration<T>(s, A, st, x, r, z, p, q);
+                    pcgIteration<T, P>(s, v, st);
                 }
             }
             NSFEM_HIP(hipGetLastError());
@@ -600,12 +778,14 @@ namespace nsfem
         }
         if (usePrediction)
         {
-            predicted = launched;
+            predictedIters = launched;
         }
         return launched;
     }
 
-    void McgsSolver::runMixed(hipStream_t s)
+    // Iterative refinement: FP64 residual, FP32 PCG correction from zero to
+    // innerTol, x64 += correction; at most maxOuter steps.
+    void McgsSolver::runMixedRestart(hipStream_t s)
     {
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
@@ -614,7 +794,8 @@ namespace nsfem
         permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
         SolveState outer = outer_.view(K, opt_.target * opt_.target);
         SolveState inner = inner_.view(K, opt_.innerTol * opt_.innerTol);
-        const McgsMatrix<double> A64 = A64_.view(n_);
+        const PcgOperands<double, double> v64 = operands<double, double>();
+        const PcgOperands<float, float> v32 = operands<float, float>();
         std::vector<int> totalIters(numSystems_, 0);
         int outerIt = 0;
         for (;; outerIt++)
@@ -622,8 +803,8 @@ namespace nsfem
             fillInt(4, 0, outer.flags, s);
             dispatchLC(lanes_, nch, [&]<int L, int C>()
                        {
-                           pcgInitKernel<double, double, L, C><<<numBlocks_, kBlock, 0, s>>>(A64, bm_, outer, b64_.get(), x64_.get(), r64_.get(),
-                                                                                             z64_.get(), color0End_, 0);
+                           pcgInitKernel<double, double, L, C><<<numBlocks_, kBlock, 0, s>>>(v64.A, bm_, outer, v64.b, v64.x, v64.r, v64.r,
+                                                                                             v64.z, v64.M.invDiag, color0End_, 0);
                            pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, outer, nullptr, outer_.totals.get());
                        });
             NSFEM_HIP(hipGetLastError());
@@ -636,8 +817,7 @@ namespace nsfem
             {
                 projectMean<float>(s, b32_.get());
             }
-            runPcg<float>(s, A32_, inner, b32_.get(), x32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(), true,
-                          outer.active, opt_.maxIters, true, false);
+            runPcg<float, float>(s, v32, inner, true, outer.active, opt_.maxIters, true, false);
             // accumulate inner iteration counts
             NSFEM_HIP(hipMemcpyAsync(hostInts_, inner_.ints.get(), (2 * K + 4) * sizeof(int), hipMemcpyDeviceToHost, s));
             addConvertKernel<float><<<elementwiseGrid(nv), 256, 0, s>>>(nv, x32_.get(), x64_.get());
@@ -685,39 +865,45 @@ namespace nsfem
             }
             return;
         }
-        if (opt_.precision == SolverPrecision::Mixed)
+        if (opt_.precision == SolverPrecision::Mixed && opt_.restart)
         {
-            runMixed(s);
+            runMixedRestart(s);
             return;
         }
         const double tol2 = opt_.target * opt_.target;
-        if (opt_.precision == SolverPrecision::F64)
-        {
-            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
-            permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
-            SolveState st = inner_.view(K, tol2);
-            runPcg<double>(s, A64_, st, b64_.get(), x64_.get(), r64_.get(), z64_.get(), p64_.get(), q64_.get(), false, nullptr,
-                           opt_.maxIters, true, true);
-            readStats(s, inner_, false, 0);
-            if (opt_.projectMean)
-            {
-                projectMean<double>(s, x64_.get());
-            }
-            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
-        }
-        else
+        SolveState st = inner_.view(K, tol2);
+        if (opt_.precision == SolverPrecision::F32)
         {
             permute(F32, F32, n_, nch, perm_.get(), ioRhs_.get(), b32_.get(), false, s);
             permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
-            SolveState st = inner_.view(K, tol2);
-            runPcg<float>(s, A32_, st, b32_.get(), x32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(), false, nullptr,
-                          opt_.maxIters, true, true);
+            runPcg<float, float>(s, operands<float, float>(), st, false, nullptr, opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
             if (opt_.projectMean)
             {
                 projectMean<float>(s, x32_.get());
             }
             permute(F32, F32, n_, nch, perm_.get(), x32_.get(), ioSol_.get(), true, s);
+        }
+        else
+        {
+            // FP64 Krylov vectors and products; the preconditioner in FP64
+            // (plain) or FP32 (mixed)
+            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            if (opt_.precision == SolverPrecision::F64)
+            {
+                runPcg<double, double>(s, operands<double, double>(), st, false, nullptr, opt_.maxIters, true, true);
+            }
+            else
+            {
+                runPcg<double, float>(s, operands<double, float>(), st, false, nullptr, opt_.maxIters, true, true);
+            }
+            readStats(s, inner_, false, 0);
+            if (opt_.projectMean)
+            {
+                projectMean<double>(s, x64_.get());
+            }
+            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
         }
         NSFEM_HIP(hipStreamSynchronize(s));
     }
@@ -743,7 +929,7 @@ namespace nsfem
 
     void McgsSolver::solveAsync(hipStream_t) { solve(); }
 
-    double McgsSolver::timeKernel(const std::string & which, int reps, double * bytesPerLaunch)
+    double McgsSolver::timeKernel(const std::string & which, int reps, double * bytesPerLaunch, double * launchesPerIteration)
     {
         if (reps < 1)
         {
@@ -756,82 +942,129 @@ namespace nsfem
         hipStream_t s = stream_;
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
-        const bool f64 = opt_.precision == SolverPrecision::F64;
-        const double ts = f64 ? 8.0 : 4.0;
         SolveState st = inner_.view(K, 0.0);
         double * totals = inner_.totals.get();
-        long nnzOff = 0;
+        double nnzOff = 0;
+        for (const auto & cs : colorStats_)
         {
-            int last = 0;
-            NSFEM_HIP(hipMemcpy(&last, (f64 ? A64_.rowStart.get() : A32_.rowStart.get()) + n_, sizeof(int), hipMemcpyDeviceToHost));
-            nnzOff = last;
+            nnzOff += cs.fullNnz;
         }
         const double n = n_;
-        const double matrixBytes = nnzOff * (ts + 4.0) + (n + 1) * 4.0 + n * ts;
+        const double n0 = colorStats_.empty() ? 0.0 : colorStats_[0].rows;
         double bytes = 0;
+        double launches = 1;
         std::function<void()> launch;
-        auto run = [&](auto tag)
+        auto run = [&](auto tagT, auto tagP)
         {
-            using T = decltype(tag);
-            const DeviceMcgs<T> & A = [&]() -> const DeviceMcgs<T> &
+            using T = decltype(tagT);
+            using P = decltype(tagP);
+            constexpr bool same = std::is_same_v<T, P>;
+            const double tsT = sizeof(T), tsP = sizeof(P);
+            const PcgOperands<T, P> v = operands<T, P>();
+            // Algorithmic bytes of one launch, x read once (ideal reuse).
+            auto sweepBytes = [&](int c, bool lower)
             {
-                if constexpr (std::is_same_v<T, double>) return A64_;
-                else return A32_;
-            }();
-            T * x = f64 ? reinterpret_cast<T *>(x64_.get()) : reinterpret_cast<T *>(x32_.get());
-            T * r = f64 ? reinterpret_cast<T *>(r64_.get()) : reinterpret_cast<T *>(r32_.get());
-            T * z = f64 ? reinterpret_cast<T *>(z64_.get()) : reinterpret_cast<T *>(z32_.get());
-            T * p = f64 ? reinterpret_cast<T *>(p64_.get()) : reinterpret_cast<T *>(p32_.get());
-            T * q = f64 ? reinterpret_cast<T *>(q64_.get()) : reinterpret_cast<T *>(q32_.get());
-            const McgsMatrix<T> Av = A.view(n_);
+                const ColorStats & cs = colorStats_[c];
+                const double nnz = lower ? cs.lowerNnz : cs.fullNnz;
+                const double cols = lower ? cs.lowerCols : cs.fullCols;
+                return nnz * (tsP + 4.0) + (cs.rows + 1) * 4.0 + (lower ? cs.rows * 4.0 : 0.0) + cs.rows * tsP +
+                       nch * (cs.rows * tsP + cs.rows * tsP + cols * tsP);
+            };
             if (which == "direction")
             {
-                // z read once, p and q read and written
-                bytes = matrixBytes + 5.0 * nch * n * ts;
+                // matrix and diagonal, z gathered once, p and q read and written
+                bytes = nnzOff * (tsT + 4.0) + (n + 1) * 4.0 + n * tsT + nch * (n * tsP + 4.0 * n * tsT);
                 launch = [=, this]()
                 {
                     dispatchLC(lanes_, nch, [&]<int L, int C>()
-                               { pcgDirectionKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, z, p, q); });
+                               { pcgDirectionKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.z, v.p, v.q); });
                 };
             }
             else if (which == "sweep")
             {
-                // one full-row GS pass over every colour (numColors launches)
-                bytes = matrixBytes + 3.0 * nch * n * ts;
+                // the colour sweeps of one preconditioner application:
+                // forward colours 1 .. C-1 (lower part), backward C-2 .. 1
+                double total = 0;
+                int count = 0;
+                for (int c = 1; c < numColors_; c++)
+                {
+                    if (colorBlockStart_[c + 1] > colorBlockStart_[c])
+                    {
+                        total += sweepBytes(c, true);
+                        count++;
+                    }
+                }
+                for (int c = numColors_ - 2; c >= 1; c--)
+                {
+                    if (colorBlockStart_[c + 1] > colorBlockStart_[c])
+                    {
+                        total += sweepBytes(c, false);
+                        count++;
+                    }
+                }
+                if (count == 0)
+                {
+                    throw std::invalid_argument("the system has a single colour: no sweep launches");
+                }
+                launches = count;
+                bytes = total / count;
+                launch = [=, this]() { sgsSweeps<P>(s, v.M, st, v.rP, v.z); };
+            }
+            else if (which == "rho")
+            {
+                // backward colour 0 (full rows) fused with r.z over all rows
+                bytes = (numColors_ > 0 ? sweepBytes(0, false) : 0.0) + nch * n * tsT + nch * (n - n0) * tsP -
+                        (same ? nch * n0 * tsP : 0.0);
                 launch = [=, this]()
                 {
-                    for (int c = 0; c < numColors_; c++)
-                    {
-                        const int cnt = colorBlockStart_[c + 1] - colorBlockStart_[c];
-                        if (cnt == 0) continue;
-                        dispatchLC(lanes_, nch, [&]<int L, int C>()
-                                   { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, bm_, colorBlockStart_[c], st, r, z, 0); });
-                    }
+                    dispatchLC(lanes_, nch, [&]<int L, int C>()
+                               { pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_); });
                 };
             }
             else if (which == "update")
             {
-                bytes = (6.0 * nch * n + n) * ts;
+                // x, r read and written, p, q read; rP written (P narrower
+                // than T); z = r / d on colour 0
+                bytes = 6.0 * nch * n * tsT + (same ? 0.0 : nch * n * tsP) + n0 * tsP + nch * n0 * tsP;
                 launch = [=, this]()
                 {
                     if (nch == 1)
-                        pcgUpdateKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
+                        pcgUpdateKernel<T, P, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                              color0End_);
                     else
-                        pcgUpdateKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, p, q, x, r, z, color0End_);
+                        pcgUpdateKernel<T, P, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                              color0End_);
+                };
+            }
+            else if (which == "finalize")
+            {
+                // one of the three per-iteration one-block finalize kernels
+                launches = 3;
+                bytes = static_cast<double>(numBlocks_) * nch * 8.0;
+                launch = [=, this]()
+                {
+                    if (nch == 1)
+                        pcgDirectionFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                    else
+                        pcgDirectionFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, st, totals);
                 };
             }
             else
             {
-                throw std::invalid_argument("unknown kernel '" + which + "' (direction, sweep, update)");
+                throw std::invalid_argument("unknown kernel '" + which + "' (direction, sweep, rho, update, finalize)");
             }
         };
-        if (f64)
+        if (opt_.precision == SolverPrecision::F64)
         {
-            run(double{});
+            run(double{}, double{});
+        }
+        else if (opt_.precision == SolverPrecision::F32 || opt_.restart)
+        {
+            run(float{}, float{});
         }
         else
         {
-            run(float{});
+            run(double{}, float{});
         }
         // A state in which every kernel does its full work: all channels
         // active, not the first iteration, zero step lengths.
@@ -841,25 +1074,42 @@ namespace nsfem
         fill(F64, 1L * K, 1.0, st.ref, s);
         launch();
         NSFEM_HIP(hipGetLastError());
-        hipEvent_t e0, e1;
-        NSFEM_HIP(hipEventCreate(&e0));
-        NSFEM_HIP(hipEventCreate(&e1));
-        NSFEM_HIP(hipEventRecord(e0, s));
+        // The repetitions are one captured graph, launched as the solver
+        // launches its iterations, so host launch costs do not enter.
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < reps; i++)
         {
             launch();
         }
+        NSFEM_HIP(hipStreamEndCapture(s, &graph));
+        NSFEM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        NSFEM_HIP(hipGraphDestroy(graph));
+        NSFEM_HIP(hipGraphLaunch(exec, s));
+        hipEvent_t e0, e1;
+        NSFEM_HIP(hipEventCreate(&e0));
+        NSFEM_HIP(hipEventCreate(&e1));
+        NSFEM_HIP(hipEventRecord(e0, s));
+        NSFEM_HIP(hipGraphLaunch(exec, s));
         NSFEM_HIP(hipEventRecord(e1, s));
         NSFEM_HIP(hipEventSynchronize(e1));
         float ms = 0;
         NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
         NSFEM_HIP(hipEventDestroy(e0));
         NSFEM_HIP(hipEventDestroy(e1));
+        NSFEM_HIP(hipGraphExecDestroy(exec));
         if (bytesPerLaunch)
         {
             *bytesPerLaunch = bytes;
         }
-        return static_cast<double>(ms) / reps;
+        if (launchesPerIteration)
+        {
+            *launchesPerIteration = launches;
+        }
+        // "sweep" launches several kernels per repetition
+        const double perRep = which == "sweep" ? launches : 1.0;
+        return static_cast<double>(ms) / (reps * perRep);
     }
 
     template void McgsSolver::runGs<double>(hipStream_t);

@@ -18,6 +18,7 @@
 
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -51,8 +52,14 @@ namespace nsfem
         int lanes = 0;           // 0: choose from the mean row length
         int rowsPerBlock = 0;    // 0: 4 row slabs per block
         int checkChunk = 16;     // iterations between host convergence polls
-        double innerTol = 1e-4;  // mixed: relative tolerance of each inner solve
-        int maxOuter = 40;       // mixed: refinement steps
+        // Mixed precision.  Default: FP64 Krylov vectors, matrix products and
+        // reductions, the symmetric GS preconditioner in FP32 (its sweeps on
+        // an FP32 copy of the matrix).  restart = true: iterative refinement,
+        // FP32 PCG corrections from zero to innerTol on the FP64 residual,
+        // at most maxOuter of them.
+        bool restart = false;
+        double innerTol = 1e-4;
+        int maxOuter = 40;
         // Pure-Neumann (singular, zero row sum) systems with zero-mean right
         // hand sides: the solution is returned with zero mean, and the FP32
         // inner right-hand sides of the mixed path are projected to zero mean.
@@ -80,6 +87,24 @@ namespace nsfem
         DeviceArray<double> totals;
         SolveState view(int k, double tol2);
         void init(int numBlocks, int numSystems, int numCh);
+    };
+
+    // Device operands of one PCG solve: Krylov vectors and matrix products
+    // in T (matrix A), the symmetric GS preconditioner in P (matrix M, the
+    // residual copy rP and the preconditioned vector z).  P == T: M is A and
+    // rP aliases r.
+    template <typename T, typename P>
+    struct PcgOperands
+    {
+        McgsMatrix<T> A;
+        McgsMatrix<P> M;
+        const T * b;
+        T * x;
+        T * r;
+        P * rP;
+        P * z;
+        T * p;
+        T * q;
     };
 
     class McgsSolver
@@ -117,33 +142,63 @@ namespace nsfem
         void setTarget(double t) { opt_.target = t; }
         void setMseCheckInterval(int k);
         void setInnerTol(double t) { opt_.innerTol = t; }
+        void setMixedRestart(bool on) { opt_.restart = on; }
         void setProjectMean(bool on) { opt_.projectMean = on; }
         const SolverOptions & options() const { return opt_; }
 
         hipStream_t stream() const { return stream_; }
         void setUseGraphs(bool on) { useGraphs_ = on; }
 
-        // Average duration (ms) of one launch of a solver kernel over reps
-        // launches, timed with HIP events on the solver's stream, and the
-        // algorithmic bytes one launch moves.  which: "direction" (A z fused
-        // with the p/q updates), "sweep" (one full-row GS pass over all
-        // colours, per colour launch averaged), "update".  Leaves the io
-        // vectors untouched; internal work vectors are overwritten.
-        double timeKernel(const std::string & which, int reps, double * bytesPerLaunch);
+        // Average duration (ms) of one launch of a solver kernel, timed with
+        // HIP events on the solver's stream while replaying the launches of
+        // one PCG iteration, the algorithmic bytes of one launch and the
+        // launches per iteration.  which: "direction", "sweep", "rho",
+        // "update", "finalize".  Leaves the io vectors untouched; internal
+        // work vectors are overwritten.
+        double timeKernel(const std::string & which, int reps, double * bytesPerLaunch, double * launchesPerIteration);
+        // Sum over one PCG iteration's launches of their timed durations (ms).
+        double iterationTime(int reps);
+
+        // Launch shape: lanes per matrix row (4, 8, 16; 0 picks from the mean
+        // row length) and rows per workgroup (0: four row slabs).  Rebuilds
+        // the block map only; the colouring and matrices are kept.
+        void reshape(int lanes, int rowsPerBlock);
+        int lanes() const { return lanes_; }
+        int rowsPerBlock() const { return rowsPerBlock_; }
+        // Times iterationTime for lanes {4, 8, 16} x {1, 2, 4, 8} row slabs per
+        // workgroup and keeps the fastest shape; returns (lanes, rowsPerBlock).
+        // Overwrites the internal work vectors (not the io vectors).
+        std::pair<int, int> autotune(int reps);
 
     private:
+        // One captured graph of a chunk of PCG iterations per precision path.
+        struct ChunkGraph
+        {
+            hipGraphExec_t exec = nullptr;
+            int k = 0;
+            double tol2 = -1;
+        };
+
         template <typename T>
         void runGs(hipStream_t s);
-        template <typename T>
-        int runPcg(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * b, T * x, T * r, T * z, T * p, T * q,
-                   bool zeroGuess, const int * gate, int maxIters, bool sync, bool usePrediction);
-        void runMixed(hipStream_t s);
-        template <typename T>
-        void sgsSweeps(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, const T * r, T * z);
-        template <typename T>
-        void launchRho(hipStream_t s, const McgsMatrix<T> & Av, SolveState st, const T * r, T * z, double * totals);
-        template <typename T>
-        void pcgIteration(hipStream_t s, const DeviceMcgs<T> & A, SolveState st, T * x, T * r, T * z, T * p, T * q);
+        template <typename T, typename P>
+        PcgOperands<T, P> operands();
+        template <typename T, typename P>
+        ChunkGraph & chunkGraph();
+        template <typename T, typename P>
+        int & predicted();
+        template <typename T, typename P>
+        int runPcg(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, bool zeroGuess, const int * gate, int maxIters, bool sync,
+                   bool usePrediction);
+        template <typename T, typename P>
+        int iterate(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, int maxIters, bool sync, bool usePrediction);
+        template <typename T, typename P>
+        void pcgIteration(hipStream_t s, const PcgOperands<T, P> & v, SolveState st);
+        template <typename P>
+        void sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z);
+        template <typename T, typename P>
+        void launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals);
+        void runMixedRestart(hipStream_t s);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
         template <typename T>
         void projectMean(hipStream_t s, T * v);
@@ -157,7 +212,19 @@ namespace nsfem
         int color0End_ = 0;
         std::vector<int> sysOffset_, sysRows_;
         std::vector<std::vector<int>> colorings_, partitions_;
+        struct RowChunk
+        {
+            int color, system, begin, end; // rows [begin, end) of the global order
+        };
+        std::vector<RowChunk> chunks_;     // colour-major
+        double meanRow_ = 0;               // off-diagonal entries per row
+        int rowsPerBlock_ = 0;
         std::vector<int> colorBlockStart_; // numColors + 1
+        struct ColorStats
+        {
+            double rows = 0, lowerNnz = 0, fullNnz = 0, lowerCols = 0, fullCols = 0;
+        };
+        std::vector<ColorStats> colorStats_;
         int numBlocks_ = 0;
 
         DeviceArray<int> perm_;
@@ -185,15 +252,8 @@ namespace nsfem
         bool lastNonFinite_ = false;
         hipStream_t stream_;
 
-        // One captured graph of checkChunk PCG iterations per precision path.
-        struct ChunkGraph
-        {
-            hipGraphExec_t exec = nullptr;
-            int k = 0;
-            double tol2 = -1;
-        };
-        ChunkGraph graph64_, graph32_;
+        ChunkGraph graph64_, graph32_, graphMixed_;
         bool useGraphs_ = true;
-        int predicted64_ = 0, predicted32_ = 0;
+        int predicted64_ = 0, predicted32_ = 0, predictedMixed_ = 0;
     };
 } // namespace nsfem

@@ -244,6 +244,12 @@ class AbstractSolver:
             call("nsfem_solver_last_iterations_system", self._h, system, ctypes.byref(out))
         return out.value
 
+    def getLastOuterIterations(self):
+        """pcg_mixed with mixedRestart: refinement steps of the last solve."""
+        out = ctypes.c_int()
+        call("nsfem_solver_last_outer_iterations", self._h, ctypes.byref(out))
+        return out.value
+
     def getRhs(self):
         return self._rhs
 
@@ -266,9 +272,27 @@ class AbstractSolver:
         call("nsfem_solver_set_option", self._h, key.encode(), float(value))
 
     def timeKernel(self, which, reps=50):
-        ms, b = ctypes.c_double(), ctypes.c_double()
-        call("nsfem_solver_time_kernel", self._h, which.encode(), int(reps), ctypes.byref(ms), ctypes.byref(b))
-        return ms.value, b.value
+        """(average ms per launch, algorithmic bytes per launch, launches per
+        PCG iteration) of one of the iteration's kernels."""
+        ms, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        call("nsfem_solver_time_kernel", self._h, which.encode(), int(reps), ctypes.byref(ms), ctypes.byref(b), ctypes.byref(n))
+        return ms.value, b.value, n.value
+
+    def reshape(self, lanes=0, rowsPerBlock=0):
+        call("nsfem_solver_reshape", self._h, int(lanes), int(rowsPerBlock))
+
+    def shape(self):
+        """(lanes per row, rows per workgroup) of the solver kernels."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        call("nsfem_solver_shape", self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def autotune(self, reps=10):
+        """Picks the fastest (lanes, rows per workgroup) by timing one PCG
+        iteration's launches; overwrites internal work vectors."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        call("nsfem_solver_autotune", self._h, int(reps), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def numColors(self, system=0):
         out = ctypes.c_int()

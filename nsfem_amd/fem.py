@@ -227,6 +227,8 @@ class SolverConfig:
     maxIterations: int = 200
     targetMse: float = 1e-6
     mseCheckInterval: int = 1
+    lanes: int = 0         # lanes per matrix row (4, 8, 16); 0 picks from the mean row length
+    rowsPerBlock: int = 0  # rows per workgroup of the solver kernels; 0 is the default
 
 
 @dataclasses.dataclass
@@ -235,8 +237,6 @@ class ChorinConfig:
     pressureSolver: SolverConfig = dataclasses.field(default_factory=SolverConfig)
     dtype: int = F64
     innerTol: float = 1e-4
-    lanesVelocity: int = 0
-    lanesPressure: int = 0
 
 
 class Chorin:
@@ -248,9 +248,11 @@ class Chorin:
         self._m1 = cfg.velocitySolver.method.encode()
         self._m2 = cfg.pressureSolver.method.encode()
         v, p = cfg.velocitySolver, cfg.pressureSolver
-        c = _lib.ChorinConfigC(_lib.SolverConfig(self._m1, v.maxIterations, v.targetMse, v.mseCheckInterval),
-                               _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval),
-                               cfg.dtype, cfg.innerTol, cfg.lanesVelocity, cfg.lanesPressure)
+        c = _lib.ChorinConfigC(_lib.SolverConfig(self._m1, v.maxIterations, v.targetMse, v.mseCheckInterval, v.lanes,
+                                                 v.rowsPerBlock),
+                               _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
+                                                 p.rowsPerBlock),
+                               cfg.dtype, cfg.innerTol)
         self._h = ctypes.c_void_p()
         call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
         self.dtype = _NP[cfg.dtype]
@@ -286,10 +288,24 @@ class Chorin:
         return u.value, p.value
 
     def time_kernel(self, which, reps=50):
-        """(average ms per launch, algorithmic bytes per launch) of a step kernel."""
-        ms, b = ctypes.c_double(), ctypes.c_double()
-        call("nsfem_chorin_time_kernel", self._h, which.encode(), int(reps), ctypes.byref(ms), ctypes.byref(b))
-        return ms.value, b.value
+        """(average ms per launch, algorithmic bytes per launch, launches per
+        solver iteration -- 1 for the once-per-step kernels) of a step kernel."""
+        ms, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        call("nsfem_chorin_time_kernel", self._h, which.encode(), int(reps), ctypes.byref(ms), ctypes.byref(b), ctypes.byref(n))
+        return ms.value, b.value, n.value
+
+    def shape(self):
+        """(velocity lanes, velocity rows per workgroup, pressure lanes,
+        pressure rows per workgroup) of the solver kernels."""
+        out = (ctypes.c_int * 4)()
+        call("nsfem_chorin_shape", self._h, out)
+        return tuple(out)
+
+    def autotune(self, reps=10):
+        """Autotunes both solvers' launch shapes (the state is kept)."""
+        out = (ctypes.c_int * 4)()
+        call("nsfem_chorin_autotune", self._h, int(reps), out)
+        return tuple(out)
 
     def assemble_convection(self, u_host):
         """Returns (A, C) values in the velocity pattern for a host velocity."""

@@ -224,3 +224,79 @@ def test_pure_neumann_laplacian_with_mean_projection(oracle, octx_h025, name):
     assert np.linalg.norm(x - xp) <= 1e-7 * np.linalg.norm(xp)
     xo, rel, it = oracle.pcg_solve(m, b, np.zeros(n), 1e-10, 5000)
     assert np.linalg.norm((xo - xo.mean()) - xp) <= 1e-7 * np.linalg.norm(xp)
+
+
+def test_kernel_timing_hooks_report_per_launch_figures(octx_h025):
+    """The measurement hook replays one iteration's launches: positive times
+    and byte counts, 2C-3 sweep launches per iteration, and the solver still
+    solves afterwards."""
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(32)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, _ = _solve("pcg", m, b, max_iters=2000, target=1e-10)
+    C = s.numColors()
+    expect = {"direction": 1, "sweep": 2 * C - 3, "rho": 1, "update": 1, "finalize": 3}
+    nnz = m.nnz
+    for k, launches in expect.items():
+        ms, by, per_it = s.timeKernel(k, reps=3)
+        assert ms > 0 and by > 0 and per_it == launches, k
+    # a direction launch streams the whole off-diagonal matrix once
+    assert s.timeKernel("direction", reps=1)[1] >= (nnz - m.shape[0]) * 12
+    s.getRhs().upload(b)
+    s.getSol().memsetZero()
+    s.solve()
+    x = s.getSol().download()
+    assert np.linalg.norm(b - m @ x) <= 1.0001e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("restart", [0, 1])
+def test_pcg_mixed_iteration_counts(octx_h025, restart):
+    """Default pcg_mixed (FP64 Krylov vectors and products, FP32 symmetric GS
+    preconditioner) converges in the FP64 solver's iteration count, to the
+    same FP64 relative residual.  restart = 1 is iterative refinement: its
+    FP32 inner solves from zero cost extra iterations."""
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(33)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s64, x64 = _solve("pcg", m, b, max_iters=5000, target=1e-10)
+    smx, x = _solve("pcg_mixed", m, b, max_iters=5000, target=1e-10, mixedRestart=restart)
+    assert np.linalg.norm(b - m @ x) <= 1.0001e-10 * np.linalg.norm(b)
+    assert smx.getLastRelRes() <= 1e-10
+    it64, itmx = s64.getLastIterations(), smx.getLastIterations()
+    if restart == 0:
+        assert abs(itmx - it64) <= 2, (itmx, it64)
+        assert smx.getLastOuterIterations() == 0
+        assert np.linalg.norm(x - x64) <= 1e-8 * np.linalg.norm(x64)
+    else:
+        assert itmx > it64
+        assert smx.getLastOuterIterations() >= 2
+
+
+def test_reshape_and_autotune_keep_results(octx_h025):
+    """Launch shapes change only the reduction order: every shape converges
+    in the same iterations (+-1) to the same solution; autotune picks a valid
+    shape and leaves a working solver."""
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(34)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x0 = _solve("pcg", m, b, max_iters=3000, target=1e-11)
+    it0 = s.getLastIterations()
+    for lanes, rows in ((4, 64), (8, 32), (16, 16), (8, 256), (0, 0)):
+        s.reshape(lanes, rows)
+        L, R = s.shape()
+        assert L in (4, 8, 16) and R % (256 // L) == 0 and R >= 256 // L
+        s.getRhs().upload(b)
+        s.getSol().memsetZero()
+        s.solve()
+        x = s.getSol().download()
+        assert abs(s.getLastIterations() - it0) <= 1
+        assert np.linalg.norm(x - x0) <= 1e-9 * np.linalg.norm(x0)
+    L, R = s.autotune(reps=2)
+    assert (L, R) == s.shape()
+    s.getRhs().upload(b)
+    s.getSol().memsetZero()
+    s.solve()
+    assert np.linalg.norm(b - m @ s.getSol().download()) <= 1.0001e-11 * np.linalg.norm(b)
+    from nsfem_amd import _lib
+    with pytest.raises(_lib.NsfemInvalidArgument):
+        s.reshape(3, 0)
