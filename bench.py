@@ -95,23 +95,42 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune):
     return ch, res, (u1, p1), (u, p)
 
 
-def kernel_table(ch, res):
+def kernel_table(ch, res, reps=30):
     """Per-kernel average launch time (HIP events on the library stream,
     replaying the launches one solver iteration makes), algorithmic bytes per
     launch, GB/s, and each kernel's estimated share of one step: launches per
     iteration times the measured iterations per step."""
-    it = res["iters"]
-    vel_iters = it["tentative"] + it["final"]
+    it = res["iters"] if res else None
+    vel_iters = it["tentative"] + it["final"] if it else None
     out = {}
-    for solver, iters in (("velocity", vel_iters), ("pressure", it["pressure"])):
+    for solver, iters in (("velocity", vel_iters), ("pressure", it["pressure"] if it else None)):
         for k in ("direction", "sweep", "rho", "update", "finalize"):
-            ms, b, per_it = ch.time_kernel(f"{solver}_{k}", reps=30)
-            out[f"{solver}_{k}"] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9,
-                                    "per_step": per_it * iters, "ms_per_step": ms * per_it * iters}
+            ms, b, per_it = ch.time_kernel(f"{solver}_{k}", reps=reps)
+            out[f"{solver}_{k}"] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "launches_per_iteration": per_it,
+                                    "per_step": per_it * iters if it else None,
+                                    "ms_per_step": ms * per_it * iters if it else None}
     for k in ("convection", "spmm"):
-        ms, b, _ = ch.time_kernel(k, reps=30)
+        ms, b, _ = ch.time_kernel(k, reps=reps)
         out[k] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9, "per_step": 1, "ms_per_step": ms}
     return out
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_chorin_500k_mixed.md")
+
+
+def measured_traffic(kind):
+    """HBM bytes per launch of a mixed-path kernel kind, from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if
+    the summary does not list it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kind:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(ctx, cond, tau, u0):
@@ -151,6 +170,12 @@ def main():
                     help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="no time steps: build the workload and time each kernel kind (--kernel-reps launches); "
+                         "for hardware-counter passes, which need few dispatches")
+    ap.add_argument("--kernel-reps", type=int, default=30)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch solver iterations from the host (no graph replay; for kernel tracing tools)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run 3 in-place scalings of an array this size (known traffic, to scale HBM byte counters)")
     args = ap.parse_args()
@@ -176,23 +201,35 @@ def main():
 
     t_setup = time.perf_counter()
     m, cond, ctx = build_workload(args.nodes, args.seed)
-    lam = fem.estimate_lambda_max(ctx)
-    tau_stable = 1.0 / (cond.viscosity * lam)
-    tau = min(args.tau, tau_stable)
-    log(f"lambda_max(M^-1 K) ~ {lam:.4g}; tau requested {args.tau:g}, used {tau:.4g} "
-        f"(tau nu lambda = {tau * cond.viscosity * lam:.3f})")
     u0 = initial_velocity(ctx, cond, 1)
-    setup_s = time.perf_counter() - t_setup
-
     target = 1e-10
     vl, vr, pl, pr = (int(v) for v in args.shape.split(","))
 
     def solvers(method):
         return (fem.SolverConfig(method, 5000, target, 1, vl, vr), fem.SolverConfig(method, 50000, target, 1, pl, pr))
 
-    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64)
-    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64)
+    eager = 1 if args.eager else 0
+    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager)
+    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager)
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
+    if args.kernels_only:
+        out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
+               "kernels": {}}
+        for name in paths:
+            ch = fem.Chorin(ctx, {"fp64": cfg64, "mixed": cfgmx}[name])
+            ch.upload_velocity(u0)
+            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps)
+            out[f"{name}_shape"] = ch.shape()
+            del ch
+        print(json.dumps(out))
+        return 0
+    lam = fem.estimate_lambda_max(ctx)
+    tau_stable = 1.0 / (cond.viscosity * lam)
+    tau = min(args.tau, tau_stable)
+    log(f"lambda_max(M^-1 K) ~ {lam:.4g}; tau requested {args.tau:g}, used {tau:.4g} "
+        f"(tau nu lambda = {tau * cond.viscosity * lam:.3f})")
+    setup_s = time.perf_counter() - t_setup
+
     results, kernels, firsts, lasts = {}, {}, {}, {}
     for name in paths:
         cfg = {"fp64": cfg64, "mixed": cfgmx}[name]
@@ -227,9 +264,11 @@ def main():
         dom_name = max(kernmx, key=lambda k: kernmx[k]["ms_per_step"])
         dom = kernmx[dom_name]
         peak = 8000.0
+        traffic = measured_traffic(dom_name) if head == "mixed" and args.nodes == 500_000 else None
         roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": peak, "unit": "GB/s",
-                    "frac": dom["GBps"] / peak, "traffic": None, "bytes_per_launch": dom["bytes"],
-                    "avg_launch_ms": dom["avg_ms"]}
+                    "frac": dom["GBps"] / peak, "traffic": traffic, "bytes_per_launch": dom["bytes"],
+                    "avg_launch_ms": dom["avg_ms"],
+                    "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None}
     out = {
         "metric": "Chorin steps/s @500k p-DOFs, mixed vs FP64 GPU, same residual; SpMV % HBM peak",
         "value": rmx["steps_per_s"],

@@ -136,10 +136,13 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *   "mixedRestart" 1: pcg_mixed as iterative refinement (FP32 PCG inner
  *                  solves from zero on the FP64 residual) instead of the
  *                  default FP64 PCG with an FP32 symmetric GS preconditioner.
- *   "innerTol"     pcg_mixed with mixedRestart: inner relative tolerance. */
+ *   "innerTol"     pcg_mixed with mixedRestart: inner relative tolerance.
+ *   "graphs"       0: launch every iteration from the host instead of
+ *                  replaying captured graphs of iteration chunks (default 1). */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
-/* Kernel timing (HIP events on the library stream), replaying the launches
- * of one PCG iteration: "direction" (A z fused with the direction update and
+/* Kernel timing (HIP events on the library stream; reps launches queued
+ * behind a 20 ms delay kernel so they run back to back and the events
+ * bracket GPU execution only), replaying the launches of one PCG iteration: "direction" (A z fused with the direction update and
  * p.q), "sweep" (the colour sweeps of one preconditioner application:
  * forward colours 1..C-1 on the lower part, backward C-2..1), "rho"
  * (backward colour 0 fused with r.z), "update" (x, r, colour-0 z, r.r),
@@ -254,7 +257,10 @@ typedef struct nsfem_chorin_config
     nsfem_solver_config velocitySolver;
     nsfem_solver_config pressureSolver;
     int dtype;       /* NSFEM_F64 (plain / mixed) or NSFEM_F32 (reference precision) */
-    double innerTol; /* pcg_mixed inner relative tolerance */
+    double innerTol; /* pcg_mixed with mixedRestart: inner relative tolerance */
+    int eagerLaunches; /* 1: launch every solver iteration from the host instead
+                        * of replaying captured graphs (for tools that cannot
+                        * trace graph launches); 0: graphs */
 } nsfem_chorin_config;
 
 typedef struct nsfem_step_stats

@@ -63,7 +63,7 @@ class SolverConfig(ctypes.Structure):
 
 class ChorinConfigC(ctypes.Structure):
     _fields_ = [("velocitySolver", SolverConfig), ("pressureSolver", SolverConfig), ("dtype", ctypes.c_int),
-                ("innerTol", ctypes.c_double)]
+                ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):

@@ -624,6 +624,10 @@ extern "C"
             {
                 s->core->setMixedRestart(value != 0);
             }
+            else if (k == "graphs")
+            {
+                s->core->setUseGraphs(value != 0);
+            }
             else
             {
                 throw std::invalid_argument("Unknown solver option: " + k);
@@ -1027,6 +1031,7 @@ extern "C"
             conv(cfg->pressureSolver, cc.pressure);
             cc.dtype = cfg->dtype;
             cc.innerTol = cfg->innerTol > 0 ? cfg->innerTol : 1e-4;
+            cc.graphs = cfg->eagerLaunches == 0;
             auto h = std::make_unique<nsfem_chorin>();
             h->core = std::make_unique<ChorinGpu>(c->ctx, cc);
             *out = h.release(); });

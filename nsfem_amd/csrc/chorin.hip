@@ -200,6 +200,8 @@ namespace nsfem
         }
         vSolver_ = std::make_unique<McgsSolver>(vsys, solverOpts(cfg_.velocity, 2));
         pSolver_ = std::make_unique<McgsSolver>(psys, solverOpts(cfg_.pressure, 1));
+        vSolver_->setUseGraphs(cfg_.graphs);
+        pSolver_->setUseGraphs(cfg_.graphs);
         for (auto & e : ev_)
         {
             NSFEM_HIP(hipEventCreate(&e));

@@ -41,6 +41,7 @@ namespace nsfem
         ChorinSolverConfig pressure;
         int dtype = F64;        // state and matrix precision (F32 = the reference's precision)
         double innerTol = 1e-4; // pcg_mixed inner tolerance
+        bool graphs = true;     // solver iterations replayed as captured graphs
     };
 
     struct ChorinStepStats

@@ -54,6 +54,16 @@ namespace nsfem
         return x * q + (x < r ? x : r) + i;
     }
 
+    // Bijection on [0, B) placing the first B0 indices evenly among the rest:
+    // slot l holds index floor(l B0 / B) where that value steps up, else the
+    // next of the remaining B - B0 indices, in order.
+    __device__ __forceinline__ int interleaveFirst(int l, int B, int B0)
+    {
+        const long c = static_cast<long>(l) * B0 / B;
+        const long c1 = static_cast<long>(l + 1) * B0 / B;
+        return c1 > c ? static_cast<int>(c) : B0 + (l - static_cast<int>(c));
+    }
+
     // Sum of v over the block (256 threads); result valid in thread 0.
     template <int K>
     __device__ __forceinline__ void blockSum(double (&v)[K], double * sm /* K * kWaves */)
@@ -192,6 +202,18 @@ namespace nsfem
         for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
         {
             c[i] = alpha * a[i] + beta * b[i];
+        }
+    }
+
+    // One wave that idles for the given number of ticks of the constant
+    // 100 MHz wall clock, then exits (holds a stream while the host queues
+    // work behind it, so that work runs back to back).
+    static __global__ void delayKernel(long ticks)
+    {
+        const long t0 = static_cast<long>(wall_clock64());
+        while (static_cast<long>(wall_clock64()) - t0 < ticks)
+        {
+            __builtin_amdgcn_s_sleep(127);
         }
     }
 
@@ -634,12 +656,15 @@ namespace nsfem
 
     // Backward colour 0 of symmetric GS (on the preconditioner matrix M, type
     // P) fused with the partial sums of rho = r.z (r of the Krylov type T).
+    // Colour-0 blocks carry the sweep work; they are interleaved evenly among
+    // the others (block map order: colour 0 first, color0Blocks of them) so
+    // that every XCD's range of logical blocks gets its share.
     template <typename P, typename T, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP,
-                                                           const T * r, P * __restrict__ z, int color0End)
+                                                           const T * r, P * __restrict__ z, int color0End, int color0Blocks)
     {
         __shared__ double sm[NCH * kWaves];
-        const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
         if (st.flags[0])
         {
             return;

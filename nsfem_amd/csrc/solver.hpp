@@ -150,8 +150,8 @@ namespace nsfem
         void setUseGraphs(bool on) { useGraphs_ = on; }
 
         // Average duration (ms) of one launch of a solver kernel, timed with
-        // HIP events on the solver's stream while replaying the launches of
-        // one PCG iteration, the algorithmic bytes of one launch and the
+        // HIP events on the solver's stream around reps back-to-back
+        // replays of the launches of one PCG iteration, the algorithmic bytes of one launch and the
         // launches per iteration.  which: "direction", "sweep", "rho",
         // "update", "finalize".  Leaves the io vectors untouched; internal
         // work vectors are overwritten.

@@ -237,6 +237,7 @@ class ChorinConfig:
     pressureSolver: SolverConfig = dataclasses.field(default_factory=SolverConfig)
     dtype: int = F64
     innerTol: float = 1e-4
+    eagerLaunches: int = 0  # 1: no graph replay of solver iterations (for tracing tools)
 
 
 class Chorin:
@@ -252,7 +253,7 @@ class Chorin:
                                                  v.rowsPerBlock),
                                _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
                                                  p.rowsPerBlock),
-                               cfg.dtype, cfg.innerTol)
+                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches)
         self._h = ctypes.c_void_p()
         call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
         self.dtype = _NP[cfg.dtype]
