@@ -372,8 +372,9 @@ namespace nsfem
         int * active;   // 1 while the channel iterates
         int * iters;    // iterations done / iteration index at convergence
         int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen
-        double * partials; // blocks * 2 * NCH
-        double tol2;       // relative tolerance squared (pcg) or target RMS (gs)
+        double * partials;  // blocks * 2 * NCH
+        double * partials2; // blocks * NCH: |r|^2 of the update, read with rho's
+        double tol2;        // relative tolerance squared (pcg) or target RMS (gs)
     };
 
 
@@ -391,11 +392,12 @@ namespace nsfem
             {
                 acc[k] = 0;
             }
+            // one system: its blocks are 0 .. nb-1 in order, no indirection;
+            // the loads of a thread are independent, so keep many in flight
             const int nb = bm.sysBlockStart[1];
-#pragma unroll 4
-            for (int i = threadIdx.x; i < nb; i += kBlock)
+#pragma unroll 16
+            for (int b = threadIdx.x; b < nb; b += kBlock)
             {
-                const int b = bm.sysBlocks[i];
 #pragma unroll
                 for (int k = 0; k < K; k++)
                 {
@@ -843,7 +845,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials, lb);
+        storePartial<NCH>(part, st.partials2, lb);
     }
 
     // Per-system RMS residuals of gsResidualKernel, stop test and flags.
@@ -943,23 +945,51 @@ namespace nsfem
 
     // rho = r.z and beta = rho / rho_old.
     template <int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgRhoFinalizeKernel(BlockMap bm, SolveState st, double * totals)
+    // withUpdate: also finish the preceding update's |r|^2 (partials2): the
+    // iteration count and the convergence test of the update come first,
+    // then rho and beta for the channels that go on.  (The sweeps and the
+    // rho kernel in between ran on the update's result either way.)
+    __global__ void __launch_bounds__(kBlock) pcgRhoFinalizeKernel(BlockMap bm, SolveState st, double * totals, int withUpdate)
     {
         __shared__ double sm[NCH * kWaves];
         if (st.flags[0])
         {
             return; // the producer skipped as well
         }
+        double * rrTotals = totals + bm.numSystems * NCH;
+        if (withUpdate)
+        {
+            systemTotals<NCH>(bm, st.partials2, rrTotals, sm);
+        }
         systemTotals<NCH>(bm, st.partials, totals, sm);
         if (threadIdx.x == 0)
         {
             const bool first = st.flags[1] != 0;
+            int anyActive = 0;
             for (int k = 0; k < bm.numSystems * NCH; k++)
             {
                 if (!st.active[k])
                 {
                     continue;
                 }
+                if (withUpdate)
+                {
+                    const double rr = rrTotals[k];
+                    st.rr[k] = rr;
+                    st.iters[k] += 1;
+                    if (!(rr == rr) || rr > 1e300)
+                    {
+                        st.flags[2] = 1;
+                        st.active[k] = 0;
+                        continue;
+                    }
+                    if (rr <= st.tol2 * st.ref[k])
+                    {
+                        st.active[k] = 0;
+                        continue;
+                    }
+                }
+                anyActive = 1;
                 const double rhoNew = totals[k];
                 st.beta[k] = (first || st.rho[k] == 0) ? 0.0 : rhoNew / st.rho[k];
                 st.rho[k] = rhoNew;
@@ -967,6 +997,10 @@ namespace nsfem
                 {
                     st.flags[2] = 1;
                 }
+            }
+            if (withUpdate && !anyActive)
+            {
+                st.flags[0] = 1;
             }
         }
     }
@@ -989,50 +1023,6 @@ namespace nsfem
                 st.alpha[k] = (st.active[k] && pq != 0) ? st.rho[k] / pq : 0.0;
             }
             st.flags[1] = 0;
-        }
-    }
-
-    // |r|^2 after the update, iteration count and convergence test.
-    template <int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgUpdateFinalizeKernel(BlockMap bm, SolveState st, double * totals)
-    {
-        __shared__ double sm[NCH * kWaves];
-        if (st.flags[0])
-        {
-            return; // the producer skipped as well
-        }
-        systemTotals<NCH>(bm, st.partials, totals, sm);
-        if (threadIdx.x == 0)
-        {
-            int anyActive = 0;
-            for (int k = 0; k < bm.numSystems * NCH; k++)
-            {
-                if (!st.active[k])
-                {
-                    continue;
-                }
-                const double rr = totals[k];
-                st.rr[k] = rr;
-                st.iters[k] += 1;
-                if (!(rr == rr) || rr > 1e300)
-                {
-                    st.flags[2] = 1;
-                    st.active[k] = 0;
-                    continue;
-                }
-                if (rr <= st.tol2 * st.ref[k])
-                {
-                    st.active[k] = 0;
-                }
-                else
-                {
-                    anyActive = 1;
-                }
-            }
-            if (!anyActive)
-            {
-                st.flags[0] = 1;
-            }
         }
     }
 

@@ -103,6 +103,7 @@ namespace nsfem
         scal.resize(static_cast<size_t>(7) * K);
         ints.resize(static_cast<size_t>(2) * K + 4);
         partials.resize(static_cast<size_t>(std::max(1, numBlocks)) * 2 * numCh);
+        partials2.resize(static_cast<size_t>(std::max(1, numBlocks)) * numCh);
         totals.resize(static_cast<size_t>(numSystems) * 2 * numCh);
         scal.zero();
         ints.zero();
@@ -123,6 +124,7 @@ namespace nsfem
         s.iters = ints.get() + K;
         s.flags = ints.get() + 2 * K;
         s.partials = partials.get();
+        s.partials2 = partials2.get();
         s.tol2 = tol2;
         return s;
     }
@@ -661,13 +663,13 @@ namespace nsfem
     }
 
     template <typename T, typename P>
-    void McgsSolver::launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals)
+    void McgsSolver::launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate)
     {
         dispatchLC(lanes_, opt_.numCh, [&]<int L, int C>()
                    {
                        pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_,
                                                                               colorBlockStart_[1]);
-                       pcgRhoFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                       pcgRhoFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
                    });
     }
 
@@ -684,15 +686,14 @@ namespace nsfem
         if (nch == 1)
         {
             pcgUpdateKernel<T, P, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag, color0End_);
-            pcgUpdateFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, st, totals);
         }
         else
         {
             pcgUpdateKernel<T, P, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag, color0End_);
-            pcgUpdateFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, st, totals);
         }
+        // the update's convergence test is finished together with rho
         sgsSweeps<P>(s, v.M, st, v.rP, v.z);
-        launchRho<T, P>(s, v, st, totals);
+        launchRho<T, P>(s, v, st, totals, true);
     }
 
     template <typename T, typename P>
@@ -709,7 +710,7 @@ namespace nsfem
                        pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, gate, totals);
                    });
         sgsSweeps<P>(s, v.M, st, v.rP, v.z);
-        launchRho<T, P>(s, v, st, totals);
+        launchRho<T, P>(s, v, st, totals, false);
         NSFEM_HIP(hipGetLastError());
         return iterate<T, P>(s, v, st, maxIters, sync, usePrediction);
     }
@@ -954,6 +955,7 @@ namespace nsfem
         const double n0 = colorStats_.empty() ? 0.0 : colorStats_[0].rows;
         double bytes = 0;
         double launches = 1;
+        bool graphTiming = false;
         std::function<void()> launch;
         auto run = [&](auto tagT, auto tagP)
         {
@@ -1042,8 +1044,8 @@ namespace nsfem
             }
             else if (which == "finalize")
             {
-                // one of the three per-iteration one-block finalize kernels
-                launches = 3;
+                // one of the two per-iteration one-block finalize kernels
+                launches = 2;
                 bytes = static_cast<double>(numBlocks_) * nch * 8.0;
                 launch = [=, this]()
                 {
@@ -1053,9 +1055,19 @@ namespace nsfem
                         pcgDirectionFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, st, totals);
                 };
             }
+            else if (which == "iteration" || which == "iteration_graph")
+            {
+                // one whole PCG iteration (all its launches); "iteration_graph"
+                // replays the repetitions as one captured graph, as the solver
+                // does, "iteration" queues them eagerly behind the delay
+                graphTiming = which == "iteration_graph";
+                bytes = 0;
+                launch = [=, this]() { pcgIteration<T, P>(s, v, st); };
+            }
             else
             {
-                throw std::invalid_argument("unknown kernel '" + which + "' (direction, sweep, rho, update, finalize)");
+                throw std::invalid_argument("unknown kernel '" + which +
+                                            "' (direction, sweep, rho, update, finalize, iteration, iteration_graph)");
             }
         };
         if (opt_.precision == SolverPrecision::F64)
@@ -1081,15 +1093,36 @@ namespace nsfem
         // The stream idles on a 20 ms delay kernel while the host queues the
         // repetitions, so they run back to back and the events bracket GPU
         // execution only (the figures match a kernel trace of the same run).
+        hipGraphExec_t exec = nullptr;
+        if (graphTiming)
+        {
+            hipGraph_t graph = nullptr;
+            NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < reps; i++)
+            {
+                launch();
+            }
+            NSFEM_HIP(hipStreamEndCapture(s, &graph));
+            NSFEM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            NSFEM_HIP(hipGraphDestroy(graph));
+            NSFEM_HIP(hipGraphLaunch(exec, s));
+        }
         NSFEM_HIP(hipStreamSynchronize(s));
         delayKernel<<<1, 64, 0, s>>>(2000000);
         hipEvent_t e0, e1;
         NSFEM_HIP(hipEventCreate(&e0));
         NSFEM_HIP(hipEventCreate(&e1));
         NSFEM_HIP(hipEventRecord(e0, s));
-        for (int i = 0; i < reps; i++)
+        if (graphTiming)
         {
-            launch();
+            NSFEM_HIP(hipGraphLaunch(exec, s));
+        }
+        else
+        {
+            for (int i = 0; i < reps; i++)
+            {
+                launch();
+            }
         }
         NSFEM_HIP(hipEventRecord(e1, s));
         NSFEM_HIP(hipEventSynchronize(e1));
@@ -1097,6 +1130,10 @@ namespace nsfem
         NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
         NSFEM_HIP(hipEventDestroy(e0));
         NSFEM_HIP(hipEventDestroy(e1));
+        if (exec)
+        {
+            NSFEM_HIP(hipGraphExecDestroy(exec));
+        }
         if (bytesPerLaunch)
         {
             *bytesPerLaunch = bytes;

@@ -83,7 +83,7 @@ namespace nsfem
     {
         DeviceArray<double> scal;    // 8 arrays of S*NCH doubles
         DeviceArray<int> ints;       // active, iters (S*NCH each), flags[4]
-        DeviceArray<double> partials;
+        DeviceArray<double> partials, partials2;
         DeviceArray<double> totals;
         SolveState view(int k, double tol2);
         void init(int numBlocks, int numSystems, int numCh);
@@ -197,7 +197,7 @@ namespace nsfem
         template <typename P>
         void sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z);
         template <typename T, typename P>
-        void launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals);
+        void launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate);
         void runMixedRestart(hipStream_t s);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
         template <typename T>
