@@ -351,8 +351,7 @@ namespace nsfem
 
     struct BlockMap
     {
-        const int2 * blocks;       // [rowBegin, rowEnd) per block, colour-major
-        const int * blockSystem;   // system of each block
+        const int4 * blocks;       // per block: rowBegin, rowEnd, system, 0 (colour-major)
         const int * sysBlockStart; // numSystems + 1
         const int * sysBlocks;     // blocks grouped by system
         const int * systemRows;    // rows per system (for RMS norms)
@@ -512,22 +511,32 @@ namespace nsfem
     __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st,
                                                               const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
     {
-        if (st.flags[0])
-        {
-            return;
-        }
+        // Latency: the block's rows, its system and the done flag are loaded
+        // together, and the first slab's row bounds are in flight before the
+        // early-exit test resolves (the test reads them, so they are issued
+        // first); the chain to the first gather is three loads deep.
         const int blk = blockOffset + logicalBlock(blockIdx.x, gridDim.x);
-        if (!systemActive<NCH>(st, bm.blockSystem[blk]))
+        const int4 range = bm.blocks[blk];
+        const int sys = range.z;
+        const int done = st.flags[0];
+        int row = range.x + static_cast<int>(threadIdx.x) / LANES;
+        bool valid = row < range.y;
+        int j0 = valid ? A.rowStart[row] : 0;
+        int j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
+        const bool act = systemActive<NCH>(st, sys);
+        if (done | !act | (j1 < j0))
         {
             return;
         }
-        const int2 range = bm.blocks[blk];
         for (int base = range.x; base < range.y; base += kBlock / LANES)
         {
-            const int row = base + static_cast<int>(threadIdx.x) / LANES;
-            const bool valid = row < range.y;
-            const int j0 = valid ? A.rowStart[row] : 0;
-            const int j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
+            if (base != range.x)
+            {
+                row = base + static_cast<int>(threadIdx.x) / LANES;
+                valid = row < range.y;
+                j0 = valid ? A.rowStart[row] : 0;
+                j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
+            }
             T acc[NCH];
             rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
             if (valid && groupLeader<LANES>())
@@ -562,9 +571,9 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        if (systemActive<NCH>(st, bm.blockSystem[lb]))
+        const int4 range = bm.blocks[lb];
+        if (systemActive<NCH>(st, range.z))
         {
-            const int2 range = bm.blocks[lb];
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
                 const int row = base + static_cast<int>(threadIdx.x) / LANES;
@@ -608,7 +617,7 @@ namespace nsfem
         {
             part[k] = 0;
         }
-        const int2 range = bm.blocks[lb];
+        const int4 range = bm.blocks[lb];
         for (int base = range.x; base < range.y; base += kBlock / LANES)
         {
             const int row = base + static_cast<int>(threadIdx.x) / LANES;
@@ -667,7 +676,17 @@ namespace nsfem
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
-        if (st.flags[0])
+        // independent loads together; the first slab's row bounds in flight
+        // before the early-exit test (which reads them)
+        const int4 range = bm.blocks[lb];
+        const int sys = range.z;
+        const int done = st.flags[0];
+        const bool c0 = range.x < color0End; // blocks never straddle colours
+        int row = range.x + static_cast<int>(threadIdx.x) / LANES;
+        bool valid = row < range.y;
+        int j0 = (c0 && valid) ? M.rowStart[row] : 0;
+        int j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+        if (done || j1 < j0)
         {
             return;
         }
@@ -677,19 +696,20 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        if (systemActive<NCH>(st, bm.blockSystem[lb]))
+        if (systemActive<NCH>(st, sys))
         {
-            const int2 range = bm.blocks[lb];
-            const bool c0 = range.x < color0End; // blocks never straddle colours
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
-                const int row = base + static_cast<int>(threadIdx.x) / LANES;
-                const bool valid = row < range.y;
+                if (base != range.x)
+                {
+                    row = base + static_cast<int>(threadIdx.x) / LANES;
+                    valid = row < range.y;
+                    j0 = (c0 && valid) ? M.rowStart[row] : 0;
+                    j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+                }
                 P acc[NCH];
                 if (c0)
                 {
-                    const int j0 = valid ? M.rowStart[row] : 0;
-                    const int j1 = valid ? M.rowStart[row + 1] : 0;
                     rowDot<P, LANES, NCH>(M, j0, j1, z, M.n, acc);
                 }
                 if (valid && groupLeader<LANES>())
@@ -725,7 +745,16 @@ namespace nsfem
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
-        if (st.flags[0])
+        // independent loads together; the first slab's row bounds in flight
+        // before the early-exit test (which reads them)
+        const int4 range = bm.blocks[lb];
+        const int s = range.z;
+        const int done = st.flags[0];
+        int row = range.x + static_cast<int>(threadIdx.x) / LANES;
+        bool valid = row < range.y;
+        int j0 = valid ? A.rowStart[row] : 0;
+        int j1 = valid ? A.rowStart[row + 1] : 0;
+        if (done || j1 < j0)
         {
             return;
         }
@@ -735,7 +764,6 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int s = bm.blockSystem[lb];
         if (systemActive<NCH>(st, s))
         {
             const bool first = st.flags[1] != 0;
@@ -745,13 +773,15 @@ namespace nsfem
             {
                 beta[c] = static_cast<T>(st.beta[s * NCH + c]);
             }
-            const int2 range = bm.blocks[lb];
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
-                const int row = base + static_cast<int>(threadIdx.x) / LANES;
-                const bool valid = row < range.y;
-                const int j0 = valid ? A.rowStart[row] : 0;
-                const int j1 = valid ? A.rowStart[row + 1] : 0;
+                if (base != range.x)
+                {
+                    row = base + static_cast<int>(threadIdx.x) / LANES;
+                    valid = row < range.y;
+                    j0 = valid ? A.rowStart[row] : 0;
+                    j1 = valid ? A.rowStart[row + 1] : 0;
+                }
                 T acc[NCH];
                 rowDot<T, LANES, NCH>(A, j0, j1, z, A.n, acc);
                 if (valid && groupLeader<LANES>())
@@ -793,6 +823,9 @@ namespace nsfem
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        // the block's rows, its system and the done flag load together
+        const int4 range = bm.blocks[lb];
+        const int s = range.z;
         if (st.flags[0])
         {
             return;
@@ -803,7 +836,6 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int s = bm.blockSystem[lb];
         if (systemActive<NCH>(st, s))
         {
             T alpha[NCH];
@@ -814,7 +846,6 @@ namespace nsfem
                 alpha[c] = static_cast<T>(st.alpha[s * NCH + c]);
                 act[c] = st.active[s * NCH + c];
             }
-            const int2 range = bm.blocks[lb];
             for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
             {
 #pragma unroll
@@ -1048,7 +1079,7 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int2 range = bm.blocks[lb];
+        const int4 range = bm.blocks[lb];
         for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
         {
 #pragma unroll
@@ -1084,8 +1115,8 @@ namespace nsfem
     __global__ void __launch_bounds__(kBlock) subtractMeanKernel(int n, BlockMap bm, T * __restrict__ v, const double * __restrict__ mean)
     {
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
-        const int s = bm.blockSystem[lb];
-        const int2 range = bm.blocks[lb];
+        const int4 range = bm.blocks[lb];
+        const int s = range.z;
         for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
         {
 #pragma unroll

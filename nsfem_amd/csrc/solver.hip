@@ -378,7 +378,7 @@ namespace nsfem
         rpb = std::max(slab, rpb / slab * slab);
         rowsPerBlock_ = rpb;
 
-        std::vector<int2> blocks;
+        std::vector<int4> blocks;
         std::vector<int> blockSystem;
         colorBlockStart_.assign(numColors_ + 1, 0);
         size_t next = 0;
@@ -390,7 +390,7 @@ namespace nsfem
                 const RowChunk & ch = chunks_[next];
                 for (int b0 = ch.begin; b0 < ch.end; b0 += rpb)
                 {
-                    blocks.push_back(int2{b0, std::min(ch.end, b0 + rpb)});
+                    blocks.push_back(int4{b0, std::min(ch.end, b0 + rpb), ch.system, 0});
                     blockSystem.push_back(ch.system);
                 }
             }
@@ -414,11 +414,10 @@ namespace nsfem
                 sysBlocks[fillPos[blockSystem[b]]++] = b;
             }
         }
-        blocks_.upload(blocks.empty() ? std::vector<int2>{int2{0, 0}} : blocks);
-        blockSystem_.upload(blockSystem.empty() ? std::vector<int>{0} : blockSystem);
+        blocks_.upload(blocks.empty() ? std::vector<int4>{int4{0, 0, 0, 0}} : blocks);
         sysBlockStart_.upload(sysBlockStart);
         sysBlocks_.upload(sysBlocks.empty() ? std::vector<int>{0} : sysBlocks);
-        bm_ = BlockMap{blocks_.get(), blockSystem_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_};
+        bm_ = BlockMap{blocks_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_};
 
         // reduction scratch sized by the block count; captured graphs hold
         // the old launch shapes and buffers

@@ -228,8 +228,8 @@ namespace nsfem
         int numBlocks_ = 0;
 
         DeviceArray<int> perm_;
-        DeviceArray<int2> blocks_;
-        DeviceArray<int> blockSystem_, sysBlockStart_, sysBlocks_, systemRows_;
+        DeviceArray<int4> blocks_;
+        DeviceArray<int> sysBlockStart_, sysBlocks_, systemRows_;
         BlockMap bm_{};
 
         DeviceMcgs<double> A64_;
