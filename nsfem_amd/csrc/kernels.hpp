@@ -25,6 +25,10 @@ namespace nsfem
 {
     constexpr int kBlock = 256;
     constexpr int kWaves = kBlock / 64;
+    // One-block finalize kernels run 1024 threads: more partial sums in
+    // flight per round trip, more systems reduced side by side.
+    constexpr int kFinBlock = 1024;
+    constexpr int kFinWaves = kFinBlock / 64;
 
     template <typename T>
     __device__ __forceinline__ T fmaT(T a, T b, T c);
@@ -64,9 +68,9 @@ namespace nsfem
         return c1 > c ? static_cast<int>(c) : B0 + (l - static_cast<int>(c));
     }
 
-    // Sum of v over the block (256 threads); result valid in thread 0.
-    template <int K>
-    __device__ __forceinline__ void blockSum(double (&v)[K], double * sm /* K * kWaves */)
+    // Sum of v over the block (W wavefronts); result valid in thread 0.
+    template <int K, int W = kWaves>
+    __device__ __forceinline__ void blockSum(double (&v)[K], double * sm /* K * W */)
     {
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -83,7 +87,7 @@ namespace nsfem
 #pragma unroll
             for (int k = 0; k < K; k++)
             {
-                sm[k * kWaves + wave] = v[k];
+                sm[k * W + wave] = v[k];
             }
         }
         __syncthreads();
@@ -93,9 +97,9 @@ namespace nsfem
             for (int k = 0; k < K; k++)
             {
                 double s = 0;
-                for (int w = 0; w < kWaves; w++)
+                for (int w = 0; w < W; w++)
                 {
-                    s += sm[k * kWaves + w];
+                    s += sm[k * W + w];
                 }
                 v[k] = s;
             }
@@ -351,12 +355,17 @@ namespace nsfem
 
     struct BlockMap
     {
-        const int4 * blocks;       // per block: rowBegin, rowEnd, system, 0 (colour-major)
+        const int4 * blocks;       // per block: rowBegin, rowEnd, system, partial slot (colour-major)
         const int * sysBlockStart; // numSystems + 1
         const int * sysBlocks;     // blocks grouped by system
         const int * systemRows;    // rows per system (for RMS norms)
         int numSystems;
+        int numBlocks;
     };
+
+    // Finalize kernels stage the per-channel solver scalars in LDS while the
+    // partial sums stream in, for up to this many (system, channel) pairs.
+    constexpr int kStaged = 64;
 
     // Per (system, channel) scalars, index s*NCH + c, plus global flags.
     struct SolveState
@@ -377,9 +386,12 @@ namespace nsfem
     };
 
 
-    // Finishes a per-system reduction inside the last block: out[s*K + k] is
-    // the sum over the blocks of system s of partials[b*K + k].  The summation
-    // order depends only on the block map, so results are reproducible.
+    // Finishes a per-system reduction in a one-block finalize kernel of
+    // kFinBlock threads: out[s*K + k] is the sum over the blocks of system s
+    // of their partials.  Producers store a block's partial at its slot (the
+    // block's position in the system-grouped order, block table entry .w), so
+    // each system's partials are contiguous: [sysBlockStart[s], sysBlockStart[s+1]).
+    // The summation order depends only on the block map: results repeat.
     template <int K>
     __device__ __forceinline__ void systemTotals(const BlockMap & bm, const double * partials, double * out, double * sm)
     {
@@ -391,11 +403,10 @@ namespace nsfem
             {
                 acc[k] = 0;
             }
-            // one system: its blocks are 0 .. nb-1 in order, no indirection;
-            // the loads of a thread are independent, so keep many in flight
-            const int nb = bm.sysBlockStart[1];
-#pragma unroll 16
-            for (int b = threadIdx.x; b < nb; b += kBlock)
+            // independent loads: keep many in flight
+            const int nb = bm.numBlocks;
+#pragma unroll 8
+            for (int b = threadIdx.x; b < nb; b += kFinBlock)
             {
 #pragma unroll
                 for (int k = 0; k < K; k++)
@@ -403,7 +414,7 @@ namespace nsfem
                     acc[k] += partials[static_cast<size_t>(b) * K + k];
                 }
             }
-            blockSum<K>(acc, sm);
+            blockSum<K, kFinWaves>(acc, sm);
             if (threadIdx.x == 0)
             {
 #pragma unroll
@@ -415,10 +426,10 @@ namespace nsfem
             __syncthreads();
             return;
         }
-        // One wavefront per system, lanes stride the system's blocks.
+        // One wavefront per system, lanes stride the system's partials.
         const int wave = threadIdx.x >> 6;
         const int lane = threadIdx.x & 63;
-        for (int s = wave; s < bm.numSystems; s += kWaves)
+        for (int s = wave; s < bm.numSystems; s += kFinWaves)
         {
             double acc[K];
 #pragma unroll
@@ -426,13 +437,14 @@ namespace nsfem
             {
                 acc[k] = 0;
             }
-            for (int i = bm.sysBlockStart[s] + lane; i < bm.sysBlockStart[s + 1]; i += 64)
+            const int i1 = bm.sysBlockStart[s + 1];
+#pragma unroll 4
+            for (int i = bm.sysBlockStart[s] + lane; i < i1; i += 64)
             {
-                const int b = bm.sysBlocks[i];
 #pragma unroll
                 for (int k = 0; k < K; k++)
                 {
-                    acc[k] += partials[static_cast<size_t>(b) * K + k];
+                    acc[k] += partials[static_cast<size_t>(i) * K + k];
                 }
             }
 #pragma unroll
@@ -596,7 +608,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials, lb);
+        storePartial<NCH>(part, st.partials, range.w);
     }
 
     // PCG set-up: r = b - A x (or r = b with x = 0), z = r / d on colour 0,
@@ -662,7 +674,7 @@ namespace nsfem
             }
         }
         blockSum<2 * NCH>(part, sm);
-        storePartial<2 * NCH>(part, st.partials, lb);
+        storePartial<2 * NCH>(part, st.partials, range.w);
     }
 
     // Backward colour 0 of symmetric GS (on the preconditioner matrix M, type
@@ -734,7 +746,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials, lb);
+        storePartial<NCH>(part, st.partials, range.w);
     }
 
     // p = z + beta p;  q = A z + beta q  (= A p);  partial sums of p.q.
@@ -811,7 +823,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials, lb);
+        storePartial<NCH>(part, st.partials, range.w);
     }
 
     // x += alpha p;  r -= alpha q;  z = r / d on colour 0 (preconditioner
@@ -876,14 +888,14 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials2, lb);
+        storePartial<NCH>(part, st.partials2, range.w);
     }
 
     // Per-system RMS residuals of gsResidualKernel, stop test and flags.
     template <int NCH>
-    __global__ void __launch_bounds__(kBlock) gsResidualFinalizeKernel(BlockMap bm, SolveState st, int iter, double * totals)
+    __global__ void __launch_bounds__(kFinBlock) gsResidualFinalizeKernel(BlockMap bm, SolveState st, int iter, double * totals)
     {
-        __shared__ double sm[NCH * kWaves];
+        __shared__ double sm[NCH * kFinWaves];
         if (st.flags[0])
         {
             return; // the producer skipped as well
@@ -935,9 +947,9 @@ namespace nsfem
     // |b|^2, |r|^2 of pcgInitKernel; resets the iteration scalars.  gate
     // (optional) switches off channels whose outer refinement has converged.
     template <int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgInitFinalizeKernel(BlockMap bm, SolveState st, const int * __restrict__ gate, double * totals)
+    __global__ void __launch_bounds__(kFinBlock) pcgInitFinalizeKernel(BlockMap bm, SolveState st, const int * __restrict__ gate, double * totals)
     {
-        __shared__ double sm[2 * NCH * kWaves];
+        __shared__ double sm[2 * NCH * kFinWaves];
         systemTotals<2 * NCH>(bm, st.partials, totals, sm);
         if (threadIdx.x == 0)
         {
@@ -980,26 +992,39 @@ namespace nsfem
     // iteration count and the convergence test of the update come first,
     // then rho and beta for the channels that go on.  (The sweeps and the
     // rho kernel in between ran on the update's result either way.)
-    __global__ void __launch_bounds__(kBlock) pcgRhoFinalizeKernel(BlockMap bm, SolveState st, double * totals, int withUpdate)
+    __global__ void __launch_bounds__(kFinBlock) pcgRhoFinalizeKernel(BlockMap bm, SolveState st, double * totals, int withUpdate)
     {
-        __shared__ double sm[NCH * kWaves];
-        if (st.flags[0])
+        __shared__ double sm[NCH * kFinWaves];
+        __shared__ double shRho[kStaged], shRef[kStaged];
+        __shared__ int shAct[kStaged], shIters[kStaged];
+        const int K = bm.numSystems * NCH;
+        const bool staged = K <= kStaged;
+        const int done = st.flags[0];
+        const int firstFlag = st.flags[1];
+        if (staged && static_cast<int>(threadIdx.x) < K)
         {
-            return; // the producer skipped as well
+            shRho[threadIdx.x] = st.rho[threadIdx.x];
+            shRef[threadIdx.x] = st.ref[threadIdx.x];
+            shAct[threadIdx.x] = st.active[threadIdx.x];
+            shIters[threadIdx.x] = st.iters[threadIdx.x];
         }
-        double * rrTotals = totals + bm.numSystems * NCH;
+        double * rrTotals = totals + K;
         if (withUpdate)
         {
             systemTotals<NCH>(bm, st.partials2, rrTotals, sm);
         }
-        systemTotals<NCH>(bm, st.partials, totals, sm);
+        systemTotals<NCH>(bm, st.partials, totals, sm); // ends with a barrier
+        if (done)
+        {
+            return; // the producers skipped as well
+        }
         if (threadIdx.x == 0)
         {
-            const bool first = st.flags[1] != 0;
+            const bool first = firstFlag != 0;
             int anyActive = 0;
-            for (int k = 0; k < bm.numSystems * NCH; k++)
+            for (int k = 0; k < K; k++)
             {
-                if (!st.active[k])
+                if (!(staged ? shAct[k] : st.active[k]))
                 {
                     continue;
                 }
@@ -1007,14 +1032,14 @@ namespace nsfem
                 {
                     const double rr = rrTotals[k];
                     st.rr[k] = rr;
-                    st.iters[k] += 1;
+                    st.iters[k] = (staged ? shIters[k] : st.iters[k]) + 1;
                     if (!(rr == rr) || rr > 1e300)
                     {
                         st.flags[2] = 1;
                         st.active[k] = 0;
                         continue;
                     }
-                    if (rr <= st.tol2 * st.ref[k])
+                    if (rr <= st.tol2 * (staged ? shRef[k] : st.ref[k]))
                     {
                         st.active[k] = 0;
                         continue;
@@ -1022,7 +1047,8 @@ namespace nsfem
                 }
                 anyActive = 1;
                 const double rhoNew = totals[k];
-                st.beta[k] = (first || st.rho[k] == 0) ? 0.0 : rhoNew / st.rho[k];
+                const double rhoOld = staged ? shRho[k] : st.rho[k];
+                st.beta[k] = (first || rhoOld == 0) ? 0.0 : rhoNew / rhoOld;
                 st.rho[k] = rhoNew;
                 if (!(rhoNew == rhoNew))
                 {
@@ -1038,20 +1064,32 @@ namespace nsfem
 
     // alpha = rho / (p.q).
     template <int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgDirectionFinalizeKernel(BlockMap bm, SolveState st, double * totals)
+    __global__ void __launch_bounds__(kFinBlock) pcgDirectionFinalizeKernel(BlockMap bm, SolveState st, double * totals)
     {
-        __shared__ double sm[NCH * kWaves];
-        if (st.flags[0])
+        __shared__ double sm[NCH * kFinWaves];
+        __shared__ double shRho[kStaged];
+        __shared__ int shAct[kStaged];
+        const int K = bm.numSystems * NCH;
+        const bool staged = K <= kStaged;
+        const int done = st.flags[0];
+        if (staged && static_cast<int>(threadIdx.x) < K)
+        {
+            shRho[threadIdx.x] = st.rho[threadIdx.x];
+            shAct[threadIdx.x] = st.active[threadIdx.x];
+        }
+        systemTotals<NCH>(bm, st.partials, totals, sm); // ends with a barrier
+        if (done)
         {
             return; // the producer skipped as well
         }
-        systemTotals<NCH>(bm, st.partials, totals, sm);
         if (threadIdx.x == 0)
         {
-            for (int k = 0; k < bm.numSystems * NCH; k++)
+            for (int k = 0; k < K; k++)
             {
                 const double pq = totals[k];
-                st.alpha[k] = (st.active[k] && pq != 0) ? st.rho[k] / pq : 0.0;
+                const double rho = staged ? shRho[k] : st.rho[k];
+                const int act = staged ? shAct[k] : st.active[k];
+                st.alpha[k] = (act && pq != 0) ? rho / pq : 0.0;
             }
             st.flags[1] = 0;
         }
@@ -1089,14 +1127,14 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, partials, lb);
+        storePartial<NCH>(part, partials, range.w);
     }
 
     // mean[s*NCH + c] from the partial sums of meanKernel.
     template <int NCH>
-    __global__ void __launch_bounds__(kBlock) meanFinalizeKernel(BlockMap bm, const double * __restrict__ partials, double * totals, double * mean)
+    __global__ void __launch_bounds__(kFinBlock) meanFinalizeKernel(BlockMap bm, const double * __restrict__ partials, double * totals, double * mean)
     {
-        __shared__ double sm[NCH * kWaves];
+        __shared__ double sm[NCH * kFinWaves];
         systemTotals<NCH>(bm, partials, totals, sm);
         if (threadIdx.x == 0)
         {

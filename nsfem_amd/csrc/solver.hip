@@ -329,6 +329,11 @@ namespace nsfem
 
         const int K = numSystems_ * opt_.numCh;
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostFlags_), 4 * sizeof(int), hipHostMallocDefault));
+        NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&pollRing_), 8 * sizeof(int), hipHostMallocDefault));
+        for (hipEvent_t & e : pollEvents_)
+        {
+            NSFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostScal_), 7 * K * sizeof(double), hipHostMallocDefault));
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostInts_), (2 * K + 4) * sizeof(int), hipHostMallocDefault));
         lastMse_.assign(K, -1.0);
@@ -350,6 +355,17 @@ namespace nsfem
         if (hostFlags_)
         {
             (void)hipHostFree(hostFlags_);
+        }
+        if (pollRing_)
+        {
+            (void)hipHostFree(pollRing_);
+        }
+        for (hipEvent_t e : pollEvents_)
+        {
+            if (e)
+            {
+                (void)hipEventDestroy(e);
+            }
         }
         if (hostScal_)
         {
@@ -390,7 +406,7 @@ namespace nsfem
                 const RowChunk & ch = chunks_[next];
                 for (int b0 = ch.begin; b0 < ch.end; b0 += rpb)
                 {
-                    blocks.push_back(int4{b0, std::min(ch.end, b0 + rpb), ch.system, 0});
+                    blocks.push_back(int4{b0, std::min(ch.end, b0 + rpb), ch.system, -1});
                     blockSystem.push_back(ch.system);
                 }
             }
@@ -411,13 +427,14 @@ namespace nsfem
             std::vector<int> fillPos(sysBlockStart.begin(), sysBlockStart.end() - 1);
             for (int b = 0; b < numBlocks_; b++)
             {
-                sysBlocks[fillPos[blockSystem[b]]++] = b;
+                sysBlocks[fillPos[blockSystem[b]]] = b;
+                blocks[b].w = fillPos[blockSystem[b]]++; // partial slot
             }
         }
         blocks_.upload(blocks.empty() ? std::vector<int4>{int4{0, 0, 0, 0}} : blocks);
         sysBlockStart_.upload(sysBlockStart);
         sysBlocks_.upload(sysBlocks.empty() ? std::vector<int>{0} : sysBlocks);
-        bm_ = BlockMap{blocks_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_};
+        bm_ = BlockMap{blocks_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_, numBlocks_};
 
         // reduction scratch sized by the block count; captured graphs hold
         // the old launch shapes and buffers
@@ -574,7 +591,7 @@ namespace nsfem
                 dispatchLC(lanes_, nch, [&]<int L, int C>()
                            {
                                gsResidualKernel<T, L, C><<<numBlocks_, kBlock, 0, s>>>(Av, bm_, st, b, x);
-                               gsResidualFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, iter, inner_.totals.get());
+                               gsResidualFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, iter, inner_.totals.get());
                            });
             }
             NSFEM_HIP(hipGetLastError());
@@ -668,7 +685,7 @@ namespace nsfem
                    {
                        pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_,
                                                                               colorBlockStart_[1]);
-                       pcgRhoFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
+                       pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
                    });
     }
 
@@ -680,7 +697,7 @@ namespace nsfem
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
                        pcgDirectionKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.z, v.p, v.q);
-                       pcgDirectionFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                       pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
                    });
         if (nch == 1)
         {
@@ -706,7 +723,7 @@ namespace nsfem
                    {
                        pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
                                                                               color0End_, zeroGuess ? 1 : 0);
-                       pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, st, gate, totals);
+                       pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, gate, totals);
                    });
         sgsSweeps<P>(s, v.M, st, v.rP, v.z);
         launchRho<T, P>(s, v, st, totals, false);
@@ -752,10 +769,15 @@ namespace nsfem
             g.k = chunk;
             g.tol2 = st.tol2;
         }
-        // The first host poll comes one chunk before the iteration count of
-        // the previous solve on this path (warm-started solves repeat it).
-        const int firstPoll = usePrediction ? std::max(chunk, predictedIters - chunk) : chunk;
+        // Lagged convergence polls: after each chunk an asynchronous copy of
+        // the flags into pinned host memory and an event are queued; before
+        // the next chunk is queued the host waits for the previous chunk's
+        // copy, which has finished by the time the current chunk runs, so the
+        // GPU never drains and at most one chunk runs past convergence (its
+        // kernels exit at once).
         int launched = 0;
+        int slot = 0;
+        bool pending[2] = {false, false};
         while (launched < maxIters)
         {
             const int k = std::min(chunk, maxIters - launched);
@@ -772,14 +794,29 @@ namespace nsfem
             }
             NSFEM_HIP(hipGetLastError());
             launched += k;
-            if (sync && launched >= firstPoll && pollDone(s, ds))
+            if (!sync)
             {
-                break;
+                continue;
             }
+            NSFEM_HIP(hipMemcpyAsync(pollRing_ + 4 * slot, st.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+            NSFEM_HIP(hipEventRecord(pollEvents_[slot], s));
+            pending[slot] = true;
+            const int prev = slot ^ 1;
+            if (pending[prev])
+            {
+                NSFEM_HIP(hipEventSynchronize(pollEvents_[prev]));
+                pending[prev] = false;
+                if (pollRing_[4 * prev] != 0)
+                {
+                    break; // converged before the chunk just queued
+                }
+            }
+            slot = prev;
         }
+        (void)ds;
         if (usePrediction)
         {
-            predictedIters = launched;
+            predictedIters = launched; // only sizes the chunks of the next solve
         }
         return launched;
     }
@@ -806,7 +843,7 @@ namespace nsfem
                        {
                            pcgInitKernel<double, double, L, C><<<numBlocks_, kBlock, 0, s>>>(v64.A, bm_, outer, v64.b, v64.x, v64.r, v64.r,
                                                                                              v64.z, v64.M.invDiag, color0End_, 0);
-                           pcgInitFinalizeKernel<C><<<1, kBlock, 0, s>>>(bm_, outer, nullptr, outer_.totals.get());
+                           pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, outer, nullptr, outer_.totals.get());
                        });
             NSFEM_HIP(hipGetLastError());
             if (pollDone(s, outer_) || outerIt >= opt_.maxOuter)
@@ -916,13 +953,13 @@ namespace nsfem
         if (opt_.numCh == 1)
         {
             meanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
-            meanFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
+            meanFinalizeKernel<1><<<1, kFinBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
             subtractMeanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
         }
         else
         {
             meanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
-            meanFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
+            meanFinalizeKernel<2><<<1, kFinBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
             subtractMeanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
         }
         NSFEM_HIP(hipGetLastError());
@@ -1049,9 +1086,9 @@ namespace nsfem
                 launch = [=, this]()
                 {
                     if (nch == 1)
-                        pcgDirectionFinalizeKernel<1><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                        pcgDirectionFinalizeKernel<1><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
                     else
-                        pcgDirectionFinalizeKernel<2><<<1, kBlock, 0, s>>>(bm_, st, totals);
+                        pcgDirectionFinalizeKernel<2><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
                 };
             }
             else if (which == "iteration" || which == "iteration_graph")

@@ -243,6 +243,8 @@ namespace nsfem
         DeviceState inner_, outer_;
         DeviceArray<double> means_;
         int * hostFlags_ = nullptr; // pinned
+        int * pollRing_ = nullptr;  // pinned, two slots of the solve flags
+        hipEvent_t pollEvents_[2] = {nullptr, nullptr};
         double * hostScal_ = nullptr;
         int * hostInts_ = nullptr;
 
