@@ -27,7 +27,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(nodes, seed):
+def build_workload(nodes, seed, gpu_assembly=False):
     import numpy as np
     from nsfem_amd import fem, mesher
     t0 = time.perf_counter()
@@ -37,10 +37,10 @@ def build_workload(nodes, seed):
     diameter = 2 * m["radius"]
     nu = (2.0 / 3.0) * peak * diameter / re
     cond = fem.DfgConditions(viscosity=nu, peakVelocity=peak, channelHeight=m["height"], profileScale=4.0)
-    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), cond)
+    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), cond, gpu_assembly=gpu_assembly)
     t2 = time.perf_counter()
     log(f"mesh {t1 - t0:.1f}s (V={len(m['xy'])}, E={len(m['triangles'])}), context {t2 - t1:.1f}s "
-        f"(N={ctx.numVelocityNodes})")
+        f"({'GPU' if gpu_assembly else 'host'} assembly, N={ctx.numVelocityNodes})")
     return m, cond, ctx
 
 
@@ -246,6 +246,7 @@ def main():
                     help="no time steps: build the workload and time each kernel kind (--kernel-reps launches); "
                          "for hardware-counter passes, which need few dispatches")
     ap.add_argument("--kernel-reps", type=int, default=30)
+    ap.add_argument("--host-assembly", action="store_true", help="assemble the matrices on the host instead of the GPU")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 500k workload: this many uneven instances (10k-250k pressure DOFs) in one batch")
     ap.add_argument("--eager", action="store_true",
@@ -277,7 +278,7 @@ def main():
         del a
 
     t_setup = time.perf_counter()
-    m, cond, ctx = build_workload(args.nodes, args.seed)
+    m, cond, ctx = build_workload(args.nodes, args.seed, gpu_assembly=not args.host_assembly)
     u0 = initial_velocity(ctx, cond, 1)
     target = 1e-10
     vl, vr, pl, pr = (int(v) for v in args.shape.split(","))

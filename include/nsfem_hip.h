@@ -201,13 +201,20 @@ typedef struct nsfem_dfg_conditions
 
 typedef struct nsfem_context nsfem_context;
 int nsfem_context_build(const nsfem_mesh * mesh, const nsfem_dfg_conditions * cond, int numThreads, nsfem_context ** out);
+/* flags: NSFEM_CONTEXT_GPU_ASSEMBLY computes the matrix values on the GPU
+ * (patterns and per-entry slot lists on the host): the values part of
+ * buildChorinCsrMatrices (cpp/libs/Fem/src/chorinCsr.cpp:296-502), bit-identical
+ * to the host assembly. */
+#define NSFEM_CONTEXT_GPU_ASSEMBLY 1
+int nsfem_context_build_ex(const nsfem_mesh * mesh, const nsfem_dfg_conditions * cond, int numThreads, int flags,
+                           nsfem_context ** out);
 /* A batch of independent instances (different meshes and conditions) as one
  * block-diagonal context: velocity [ux of all; uy of all], pressure and
  * internal-pressure vectors concatenated in instance order; the Chorin solvers
  * keep one system per instance (own colouring, own convergence).  No
  * counterpart in the reference (one mesh per run there). */
 int nsfem_context_build_batch(const nsfem_mesh * const * meshes, const nsfem_dfg_conditions * conds, int count, int numThreads,
-                              nsfem_context ** out);
+                              int flags, nsfem_context ** out);
 int nsfem_context_num_systems(const nsfem_context * c, int * out);
 /* Per instance: offset into one velocity component and into the pressure vector. */
 int nsfem_context_system_offsets(const nsfem_context * c, int * velocityOffsets, int * pressureOffsets);

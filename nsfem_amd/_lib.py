@@ -18,6 +18,7 @@ NSFEM_ENONFINITE = 4
 
 F32 = 0
 F64 = 1
+NSFEM_CONTEXT_GPU_ASSEMBLY = 1
 
 MAT_VELOCITY_MASS = 0
 MAT_VELOCITY_STIFFNESS = 1
@@ -136,7 +137,8 @@ _SIGNATURES = {
     "nsfem_mesh_arrays": [_vp, _vp, _vp, _vp],
     "nsfem_mesh_group_name": [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
     "nsfem_context_build": [_vp, ctypes.POINTER(DfgConditions), ctypes.c_int, ctypes.POINTER(_vp)],
-    "nsfem_context_build_batch": [_vp, ctypes.POINTER(DfgConditions), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+    "nsfem_context_build_ex": [_vp, ctypes.POINTER(DfgConditions), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+    "nsfem_context_build_batch": [_vp, ctypes.POINTER(DfgConditions), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
     "nsfem_context_num_systems": [_vp, _c_int_p],
     "nsfem_context_system_offsets": [_vp, _vp, _vp],
     "nsfem_context_destroy": [_vp],

@@ -13,11 +13,14 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnsfem_hip.so")
 
-HIP_SOURCES = ["ops.hip", "solver.hip", "chorin.hip", "capi.hip"]
+HIP_SOURCES = ["ops.hip", "solver.hip", "chorin.hip", "capi.hip", "assembly.hip"]
+# The GPU assembly evaluates the element integrals the host assembly shares
+# with it; without FP contraction both run the same IEEE operations.
+HIP_EXTRA_FLAGS = {"assembly.hip": ["-ffp-contract=off"]}
 CPP_SOURCES = ["host/mesh.cpp", "host/csr.cpp", "host/fem.cpp", "host/context.cpp"]
 APPS = ["navier_stokes", "solver_test"]
 HEADERS = ["common.hpp", "kernels.hpp", "ops.hpp", "solver.hpp", "chorin.hpp", "host/mesh.hpp", "host/csr.hpp",
-           "host/fem.hpp", "host/context.hpp"]
+           "host/fem.hpp", "host/context.hpp", "host/element_math.hpp", "assembly.hpp"]
 
 ARCH = os.environ.get("NSFEM_OFFLOAD_ARCH", "gfx950")
 
@@ -57,7 +60,7 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
-        tasks.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        tasks.append([hipcc, *HIPCC_FLAGS, *HIP_EXTRA_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in CPP_SOURCES:
         obj = os.path.join(BUILD, src.replace("/", "_") + ".o")

@@ -808,8 +808,15 @@ extern "C"
 
     int nsfem_context_build(const nsfem_mesh * mesh, const nsfem_dfg_conditions * cond, int numThreads, nsfem_context ** out)
     {
+        return nsfem_context_build_ex(mesh, cond, numThreads, 0, out);
+    }
+
+    int nsfem_context_build_ex(const nsfem_mesh * mesh, const nsfem_dfg_conditions * cond, int numThreads, int flags,
+                               nsfem_context ** out)
+    {
         return guarded([&]
                        {
+            if (flags & ~NSFEM_CONTEXT_GPU_ASSEMBLY) throw std::invalid_argument("unknown context flags");
             need(mesh, "mesh");
             need(cond, "cond");
             need(out, "out");
@@ -822,15 +829,17 @@ extern "C"
             auto h = std::make_unique<nsfem_context>();
             h->velocityMesh = createMesh(mesh->tri, ElementType::P2);
             h->pressureMesh = createMesh(mesh->tri, ElementType::P1);
-            h->ctx = buildChorinContext(h->velocityMesh, h->pressureMesh, c, numThreads > 0 ? numThreads : defaultThreads());
+            h->ctx = buildChorinContext(h->velocityMesh, h->pressureMesh, c, numThreads > 0 ? numThreads : defaultThreads(),
+                                        (flags & NSFEM_CONTEXT_GPU_ASSEMBLY) != 0);
             *out = h.release(); });
     }
 
     int nsfem_context_build_batch(const nsfem_mesh * const * meshes, const nsfem_dfg_conditions * conds, int count, int numThreads,
-                                  nsfem_context ** out)
+                                  int flags, nsfem_context ** out)
     {
         return guarded([&]
                        {
+            if (flags & ~NSFEM_CONTEXT_GPU_ASSEMBLY) throw std::invalid_argument("unknown context flags");
             need(meshes, "meshes");
             need(conds, "conds");
             need(out, "out");
@@ -845,7 +854,7 @@ extern "C"
                 DfgConditions c{conds[k].viscosity, conds[k].peakVelocity, conds[k].channelHeight, conds[k].profileScale};
                 ConcreteMesh vm = createMesh(meshes[k]->tri, ElementType::P2);
                 ConcreteMesh pm = createMesh(meshes[k]->tri, ElementType::P1);
-                parts.push_back(buildChorinContext(vm, pm, c, nt));
+                parts.push_back(buildChorinContext(vm, pm, c, nt, (flags & NSFEM_CONTEXT_GPU_ASSEMBLY) != 0));
             }
             auto h = std::make_unique<nsfem_context>();
             h->ctx = concatenateContexts(parts);

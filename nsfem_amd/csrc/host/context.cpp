@@ -1,5 +1,7 @@
 #include "context.hpp"
 
+#include "../assembly.hpp"
+
 #include <algorithm>
 #include <stdexcept>
 
@@ -62,7 +64,8 @@ namespace nsfem
         return result;
     }
 
-    ChorinContext buildChorinContext(const ConcreteMesh & vm, const ConcreteMesh & pm, const DfgConditions & cond, int nThreads)
+    ChorinContext buildChorinContext(const ConcreteMesh & vm, const ConcreteMesh & pm, const DfgConditions & cond, int nThreads,
+                                     bool gpuAssembly)
     {
         if (vm.numElements != pm.numElements || vm.groups.size() != pm.groups.size())
         {
@@ -99,14 +102,28 @@ namespace nsfem
             throw std::invalid_argument("No internal pressure nodes");
         }
 
-        auto mats = buildChorinMatrices(vm, pm, nThreads);
-        ctx.velocityMass = std::move(mats.velocityMass);
-        ctx.velocityStiffness = std::move(mats.velocityStiffness);
-        ctx.pressureStiffness = std::move(mats.pressureStiffness);
-        ctx.velocityPressureDiv = std::move(mats.velocityPressureDiv);
-        ctx.pressureVelocityDiv = std::move(mats.pressureVelocityDiv);
+        if (gpuAssembly)
+        {
+            AssemblyTopology topo = buildAssemblyTopology(vm, pm, nThreads);
+            auto mats = assembleChorinMatricesGpu(vm, pm, topo);
+            ctx.velocityMass = std::move(mats.velocityMass);
+            ctx.velocityStiffness = std::move(mats.velocityStiffness);
+            ctx.pressureStiffness = std::move(mats.pressureStiffness);
+            ctx.velocityPressureDiv = std::move(mats.velocityPressureDiv);
+            ctx.pressureVelocityDiv = std::move(mats.pressureVelocityDiv);
+            ctx.convection = buildConvectionElementData(vm, ctx.velocityStiffness, &topo.velocitySlots);
+        }
+        else
+        {
+            auto mats = buildChorinMatrices(vm, pm, nThreads);
+            ctx.velocityMass = std::move(mats.velocityMass);
+            ctx.velocityStiffness = std::move(mats.velocityStiffness);
+            ctx.pressureStiffness = std::move(mats.pressureStiffness);
+            ctx.velocityPressureDiv = std::move(mats.velocityPressureDiv);
+            ctx.pressureVelocityDiv = std::move(mats.pressureVelocityDiv);
+            ctx.convection = buildConvectionElementData(vm, ctx.velocityStiffness);
+        }
         ctx.pressureStiffnessInternal = ctx.pressureStiffness.slice(ctx.internalPressureNodes, ctx.internalPressureNodes);
-        ctx.convection = buildConvectionElementData(vm, ctx.velocityStiffness);
         return ctx;
     }
 

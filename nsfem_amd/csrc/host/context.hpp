@@ -69,8 +69,11 @@ namespace nsfem
 
     std::vector<int> extractInternalNodes(int numNodes, const std::vector<DirichletNode> & sorted);
 
+    // gpuAssembly: the matrix values (and the convection slot lists) come
+    // from assembleChorinMatricesGpu over host-built patterns; identical
+    // values to the host assembly.
     ChorinContext buildChorinContext(const ConcreteMesh & velocityMesh, const ConcreteMesh & pressureMesh,
-                                     const DfgConditions & cond, int nThreads);
+                                     const DfgConditions & cond, int nThreads, bool gpuAssembly = false);
 
     // Concatenates independent instances (different meshes and conditions)
     // into one block-diagonal context: velocity planar over the whole batch

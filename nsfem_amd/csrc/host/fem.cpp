@@ -1,5 +1,7 @@
 #include "fem.hpp"
 
+#include "element_math.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <functional>
@@ -117,51 +119,35 @@ namespace nsfem
         return g;
     }
 
+    namespace
+    {
+        elem::Geometry elementGeometry(const AffineMap & t)
+        {
+            return elem::geometry(t.m[0][0], t.m[0][1], t.m[1][0], t.m[1][1]);
+        }
+    } // namespace
+
     void localMass(const AffineMap & t, int nDof, double * dst)
     {
-        const ElementGeometry g = geometryOf(t);
-        const double absDet = std::abs(g.det);
-        double phi[6];
-        std::fill(dst, dst + nDof * nDof, 0.0);
-        for (const auto & q : quadratureDeg4())
+        if (nDof == 3)
         {
-            shapeOf(nDof, q.x, q.y, phi);
-            for (int i = 0; i < nDof; i++)
-            {
-                for (int j = 0; j < nDof; j++)
-                {
-                    dst[i * nDof + j] += phi[i] * phi[j] * q.w;
-                }
-            }
+            elem::mass<3>(elementGeometry(t), dst);
         }
-        for (int i = 0; i < nDof * nDof; i++)
+        else
         {
-            dst[i] *= absDet;
+            elem::mass<6>(elementGeometry(t), dst);
         }
     }
 
     void localStiffness(const AffineMap & t, int nDof, double * dst)
     {
-        const ElementGeometry g = geometryOf(t);
-        const double invAbsDet = 1.0 / std::abs(g.det);
-        // BtB = B^T B
-        const double a00 = g.b00 * g.b00 + g.b10 * g.b10;
-        const double a01 = g.b00 * g.b01 + g.b10 * g.b11;
-        const double a11 = g.b01 * g.b01 + g.b11 * g.b11;
-        double gx[6], gy[6];
-        std::fill(dst, dst + nDof * nDof, 0.0);
-        for (const auto & q : quadratureDeg4())
+        if (nDof == 3)
         {
-            gradOf(nDof, q.x, q.y, gx, gy);
-            const double w = q.w * invAbsDet;
-            for (int i = 0; i < nDof; i++)
-            {
-                for (int j = 0; j < nDof; j++)
-                {
-                    const double v = gx[i] * (a00 * gx[j] + a01 * gy[j]) + gy[i] * (a01 * gx[j] + a11 * gy[j]);
-                    dst[i * nDof + j] += v * w;
-                }
-            }
+            elem::stiffness<3>(elementGeometry(t), dst);
+        }
+        else
+        {
+            elem::stiffness<6>(elementGeometry(t), dst);
         }
     }
 
@@ -197,28 +183,13 @@ namespace nsfem
 
     void localDivergence(const AffineMap & t, bool swap, double * dstX, double * dstY)
     {
-        const ElementGeometry g = geometryOf(t);
-        const double sgn = g.det < 0 ? -1.0 : 1.0;
-        const int dofM = swap ? 3 : 6; // differentiated element
-        const int dofS = swap ? 6 : 3; // test element
-        double phi[6], gx[6], gy[6];
-        std::fill(dstX, dstX + dofS * dofM, 0.0);
-        std::fill(dstY, dstY + dofS * dofM, 0.0);
-        for (const auto & q : quadratureDeg4())
+        if (swap)
         {
-            shapeOf(dofS, q.x, q.y, phi);
-            gradOf(dofM, q.x, q.y, gx, gy);
-            const double w = q.w * sgn;
-            for (int r = 0; r < dofS; r++)
-            {
-                for (int c = 0; c < dofM; c++)
-                {
-                    const double ggx = g.b00 * gx[c] + g.b01 * gy[c];
-                    const double ggy = g.b10 * gx[c] + g.b11 * gy[c];
-                    dstX[r * dofM + c] += ggx * phi[r] * w;
-                    dstY[r * dofM + c] += ggy * phi[r] * w;
-                }
-            }
+            elem::divergence<true>(elementGeometry(t), dstX, dstY);
+        }
+        else
+        {
+            elem::divergence<false>(elementGeometry(t), dstX, dstY);
         }
     }
 
@@ -518,7 +489,159 @@ namespace nsfem
             (void)E; });
     }
 
-    ConvectionElementData buildConvectionElementData(const ConcreteMesh & vm, const CsrD & pattern)
+    namespace
+    {
+        // Row-parallel pattern + slot-list builder: rowFn(r, pairs) appends
+        // (column, slot) pairs; a row's entries are its distinct columns and
+        // each entry's slots come out in ascending slot order (elements are
+        // visited in ascending order, one slot per element and entry).
+        using SlotRowFn = std::function<void(int, std::vector<std::pair<int, int>> &)>;
+
+        void buildRowsWithSlots(int rows, int cols, int nThreads, const SlotRowFn & rowFn, CsrD & pattern, SlotList & slots)
+        {
+            nThreads = std::max(1, std::min(nThreads, rows));
+            struct Part
+            {
+                std::vector<int> rowCounts, column, entryCounts, slot;
+            };
+            std::vector<Part> parts(nThreads);
+            std::vector<std::exception_ptr> errors(nThreads);
+            std::vector<std::thread> threads;
+            for (int t = 0; t < nThreads; t++)
+            {
+                threads.emplace_back([&, t]
+                                     {
+                    try
+                    {
+                        const int r0 = static_cast<int>(static_cast<long long>(rows) * t / nThreads);
+                        const int r1 = static_cast<int>(static_cast<long long>(rows) * (t + 1) / nThreads);
+                        Part & p = parts[t];
+                        std::vector<std::pair<int, int>> pairs;
+                        for (int r = r0; r < r1; r++)
+                        {
+                            pairs.clear();
+                            rowFn(r, pairs);
+                            std::sort(pairs.begin(), pairs.end());
+                            int count = 0;
+                            for (size_t i = 0; i < pairs.size(); i++)
+                            {
+                                if (pairs[i].first < 0 || pairs[i].first >= cols)
+                                {
+                                    throw std::runtime_error("assembly: column out of range");
+                                }
+                                if (i == 0 || pairs[i].first != pairs[i - 1].first)
+                                {
+                                    p.column.push_back(pairs[i].first);
+                                    p.entryCounts.push_back(0);
+                                    count++;
+                                }
+                                p.entryCounts.back()++;
+                                p.slot.push_back(pairs[i].second);
+                            }
+                            p.rowCounts.push_back(count);
+                        }
+                    }
+                    catch (...)
+                    {
+                        errors[t] = std::current_exception();
+                    } });
+            }
+            for (auto & th : threads)
+            {
+                th.join();
+            }
+            for (auto & e : errors)
+            {
+                if (e)
+                {
+                    std::rethrow_exception(e);
+                }
+            }
+            pattern = CsrD{};
+            pattern.rows = rows;
+            pattern.cols = cols;
+            pattern.rowStart.assign(rows + 1, 0);
+            slots.start.assign(1, 0);
+            int r = 0;
+            for (auto & p : parts)
+            {
+                for (int c : p.rowCounts)
+                {
+                    pattern.rowStart[r + 1] = pattern.rowStart[r] + c;
+                    r++;
+                }
+                pattern.column.insert(pattern.column.end(), p.column.begin(), p.column.end());
+                for (int c : p.entryCounts)
+                {
+                    slots.start.push_back(slots.start.back() + c);
+                }
+                slots.slot.insert(slots.slot.end(), p.slot.begin(), p.slot.end());
+                p = Part{};
+            }
+        }
+    } // namespace
+
+    AssemblyTopology buildAssemblyTopology(const ConcreteMesh & vm, const ConcreteMesh & pm, int nThreads)
+    {
+        if (vm.numElements != pm.numElements || vm.type != ElementType::P2 || pm.type != ElementType::P1)
+        {
+            throw std::invalid_argument("Assembly needs a P2 velocity mesh and a P1 pressure mesh over the same elements");
+        }
+        const int N = static_cast<int>(vm.nodes.size());
+        const int V = static_cast<int>(pm.nodes.size());
+        const int E = vm.numElements;
+        if (static_cast<long long>(E) * 36 > 0x7fffffffLL || 2LL * N > 0x7fffffffLL)
+        {
+            throw std::invalid_argument("mesh too large for 32-bit slot indices");
+        }
+        const Incidence incV = buildIncidence(vm);
+        const Incidence incP = buildIncidence(pm);
+        const int * ev = vm.elements.data();
+        const int * ep = pm.elements.data();
+        AssemblyTopology t;
+        // velocity (M, K): row r, element e with r at local a: (ids6[b], e*36 + a*6 + b)
+        buildRowsWithSlots(N, N, nThreads, [&](int r, auto & pr)
+                           {
+            for (int k = incV.start[r]; k < incV.start[r + 1]; k++)
+            {
+                const int e = incV.element[k], a = incV.local[k];
+                for (int b = 0; b < 6; b++) pr.emplace_back(ev[static_cast<size_t>(e) * 6 + b], e * 36 + a * 6 + b);
+            } }, t.velocity, t.velocitySlots);
+        // pressure stiffness: (ids3[b], e*9 + a*3 + b)
+        buildRowsWithSlots(V, V, nThreads, [&](int r, auto & pr)
+                           {
+            for (int k = incP.start[r]; k < incP.start[r + 1]; k++)
+            {
+                const int e = incP.element[k], a = incP.local[k];
+                for (int b = 0; b < 3; b++) pr.emplace_back(ep[static_cast<size_t>(e) * 3 + b], e * 9 + a * 3 + b);
+            } }, t.pressure, t.pressureSlots);
+        // VPD: pressure row, (ids6[b] + comp N, e*36 + comp*18 + a*6 + b)
+        buildRowsWithSlots(V, 2 * N, nThreads, [&](int r, auto & pr)
+                           {
+            for (int k = incP.start[r]; k < incP.start[r + 1]; k++)
+            {
+                const int e = incP.element[k], a = incP.local[k];
+                for (int b = 0; b < 6; b++)
+                {
+                    const int id = ev[static_cast<size_t>(e) * 6 + b];
+                    pr.emplace_back(id, e * 36 + a * 6 + b);
+                    pr.emplace_back(id + N, e * 36 + 18 + a * 6 + b);
+                }
+            } }, t.velocityPressureDiv, t.vpdSlots);
+        // PVD: velocity row (x, then y + N), (ids3[b], e*36 + comp*18 + a*3 + b)
+        buildRowsWithSlots(2 * N, V, nThreads, [&](int row, auto & pr)
+                           {
+            const int comp = row >= N ? 1 : 0;
+            const int r = row - comp * N;
+            for (int k = incV.start[r]; k < incV.start[r + 1]; k++)
+            {
+                const int e = incV.element[k], a = incV.local[k];
+                for (int b = 0; b < 3; b++) pr.emplace_back(ep[static_cast<size_t>(e) * 3 + b], e * 36 + comp * 18 + a * 3 + b);
+            } }, t.pressureVelocityDiv, t.pvdSlots);
+        return t;
+    }
+
+    ConvectionElementData buildConvectionElementData(const ConcreteMesh & vm, const CsrD & pattern, const SlotList * slots)
     {
         ConvectionElementData d;
         const int E = vm.numElements;
@@ -536,6 +659,16 @@ namespace nsfem
             dst[4] = g.det < 0 ? -1.0 : 1.0;
         }
         const int nnz = pattern.nnz();
+        if (slots)
+        {
+            if (static_cast<int>(slots->start.size()) != nnz + 1)
+            {
+                throw std::invalid_argument("slot lists do not match the velocity pattern");
+            }
+            d.contribStart = slots->start;
+            d.contribIndex = slots->slot;
+            return d;
+        }
         d.contribStart.assign(nnz + 1, 0);
         std::vector<int> slot(static_cast<size_t>(E) * 36);
         for (int e = 0; e < E; e++)

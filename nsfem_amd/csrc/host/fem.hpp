@@ -84,7 +84,29 @@ namespace nsfem
         std::vector<int> contribStart;    // nnz + 1
         std::vector<int> contribIndex;    // 36 E: e*36 + r*6 + c, grouped by CSR entry
     };
-    ConvectionElementData buildConvectionElementData(const ConcreteMesh & velocityMesh, const CsrD & velocityPattern);
+    // slots (optional): the velocity slot lists of buildAssemblyTopology,
+    // which are exactly the contributor lists (not recomputed then).
+    struct SlotList;
+    ConvectionElementData buildConvectionElementData(const ConcreteMesh & velocityMesh, const CsrD & velocityPattern,
+                                                     const SlotList * slots = nullptr);
+
+    // Topology of the Chorin matrices for assembly elsewhere (the GPU): the
+    // CSR patterns (values empty) and, for every entry, the element-local
+    // slots that land on it, in ascending element order (the order the host
+    // assembly sums them in).  Slots: velocity e*36 + a*6 + b (P2 local a, b);
+    // pressure e*9 + a*3 + b; VPD e*36 + comp*18 + a*6 + b (P1 row a, P2 col
+    // b, comp 0 = x, 1 = y); PVD e*36 + comp*18 + a*3 + b (P2 row a, P1 col b).
+    struct SlotList
+    {
+        std::vector<int> start; // nnz + 1
+        std::vector<int> slot;
+    };
+    struct AssemblyTopology
+    {
+        CsrD velocity, pressure, velocityPressureDiv, pressureVelocityDiv; // patterns
+        SlotList velocitySlots, pressureSlots, vpdSlots, pvdSlots;
+    };
+    AssemblyTopology buildAssemblyTopology(const ConcreteMesh & velocityMesh, const ConcreteMesh & pressureMesh, int nThreads);
 
     int defaultThreads();
 } // namespace nsfem

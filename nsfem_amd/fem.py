@@ -103,18 +103,19 @@ MATRICES = {
 class ChorinContext:
     """Host-side set-up of one mesh: P2/P1 numbering, Dirichlet lists, FP64 matrices."""
 
-    def __init__(self, mesh: TriangleMesh, cond: DfgConditions, threads=0, _handle=None):
+    def __init__(self, mesh: TriangleMesh, cond: DfgConditions, threads=0, gpu_assembly=False, _handle=None):
         self.cond = cond
         if _handle is None:
             c = _lib.DfgConditions(cond.viscosity, cond.peakVelocity, cond.channelHeight, cond.profileScale)
             self._h = ctypes.c_void_p()
-            call("nsfem_context_build", mesh.handle, ctypes.byref(c), threads, ctypes.byref(self._h))
+            flags = _lib.NSFEM_CONTEXT_GPU_ASSEMBLY if gpu_assembly else 0
+            call("nsfem_context_build_ex", mesh.handle, ctypes.byref(c), threads, flags, ctypes.byref(self._h))
         else:
             self._h = _handle
         self._init_sizes()
 
     @classmethod
-    def batch(cls, meshes, conds, threads=0):
+    def batch(cls, meshes, conds, threads=0, gpu_assembly=False):
         """Independent instances as one block-diagonal context (velocity
         [ux of all; uy of all]); the Chorin solvers keep one system per instance."""
         if len(meshes) != len(conds) or not meshes:
@@ -123,7 +124,8 @@ class ChorinContext:
             _lib.DfgConditions(c.viscosity, c.peakVelocity, c.channelHeight, c.profileScale) for c in conds])
         hs = (ctypes.c_void_p * len(meshes))(*[m.handle.value for m in meshes])
         h = ctypes.c_void_p()
-        call("nsfem_context_build_batch", ctypes.cast(hs, ctypes.c_void_p), arr, len(meshes), threads, ctypes.byref(h))
+        flags = _lib.NSFEM_CONTEXT_GPU_ASSEMBLY if gpu_assembly else 0
+        call("nsfem_context_build_batch", ctypes.cast(hs, ctypes.c_void_p), arr, len(meshes), threads, flags, ctypes.byref(h))
         return cls(None, None, _handle=h)
 
     def num_systems(self):
