@@ -154,11 +154,11 @@ int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes,
                              double * launchesPerIteration);
 int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
-/* Launch shape of the solver kernels: lanes per matrix row (4, 8, 16; 0
+/* Launch shape of the solver kernels: lanes per matrix row (2, 4, 8, 16; 0
  * picks from the mean row length) and rows per workgroup (0: default).
  * reshape rebuilds the workgroup map only (colouring and matrices kept).
- * autotune times one PCG iteration's launches (pcg solvers, reps graph
- * replays per kernel) for lanes {4, 8, 16} x {1, 2, 4, 8} row slabs and keeps
+ * autotune times one PCG iteration's launches (pcg solvers, reps back-to-back
+ * replays per kernel) for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs and keeps
  * the fastest; it overwrites the internal work vectors, not rhs/sol.  No
  * reference counterpart (the reference's launch shapes are fixed). */
 int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock);
@@ -255,7 +255,7 @@ typedef struct nsfem_solver_config
     int maxIterations;
     double targetMse;
     int mseCheckInterval;
-    int lanes;        /* lanes per matrix row: 4, 8, 16 (0: from the mean row length) */
+    int lanes;        /* lanes per matrix row: 2, 4, 8, 16 (0: from the mean row length) */
     int rowsPerBlock; /* rows per workgroup of the solver kernels (0: default) */
 } nsfem_solver_config;
 

@@ -493,14 +493,31 @@ namespace nsfem
         {
             acc[c] = T(0);
         }
-        for (int j = j0 + lane; j < j1; j += LANES)
+        // two strided entries per step: their loads, then their gathers, go
+        // out together (the terms are added in the same order as one by one)
+        for (int j = j0 + lane; j < j1; j += 2 * LANES)
         {
-            const T a = A.values[j];
-            const int col = A.column[j];
+            const int jb = j + LANES;
+            const bool hasB = jb < j1;
+            const T a0 = A.values[j];
+            const int c0 = A.column[j];
+            const T a1 = hasB ? A.values[jb] : T(0);
+            const int c1 = hasB ? A.column[jb] : c0;
+            T x0[NCH], x1[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                acc[c] = fmaT(a, static_cast<T>(v[col + c * ld]), acc[c]);
+                x0[c] = static_cast<T>(v[c0 + c * ld]);
+                x1[c] = static_cast<T>(v[c1 + c * ld]);
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                acc[c] = fmaT(a0, x0[c], acc[c]);
+                if (hasB)
+                {
+                    acc[c] = fmaT(a1, x1[c], acc[c]);
+                }
             }
         }
 #pragma unroll
@@ -549,16 +566,31 @@ namespace nsfem
                 j0 = valid ? A.rowStart[row] : 0;
                 j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
             }
-            T acc[NCH];
-            rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
-            if (valid && groupLeader<LANES>())
+            // the row's own operands load alongside the gathers
+            const bool lead = valid && groupLeader<LANES>();
+            T inv = T(0), bi[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
             {
-                const T inv = A.invDiag[row];
+                bi[c] = T(0);
+            }
+            if (lead)
+            {
+                inv = A.invDiag[row];
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = row + static_cast<long>(c) * A.n;
-                    x[i] = (b[i] - acc[c]) * inv;
+                    bi[c] = b[row + static_cast<long>(c) * A.n];
+                }
+            }
+            T acc[NCH];
+            rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+            if (lead)
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    x[row + static_cast<long>(c) * A.n] = (bi[c] - acc[c]) * inv;
                 }
             }
         }
@@ -719,12 +751,36 @@ namespace nsfem
                     j0 = (c0 && valid) ? M.rowStart[row] : 0;
                     j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
                 }
+                // the row's own operands load alongside the gathers
+                const bool lead = valid && groupLeader<LANES>();
+                P inv = P(0), own[NCH];
+                double rr[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    own[c] = P(0);
+                    rr[c] = 0;
+                }
+                if (lead)
+                {
+                    if (c0)
+                    {
+                        inv = M.invDiag[row];
+                    }
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const long i = row + static_cast<long>(c) * M.n;
+                        own[c] = c0 ? rP[i] : z[i];
+                        rr[c] = static_cast<double>(r[i]);
+                    }
+                }
                 P acc[NCH];
                 if (c0)
                 {
                     rowDot<P, LANES, NCH>(M, j0, j1, z, M.n, acc);
                 }
-                if (valid && groupLeader<LANES>())
+                if (lead)
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
@@ -733,14 +789,14 @@ namespace nsfem
                         P zi;
                         if (c0)
                         {
-                            zi = (rP[i] - acc[c]) * M.invDiag[row];
+                            zi = (own[c] - acc[c]) * inv;
                             z[i] = zi;
                         }
                         else
                         {
-                            zi = z[i];
+                            zi = own[c];
                         }
-                        part[c] = fma(static_cast<double>(r[i]), static_cast<double>(zi), part[c]);
+                        part[c] = fma(rr[c], static_cast<double>(zi), part[c]);
                     }
                 }
             }
@@ -794,16 +850,39 @@ namespace nsfem
                     j0 = valid ? A.rowStart[row] : 0;
                     j1 = valid ? A.rowStart[row + 1] : 0;
                 }
+                // the row's own operands load alongside the gathers
+                const bool lead = valid && groupLeader<LANES>();
+                T dg = T(0), zr[NCH], pr[NCH], qr[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    zr[c] = pr[c] = qr[c] = T(0);
+                }
+                if (lead)
+                {
+                    dg = A.diag[row];
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const long i = row + static_cast<long>(c) * A.n;
+                        zr[c] = static_cast<T>(z[i]);
+                        if (!first)
+                        {
+                            pr[c] = p[i];
+                            qr[c] = q[i];
+                        }
+                    }
+                }
                 T acc[NCH];
                 rowDot<T, LANES, NCH>(A, j0, j1, z, A.n, acc);
-                if (valid && groupLeader<LANES>())
+                if (lead)
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
                         const long i = row + static_cast<long>(c) * A.n;
-                        const T zi = static_cast<T>(z[i]);
-                        const T az = fmaT(A.diag[row], zi, acc[c]);
+                        const T zi = zr[c];
+                        const T az = fmaT(dg, zi, acc[c]);
                         T pi, qi;
                         if (first)
                         {
@@ -812,8 +891,8 @@ namespace nsfem
                         }
                         else
                         {
-                            pi = fmaT(beta[c], p[i], zi);
-                            qi = fmaT(beta[c], q[i], az);
+                            pi = fmaT(beta[c], pr[c], zi);
+                            qi = fmaT(beta[c], qr[c], az);
                         }
                         p[i] = pi;
                         q[i] = qi;

@@ -29,10 +29,11 @@ namespace nsfem
             };
             switch (lanes)
             {
+            case 2: byC.template operator()<2>(); break;
             case 4: byC.template operator()<4>(); break;
             case 8: byC.template operator()<8>(); break;
             case 16: byC.template operator()<16>(); break;
-            default: throw std::invalid_argument("solver lanes must be 4, 8 or 16");
+            default: throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
             }
         }
 
@@ -165,9 +166,9 @@ namespace nsfem
             partitions_.push_back(layouts.back().partitionStart);
         }
         meanRow_ = n_ > 0 ? nnzOff / n_ : 0.0;
-        if (opt_.lanes != 0 && opt_.lanes != 4 && opt_.lanes != 8 && opt_.lanes != 16)
+        if (opt_.lanes != 0 && opt_.lanes != 2 && opt_.lanes != 4 && opt_.lanes != 8 && opt_.lanes != 16)
         {
-            throw std::invalid_argument("solver lanes must be 4, 8 or 16");
+            throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
         }
 
         // Colour-major global order: colour c, then system s, then the
@@ -379,9 +380,9 @@ namespace nsfem
 
     void McgsSolver::reshape(int lanes, int rowsPerBlock)
     {
-        if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16)
+        if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
         {
-            throw std::invalid_argument("solver lanes must be 4, 8 or 16");
+            throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
         }
         if (rowsPerBlock < 0)
         {
@@ -474,7 +475,7 @@ namespace nsfem
         }
         double best = 0;
         int bestLanes = lanes_, bestRows = rowsPerBlock_;
-        for (int lanes : {4, 8, 16})
+        for (int lanes : {2, 4, 8, 16})
         {
             for (int slabs : {1, 2, 4, 8})
             {

@@ -159,13 +159,13 @@ namespace nsfem
         // Sum over one PCG iteration's launches of their timed durations (ms).
         double iterationTime(int reps);
 
-        // Launch shape: lanes per matrix row (4, 8, 16; 0 picks from the mean
+        // Launch shape: lanes per matrix row (2, 4, 8, 16; 0 picks from the mean
         // row length) and rows per workgroup (0: four row slabs).  Rebuilds
         // the block map only; the colouring and matrices are kept.
         void reshape(int lanes, int rowsPerBlock);
         int lanes() const { return lanes_; }
         int rowsPerBlock() const { return rowsPerBlock_; }
-        // Times iterationTime for lanes {4, 8, 16} x {1, 2, 4, 8} row slabs per
+        // Times iterationTime for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs per
         // workgroup and keeps the fastest shape; returns (lanes, rowsPerBlock).
         // Overwrites the internal work vectors (not the io vectors).
         std::pair<int, int> autotune(int reps);
