@@ -95,7 +95,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune):
     return ch, res, (u1, p1), (u, p)
 
 
-def kernel_table(ch, res, reps=30):
+def kernel_table(ch, res, reps=30, with_iteration=True):
     """Per-kernel average launch time (HIP events on the library stream,
     replaying the launches one solver iteration makes), algorithmic bytes per
     launch, GB/s, and each kernel's estimated share of one step: launches per
@@ -104,7 +104,8 @@ def kernel_table(ch, res, reps=30):
     vel_iters = it["tentative"] + it["final"] if it else None
     out = {}
     for solver, iters in (("velocity", vel_iters), ("pressure", it["pressure"] if it else None)):
-        for k in ("direction", "sweep", "rho", "update", "finalize", "iteration"):
+        kinds = ("direction", "sweep", "rho", "update", "finalize") + (("iteration",) if with_iteration else ())
+        for k in kinds:
             ms, b, per_it = ch.time_kernel(f"{solver}_{k}", reps=reps)
             out[f"{solver}_{k}"] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9 if b else None,
                                     "launches_per_iteration": per_it,
@@ -296,12 +297,14 @@ def main():
         for name in paths:
             ch = fem.Chorin(ctx, {"fp64": cfg64, "mixed": cfgmx}[name])
             ch.upload_velocity(u0)
-            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps)
+            # no whole-iteration replays: with no step run the solver state is
+            # zero and those iterations would stop at once
+            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps, with_iteration=False)
             out[f"{name}_shape"] = ch.shape()
             del ch
         print(json.dumps(out))
         return 0
-    lam = fem.estimate_lambda_max(ctx)
+    lam = fem.estimate_lambda_max(ctx, graphs=not args.eager)
     tau_stable = 1.0 / (cond.viscosity * lam)
     tau = min(args.tau, tau_stable)
     log(f"lambda_max(M^-1 K) ~ {lam:.4g}; tau requested {args.tau:g}, used {tau:.4g} "

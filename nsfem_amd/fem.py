@@ -361,7 +361,7 @@ def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t:
     return out, stats
 
 
-def estimate_lambda_max(ctx: ChorinContext, iters=40, seed=1):
+def estimate_lambda_max(ctx: ChorinContext, iters=40, seed=1, graphs=True):
     """Largest eigenvalue of M^-1 K (velocity mass / stiffness) by power
     iteration on the GPU; bounds the forward-Euler diffusion step,
     tau * nu * lambda_max <= 2."""
@@ -370,6 +370,8 @@ def estimate_lambda_max(ctx: ChorinContext, iters=40, seed=1):
     K = cu.csrF(*ctx.matrix("velocityStiffness")[:2], *ctx.matrix("velocityStiffness")[2:])
     M = cu.csrF(*ctx.matrix("velocityMass")[:2], *ctx.matrix("velocityMass")[2:])
     solver = cu.createSolver("pcg", 1, ctx.scipy("velocityMass"), 500, 1e-10, 1)
+    if not graphs:
+        solver.setOption("graphs", 0)
     n = ctx.numVelocityNodes
     x = cu.vec(rng.standard_normal(n))
     y = cu.vec(n, np.float64)
