@@ -314,8 +314,10 @@ namespace nsfem
     }
 
     // ------------------------------------------------------------------
-    // Colour permutation of planar vectors.
-    // fwd: dst[c*n + i] = src[c*n + perm[i]];   inv: dst[c*n + perm[i]] = src[c*n + i]
+    // Colour permutation between planar io vectors (channel c of row r at
+    // c*n + r) and the solvers' interleaved internal vectors (row i of the
+    // colour order, channel c at i*nch + c).
+    // fwd: dst[i*nch + c] = src[c*n + perm[i]];   inv: dst[c*n + perm[i]] = src[i*nch + c]
     // ------------------------------------------------------------------
     template <typename S, typename D>
     __global__ void permuteKernel(int n, int nch, const int * __restrict__ perm, const S * __restrict__ src,
@@ -324,8 +326,8 @@ namespace nsfem
         const long total = static_cast<long>(n) * nch;
         for (long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; t < total; t += static_cast<long>(gridDim.x) * blockDim.x)
         {
-            const int c = static_cast<int>(t / n);
-            const int i = static_cast<int>(t - static_cast<long>(c) * n);
+            const int i = static_cast<int>(t / nch);
+            const int c = static_cast<int>(t - static_cast<long>(i) * nch);
             const long p = static_cast<long>(c) * n + perm[i];
             if (inverse)
             {
@@ -480,12 +482,12 @@ namespace nsfem
         return a != 0;
     }
 
-    // Lane-group sum over j in [j0, j1) of val[j] * v[col[j] + c*ld], in the
+    // Lane-group sum over j in [j0, j1) of val[j] * v[col[j]*NCH + c], in the
     // matrix type T (v may be stored narrower, e.g. an FP32 preconditioned
-    // vector under an FP64 matrix).
+    // vector under an FP64 matrix).  Solver vectors interleave their
+    // channels, so one gathered column is one contiguous access.
     template <typename T, int LANES, int NCH, typename V>
-    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const V * __restrict__ v, long ld,
-                                           T (&acc)[NCH])
+    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const V * __restrict__ v, T (&acc)[NCH])
     {
         const int lane = threadIdx.x & (LANES - 1);
 #pragma unroll
@@ -507,8 +509,8 @@ namespace nsfem
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                x0[c] = static_cast<T>(v[c0 + c * ld]);
-                x1[c] = static_cast<T>(v[c1 + c * ld]);
+                x0[c] = static_cast<T>(v[static_cast<long>(c0) * NCH + c]);
+                x1[c] = static_cast<T>(v[static_cast<long>(c1) * NCH + c]);
             }
 #pragma unroll
             for (int c = 0; c < NCH; c++)
@@ -580,17 +582,17 @@ namespace nsfem
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    bi[c] = b[row + static_cast<long>(c) * A.n];
+                    bi[c] = b[static_cast<long>(row) * NCH + c];
                 }
             }
             T acc[NCH];
-            rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+            rowDot<T, LANES, NCH>(A, j0, j1, x, acc);
             if (lead)
             {
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    x[row + static_cast<long>(c) * A.n] = (bi[c] - acc[c]) * inv;
+                    x[static_cast<long>(row) * NCH + c] = (bi[c] - acc[c]) * inv;
                 }
             }
         }
@@ -625,13 +627,13 @@ namespace nsfem
                 const int j0 = valid ? A.rowStart[row] : 0;
                 const int j1 = valid ? A.rowStart[row + 1] : 0;
                 T acc[NCH];
-                rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+                rowDot<T, LANES, NCH>(A, j0, j1, x, acc);
                 if (valid && groupLeader<LANES>())
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
-                        const long i = row + static_cast<long>(c) * A.n;
+                        const long i = static_cast<long>(row) * NCH + c;
                         const T ax = fmaT(A.diag[row], x[i], acc[c]);
                         const double r = static_cast<double>(ax - b[i]);
                         part[c] = fma(r, r, part[c]);
@@ -671,14 +673,14 @@ namespace nsfem
             {
                 const int j0 = valid ? A.rowStart[row] : 0;
                 const int j1 = valid ? A.rowStart[row + 1] : 0;
-                rowDot<T, LANES, NCH>(A, j0, j1, x, A.n, acc);
+                rowDot<T, LANES, NCH>(A, j0, j1, x, acc);
             }
             if (valid && groupLeader<LANES>())
             {
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = row + static_cast<long>(c) * A.n;
+                    const long i = static_cast<long>(row) * NCH + c;
                     const T bi = b[i];
                     T ri;
                     if (zeroGuess)
@@ -770,7 +772,7 @@ namespace nsfem
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
-                        const long i = row + static_cast<long>(c) * M.n;
+                        const long i = static_cast<long>(row) * NCH + c;
                         own[c] = c0 ? rP[i] : z[i];
                         rr[c] = static_cast<double>(r[i]);
                     }
@@ -778,14 +780,14 @@ namespace nsfem
                 P acc[NCH];
                 if (c0)
                 {
-                    rowDot<P, LANES, NCH>(M, j0, j1, z, M.n, acc);
+                    rowDot<P, LANES, NCH>(M, j0, j1, z, acc);
                 }
                 if (lead)
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
-                        const long i = row + static_cast<long>(c) * M.n;
+                        const long i = static_cast<long>(row) * NCH + c;
                         P zi;
                         if (c0)
                         {
@@ -864,7 +866,7 @@ namespace nsfem
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
-                        const long i = row + static_cast<long>(c) * A.n;
+                        const long i = static_cast<long>(row) * NCH + c;
                         zr[c] = static_cast<T>(z[i]);
                         if (!first)
                         {
@@ -874,13 +876,13 @@ namespace nsfem
                     }
                 }
                 T acc[NCH];
-                rowDot<T, LANES, NCH>(A, j0, j1, z, A.n, acc);
+                rowDot<T, LANES, NCH>(A, j0, j1, z, acc);
                 if (lead)
                 {
 #pragma unroll
                     for (int c = 0; c < NCH; c++)
                     {
-                        const long i = row + static_cast<long>(c) * A.n;
+                        const long i = static_cast<long>(row) * NCH + c;
                         const T zi = zr[c];
                         const T az = fmaT(dg, zi, acc[c]);
                         T pi, qi;
@@ -942,7 +944,7 @@ namespace nsfem
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = row + static_cast<long>(c) * n;
+                    const long i = static_cast<long>(row) * NCH + c;
                     T ri = r[i];
                     if (act[c])
                     {
@@ -1202,7 +1204,7 @@ namespace nsfem
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                part[c] += static_cast<double>(v[row + static_cast<long>(c) * n]);
+                part[c] += static_cast<double>(v[static_cast<long>(row) * NCH + c]);
             }
         }
         blockSum<NCH>(part, sm);
@@ -1239,7 +1241,7 @@ namespace nsfem
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                const long i = row + static_cast<long>(c) * n;
+                const long i = static_cast<long>(row) * NCH + c;
                 v[i] = static_cast<T>(static_cast<double>(v[i]) - mean[s * NCH + c]);
             }
         }
