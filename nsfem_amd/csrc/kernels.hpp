@@ -931,41 +931,42 @@ namespace nsfem
         }
         if (systemActive<NCH>(st, s))
         {
-            T alpha[NCH];
-            int act[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
+            // flat over the block's interleaved elements, one per thread: the
+            // stride (kBlock) is a multiple of NCH, so a thread keeps one channel
+            static_assert(kBlock % NCH == 0, "channels must divide the block");
+            const int c = static_cast<int>(threadIdx.x) % NCH;
+            const T alpha = static_cast<T>(st.alpha[s * NCH + c]);
+            const int act = st.active[s * NCH + c];
+            double sum = 0;
+            const long end = static_cast<long>(range.y) * NCH;
+            for (long i = static_cast<long>(range.x) * NCH + threadIdx.x; i < end; i += kBlock)
             {
-                alpha[c] = static_cast<T>(st.alpha[s * NCH + c]);
-                act[c] = st.active[s * NCH + c];
-            }
-            for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
-            {
-#pragma unroll
-                for (int c = 0; c < NCH; c++)
+                const int row = static_cast<int>(i / NCH);
+                T ri = r[i];
+                if (act)
                 {
-                    const long i = static_cast<long>(row) * NCH + c;
-                    T ri = r[i];
-                    if (act[c])
-                    {
-                        x[i] = fmaT(alpha[c], p[i], x[i]);
-                        ri = fmaT(-alpha[c], q[i], ri);
-                        r[i] = ri;
-                    }
-                    const P rp = static_cast<P>(ri);
-                    if constexpr (!std::is_same_v<T, P>)
-                    {
-                        if (act[c])
-                        {
-                            rP[i] = rp;
-                        }
-                    }
-                    if (row < color0End)
-                    {
-                        z[i] = rp * invDiagP[row];
-                    }
-                    part[c] = fma(static_cast<double>(ri), static_cast<double>(ri), part[c]);
+                    x[i] = fmaT(alpha, p[i], x[i]);
+                    ri = fmaT(-alpha, q[i], ri);
+                    r[i] = ri;
                 }
+                const P rp = static_cast<P>(ri);
+                if constexpr (!std::is_same_v<T, P>)
+                {
+                    if (act)
+                    {
+                        rP[i] = rp;
+                    }
+                }
+                if (row < color0End)
+                {
+                    z[i] = rp * invDiagP[row];
+                }
+                sum = fma(static_cast<double>(ri), static_cast<double>(ri), sum);
+            }
+#pragma unroll
+            for (int cc = 0; cc < NCH; cc++)
+            {
+                part[cc] = cc == c ? sum : 0.0;
             }
         }
         blockSum<NCH>(part, sm);
