@@ -229,6 +229,57 @@ def run_batch(args):
     return 0
 
 
+def run_poisson(args):
+    """Pressure-Poisson kernel alone: the pure-Neumann P1 Laplacian of a
+    seeded channel mesh, uniform(-1, 1) right-hand side projected to zero
+    mean, PCG to 1e-10 relative residual, FP64 and mixed."""
+    import numpy as np
+    from nsfem_amd import cu, fem, mesher
+    t0 = time.perf_counter()
+    m = mesher.channel_mesh(args.poisson, seed=args.seed)
+    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), fem.DfgConditions(0.01, 1.0, m["height"], 4.0), gpu_assembly=True)
+    K = ctx.scipy("pressureStiffness")
+    setup_s = time.perf_counter() - t0
+    n = K.shape[0]
+    rng = np.random.default_rng(1)
+    b = rng.uniform(-1, 1, n)
+    b -= b.mean()
+    out = {}
+    for name, method in (("fp64", "pcg"), ("mixed", "pcg_mixed")):
+        s = cu.createSolver(method, 1, K, 50000, 1e-10, 1)
+        s.setOption("projectMean", 1)
+        shape = s.autotune(10) if not args.no_autotune else s.shape()
+        times, iters = [], []
+        for rep in range(args.warmup + args.steps):
+            s.getRhs().upload(b)
+            s.getSol().memsetZero()
+            t = time.perf_counter()
+            s.solve()
+            dt = time.perf_counter() - t
+            if rep >= args.warmup:
+                times.append(dt)
+                iters.append(s.getLastIterations())
+        x = s.getSol().download()
+        rel = float(np.linalg.norm(b - K @ x) / np.linalg.norm(b))
+        out[name] = {"solves_per_s": 1.0 / float(np.mean(times)), "ms_per_solve": 1e3 * float(np.mean(times)),
+                     "iterations": float(np.mean(iters)), "true_relres": rel, "shape": shape,
+                     "ms_per_iteration": 1e3 * float(np.mean(times)) / float(np.mean(iters))}
+        log(f"{name}: {out[name]['ms_per_solve']:.1f} ms per solve, {out[name]['iterations']:.0f} iterations, "
+            f"true relres {rel:.2e}")
+        del s
+    res = {
+        "metric": "pressure-Poisson PCG solves/s, mixed vs FP64 GPU, 1e-10 relative residual",
+        "value": out["mixed"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out["mixed"]["ms_per_solve"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic: seeded mesh and right-hand side",
+        "config": {"workload": f"poisson_neumann_{n}", "rows": n, "nnz": int(K.nnz), "seed": args.seed},
+        "fp64": out["fp64"], "mixed": out["mixed"],
+        "mixed_over_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], "setup_s": setup_s,
+    }
+    print(json.dumps(res))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +299,8 @@ def main():
                          "for hardware-counter passes, which need few dispatches")
     ap.add_argument("--kernel-reps", type=int, default=30)
     ap.add_argument("--host-assembly", action="store_true", help="assemble the matrices on the host instead of the GPU")
+    ap.add_argument("--poisson", type=int, default=0,
+                    help="instead of the Chorin step: the pure-Neumann P1 Laplacian with about this many rows alone")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 500k workload: this many uneven instances (10k-250k pressure DOFs) in one batch")
     ap.add_argument("--eager", action="store_true",
@@ -265,6 +318,8 @@ def main():
 
     if args.batch > 0:
         return run_batch(args)
+    if args.poisson > 0:
+        return run_poisson(args)
 
     if args.copy_calibration_mib > 0:
         # a plain streaming kernel of known traffic: each launch reads and
