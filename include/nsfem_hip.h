@@ -95,13 +95,17 @@ int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * d
  *                      checked every mseCheckInterval sweeps, stop below targetMse.
  *   name "pcg"       : CG preconditioned by a symmetric multicolour GS sweep;
  *                      targetMse is a relative residual |b-Ax| / |b|.
- *   name "pcg_mixed" : FP64 iterative refinement around FP32 inner PCG
- *                      (values must be NSFEM_F64).
+ *   name "pcg_mixed" : FP64 PCG (vectors, products, reductions) with the
+ *                      symmetric GS preconditioner in FP32 (values must be
+ *                      NSFEM_F64); option "mixedRestart" switches to FP64
+ *                      iterative refinement around FP32 inner PCG.
  *   name "dss"       : rejected with NSFEM_EINVAL (no direct solver here).
  * The compute precision is valueDtype ("gs"/"pcg").  Colouring is the
  * reference's smallest-last greedy colouring (graphs.cpp:64-176).
  * rhs/sol are device buffers with stable addresses, numCh * rows elements of
  * the io dtype (NSFEM_F32 for an F32 solver, NSFEM_F64 otherwise).
+ * numCh: 1 or 2 (the reference's solvers), or 4, 8, 16, 32 right-hand sides
+ * sharing the matrix (no reference counterpart; one lane per channel).
  * ---------------------------------------------------------------------- */
 typedef struct nsfem_solver nsfem_solver;
 int nsfem_solver_create(const char * name, int numCh, int rows, int nnz, const int * rowStart, const int * column, const void * values,
@@ -138,7 +142,12 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  default FP64 PCG with an FP32 symmetric GS preconditioner.
  *   "innerTol"     pcg_mixed with mixedRestart: inner relative tolerance.
  *   "graphs"       0: launch every iteration from the host instead of
- *                  replaying captured graphs of iteration chunks (default 1). */
+ *                  replaying captured graphs of iteration chunks (default 1).
+ *   "fusedAlpha"   pcg: 1 finishes the step length alpha inside the update
+ *                  kernel (one launch less per iteration; every update
+ *                  workgroup re-reads the direction partial sums of its
+ *                  system; 1 and 2 channels), 0 (default) or -1 in a one-block
+ *                  finalize kernel. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events
@@ -159,8 +168,10 @@ int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
  * reshape rebuilds the workgroup map only (colouring and matrices kept).
  * autotune times one PCG iteration's launches (pcg solvers, reps back-to-back
  * replays per kernel) for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs and keeps
- * the fastest; it overwrites the internal work vectors, not rhs/sol.  No
- * reference counterpart (the reference's launch shapes are fixed). */
+ * the fastest; it overwrites the internal work vectors, not rhs/sol.  Solvers
+ * of 4 to 32 channels run one lane per channel (lanes = numCh) and keep at
+ * most about 16k workgroups.  No reference counterpart (the reference's
+ * launch shapes are fixed). */
 int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock);
 int nsfem_solver_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock);
 int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock);
@@ -297,6 +308,9 @@ int nsfem_chorin_reset(nsfem_chorin * ch);
  * as nsfem_solver_autotune for both solvers (the state u, p is kept). */
 int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
 int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
+/* nsfem_solver_set_option for the step's solvers: which = 0 velocity,
+ * 1 pressure, 2 both.  No reference counterpart. */
+int nsfem_chorin_set_solver_option(nsfem_chorin * ch, int which, const char * key, double value);
 /* Kernel timing inside a Chorin context: "convection", "spmm",
  * "velocity_<k>", "pressure_<k>" (k as in nsfem_solver_time_kernel). */
 int nsfem_chorin_time_kernel(nsfem_chorin * ch, const char * which, int reps, double * avgMs, double * bytes,

@@ -66,6 +66,36 @@ namespace
         }
     }
 
+    void applySolverOption(McgsSolver & solver, const std::string & k, double value)
+    {
+        if (k == "innerTol")
+        {
+            if (!(value > 0 && value < 1)) throw std::invalid_argument("innerTol must be in (0, 1)");
+            solver.setInnerTol(value);
+        }
+        else if (k == "projectMean")
+        {
+            solver.setProjectMean(value != 0);
+        }
+        else if (k == "mixedRestart")
+        {
+            solver.setMixedRestart(value != 0);
+        }
+        else if (k == "graphs")
+        {
+            solver.setUseGraphs(value != 0);
+        }
+        else if (k == "fusedAlpha")
+        {
+            if (value != -1 && value != 0 && value != 1) throw std::invalid_argument("fusedAlpha must be -1, 0 or 1");
+            solver.setFusedAlpha(static_cast<int>(value));
+        }
+        else
+        {
+            throw std::invalid_argument("Unknown solver option: " + k);
+        }
+    }
+
     CsrD hostCsr(int rows, int cols, int nnz, const int * rowStart, const int * column, const void * values, int dtype)
     {
         checkDtype(dtype);
@@ -610,28 +640,7 @@ extern "C"
                        {
             need(s, "solver");
             need(key, "key");
-            const std::string k(key);
-            if (k == "innerTol")
-            {
-                if (!(value > 0 && value < 1)) throw std::invalid_argument("innerTol must be in (0, 1)");
-                s->core->setInnerTol(value);
-            }
-            else if (k == "projectMean")
-            {
-                s->core->setProjectMean(value != 0);
-            }
-            else if (k == "mixedRestart")
-            {
-                s->core->setMixedRestart(value != 0);
-            }
-            else if (k == "graphs")
-            {
-                s->core->setUseGraphs(value != 0);
-            }
-            else
-            {
-                throw std::invalid_argument("Unknown solver option: " + k);
-            } });
+            applySolverOption(*s->core, key, value); });
     }
 
     int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes,
@@ -1149,6 +1158,17 @@ extern "C"
             shape[1] = v.rowsPerBlock();
             shape[2] = p.lanes();
             shape[3] = p.rowsPerBlock(); });
+    }
+
+    int nsfem_chorin_set_solver_option(nsfem_chorin * ch, int which, const char * key, double value)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            need(key, "key");
+            if (which < 0 || which > 2) throw std::invalid_argument("which must be 0 (velocity), 1 (pressure) or 2 (both)");
+            if (which != 1) applySolverOption(ch->core->velocitySolver(), key, value);
+            if (which != 0) applySolverOption(ch->core->pressureSolver(), key, value); });
     }
 
     int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape)

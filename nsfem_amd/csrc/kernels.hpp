@@ -397,77 +397,83 @@ namespace nsfem
     template <int K>
     __device__ __forceinline__ void systemTotals(const BlockMap & bm, const double * partials, double * out, double * sm)
     {
-        if (bm.numSystems == 1)
+        // at most 8 sums at a time keep the accumulators in registers
+        constexpr int G = K < 8 ? K : 8;
+        static_assert(K % G == 0, "sum groups must divide K");
+        for (int k0 = 0; k0 < K; k0 += G)
         {
-            double acc[K];
-#pragma unroll
-            for (int k = 0; k < K; k++)
+            if (bm.numSystems == 1)
             {
-                acc[k] = 0;
-            }
-            // independent loads: keep many in flight
-            const int nb = bm.numBlocks;
-#pragma unroll 8
-            for (int b = threadIdx.x; b < nb; b += kFinBlock)
-            {
+                double acc[G];
 #pragma unroll
-                for (int k = 0; k < K; k++)
+                for (int k = 0; k < G; k++)
                 {
-                    acc[k] += partials[static_cast<size_t>(b) * K + k];
+                    acc[k] = 0;
                 }
-            }
-            blockSum<K, kFinWaves>(acc, sm);
-            if (threadIdx.x == 0)
-            {
-#pragma unroll
-                for (int k = 0; k < K; k++)
+                // independent loads: keep many in flight
+                const int nb = bm.numBlocks;
+#pragma unroll 8
+                for (int b = threadIdx.x; b < nb; b += kFinBlock)
                 {
-                    out[k] = acc[k];
+#pragma unroll
+                    for (int k = 0; k < G; k++)
+                    {
+                        acc[k] += partials[static_cast<size_t>(b) * K + k0 + k];
+                    }
+                }
+                blockSum<G, kFinWaves>(acc, sm);
+                if (threadIdx.x == 0)
+                {
+#pragma unroll
+                    for (int k = 0; k < G; k++)
+                    {
+                        out[k0 + k] = acc[k];
+                    }
+                }
+                __syncthreads();
+                continue;
+            }
+            // One wavefront per system, lanes stride the system's partials.
+            const int wave = threadIdx.x >> 6;
+            const int lane = threadIdx.x & 63;
+            for (int s = wave; s < bm.numSystems; s += kFinWaves)
+            {
+                double acc[G];
+#pragma unroll
+                for (int k = 0; k < G; k++)
+                {
+                    acc[k] = 0;
+                }
+                const int i1 = bm.sysBlockStart[s + 1];
+#pragma unroll 4
+                for (int i = bm.sysBlockStart[s] + lane; i < i1; i += 64)
+                {
+#pragma unroll
+                    for (int k = 0; k < G; k++)
+                    {
+                        acc[k] += partials[static_cast<size_t>(i) * K + k0 + k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < G; k++)
+                {
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1)
+                    {
+                        acc[k] += __shfl_xor(acc[k], off, 64);
+                    }
+                }
+                if (lane == 0)
+                {
+#pragma unroll
+                    for (int k = 0; k < G; k++)
+                    {
+                        out[s * K + k0 + k] = acc[k];
+                    }
                 }
             }
             __syncthreads();
-            return;
         }
-        // One wavefront per system, lanes stride the system's partials.
-        const int wave = threadIdx.x >> 6;
-        const int lane = threadIdx.x & 63;
-        for (int s = wave; s < bm.numSystems; s += kFinWaves)
-        {
-            double acc[K];
-#pragma unroll
-            for (int k = 0; k < K; k++)
-            {
-                acc[k] = 0;
-            }
-            const int i1 = bm.sysBlockStart[s + 1];
-#pragma unroll 4
-            for (int i = bm.sysBlockStart[s] + lane; i < i1; i += 64)
-            {
-#pragma unroll
-                for (int k = 0; k < K; k++)
-                {
-                    acc[k] += partials[static_cast<size_t>(i) * K + k];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < K; k++)
-            {
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1)
-                {
-                    acc[k] += __shfl_xor(acc[k], off, 64);
-                }
-            }
-            if (lane == 0)
-            {
-#pragma unroll
-                for (int k = 0; k < K; k++)
-                {
-                    out[s * K + k] = acc[k];
-                }
-            }
-        }
-        __syncthreads();
     }
 
     template <int NCH>
@@ -539,7 +545,7 @@ namespace nsfem
     // sweep, backward half of symmetric GS); lowerOnly = 1: lower-colour part
     // only (forward half of symmetric GS started from zero).
     template <typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st,
+    __device__ __forceinline__ void mcgsColorNarrow(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st,
                                                               const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
     {
         // Latency: the block's rows, its system and the done flag are loaded
@@ -602,7 +608,7 @@ namespace nsfem
     // per system and channel; a system stops when all its channels are below
     // the target (gaussSeidel.cu:267-290, 354-380).  st.tol2 holds the target.
     template <typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) gsResidualKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
+    __device__ __forceinline__ void gsResidualNarrow(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
                                                                const T * __restrict__ x)
     {
         __shared__ double sm[NCH * kWaves];
@@ -651,7 +657,7 @@ namespace nsfem
     // the preconditioner (the symmetric GS sweeps).  P narrower than T: the
     // residual is also stored as rP = (P) r for the sweeps; P == T: rP == r.
     template <typename T, typename P, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgInitKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
+    __device__ __forceinline__ void pcgInitNarrow(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
                                                             T * __restrict__ x, T * r, P * rP, P * __restrict__ z,
                                                             const P * __restrict__ invDiagP, int color0End, int zeroGuess)
     {
@@ -717,7 +723,7 @@ namespace nsfem
     // the others (block map order: colour 0 first, color0Blocks of them) so
     // that every XCD's range of logical blocks gets its share.
     template <typename P, typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP,
+    __device__ __forceinline__ void pcgRhoNarrow(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP,
                                                            const T * r, P * __restrict__ z, int color0End, int color0Blocks)
     {
         __shared__ double sm[NCH * kWaves];
@@ -810,7 +816,7 @@ namespace nsfem
     // p = z + beta p;  q = A z + beta q  (= A p);  partial sums of p.q.
     // z is the preconditioned vector (type P), p and q are of the Krylov type T.
     template <typename T, typename P, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
+    __device__ __forceinline__ void pcgDirectionNarrow(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
                                                                  T * __restrict__ p, T * __restrict__ q)
     {
         __shared__ double sm[NCH * kWaves];
@@ -907,46 +913,496 @@ namespace nsfem
         storePartial<NCH>(part, st.partials, range.w);
     }
 
+    // ------------------------------------------------------------------
+    // Wide channel tiles: 4 to 32 right-hand sides sharing one matrix.  NCH
+    // lanes per row, lane c on channel c walks the whole row in order, so a
+    // gathered column is NCH contiguous values across the lanes, the row's
+    // entries are one broadcast load per lane group, and no lane reduction
+    // is needed.  1 and 2 channels keep the lane-split rows above.
+    // ------------------------------------------------------------------
+    template <int NCH>
+    constexpr bool kWide = NCH > 2;
+
+    // sum over j in [j0, j1) of val[j] * v[col[j]*NCH + c], in entry order;
+    // four entries' loads, then their gathers, go out together.
+    template <typename T, int NCH, typename V>
+    __device__ __forceinline__ T rowDotWide(const McgsMatrix<T> & A, int j0, int j1, const V * __restrict__ v, int c)
+    {
+        T acc = T(0);
+        int j = j0;
+        for (; j + 4 <= j1; j += 4)
+        {
+            T a[4], xv[4];
+            int col[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+            {
+                a[k] = A.values[j + k];
+                col[k] = A.column[j + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+            {
+                xv[k] = static_cast<T>(v[static_cast<long>(col[k]) * NCH + c]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+            {
+                acc = fmaT(a[k], xv[k], acc);
+            }
+        }
+        for (; j < j1; j++)
+        {
+            acc = fmaT(A.values[j], static_cast<T>(v[static_cast<long>(A.column[j]) * NCH + c]), acc);
+        }
+        return acc;
+    }
+
+    // Block sum per channel when every thread holds one channel's value,
+    // channel = threadIdx.x % NCH (lanes of one channel are NCH apart in a
+    // wavefront).  The total of channel c is returned in thread c (c < NCH).
+    template <int NCH>
+    __device__ __forceinline__ double channelBlockSum(double v, double * sm /* NCH * kWaves */)
+    {
+#pragma unroll
+        for (int off = 32; off >= NCH; off >>= 1)
+        {
+            v += __shfl_xor(v, off, 64);
+        }
+        const int wave = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        if (lane < NCH)
+        {
+            sm[lane * kWaves + wave] = v;
+        }
+        __syncthreads();
+        double s = 0;
+        if (static_cast<int>(threadIdx.x) < NCH)
+        {
+#pragma unroll
+            for (int w = 0; w < kWaves; w++)
+            {
+                s += sm[threadIdx.x * kWaves + w];
+            }
+        }
+        __syncthreads();
+        return s;
+    }
+
+    // partials[slot*K + offset + c] = total of channel c (threads c < NCH).
+    template <int NCH, int K>
+    __device__ __forceinline__ void storeChannelPartial(double total, double * partials, int slot, int offset)
+    {
+        if (static_cast<int>(threadIdx.x) < NCH)
+        {
+            partials[static_cast<size_t>(slot) * K + offset + threadIdx.x] = total;
+        }
+    }
+
+    template <typename T, int NCH>
+    __device__ __forceinline__ void mcgsColorWide(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st, const T * __restrict__ b,
+                                                  T * __restrict__ x, int lowerOnly)
+    {
+        constexpr int slab = kBlock / NCH;
+        const int blk = blockOffset + logicalBlock(blockIdx.x, gridDim.x);
+        const int4 range = bm.blocks[blk];
+        const int sys = range.z;
+        const int done = st.flags[0];
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        int row = range.x + static_cast<int>(threadIdx.x) / NCH;
+        bool valid = row < range.y;
+        int j0 = valid ? A.rowStart[row] : 0;
+        int j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
+        const bool act = systemActive<NCH>(st, sys);
+        if (done | !act | (j1 < j0))
+        {
+            return;
+        }
+        for (int base = range.x; base < range.y; base += slab)
+        {
+            if (base != range.x)
+            {
+                row = base + static_cast<int>(threadIdx.x) / NCH;
+                valid = row < range.y;
+                j0 = valid ? A.rowStart[row] : 0;
+                j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
+            }
+            if (valid)
+            {
+                const long i = static_cast<long>(row) * NCH + c;
+                const T inv = A.invDiag[row];
+                const T bi = b[i];
+                const T acc = rowDotWide<T, NCH>(A, j0, j1, x, c);
+                x[i] = (bi - acc) * inv;
+            }
+        }
+    }
+
+    template <typename T, int NCH>
+    __device__ __forceinline__ void gsResidualWide(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b,
+                                                   const T * __restrict__ x)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        if (st.flags[0])
+        {
+            return; // the whole grid and its finalize skip
+        }
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        double part = 0;
+        const int4 range = bm.blocks[lb];
+        if (systemActive<NCH>(st, range.z))
+        {
+            for (int base = range.x; base < range.y; base += kBlock / NCH)
+            {
+                const int row = base + static_cast<int>(threadIdx.x) / NCH;
+                if (row < range.y)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    const T acc = rowDotWide<T, NCH>(A, A.rowStart[row], A.rowStart[row + 1], x, c);
+                    const T ax = fmaT(A.diag[row], x[i], acc);
+                    const double r = static_cast<double>(ax - b[i]);
+                    part = fma(r, r, part);
+                }
+            }
+        }
+        const double total = channelBlockSum<NCH>(part, sm);
+        storeChannelPartial<NCH, NCH>(total, st.partials, range.w, 0);
+    }
+
+    template <typename T, typename P, int NCH>
+    __device__ __forceinline__ void pcgInitWide(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b, T * __restrict__ x,
+                                                T * r, P * rP, P * __restrict__ z, const P * __restrict__ invDiagP, int color0End,
+                                                int zeroGuess)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        double bb = 0, rr = 0;
+        const int4 range = bm.blocks[lb];
+        for (int base = range.x; base < range.y; base += kBlock / NCH)
+        {
+            const int row = base + static_cast<int>(threadIdx.x) / NCH;
+            if (row < range.y)
+            {
+                const long i = static_cast<long>(row) * NCH + c;
+                const T bi = b[i];
+                T ri;
+                if (zeroGuess)
+                {
+                    x[i] = T(0);
+                    ri = bi;
+                }
+                else
+                {
+                    const T acc = rowDotWide<T, NCH>(A, A.rowStart[row], A.rowStart[row + 1], x, c);
+                    ri = bi - fmaT(A.diag[row], x[i], acc);
+                }
+                r[i] = ri;
+                const P rp = static_cast<P>(ri);
+                if constexpr (!std::is_same_v<T, P>)
+                {
+                    rP[i] = rp;
+                }
+                if (row < color0End)
+                {
+                    z[i] = rp * invDiagP[row];
+                }
+                bb = fma(static_cast<double>(bi), static_cast<double>(bi), bb);
+                rr = fma(static_cast<double>(ri), static_cast<double>(ri), rr);
+            }
+        }
+        const double tb = channelBlockSum<NCH>(bb, sm);
+        const double tr = channelBlockSum<NCH>(rr, sm);
+        storeChannelPartial<NCH, 2 * NCH>(tb, st.partials, range.w, 0);
+        storeChannelPartial<NCH, 2 * NCH>(tr, st.partials, range.w, NCH);
+    }
+
+    template <typename P, typename T, int NCH>
+    __device__ __forceinline__ void pcgRhoWide(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP, const T * r,
+                                               P * __restrict__ z, int color0End, int color0Blocks)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
+        const int4 range = bm.blocks[lb];
+        const int sys = range.z;
+        const int done = st.flags[0];
+        const bool c0 = range.x < color0End; // blocks never straddle colours
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        int row = range.x + static_cast<int>(threadIdx.x) / NCH;
+        bool valid = row < range.y;
+        int j0 = (c0 && valid) ? M.rowStart[row] : 0;
+        int j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+        if (done || j1 < j0)
+        {
+            return;
+        }
+        double part = 0;
+        if (systemActive<NCH>(st, sys))
+        {
+            for (int base = range.x; base < range.y; base += kBlock / NCH)
+            {
+                if (base != range.x)
+                {
+                    row = base + static_cast<int>(threadIdx.x) / NCH;
+                    valid = row < range.y;
+                    j0 = (c0 && valid) ? M.rowStart[row] : 0;
+                    j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+                }
+                if (valid)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    const double ri = static_cast<double>(r[i]);
+                    P zi;
+                    if (c0)
+                    {
+                        const P inv = M.invDiag[row];
+                        const P own = rP[i];
+                        const P acc = rowDotWide<P, NCH>(M, j0, j1, z, c);
+                        zi = (own - acc) * inv;
+                        z[i] = zi;
+                    }
+                    else
+                    {
+                        zi = z[i];
+                    }
+                    part = fma(ri, static_cast<double>(zi), part);
+                }
+            }
+        }
+        const double total = channelBlockSum<NCH>(part, sm);
+        storeChannelPartial<NCH, NCH>(total, st.partials, range.w, 0);
+    }
+
+    template <typename T, typename P, int NCH>
+    __device__ __forceinline__ void pcgDirectionWide(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
+                                                     T * __restrict__ p, T * __restrict__ q)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int lb = logicalBlock(blockIdx.x, gridDim.x);
+        const int4 range = bm.blocks[lb];
+        const int s = range.z;
+        const int done = st.flags[0];
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        int row = range.x + static_cast<int>(threadIdx.x) / NCH;
+        bool valid = row < range.y;
+        int j0 = valid ? A.rowStart[row] : 0;
+        int j1 = valid ? A.rowStart[row + 1] : 0;
+        if (done || j1 < j0)
+        {
+            return;
+        }
+        double part = 0;
+        if (systemActive<NCH>(st, s))
+        {
+            const bool first = st.flags[1] != 0;
+            const T beta = static_cast<T>(st.beta[s * NCH + c]);
+            for (int base = range.x; base < range.y; base += kBlock / NCH)
+            {
+                if (base != range.x)
+                {
+                    row = base + static_cast<int>(threadIdx.x) / NCH;
+                    valid = row < range.y;
+                    j0 = valid ? A.rowStart[row] : 0;
+                    j1 = valid ? A.rowStart[row + 1] : 0;
+                }
+                if (valid)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    const T dg = A.diag[row];
+                    const T zi = static_cast<T>(z[i]);
+                    T pr = T(0), qr = T(0);
+                    if (!first)
+                    {
+                        pr = p[i];
+                        qr = q[i];
+                    }
+                    const T acc = rowDotWide<T, NCH>(A, j0, j1, z, c);
+                    const T az = fmaT(dg, zi, acc);
+                    const T pi = first ? zi : fmaT(beta, pr, zi);
+                    const T qi = first ? az : fmaT(beta, qr, az);
+                    p[i] = pi;
+                    q[i] = qi;
+                    part = fma(static_cast<double>(pi), static_cast<double>(qi), part);
+                }
+            }
+        }
+        const double total = channelBlockSum<NCH>(part, sm);
+        storeChannelPartial<NCH, NCH>(total, st.partials, range.w, 0);
+    }
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st, const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            mcgsColorWide<T, NCH>(A, bm, blockOffset, st, b, x, lowerOnly);
+        }
+        else
+        {
+            mcgsColorNarrow<T, LANES, NCH>(A, bm, blockOffset, st, b, x, lowerOnly);
+        }
+    }
+
+    template <typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) gsResidualKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b, const T * __restrict__ x)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            gsResidualWide<T, NCH>(A, bm, st, b, x);
+        }
+        else
+        {
+            gsResidualNarrow<T, LANES, NCH>(A, bm, st, b, x);
+        }
+    }
+
+    template <typename T, typename P, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgInitKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const T * __restrict__ b, T * __restrict__ x, T * r, P * rP, P * __restrict__ z, const P * __restrict__ invDiagP, int color0End, int zeroGuess)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            pcgInitWide<T, P, NCH>(A, bm, st, b, x, r, rP, z, invDiagP, color0End, zeroGuess);
+        }
+        else
+        {
+            pcgInitNarrow<T, P, LANES, NCH>(A, bm, st, b, x, r, rP, z, invDiagP, color0End, zeroGuess);
+        }
+    }
+
+    template <typename P, typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP, const T * r, P * __restrict__ z, int color0End, int color0Blocks)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            pcgRhoWide<P, T, NCH>(M, bm, st, rP, r, z, color0End, color0Blocks);
+        }
+        else
+        {
+            pcgRhoNarrow<P, T, LANES, NCH>(M, bm, st, rP, r, z, color0End, color0Blocks);
+        }
+    }
+
+    template <typename T, typename P, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z, T * __restrict__ p, T * __restrict__ q)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            pcgDirectionWide<T, P, NCH>(A, bm, st, z, p, q);
+        }
+        else
+        {
+            pcgDirectionNarrow<T, P, LANES, NCH>(A, bm, st, z, p, q);
+        }
+    }
+
     // x += alpha p;  r -= alpha q;  z = r / d on colour 0 (preconditioner
     // type P; rP = (P) r when P is narrower than T); partial sums of |r|^2.
+    // fusedAlpha = 0: alpha comes from pcgDirectionFinalizeKernel.
+    // fusedAlpha = 1: the block finishes p.q itself from the direction
+    // kernel's partials of its system (every block of a system sums them in
+    // the same order, so all hold the same alpha = rho / p.q) while its first
+    // elements' operands are already loading: one launch less per iteration.
+    // (1 and 2 channels only.)
     template <typename T, typename P, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(int n, BlockMap bm, SolveState st, const T * __restrict__ p,
                                                               const T * __restrict__ q, T * __restrict__ x, T * r, P * rP,
-                                                              P * __restrict__ z, const P * __restrict__ invDiagP, int color0End)
+                                                              P * __restrict__ z, const P * __restrict__ invDiagP, int color0End,
+                                                              int fusedAlpha)
     {
         __shared__ double sm[NCH * kWaves];
+        __shared__ double shAlpha[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         // the block's rows, its system and the done flag load together
         const int4 range = bm.blocks[lb];
         const int s = range.z;
+        if (lb == 0 && threadIdx.x == 0)
+        {
+            st.flags[1] = 0; // the direction kernel before has used p = z
+        }
         if (st.flags[0])
         {
             return;
         }
-        double part[NCH];
-#pragma unroll
-        for (int c = 0; c < NCH; c++)
-        {
-            part[c] = 0;
-        }
+        // flat over the block's interleaved elements, one per thread: the
+        // stride (kBlock) is a multiple of NCH, so a thread keeps one channel
+        static_assert(kBlock % NCH == 0, "channels must divide the block");
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        double sum = 0;
         if (systemActive<NCH>(st, s))
         {
-            // flat over the block's interleaved elements, one per thread: the
-            // stride (kBlock) is a multiple of NCH, so a thread keeps one channel
-            static_assert(kBlock % NCH == 0, "channels must divide the block");
-            const int c = static_cast<int>(threadIdx.x) % NCH;
-            const T alpha = static_cast<T>(st.alpha[s * NCH + c]);
             const int act = st.active[s * NCH + c];
-            double sum = 0;
+            const long i0 = static_cast<long>(range.x) * NCH + threadIdx.x;
             const long end = static_cast<long>(range.y) * NCH;
-            for (long i = static_cast<long>(range.x) * NCH + threadIdx.x; i < end; i += kBlock)
+            // the first element's operands go out before alpha is known
+            T r0 = T(0), x0 = T(0), p0 = T(0), q0 = T(0);
+            P d0 = P(0);
+            const int row0 = static_cast<int>(i0 / NCH);
+            if (i0 < end)
             {
-                const int row = static_cast<int>(i / NCH);
-                T ri = r[i];
+                r0 = r[i0];
                 if (act)
                 {
-                    x[i] = fmaT(alpha, p[i], x[i]);
-                    ri = fmaT(-alpha, q[i], ri);
+                    x0 = x[i0];
+                    p0 = p[i0];
+                    q0 = q[i0];
+                }
+                if (row0 < color0End)
+                {
+                    d0 = invDiagP[row0];
+                }
+            }
+            T alpha;
+            if (!kWide<NCH> && fusedAlpha)
+            {
+                double rho[NCH];
+#pragma unroll
+                for (int k = 0; k < NCH; k++)
+                {
+                    rho[k] = st.rho[s * NCH + k];
+                }
+                int j0 = 0, j1 = bm.numBlocks;
+                if (bm.numSystems > 1)
+                {
+                    j0 = bm.sysBlockStart[s];
+                    j1 = bm.sysBlockStart[s + 1];
+                }
+                double pq[NCH];
+#pragma unroll
+                for (int k = 0; k < NCH; k++)
+                {
+                    pq[k] = 0;
+                }
+#pragma unroll 8
+                for (int j = j0 + static_cast<int>(threadIdx.x); j < j1; j += kBlock)
+                {
+#pragma unroll
+                    for (int k = 0; k < NCH; k++)
+                    {
+                        pq[k] += st.partials[static_cast<size_t>(j) * NCH + k];
+                    }
+                }
+                blockSum<NCH>(pq, sm);
+                if (threadIdx.x == 0)
+                {
+#pragma unroll
+                    for (int k = 0; k < NCH; k++)
+                    {
+                        shAlpha[k] = (st.active[s * NCH + k] && pq[k] != 0) ? rho[k] / pq[k] : 0.0;
+                    }
+                }
+                __syncthreads();
+                alpha = static_cast<T>(shAlpha[c]);
+            }
+            else
+            {
+                alpha = static_cast<T>(st.alpha[s * NCH + c]);
+            }
+            auto finish = [&](long i, int row, T ri, T xi, T pi, T qi, P di)
+            {
+                if (act)
+                {
+                    x[i] = fmaT(alpha, pi, xi);
+                    ri = fmaT(-alpha, qi, ri);
                     r[i] = ri;
                 }
                 const P rp = static_cast<P>(ri);
@@ -959,18 +1415,45 @@ namespace nsfem
                 }
                 if (row < color0End)
                 {
-                    z[i] = rp * invDiagP[row];
+                    z[i] = rp * di;
                 }
                 sum = fma(static_cast<double>(ri), static_cast<double>(ri), sum);
+            };
+            if (i0 < end)
+            {
+                finish(i0, row0, r0, x0, p0, q0, d0);
             }
+            for (long i = i0 + kBlock; i < end; i += kBlock)
+            {
+                const int row = static_cast<int>(i / NCH);
+                const T ri = r[i];
+                T xi = T(0), pi = T(0), qi = T(0);
+                if (act)
+                {
+                    xi = x[i];
+                    pi = p[i];
+                    qi = q[i];
+                }
+                const P di = row < color0End ? invDiagP[row] : P(0);
+                finish(i, row, ri, xi, pi, qi, di);
+            }
+        }
+        if constexpr (kWide<NCH>)
+        {
+            const double total = channelBlockSum<NCH>(sum, sm);
+            storeChannelPartial<NCH, NCH>(total, st.partials2, range.w, 0);
+        }
+        else
+        {
+            double part[NCH];
 #pragma unroll
             for (int cc = 0; cc < NCH; cc++)
             {
                 part[cc] = cc == c ? sum : 0.0;
             }
+            blockSum<NCH>(part, sm);
+            storePartial<NCH>(part, st.partials2, range.w);
         }
-        blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials2, range.w);
     }
 
     // Per-system RMS residuals of gsResidualKernel, stop test and flags.

@@ -12,6 +12,16 @@ namespace nsfem
         template <typename F>
         void dispatchLC(int lanes, int nch, F && f)
         {
+            // wide channel tiles: one lane per channel, the lane count is the
+            // channel count
+            switch (nch)
+            {
+            case 4: f.template operator()<4, 4>(); return;
+            case 8: f.template operator()<8, 8>(); return;
+            case 16: f.template operator()<16, 16>(); return;
+            case 32: f.template operator()<32, 32>(); return;
+            default: break;
+            }
             auto byC = [&]<int L>()
             {
                 if (nch == 1)
@@ -24,7 +34,7 @@ namespace nsfem
                 }
                 else
                 {
-                    throw std::invalid_argument("Only 1 and 2 channel solvers are supported");
+                    throw std::invalid_argument("solver channels must be 1, 2, 4, 8, 16 or 32");
                 }
             };
             switch (lanes)
@@ -36,6 +46,24 @@ namespace nsfem
             default: throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
             }
         }
+
+        // f<C>() for the channel count
+        template <typename F>
+        void dispatchC(int nch, F && f)
+        {
+            switch (nch)
+            {
+            case 1: f.template operator()<1>(); break;
+            case 2: f.template operator()<2>(); break;
+            case 4: f.template operator()<4>(); break;
+            case 8: f.template operator()<8>(); break;
+            case 16: f.template operator()<16>(); break;
+            case 32: f.template operator()<32>(); break;
+            default: throw std::invalid_argument("solver channels must be 1, 2, 4, 8, 16 or 32");
+            }
+        }
+
+        bool validChannelCount(int nch) { return nch == 1 || nch == 2 || nch == 4 || nch == 8 || nch == 16 || nch == 32; }
 
         template <typename T>
         void uploadMatrix(DeviceMcgs<T> & d, const std::vector<int> & rowStart, const std::vector<int> & rowSplit,
@@ -136,9 +164,9 @@ namespace nsfem
         {
             throw std::invalid_argument("Solver needs at least one matrix");
         }
-        if (opt_.numCh < 1 || opt_.numCh > 2)
+        if (!validChannelCount(opt_.numCh))
         {
-            throw std::invalid_argument("Only 1 and 2 ch solvers are supported");
+            throw std::invalid_argument("solver channels must be 1, 2, 4, 8, 16 or 32");
         }
         if (opt_.mseCheckInterval < 1)
         {
@@ -380,7 +408,7 @@ namespace nsfem
 
     void McgsSolver::reshape(int lanes, int rowsPerBlock)
     {
-        if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+        if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && !(opt_.numCh > 2 && lanes == opt_.numCh))
         {
             throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
         }
@@ -389,9 +417,10 @@ namespace nsfem
             throw std::invalid_argument("rowsPerBlock must be >= 0");
         }
         NSFEM_HIP(hipStreamSynchronize(stream_));
-        lanes_ = lanes > 0 ? lanes : std::min(16, lanesForMeanRow(meanRow_));
+        // wide channel tiles run one lane per channel: the lane count is fixed
+        lanes_ = opt_.numCh > 2 ? opt_.numCh : (lanes > 0 ? lanes : std::min(16, lanesForMeanRow(meanRow_)));
         const int slab = kBlock / lanes_;
-        int rpb = rowsPerBlock > 0 ? rowsPerBlock : 4 * slab;
+        int rpb = rowsPerBlock > 0 ? rowsPerBlock : std::max(4 * slab, minRowsPerBlock());
         rpb = std::max(slab, rpb / slab * slab);
         rowsPerBlock_ = rpb;
 
@@ -455,6 +484,43 @@ namespace nsfem
         syncLibraryStream();
     }
 
+    int McgsSolver::minRowsPerBlock() const
+    {
+        // Wide channel tiles do NCH times the work per row: the default and
+        // autotuned shapes keep at most about 16k workgroups, so that the
+        // one-workgroup finalizes read a bounded number of partial sums.
+        if (opt_.numCh <= 2)
+        {
+            return 1;
+        }
+        const int slab = kBlock / opt_.numCh;
+        const int need = (n_ + 16383) / 16384;
+        return (need + slab - 1) / slab * slab;
+    }
+
+    void McgsSolver::setFusedAlpha(int mode)
+    {
+        if (mode < -1 || mode > 1)
+        {
+            throw std::invalid_argument("fusedAlpha must be -1 (auto), 0 or 1");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        fusedAlphaMode_ = mode;
+        for (ChunkGraph * g : {&graph64_, &graph32_, &graphMixed_})
+        {
+            if (g->exec)
+            {
+                NSFEM_HIP(hipGraphExecDestroy(g->exec));
+                g->exec = nullptr;
+            }
+        }
+    }
+
+    bool McgsSolver::fusedAlpha() const
+    {
+        return fusedAlphaMode_ == 1 && opt_.numCh <= 2;
+    }
+
     double McgsSolver::iterationTime(int reps)
     {
         double total = 0;
@@ -475,11 +541,16 @@ namespace nsfem
         }
         double best = 0;
         int bestLanes = lanes_, bestRows = rowsPerBlock_;
-        for (int lanes : {2, 4, 8, 16})
+        const std::vector<int> laneChoices = opt_.numCh > 2 ? std::vector<int>{opt_.numCh} : std::vector<int>{2, 4, 8, 16};
+        for (int lanes : laneChoices)
         {
             for (int slabs : {1, 2, 4, 8})
             {
-                reshape(lanes, slabs * (kBlock / lanes));
+                const int slab = kBlock / lanes;
+                // wide tiles: from the fewest rows per workgroup that keep the
+                // finalize kernels' partial sums bounded
+                const int base = opt_.numCh > 2 ? std::max(slab, minRowsPerBlock()) : slab;
+                reshape(lanes, slabs * base);
                 const double t = iterationTime(reps);
                 if (best == 0 || t < best)
                 {
@@ -695,19 +766,20 @@ namespace nsfem
     {
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
+        const int fused = fusedAlpha() ? 1 : 0;
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
                        pcgDirectionKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.z, v.p, v.q);
-                       pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
+                       if (!fused)
+                       {
+                           pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
+                       }
                    });
-        if (nch == 1)
-        {
-            pcgUpdateKernel<T, P, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag, color0End_);
-        }
-        else
-        {
-            pcgUpdateKernel<T, P, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag, color0End_);
-        }
+        dispatchC(nch, [&]<int C>()
+                  {
+                      pcgUpdateKernel<T, P, C><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                            color0End_, fused);
+                  });
         // the update's convergence test is finished together with rho
         sgsSweeps<P>(s, v.M, st, v.rP, v.z);
         launchRho<T, P>(s, v, st, totals, true);
@@ -951,18 +1023,12 @@ namespace nsfem
     void McgsSolver::projectMean(hipStream_t s, T * v)
     {
         // reuse the inner state's reduction scratch (no solve is in flight)
-        if (opt_.numCh == 1)
-        {
-            meanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
-            meanFinalizeKernel<1><<<1, kFinBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
-            subtractMeanKernel<T, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
-        }
-        else
-        {
-            meanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
-            meanFinalizeKernel<2><<<1, kFinBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
-            subtractMeanKernel<T, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
-        }
+        dispatchC(opt_.numCh, [&]<int C>()
+                  {
+                      meanKernel<T, C><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, inner_.partials.get());
+                      meanFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, inner_.partials.get(), inner_.totals.get(), means_.get());
+                      subtractMeanKernel<T, C><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, v, means_.get());
+                  });
         NSFEM_HIP(hipGetLastError());
     }
 
@@ -1069,28 +1135,24 @@ namespace nsfem
                 // x, r read and written, p, q read; rP written (P narrower
                 // than T); z = r / d on colour 0
                 bytes = 6.0 * nch * n * tsT + (same ? 0.0 : nch * n * tsP) + n0 * tsP + nch * n0 * tsP;
+                const int fused = fusedAlpha() ? 1 : 0;
                 launch = [=, this]()
                 {
-                    if (nch == 1)
-                        pcgUpdateKernel<T, P, 1><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                              color0End_);
-                    else
-                        pcgUpdateKernel<T, P, 2><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                              color0End_);
+                    dispatchC(nch, [&]<int C>()
+                              {
+                                  pcgUpdateKernel<T, P, C><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z,
+                                                                                        v.M.invDiag, color0End_, fused);
+                              });
                 };
             }
             else if (which == "finalize")
             {
-                // one of the two per-iteration one-block finalize kernels
-                launches = 2;
+                // one of the per-iteration one-block finalize kernels (two,
+                // one when the update kernel finishes alpha itself)
+                launches = fusedAlpha() ? 1 : 2;
                 bytes = static_cast<double>(numBlocks_) * nch * 8.0;
                 launch = [=, this]()
-                {
-                    if (nch == 1)
-                        pcgDirectionFinalizeKernel<1><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
-                    else
-                        pcgDirectionFinalizeKernel<2><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
-                };
+                { dispatchC(nch, [&]<int C>() { pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals); }); };
             }
             else if (which == "iteration" || which == "iteration_graph")
             {

@@ -148,6 +148,14 @@ namespace nsfem
 
         hipStream_t stream() const { return stream_; }
         void setUseGraphs(bool on) { useGraphs_ = on; }
+        // Step length alpha finished inside the update kernel (1) or by its
+        // own one-block finalize kernel (0, default; -1 picks the same).
+        // Every update block then re-reads all direction partials of its
+        // system: at 500k rows that costs about what the launch saves.
+        void setFusedAlpha(int mode);
+        bool fusedAlpha() const;
+        // Fewest rows per workgroup of the default and autotuned shapes.
+        int minRowsPerBlock() const;
 
         // Average duration (ms) of one launch of a solver kernel, timed with
         // HIP events on the solver's stream around reps back-to-back
@@ -160,13 +168,17 @@ namespace nsfem
         double iterationTime(int reps);
 
         // Launch shape: lanes per matrix row (2, 4, 8, 16; 0 picks from the mean
-        // row length) and rows per workgroup (0: four row slabs).  Rebuilds
-        // the block map only; the colouring and matrices are kept.
+        // row length; wide channel tiles always run one lane per channel) and
+        // rows per workgroup (0: four row slabs, for wide tiles at least
+        // minRowsPerBlock()).  Rebuilds the block map only; the colouring and
+        // matrices are kept.
         void reshape(int lanes, int rowsPerBlock);
         int lanes() const { return lanes_; }
         int rowsPerBlock() const { return rowsPerBlock_; }
         // Times iterationTime for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs per
-        // workgroup and keeps the fastest shape; returns (lanes, rowsPerBlock).
+        // workgroup (wide tiles: their own lanes x {1, 2, 4, 8} times
+        // minRowsPerBlock()) and keeps the fastest shape; returns (lanes,
+        // rowsPerBlock).
         // Overwrites the internal work vectors (not the io vectors).
         std::pair<int, int> autotune(int reps);
 
@@ -226,6 +238,7 @@ namespace nsfem
         };
         std::vector<ColorStats> colorStats_;
         int numBlocks_ = 0;
+        int fusedAlphaMode_ = 0;
 
         DeviceArray<int> perm_;
         DeviceArray<int4> blocks_;

@@ -310,6 +310,12 @@ class Chorin:
         call("nsfem_chorin_autotune", self._h, int(reps), out)
         return tuple(out)
 
+    def set_solver_option(self, key, value, which="both"):
+        """Solver.setOption for the step's solvers: which = "velocity",
+        "pressure" or "both"."""
+        w = {"velocity": 0, "pressure": 1, "both": 2}[which]
+        call("nsfem_chorin_set_solver_option", self._h, w, key.encode(), float(value))
+
     def assemble_convection(self, u_host):
         """Returns (A, C) values in the velocity pattern for a host velocity."""
         from .cu import vec

@@ -235,11 +235,15 @@ def test_kernel_timing_hooks_report_per_launch_figures(octx_h025):
     b = rng.uniform(-1, 1, m.shape[0])
     s, _ = _solve("pcg", m, b, max_iters=2000, target=1e-10)
     C = s.numColors()
-    expect = {"direction": 1, "sweep": 2 * C - 3, "rho": 1, "update": 1, "finalize": 2}
     nnz = m.nnz
-    for k, launches in expect.items():
-        ms, by, per_it = s.timeKernel(k, reps=3)
-        assert ms > 0 and by > 0 and per_it == launches, k
+    # alpha by its own finalize kernel by default, or inside the update kernel
+    for fused in (-1, 0, 1):
+        s.setOption("fusedAlpha", fused)
+        expect = {"direction": 1, "sweep": 2 * C - 3, "rho": 1, "update": 1, "finalize": 1 if fused == 1 else 2}
+        for k, launches in expect.items():
+            ms, by, per_it = s.timeKernel(k, reps=3)
+            assert ms > 0 and by > 0 and per_it == launches, (k, fused)
+    s.setOption("fusedAlpha", -1)
     # a direction launch streams the whole off-diagonal matrix once
     assert s.timeKernel("direction", reps=1)[1] >= (nnz - m.shape[0]) * 12
     s.getRhs().upload(b)
@@ -300,3 +304,133 @@ def test_reshape_and_autotune_keep_results(octx_h025):
     from nsfem_amd import _lib
     with pytest.raises(_lib.NsfemInvalidArgument):
         s.reshape(3, 0)
+
+
+@pytest.mark.parametrize("name,nch", [("pcg", 1), ("pcg", 2), ("pcg_mixed", 1), ("pcg_mixed", 2)])
+def test_fused_alpha_matches_finalize_kernel(octx_h025, name, nch):
+    """alpha finished inside the update kernel (every block sums its system's
+    p.q partials) or by the one-block finalize kernel: the two sums differ
+    only in order, so both converge in the same iterations (+-1) to the same
+    solution, and the fused mode repeats bit for bit."""
+    m = octx_h025["Kp_int"]
+    n = m.shape[0]
+    rng = np.random.default_rng(35)
+    b = rng.uniform(-1, 1, nch * n)
+    runs = {}
+    for fused in (0, 1, 1):
+        s, x = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10, fusedAlpha=fused)
+        its = s.getLastIterations()
+        if fused in runs:
+            assert its == runs[fused][1] and np.array_equal(x, runs[fused][0])
+        runs[fused] = (x, its)
+        for c in range(nch):
+            bc = b[c * n:(c + 1) * n]
+            assert np.linalg.norm(bc - m @ x[c * n:(c + 1) * n]) <= 1.0001e-10 * np.linalg.norm(bc)
+    (x0, it0), (x1, it1) = runs[0], runs[1]
+    assert abs(it0 - it1) <= 1
+    assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def test_fused_alpha_batch_of_uneven_systems(oracle):
+    """Fused alpha per system in a batch: each block sums only its own
+    system's partials, and every system converges as with the finalize
+    kernel (pcg: in the oracle's iterations +-1; pcg_mixed needs more
+    iterations than FP64 PCG at 1e-11, so it is compared with its own
+    unfused run)."""
+    from nsfem_amd import cu, fem, mesher
+    mats = []
+    for k, n in enumerate([600, 2500, 1300]):
+        m = mesher.channel_mesh(n, seed=100 + k)
+        ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), fem.DfgConditions(0.01, 1.0, 1.0, 4.0))
+        mats.append(ctx.scipy("pressureStiffnessInternal"))
+    rng = np.random.default_rng(36)
+    bs = [rng.uniform(-1, 1, m.shape[0]) for m in mats]
+    for name in ("pcg", "pcg_mixed"):
+        its = {}
+        for fused in (0, 1):
+            s = cu.createBatchSolver(name, 1, mats, 3000, 1e-11, 1)
+            s.setOption("fusedAlpha", fused)
+            s.getRhs().upload(np.concatenate(bs))
+            s.getSol().memsetZero()
+            s.solve()
+            x = s.getSol().download()
+            its[fused] = [s.getLastIterations(k) for k in range(len(mats))]
+            for k, (m, b) in enumerate(zip(mats, bs)):
+                o = s.system_offsets[k]
+                xk = x[o:o + m.shape[0]]
+                assert np.linalg.norm(b - m @ xk) <= 1.0001e-11 * np.linalg.norm(b)
+                if name == "pcg":
+                    xo, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-11, 3000)
+                    assert abs(its[fused][k] - it) <= 1, (fused, k)
+        assert all(abs(a - b) <= 1 for a, b in zip(its[0], its[1])), its
+
+
+@pytest.mark.parametrize("name", ["pcg", "pcg_mixed"])
+@pytest.mark.parametrize("nch", [4, 8, 32])
+def test_wide_channel_tiles_match_single_channel_solves(octx_h025, name, nch):
+    """4 to 32 right-hand sides sharing the matrix, one lane per channel:
+    every column reaches the tolerance and equals its own one-channel solve
+    (columns scaled over six decades, so each stops on its own test).  FP64
+    pcg needs the iterations of the slowest column; pcg_mixed counts jitter
+    by up to ~30 % on a system this small (the FP32 preconditioner's
+    rounding), so only its results are compared."""
+    m = octx_h025["Kp_int"]
+    n = m.shape[0]
+    rng = np.random.default_rng(37)
+    scale = np.repeat(10.0 ** rng.uniform(-3, 3, nch), n)
+    b = rng.uniform(-1, 1, nch * n) * scale
+    s, x = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10)
+    for c in range(nch):
+        bc = b[c * n:(c + 1) * n]
+        assert np.linalg.norm(bc - m @ x[c * n:(c + 1) * n]) <= 1.0001e-10 * np.linalg.norm(bc), c
+        assert s.getLastRelRes(c) <= 1e-10
+    its = []
+    for c in (0, nch // 2, nch - 1):
+        s1, x1 = _solve(name, m, b[c * n:(c + 1) * n], max_iters=3000, target=1e-10)
+        its.append(s1.getLastIterations())
+        assert np.linalg.norm(x[c * n:(c + 1) * n] - x1) <= 1e-8 * np.linalg.norm(x1), c
+    if name == "pcg":
+        assert s.getLastIterations() >= max(its) - 1
+
+
+@pytest.mark.parametrize("nch", [4, 16])
+def test_wide_channel_gs_matches_oracle_sweeps(oracle, octx_c3, nch):
+    """Wide-tile GS sweeps: 25 sweeps of every column equal the oracle's
+    sequential GS in colouring order.  The RMS residual (~1e-10 of |b|) is
+    a difference of O(1) terms: it agrees to ~1e-6 relative."""
+    m = octx_c3["M"]
+    n = m.shape[0]
+    rng = np.random.default_rng(38)
+    b = rng.uniform(-1, 1, nch * n)
+    s, x = _solve("gs", m, b, nch=nch, max_iters=25, target=1e-30, interval=5)
+    L = oracle.Mcgs(m)
+    for c in range(nch):
+        xo, mse, it = oracle.gs_solve(m, b[c * n:(c + 1) * n], np.zeros(n), 25, 0.0, 5, L)
+        assert s.getLastIterations() == it
+        assert abs(s.getLastMse(c) - mse) <= 1e-6 * mse
+        assert np.max(np.abs(x[c * n:(c + 1) * n] - xo)) <= 1e-12 * np.max(np.abs(xo)), c
+
+
+def test_wide_channel_pure_neumann_batch(octx_h025):
+    """8 zero-mean right-hand sides on the singular Laplacian with mean
+    projection (the multi-right-hand-side stress configuration in small):
+    pcg_mixed to 1e-8 per column, zero-mean solutions; autotuning keeps the
+    one-lane-per-channel shape."""
+    from nsfem_amd import cu
+    m = octx_h025["Kp"]
+    n = m.shape[0]
+    nch = 8
+    rng = np.random.default_rng(39)
+    b = rng.uniform(-1, 1, (nch, n))
+    b -= b.mean(axis=1, keepdims=True)
+    s = cu.createSolver("pcg_mixed", nch, m, 5000, 1e-8, 1)
+    s.setOption("projectMean", 1)
+    L, R = s.autotune(reps=2)
+    assert L == nch and R % (256 // nch) == 0
+    s.getRhs().upload(b.ravel())
+    s.getSol().memsetZero()
+    s.solve()
+    x = s.getSol().download().reshape(nch, n)
+    for c in range(nch):
+        assert abs(x[c].mean()) <= 1e-12 * np.max(np.abs(x[c]))
+        assert np.linalg.norm(b[c] - m @ x[c]) <= 1.0001e-8 * np.linalg.norm(b[c])
