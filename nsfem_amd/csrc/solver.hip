@@ -1047,7 +1047,9 @@ namespace nsfem
         hipStream_t s = stream_;
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
-        SolveState st = inner_.view(K, 0.0);
+        // a negative tolerance: the convergence test of a timed iteration
+        // never stops it, whatever the work vectors hold
+        SolveState st = inner_.view(K, -1.0);
         double * totals = inner_.totals.get();
         double nnzOff = 0;
         for (const auto & cs : colorStats_)
@@ -1067,6 +1069,18 @@ namespace nsfem
             constexpr bool same = std::is_same_v<T, P>;
             const double tsT = sizeof(T), tsP = sizeof(P);
             const PcgOperands<T, P> v = operands<T, P>();
+            // finite work vectors (a fresh solver's memory is arbitrary)
+            const long nv = static_cast<long>(n_) * nch;
+            const int dtT = std::is_same_v<T, double> ? F64 : F32;
+            const int dtP = std::is_same_v<P, double> ? F64 : F32;
+            for (T * ptr : {v.x, v.r, v.p, v.q})
+            {
+                fill(dtT, nv, 0.0, ptr, s);
+            }
+            for (P * ptr : {v.rP, v.z})
+            {
+                fill(dtP, nv, 0.0, ptr, s);
+            }
             // Algorithmic bytes of one launch, x read once (ideal reuse).
             auto sweepBytes = [&](int c, bool lower)
             {
