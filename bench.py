@@ -89,7 +89,9 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune):
            "ms": {"convection": agg["msConvection"], "tentative": agg["msTentative"], "pressure": agg["msPressure"],
                   "final": agg["msFinal"]},
            "shape": {"velocity_lanes": shape[0], "velocity_rows_per_block": shape[1], "pressure_lanes": shape[2],
-                     "pressure_rows_per_block": shape[3], "autotuned": bool(autotune), "autotune_s": tune_s}}
+                     "pressure_rows_per_block": shape[3], "velocity_sweep_lanes": shape[4],
+                     "velocity_sweep_rows_per_block": shape[5], "pressure_sweep_lanes": shape[6],
+                     "pressure_sweep_rows_per_block": shape[7], "autotuned": bool(autotune), "autotune_s": tune_s}}
     log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
         f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
     return ch, res, (u1, p1), (u, p)
@@ -249,6 +251,7 @@ def run_poisson(args):
         s = cu.createSolver(method, 1, K, 50000, 1e-10, 1)
         s.setOption("projectMean", 1)
         shape = s.autotune(10) if not args.no_autotune else s.shape()
+        shape = tuple(shape) + s.sweepShape()
         times, iters = [], []
         for rep in range(args.warmup + args.steps):
             s.getRhs().upload(b)
@@ -332,7 +335,8 @@ def run_multi_rhs(args):
         s = cu.createSolver("pcg_mixed", width, K, 50000, tol, 1)
         s.setOption("projectMean", 1)
         if shape is not None:
-            s.reshape(*shape)
+            s.reshape(*shape[:2])
+            s.reshapeSweeps(*shape[2:])
         elif not args.no_autotune:
             s.autotune(5)
         return s
@@ -345,9 +349,9 @@ def run_multi_rhs(args):
         for k in ("direction", "sweep", "rho", "update", "finalize"):
             ms, by, per_it = s.timeKernel(k, reps=10)
             kern[k] = {"avg_ms": ms, "bytes": by, "launches_per_iteration": per_it, "gbps": by / (ms * 1e-3) / 1e9}
-        tiles[width] = {"shape": s.shape(), "ms_per_iteration": ms_it, "ms_per_column_iteration": ms_it / width,
-                        "kernels": kern}
-        log(f"  tile {width:2d}: shape {s.shape()}, {ms_it:.3f} ms per iteration, {ms_it / width * 1e3:.1f} us per column, "
+        tiles[width] = {"shape": s.shape() + s.sweepShape(), "ms_per_iteration": ms_it,
+                        "ms_per_column_iteration": ms_it / width, "kernels": kern}
+        log(f"  tile {width:2d}: shape {tiles[width]['shape']}, {ms_it:.3f} ms per iteration, {ms_it / width * 1e3:.1f} us per column, "
             f"direction {kern['direction']['gbps']:.0f} GB/s, sweep {kern['sweep']['gbps']:.0f} GB/s")
         del s
     best = min(tiles, key=lambda w: tiles[w]["ms_per_column_iteration"])
@@ -400,7 +404,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--shape", default="0,0,0,0",
-                    help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default)")
+                    help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default), "
+                         "optionally followed by the same four for the colour sweeps")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
     ap.add_argument("--kernels-only", action="store_true",
@@ -451,10 +456,14 @@ def main():
     m, cond, ctx = build_workload(args.nodes, args.seed, gpu_assembly=not args.host_assembly)
     u0 = initial_velocity(ctx, cond, 1)
     target = 1e-10
-    vl, vr, pl, pr = (int(v) for v in args.shape.split(","))
+    shape = [int(v) for v in args.shape.split(",")]
+    if len(shape) not in (4, 8):
+        raise SystemExit("--shape takes 4 or 8 numbers")
+    vl, vr, pl, pr, vsl, vsr, psl, psr = shape + [0] * (8 - len(shape))
 
     def solvers(method):
-        return (fem.SolverConfig(method, 5000, target, 1, vl, vr), fem.SolverConfig(method, 50000, target, 1, pl, pr))
+        return (fem.SolverConfig(method, 5000, target, 1, vl, vr, vsl, vsr),
+                fem.SolverConfig(method, 50000, target, 1, pl, pr, psl, psr))
 
     eager = 1 if args.eager else 0
     cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager)
@@ -466,9 +475,7 @@ def main():
         for name in paths:
             ch = fem.Chorin(ctx, {"fp64": cfg64, "mixed": cfgmx}[name])
             ch.upload_velocity(u0)
-            # no whole-iteration replays: with no step run the solver state is
-            # zero and those iterations would stop at once
-            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps, with_iteration=False)
+            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps)
             out[f"{name}_shape"] = ch.shape()
             del ch
         print(json.dumps(out))

@@ -165,15 +165,19 @@ int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, dou
 int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
 /* Launch shape of the solver kernels: lanes per matrix row (2, 4, 8, 16; 0
  * picks from the mean row length) and rows per workgroup (0: default).
- * reshape rebuilds the workgroup map only (colouring and matrices kept).
- * autotune times one PCG iteration's launches (pcg solvers, reps back-to-back
- * replays per kernel) for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs and keeps
- * the fastest; it overwrites the internal work vectors, not rhs/sol.  Solvers
- * of 4 to 32 channels run one lane per channel (lanes = numCh) and keep at
- * most about 16k workgroups.  No reference counterpart (the reference's
- * launch shapes are fixed). */
+ * reshape rebuilds the workgroup map only (colouring and matrices kept) and
+ * gives the colour sweeps the same shape; reshape_sweeps then sets the
+ * sweeps' own (they write no partial sums).  autotune times one PCG
+ * iteration replayed from a captured graph (pcg solvers, reps iterations)
+ * for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs, keeps the fastest, then
+ * tries the sweeps on lanes x {1, 2, 4} slabs of their own; it overwrites the
+ * internal work vectors, not rhs/sol.  Solvers of 4 to 32 channels run one
+ * lane per channel (lanes = numCh) and keep at most about 16k workgroups.
+ * No reference counterpart (the reference's launch shapes are fixed). */
 int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock);
 int nsfem_solver_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock);
+int nsfem_solver_reshape_sweeps(nsfem_solver * s, int lanes, int rowsPerBlock);
+int nsfem_solver_sweep_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock);
 int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock);
 int nsfem_solver_coloring(const nsfem_solver * s, int system, int * coloring, int * partitionStart);
 
@@ -268,6 +272,8 @@ typedef struct nsfem_solver_config
     int mseCheckInterval;
     int lanes;        /* lanes per matrix row: 2, 4, 8, 16 (0: from the mean row length) */
     int rowsPerBlock; /* rows per workgroup of the solver kernels (0: default) */
+    int sweepLanes;   /* the colour sweeps' own shape (0, 0: as lanes, rowsPerBlock) */
+    int sweepRowsPerBlock;
 } nsfem_solver_config;
 
 typedef struct nsfem_chorin_config
@@ -303,9 +309,10 @@ int nsfem_chorin_download(nsfem_chorin * ch, void * velocity, void * pressure);
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);
-/* Launch shapes of the step's solvers, shape[4] = {velocity lanes, velocity
- * rows per workgroup, pressure lanes, pressure rows per workgroup}; autotune
- * as nsfem_solver_autotune for both solvers (the state u, p is kept). */
+/* Launch shapes of the step's solvers, shape[8] = {velocity lanes, velocity
+ * rows per workgroup, pressure lanes, pressure rows per workgroup, then the
+ * same four for the colour sweeps}; autotune as nsfem_solver_autotune for
+ * both solvers (the state u, p is kept). */
 int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
 int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
 /* nsfem_solver_set_option for the step's solvers: which = 0 velocity,

@@ -59,7 +59,8 @@ class DfgConditions(ctypes.Structure):
 class SolverConfig(ctypes.Structure):
     _fields_ = [("method", ctypes.c_char_p), ("maxIterations", ctypes.c_int),
                 ("targetMse", ctypes.c_double), ("mseCheckInterval", ctypes.c_int),
-                ("lanes", ctypes.c_int), ("rowsPerBlock", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("rowsPerBlock", ctypes.c_int),
+                ("sweepLanes", ctypes.c_int), ("sweepRowsPerBlock", ctypes.c_int)]
 
 
 class ChorinConfigC(ctypes.Structure):
@@ -127,6 +128,8 @@ _SIGNATURES = {
     "nsfem_solver_num_colors": [_vp, ctypes.c_int, _c_int_p],
     "nsfem_solver_reshape": [_vp, ctypes.c_int, ctypes.c_int],
     "nsfem_solver_shape": [_vp, _c_int_p, _c_int_p],
+    "nsfem_solver_reshape_sweeps": [_vp, ctypes.c_int, ctypes.c_int],
+    "nsfem_solver_sweep_shape": [_vp, _c_int_p, _c_int_p],
     "nsfem_solver_autotune": [_vp, ctypes.c_int, _c_int_p, _c_int_p],
     "nsfem_solver_coloring": [_vp, ctypes.c_int, _vp, _vp],
     "nsfem_mcgs_coloring": [ctypes.c_int, _vp, _vp, _vp, _vp, _c_int_p],

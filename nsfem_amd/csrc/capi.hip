@@ -671,6 +671,23 @@ extern "C"
             if (rowsPerBlock) *rowsPerBlock = s->core->rowsPerBlock(); });
     }
 
+    int nsfem_solver_reshape_sweeps(nsfem_solver * s, int lanes, int rowsPerBlock)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            s->core->reshapeSweeps(lanes, rowsPerBlock); });
+    }
+
+    int nsfem_solver_sweep_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            if (lanes) *lanes = s->core->sweepLanes();
+            if (rowsPerBlock) *rowsPerBlock = s->core->sweepRowsPerBlock(); });
+    }
+
     int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock)
     {
         return guarded([&]
@@ -1042,7 +1059,9 @@ extern "C"
                 d.checkInterval = s.mseCheckInterval;
                 d.lanes = s.lanes;
                 d.rowsPerBlock = s.rowsPerBlock;
-                if (d.rowsPerBlock < 0) throw std::invalid_argument("rowsPerBlock must be >= 0");
+                d.sweepLanes = s.sweepLanes;
+                d.sweepRowsPerBlock = s.sweepRowsPerBlock;
+                if (d.rowsPerBlock < 0 || d.sweepRowsPerBlock < 0) throw std::invalid_argument("rowsPerBlock must be >= 0");
                 if (d.checkInterval < 1) throw std::invalid_argument("Invalid MSE check interval for solver: " + std::to_string(d.checkInterval));
             };
             conv(cfg->velocitySolver, cc.velocity);
@@ -1146,18 +1165,22 @@ extern "C"
             ch->core->resetState(); });
     }
 
+    void chorinShape(const nsfem_chorin & ch, int * shape)
+    {
+        const McgsSolver & v = ch.core->velocitySolver();
+        const McgsSolver & p = ch.core->pressureSolver();
+        const int s[8] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
+                          v.sweepLanes(), v.sweepRowsPerBlock(), p.sweepLanes(), p.sweepRowsPerBlock()};
+        std::copy(s, s + 8, shape);
+    }
+
     int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape)
     {
         return guarded([&]
                        {
             need(ch, "chorin");
             need(shape, "shape");
-            McgsSolver & v = ch->core->velocitySolver();
-            McgsSolver & p = ch->core->pressureSolver();
-            shape[0] = v.lanes();
-            shape[1] = v.rowsPerBlock();
-            shape[2] = p.lanes();
-            shape[3] = p.rowsPerBlock(); });
+            chorinShape(*ch, shape); });
     }
 
     int nsfem_chorin_set_solver_option(nsfem_chorin * ch, int which, const char * key, double value)
@@ -1181,12 +1204,7 @@ extern "C"
             ch->core->pressureSolver().autotune(reps);
             if (shape)
             {
-                McgsSolver & v = ch->core->velocitySolver();
-                McgsSolver & p = ch->core->pressureSolver();
-                shape[0] = v.lanes();
-                shape[1] = v.rowsPerBlock();
-                shape[2] = p.lanes();
-                shape[3] = p.rowsPerBlock();
+                chorinShape(*ch, shape);
             } });
     }
 

@@ -183,6 +183,8 @@ namespace nsfem
             o.innerTol = cfg_.innerTol;
             o.lanes = sc.lanes;
             o.rowsPerBlock = sc.rowsPerBlock;
+            o.sweepLanes = sc.sweepLanes;
+            o.sweepRowsPerBlock = sc.sweepRowsPerBlock;
             return o;
         };
         // One solver system per instance (per-instance colouring and

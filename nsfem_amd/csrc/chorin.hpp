@@ -33,6 +33,8 @@ namespace nsfem
         int checkInterval = 1;
         int lanes = 0;        // 0: from the mean row length
         int rowsPerBlock = 0; // 0: solver default
+        int sweepLanes = 0;   // colour sweeps (0: as lanes / rowsPerBlock)
+        int sweepRowsPerBlock = 0;
     };
 
     struct ChorinConfig

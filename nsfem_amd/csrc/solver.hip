@@ -355,6 +355,11 @@ namespace nsfem
         }
         means_.resize(static_cast<size_t>(numSystems_) * opt_.numCh);
         reshape(opt_.lanes, opt_.rowsPerBlock);
+        if (opt_.sweepLanes != 0 || opt_.sweepRowsPerBlock != 0)
+        {
+            reshapeSweeps(opt_.sweepLanes != 0 ? opt_.sweepLanes : lanes_,
+                          opt_.sweepRowsPerBlock != 0 ? opt_.sweepRowsPerBlock : (opt_.sweepLanes != 0 ? 0 : rowsPerBlock_));
+        }
 
         const int K = numSystems_ * opt_.numCh;
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostFlags_), 4 * sizeof(int), hipHostMallocDefault));
@@ -406,7 +411,7 @@ namespace nsfem
         }
     }
 
-    void McgsSolver::reshape(int lanes, int rowsPerBlock)
+    std::pair<int, int> McgsSolver::resolveShape(int lanes, int rowsPerBlock, bool sweeps) const
     {
         if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && !(opt_.numCh > 2 && lanes == opt_.numCh))
         {
@@ -416,21 +421,26 @@ namespace nsfem
         {
             throw std::invalid_argument("rowsPerBlock must be >= 0");
         }
-        NSFEM_HIP(hipStreamSynchronize(stream_));
         // wide channel tiles run one lane per channel: the lane count is fixed
-        lanes_ = opt_.numCh > 2 ? opt_.numCh : (lanes > 0 ? lanes : std::min(16, lanesForMeanRow(meanRow_)));
-        const int slab = kBlock / lanes_;
-        int rpb = rowsPerBlock > 0 ? rowsPerBlock : std::max(4 * slab, minRowsPerBlock());
+        const int L = opt_.numCh > 2 ? opt_.numCh : (lanes > 0 ? lanes : std::min(16, lanesForMeanRow(meanRow_)));
+        const int slab = kBlock / L;
+        // the sweeps write no partial sums: any row count goes
+        int rpb = rowsPerBlock > 0 ? rowsPerBlock : (sweeps ? 4 * slab : std::max(4 * slab, minRowsPerBlock()));
         rpb = std::max(slab, rpb / slab * slab);
-        rowsPerBlock_ = rpb;
+        return {L, rpb};
+    }
 
-        std::vector<int4> blocks;
-        std::vector<int> blockSystem;
-        colorBlockStart_.assign(numColors_ + 1, 0);
+    // Workgroups of at most rpb rows, colour-major, never straddling a
+    // (colour, system) chunk; colorStart[c] = first workgroup of colour c.
+    void McgsSolver::buildBlocks(int rpb, std::vector<int4> & blocks, std::vector<int> & blockSystem, std::vector<int> & colorStart) const
+    {
+        blocks.clear();
+        blockSystem.clear();
+        colorStart.assign(numColors_ + 1, 0);
         size_t next = 0;
         for (int c = 0; c < numColors_; c++)
         {
-            colorBlockStart_[c] = static_cast<int>(blocks.size());
+            colorStart[c] = static_cast<int>(blocks.size());
             for (; next < chunks_.size() && chunks_[next].color == c; next++)
             {
                 const RowChunk & ch = chunks_[next];
@@ -441,7 +451,47 @@ namespace nsfem
                 }
             }
         }
-        colorBlockStart_[numColors_] = static_cast<int>(blocks.size());
+        colorStart[numColors_] = static_cast<int>(blocks.size());
+    }
+
+    void McgsSolver::reshapeSweeps(int lanes, int rowsPerBlock)
+    {
+        const auto [L, rpb] = resolveShape(lanes, rowsPerBlock, true);
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        sweepLanes_ = L;
+        sweepRows_ = rpb;
+        std::vector<int4> blocks;
+        std::vector<int> blockSystem;
+        buildBlocks(rpb, blocks, blockSystem, colorSweepBlockStart_);
+        sweepBlocks_.upload(blocks.empty() ? std::vector<int4>{int4{0, 0, 0, 0}} : blocks);
+        sweepBm_ = bm_;
+        sweepBm_.blocks = sweepBlocks_.get();
+        destroyGraphs();
+        syncLibraryStream();
+    }
+
+    void McgsSolver::destroyGraphs()
+    {
+        for (ChunkGraph * g : {&graph64_, &graph32_, &graphMixed_})
+        {
+            if (g->exec)
+            {
+                NSFEM_HIP(hipGraphExecDestroy(g->exec));
+                g->exec = nullptr;
+            }
+        }
+    }
+
+    void McgsSolver::reshape(int lanes, int rowsPerBlock)
+    {
+        const auto [L, rpb] = resolveShape(lanes, rowsPerBlock, false);
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        lanes_ = L;
+        rowsPerBlock_ = rpb;
+
+        std::vector<int4> blocks;
+        std::vector<int> blockSystem;
+        buildBlocks(rpb, blocks, blockSystem, colorBlockStart_);
         numBlocks_ = static_cast<int>(blocks.size());
 
         std::vector<int> sysBlockStart(numSystems_ + 1, 0), sysBlocks(numBlocks_);
@@ -473,15 +523,8 @@ namespace nsfem
         {
             outer_.init(numBlocks_, numSystems_, opt_.numCh);
         }
-        for (ChunkGraph * g : {&graph64_, &graph32_, &graphMixed_})
-        {
-            if (g->exec)
-            {
-                NSFEM_HIP(hipGraphExecDestroy(g->exec));
-                g->exec = nullptr;
-            }
-        }
-        syncLibraryStream();
+        // the sweeps take the same shape until reshapeSweeps
+        reshapeSweeps(lanes_, rowsPerBlock_);
     }
 
     int McgsSolver::minRowsPerBlock() const
@@ -506,14 +549,7 @@ namespace nsfem
         }
         NSFEM_HIP(hipStreamSynchronize(stream_));
         fusedAlphaMode_ = mode;
-        for (ChunkGraph * g : {&graph64_, &graph32_, &graphMixed_})
-        {
-            if (g->exec)
-            {
-                NSFEM_HIP(hipGraphExecDestroy(g->exec));
-                g->exec = nullptr;
-            }
-        }
+        destroyGraphs();
     }
 
     bool McgsSolver::fusedAlpha() const
@@ -539,9 +575,16 @@ namespace nsfem
         {
             throw std::invalid_argument("autotuning is available for pcg solvers");
         }
+        // The objective is one PCG iteration replayed from a captured graph,
+        // as the solver runs it: kernel durations and the gaps between the
+        // dependent launches.
+        // (Eager replays when graphs are off, e.g. under a kernel tracer.)
+        const char * kind = useGraphs_ ? "iteration_graph" : "iteration";
+        auto iterationMs = [&]() { return timeKernel(kind, reps, nullptr, nullptr); };
+        const std::vector<int> laneChoices = opt_.numCh > 2 ? std::vector<int>{opt_.numCh} : std::vector<int>{2, 4, 8, 16};
+        // 1. one shape for every kernel
         double best = 0;
         int bestLanes = lanes_, bestRows = rowsPerBlock_;
-        const std::vector<int> laneChoices = opt_.numCh > 2 ? std::vector<int>{opt_.numCh} : std::vector<int>{2, 4, 8, 16};
         for (int lanes : laneChoices)
         {
             for (int slabs : {1, 2, 4, 8})
@@ -551,7 +594,7 @@ namespace nsfem
                 // finalize kernels' partial sums bounded
                 const int base = opt_.numCh > 2 ? std::max(slab, minRowsPerBlock()) : slab;
                 reshape(lanes, slabs * base);
-                const double t = iterationTime(reps);
+                const double t = iterationMs();
                 if (best == 0 || t < best)
                 {
                     best = t;
@@ -561,6 +604,23 @@ namespace nsfem
             }
         }
         reshape(bestLanes, bestRows);
+        // 2. the colour sweeps (no partial sums) on a shape of their own
+        int sweepLanes = bestLanes, sweepRows = bestRows;
+        for (int lanes : laneChoices)
+        {
+            for (int slabs : {1, 2, 4})
+            {
+                reshapeSweeps(lanes, slabs * (kBlock / lanes));
+                const double t = iterationMs();
+                if (t < best)
+                {
+                    best = t;
+                    sweepLanes = sweepLanes_;
+                    sweepRows = sweepRows_;
+                }
+            }
+        }
+        reshapeSweeps(sweepLanes, sweepRows);
         return {bestLanes, bestRows};
     }
 
@@ -650,13 +710,13 @@ namespace nsfem
         {
             for (int c = 0; c < numColors_; c++)
             {
-                const int cnt = colorBlockStart_[c + 1] - colorBlockStart_[c];
+                const int cnt = colorSweepBlockStart_[c + 1] - colorSweepBlockStart_[c];
                 if (cnt == 0)
                 {
                     continue;
                 }
-                dispatchLC(lanes_, nch, [&]<int L, int C>()
-                           { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, bm_, colorBlockStart_[c], st, b, x, 0); });
+                dispatchLC(sweepLanes_, nch, [&]<int L, int C>()
+                           { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, sweepBm_, colorSweepBlockStart_[c], st, b, x, 0); });
             }
             if (iter % opt_.mseCheckInterval == 0)
             {
@@ -729,24 +789,24 @@ namespace nsfem
         // forward colours 1 .. C-1 on the lower part (colour 0 was set to r/d)
         for (int c = 1; c < numColors_; c++)
         {
-            const int cnt = colorBlockStart_[c + 1] - colorBlockStart_[c];
+            const int cnt = colorSweepBlockStart_[c + 1] - colorSweepBlockStart_[c];
             if (cnt == 0)
             {
                 continue;
             }
-            dispatchLC(lanes_, nch, [&]<int L, int C>()
-                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, bm_, colorBlockStart_[c], st, r, z, 1); });
+            dispatchLC(sweepLanes_, nch, [&]<int L, int C>()
+                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, sweepBm_, colorSweepBlockStart_[c], st, r, z, 1); });
         }
         // backward colours C-2 .. 1 on full rows (colour C-1 is unchanged)
         for (int c = numColors_ - 2; c >= 1; c--)
         {
-            const int cnt = colorBlockStart_[c + 1] - colorBlockStart_[c];
+            const int cnt = colorSweepBlockStart_[c + 1] - colorSweepBlockStart_[c];
             if (cnt == 0)
             {
                 continue;
             }
-            dispatchLC(lanes_, nch, [&]<int L, int C>()
-                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, bm_, colorBlockStart_[c], st, r, z, 0); });
+            dispatchLC(sweepLanes_, nch, [&]<int L, int C>()
+                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, sweepBm_, colorSweepBlockStart_[c], st, r, z, 0); });
         }
     }
 
@@ -1108,7 +1168,7 @@ namespace nsfem
                 int count = 0;
                 for (int c = 1; c < numColors_; c++)
                 {
-                    if (colorBlockStart_[c + 1] > colorBlockStart_[c])
+                    if (colorSweepBlockStart_[c + 1] > colorSweepBlockStart_[c])
                     {
                         total += sweepBytes(c, true);
                         count++;
@@ -1116,7 +1176,7 @@ namespace nsfem
                 }
                 for (int c = numColors_ - 2; c >= 1; c--)
                 {
-                    if (colorBlockStart_[c + 1] > colorBlockStart_[c])
+                    if (colorSweepBlockStart_[c + 1] > colorSweepBlockStart_[c])
                     {
                         total += sweepBytes(c, false);
                         count++;

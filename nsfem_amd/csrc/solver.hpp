@@ -51,6 +51,8 @@ namespace nsfem
         int mseCheckInterval = 1;
         int lanes = 0;           // 0: choose from the mean row length
         int rowsPerBlock = 0;    // 0: 4 row slabs per block
+        int sweepLanes = 0;      // colour sweeps' own shape (0: as above)
+        int sweepRowsPerBlock = 0;
         int checkChunk = 16;     // iterations between host convergence polls
         // Mixed precision.  Default: FP64 Krylov vectors, matrix products and
         // reductions, the symmetric GS preconditioner in FP32 (its sweeps on
@@ -175,10 +177,16 @@ namespace nsfem
         void reshape(int lanes, int rowsPerBlock);
         int lanes() const { return lanes_; }
         int rowsPerBlock() const { return rowsPerBlock_; }
-        // Times iterationTime for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs per
-        // workgroup (wide tiles: their own lanes x {1, 2, 4, 8} times
-        // minRowsPerBlock()) and keeps the fastest shape; returns (lanes,
-        // rowsPerBlock).
+        // The colour sweeps' own shape (they write no partial sums, so any
+        // rows per workgroup go); reshape sets it to the main shape.
+        void reshapeSweeps(int lanes, int rowsPerBlock);
+        int sweepLanes() const { return sweepLanes_; }
+        int sweepRowsPerBlock() const { return sweepRows_; }
+        // Times one graph-replayed PCG iteration for lanes {2, 4, 8, 16} x
+        // {1, 2, 4, 8} row slabs per workgroup (wide tiles: their own lanes x
+        // {1, 2, 4, 8} times minRowsPerBlock()) and keeps the fastest shape,
+        // then the sweeps' own shape over lanes x {1, 2, 4} slabs; returns
+        // (lanes, rowsPerBlock).
         // Overwrites the internal work vectors (not the io vectors).
         std::pair<int, int> autotune(int reps);
 
@@ -239,6 +247,14 @@ namespace nsfem
         std::vector<ColorStats> colorStats_;
         int numBlocks_ = 0;
         int fusedAlphaMode_ = 0;
+        // the colour sweeps' workgroup map
+        int sweepLanes_ = 8, sweepRows_ = 0;
+        std::vector<int> colorSweepBlockStart_;
+        DeviceArray<int4> sweepBlocks_;
+        BlockMap sweepBm_{};
+        std::pair<int, int> resolveShape(int lanes, int rowsPerBlock, bool sweeps) const;
+        void buildBlocks(int rpb, std::vector<int4> & blocks, std::vector<int> & blockSystem, std::vector<int> & colorStart) const;
+        void destroyGraphs();
 
         DeviceArray<int> perm_;
         DeviceArray<int4> blocks_;

@@ -287,9 +287,19 @@ class AbstractSolver:
         call("nsfem_solver_shape", self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
+    def reshapeSweeps(self, lanes=0, rowsPerBlock=0):
+        call("nsfem_solver_reshape_sweeps", self._h, int(lanes), int(rowsPerBlock))
+
+    def sweepShape(self):
+        """(lanes per row, rows per workgroup) of the colour sweeps."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        call("nsfem_solver_sweep_shape", self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
     def autotune(self, reps=10):
-        """Picks the fastest (lanes, rows per workgroup) by timing one PCG
-        iteration's launches; overwrites internal work vectors."""
+        """Picks the fastest (lanes, rows per workgroup) by timing one
+        graph-replayed PCG iteration, then the sweeps' own shape
+        (sweepShape()); overwrites internal work vectors."""
         a, b = ctypes.c_int(), ctypes.c_int()
         call("nsfem_solver_autotune", self._h, int(reps), ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value

@@ -231,6 +231,8 @@ class SolverConfig:
     mseCheckInterval: int = 1
     lanes: int = 0         # lanes per matrix row (2, 4, 8, 16); 0 picks from the mean row length
     rowsPerBlock: int = 0  # rows per workgroup of the solver kernels; 0 is the default
+    sweepLanes: int = 0    # the colour sweeps' own shape; 0, 0: as lanes, rowsPerBlock
+    sweepRowsPerBlock: int = 0
 
 
 @dataclasses.dataclass
@@ -252,9 +254,9 @@ class Chorin:
         self._m2 = cfg.pressureSolver.method.encode()
         v, p = cfg.velocitySolver, cfg.pressureSolver
         c = _lib.ChorinConfigC(_lib.SolverConfig(self._m1, v.maxIterations, v.targetMse, v.mseCheckInterval, v.lanes,
-                                                 v.rowsPerBlock),
+                                                 v.rowsPerBlock, v.sweepLanes, v.sweepRowsPerBlock),
                                _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
-                                                 p.rowsPerBlock),
+                                                 p.rowsPerBlock, p.sweepLanes, p.sweepRowsPerBlock),
                                cfg.dtype, cfg.innerTol, cfg.eagerLaunches)
         self._h = ctypes.c_void_p()
         call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
@@ -299,14 +301,16 @@ class Chorin:
 
     def shape(self):
         """(velocity lanes, velocity rows per workgroup, pressure lanes,
-        pressure rows per workgroup) of the solver kernels."""
-        out = (ctypes.c_int * 4)()
+        pressure rows per workgroup, then the same four for the colour
+        sweeps) of the solver kernels."""
+        out = (ctypes.c_int * 8)()
         call("nsfem_chorin_shape", self._h, out)
         return tuple(out)
 
     def autotune(self, reps=10):
-        """Autotunes both solvers' launch shapes (the state is kept)."""
-        out = (ctypes.c_int * 4)()
+        """Autotunes both solvers' launch shapes (the state is kept); returns
+        shape()."""
+        out = (ctypes.c_int * 8)()
         call("nsfem_chorin_autotune", self._h, int(reps), out)
         return tuple(out)
 

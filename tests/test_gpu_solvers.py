@@ -289,6 +289,19 @@ def test_reshape_and_autotune_keep_results(octx_h025):
         s.reshape(lanes, rows)
         L, R = s.shape()
         assert L in (2, 4, 8, 16) and R % (256 // L) == 0 and R >= 256 // L
+        assert s.sweepShape() == (L, R)
+        s.getRhs().upload(b)
+        s.getSol().memsetZero()
+        s.solve()
+        x = s.getSol().download()
+        assert abs(s.getLastIterations() - it0) <= 1
+        assert np.linalg.norm(x - x0) <= 1e-9 * np.linalg.norm(x0)
+    # the colour sweeps on shapes of their own
+    for lanes, rows in ((2, 64), (16, 512), (4, 0)):
+        s.reshapeSweeps(lanes, rows)
+        SL, SR = s.sweepShape()
+        slab = 256 // lanes  # rows per slab; a workgroup holds whole slabs
+        assert SL == lanes and SR % slab == 0 and (rows == 0 or SR == max(slab, rows // slab * slab))
         s.getRhs().upload(b)
         s.getSol().memsetZero()
         s.solve()
@@ -297,6 +310,8 @@ def test_reshape_and_autotune_keep_results(octx_h025):
         assert np.linalg.norm(x - x0) <= 1e-9 * np.linalg.norm(x0)
     L, R = s.autotune(reps=2)
     assert (L, R) == s.shape()
+    SL, SR = s.sweepShape()
+    assert SL in (2, 4, 8, 16) and SR % (256 // SL) == 0
     s.getRhs().upload(b)
     s.getSol().memsetZero()
     s.solve()
