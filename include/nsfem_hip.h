@@ -306,6 +306,15 @@ int nsfem_chorin_step(nsfem_chorin * ch, nsfem_step_stats * stats);
 /* Device state: velocity 2N ([ux; uy]), pressure V, in the chorin dtype. */
 int nsfem_chorin_state(nsfem_chorin * ch, void ** velocity, void ** pressure);
 int nsfem_chorin_download(nsfem_chorin * ch, void * velocity, void * pressure);
+/* Output path, replacing the two blocking per-step downloads
+ * (chorinCuda.cu:327, 363) for frame-strided output: download_async
+ * snapshots (u, p) after the last step on the device and copies the
+ * snapshot to library-owned pinned host memory on a stream of its own while
+ * later steps run; download_wait waits for that copy and writes it to the
+ * caller's buffers (either may be NULL).  One download at a time
+ * (NSFEM_EINVAL otherwise). */
+int nsfem_chorin_download_async(nsfem_chorin * ch);
+int nsfem_chorin_download_wait(nsfem_chorin * ch, void * velocity, void * pressure);
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);

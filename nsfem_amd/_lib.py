@@ -166,6 +166,8 @@ _SIGNATURES = {
     "nsfem_chorin_assemble_convection": [_vp, _vp, _vp, _vp],
     "nsfem_chorin_shape": [_vp, _vp],
     "nsfem_chorin_autotune": [_vp, ctypes.c_int, _vp],
+    "nsfem_chorin_download_async": [_vp],
+    "nsfem_chorin_download_wait": [_vp, _vp, _vp],
     "nsfem_chorin_set_solver_option": [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_double],
     "nsfem_chorin_time_kernel": [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p],
 }

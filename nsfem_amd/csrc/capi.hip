@@ -1145,6 +1145,22 @@ extern "C"
             syncLibraryStream(); });
     }
 
+    int nsfem_chorin_download_async(nsfem_chorin * ch)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            ch->core->downloadAsync(); });
+    }
+
+    int nsfem_chorin_download_wait(nsfem_chorin * ch, void * velocity, void * pressure)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            ch->core->downloadWait(velocity, pressure); });
+    }
+
     int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity)
     {
         return guarded([&]

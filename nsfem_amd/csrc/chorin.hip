@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <functional>
 
 #include "kernels.hpp"
@@ -214,9 +215,81 @@ namespace nsfem
 
     ChorinGpu::~ChorinGpu()
     {
+        if (dlPending_)
+        {
+            (void)hipEventSynchronize(copyDone_);
+        }
         for (auto & e : ev_)
         {
             (void)hipEventDestroy(e);
+        }
+        for (hipEvent_t e : {snapDone_, copyDone_})
+        {
+            if (e)
+            {
+                (void)hipEventDestroy(e);
+            }
+        }
+        if (copyStream_)
+        {
+            (void)hipStreamDestroy(copyStream_);
+        }
+        for (void * h : {hostU_, hostP_})
+        {
+            if (h)
+            {
+                (void)hipHostFree(h);
+            }
+        }
+    }
+
+    void ChorinGpu::downloadAsync()
+    {
+        if (dlPending_)
+        {
+            throw std::invalid_argument("a state download is still pending (wait for it first)");
+        }
+        const size_t bu = 2 * tsize_ * static_cast<size_t>(N_);
+        const size_t bp = tsize_ * static_cast<size_t>(V_);
+        if (!copyStream_)
+        {
+            // set up on first use: snapshot buffers, pinned staging, a copy
+            // stream (default priority) and its events
+            snapU_.resize(bu);
+            snapP_.resize(bp);
+            NSFEM_HIP(hipHostMalloc(&hostU_, bu, hipHostMallocDefault));
+            NSFEM_HIP(hipHostMalloc(&hostP_, bp, hipHostMallocDefault));
+            NSFEM_HIP(hipStreamCreateWithFlags(&copyStream_, hipStreamNonBlocking));
+            NSFEM_HIP(hipEventCreateWithFlags(&snapDone_, hipEventDisableTiming));
+            NSFEM_HIP(hipEventCreateWithFlags(&copyDone_, hipEventDisableTiming));
+        }
+        const hipStream_t s = libraryStream();
+        // the snapshot is ordered after the last step and before the next
+        NSFEM_HIP(hipMemcpyAsync(snapU_.get(), u_.get(), bu, hipMemcpyDeviceToDevice, s));
+        NSFEM_HIP(hipMemcpyAsync(snapP_.get(), p_.get(), bp, hipMemcpyDeviceToDevice, s));
+        NSFEM_HIP(hipEventRecord(snapDone_, s));
+        NSFEM_HIP(hipStreamWaitEvent(copyStream_, snapDone_, 0));
+        NSFEM_HIP(hipMemcpyAsync(hostU_, snapU_.get(), bu, hipMemcpyDeviceToHost, copyStream_));
+        NSFEM_HIP(hipMemcpyAsync(hostP_, snapP_.get(), bp, hipMemcpyDeviceToHost, copyStream_));
+        NSFEM_HIP(hipEventRecord(copyDone_, copyStream_));
+        dlPending_ = true;
+    }
+
+    void ChorinGpu::downloadWait(void * velocity, void * pressure)
+    {
+        if (!dlPending_)
+        {
+            throw std::invalid_argument("no state download is pending");
+        }
+        NSFEM_HIP(hipEventSynchronize(copyDone_));
+        dlPending_ = false;
+        if (velocity)
+        {
+            std::memcpy(velocity, hostU_, 2 * tsize_ * static_cast<size_t>(N_));
+        }
+        if (pressure)
+        {
+            std::memcpy(pressure, hostP_, tsize_ * static_cast<size_t>(V_));
         }
     }
 

@@ -76,6 +76,14 @@ namespace nsfem
         void resetState();
         void imposeDirichlet(hipStream_t s);
 
+        // Output path: a device snapshot of (u, p) after the last step,
+        // copied to library-owned pinned host memory on a stream of its own
+        // while later steps run; downloadWait waits for it and copies it to
+        // the caller's buffers (either may be null).  One at a time.
+        void downloadAsync();
+        void downloadWait(void * velocity, void * pressure);
+        bool downloadPending() const { return dlPending_; }
+
         // Average launch duration (ms) and algorithmic bytes per launch of a
         // step kernel: "convection" (element + gather kernels), "spmm" (A u),
         // "velocity_<k>" / "pressure_<k>" with k a solver kernel name.
@@ -112,5 +120,12 @@ namespace nsfem
 
         std::unique_ptr<McgsSolver> vSolver_, pSolver_;
         hipEvent_t ev_[5];
+
+        DeviceArray<char> snapU_, snapP_;
+        void * hostU_ = nullptr; // pinned
+        void * hostP_ = nullptr;
+        hipStream_t copyStream_ = nullptr;
+        hipEvent_t snapDone_ = nullptr, copyDone_ = nullptr;
+        bool dlPending_ = false;
     };
 } // namespace nsfem

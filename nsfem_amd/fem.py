@@ -278,6 +278,17 @@ class Chorin:
         call("nsfem_chorin_download", self._h, _ptr(u), _ptr(p))
         return u, p
 
+    def download_async(self):
+        """Starts copying the state after the last step to host memory while
+        later steps run; download_wait() returns it."""
+        call("nsfem_chorin_download_async", self._h)
+
+    def download_wait(self):
+        u = np.empty(2 * self.N, self.dtype)
+        p = np.empty(self.V, self.dtype)
+        call("nsfem_chorin_download_wait", self._h, _ptr(u), _ptr(p))
+        return u, p
+
     def upload_velocity(self, u):
         u = np.ascontiguousarray(u, self.dtype)
         if u.size != 2 * self.N:
@@ -349,7 +360,9 @@ class TimeStepSolution:
 def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t: float, cfg: ChorinConfig,
                     frame_step: int = 1, verbose: bool = False):
     """solveNsChorinCuda: numTimeSteps = ceil(maxT / tau0), tau = maxT / numTimeSteps,
-    numTimeSteps + 1 steps from u = Dirichlet values; every frame_step-th state is kept."""
+    numTimeSteps + 1 steps from u = Dirichlet values; every frame_step-th state
+    is kept (the frames the reference renders, navierStokes.cpp:109), each
+    copied to the host while the following steps run."""
     ctx = ChorinContext(mesh, cond)
     ch = Chorin(ctx, cfg)
     n_steps = int(math.ceil(max_t / tau0))
@@ -357,17 +370,22 @@ def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t:
     ch.set_time_step(tau)
     out = []
     stats = []
+    pending = None  # time of the state being downloaded
     for it in range(n_steps + 1):
         t0 = time.perf_counter()
         st = ch.step()
         st["wall_ms"] = 1e3 * (time.perf_counter() - t0)
         stats.append(st)
         if it % frame_step == 0:
-            u, p = ch.download()
-            out.append(TimeStepSolution((it + 1) * tau, u, p))
+            if pending is not None:
+                out.append(TimeStepSolution(pending, *ch.download_wait()))
+            ch.download_async()
+            pending = (it + 1) * tau
         if verbose:
             print(f"{it} / {n_steps}: {st['wall_ms']:.3f} ms  tent iters {st['itersTent']} pressure iters "
                   f"{st['itersPressure']} final iters {st['itersFinal']}", flush=True)
+    if pending is not None:
+        out.append(TimeStepSolution(pending, *ch.download_wait()))
     return out, stats
 
 

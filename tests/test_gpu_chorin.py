@@ -135,6 +135,38 @@ def test_solve_ns_chorin_time_loop():
                                         _cfg(("pcg", 500, 1e-10, 1), ("pcg", 2000, 1e-10, 1)))
     assert len(stats) == 5 and len(frames) == 5
     assert all(np.isfinite(f.velocity).all() for f in frames)
+    # every second frame: the same states, downloaded while later steps ran
+    frames2, _ = fem.solve_ns_chorin(mesh, fem.DfgConditions(1e-3, 1.0), 0.0009, 0.003,
+                                     _cfg(("pcg", 500, 1e-10, 1), ("pcg", 2000, 1e-10, 1)), frame_step=2)
+    assert [f.time for f in frames2] == [frames[i].time for i in (0, 2, 4)]
+    for a, i in zip(frames2, (0, 2, 4)):
+        assert np.array_equal(a.velocity, frames[i].velocity) and np.array_equal(a.pressure, frames[i].pressure)
+
+
+def test_async_download_snapshots_the_state():
+    """download_async copies the state after the last step while later steps
+    run: download_wait returns that state bit for bit, not a later one; a
+    second async download before the wait is refused."""
+    from nsfem_amd import _lib, fem
+    ctx = _ctx()
+    ch = fem.Chorin(ctx, _cfg(("pcg", 500, 1e-10, 1), ("pcg", 2000, 1e-10, 1)))
+    ch.set_time_step(5e-4)
+    ch.step()
+    u1, p1 = ch.download()
+    ch.download_async()
+    with pytest.raises(_lib.NsfemInvalidArgument, match="pending"):
+        ch.download_async()
+    ch.step()
+    ch.step()
+    ua, pa = ch.download_wait()
+    assert np.array_equal(ua, u1) and np.array_equal(pa, p1)
+    u3, p3 = ch.download()
+    assert not np.array_equal(u3, u1)
+    with pytest.raises(_lib.NsfemInvalidArgument, match="no state download"):
+        ch.download_wait()
+    ch.download_async()
+    ua, pa = ch.download_wait()
+    assert np.array_equal(ua, u3) and np.array_equal(pa, p3)
 
 
 def test_batched_instances_match_single_runs():
