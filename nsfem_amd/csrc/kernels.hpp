@@ -363,7 +363,35 @@ namespace nsfem
         const int * systemRows;    // rows per system (for RMS norms)
         int numSystems;
         int numBlocks;
+        // flatRows > 0 (one system): block b is rows [b*rowsPerBlock, ...)
+        // of [0, flatRows) (of the launch's colour for the sweeps), its slot
+        // is b; the kernels compute the range instead of loading it.
+        int flatRows;
+        int rowsPerBlock;
     };
+
+    // The rows, system and partial slot of block lb.
+    __device__ __forceinline__ int4 blockRange(const BlockMap & bm, int lb)
+    {
+        if (bm.flatRows > 0)
+        {
+            const int r0 = lb * bm.rowsPerBlock;
+            return make_int4(r0, min(bm.flatRows, r0 + bm.rowsPerBlock), 0, lb);
+        }
+        return bm.blocks[lb];
+    }
+
+    // Block lb of a colour sweep launch: rows [rowBegin, rowEnd) of the colour
+    // (flat map) or table entry blockOffset + lb.
+    __device__ __forceinline__ int4 sweepRange(const BlockMap & bm, int blockOffset, int lb, int rowBegin, int rowEnd)
+    {
+        if (bm.flatRows > 0)
+        {
+            const int r0 = rowBegin + lb * bm.rowsPerBlock;
+            return make_int4(r0, min(rowEnd, r0 + bm.rowsPerBlock), 0, 0);
+        }
+        return bm.blocks[blockOffset + lb];
+    }
 
     // Finalize kernels stage the per-channel solver scalars in LDS while the
     // partial sums stream in, for up to this many (system, channel) pairs.
@@ -545,15 +573,14 @@ namespace nsfem
     // sweep, backward half of symmetric GS); lowerOnly = 1: lower-colour part
     // only (forward half of symmetric GS started from zero).
     template <typename T, int LANES, int NCH>
-    __device__ __forceinline__ void mcgsColorNarrow(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st,
+    __device__ __forceinline__ void mcgsColorNarrow(McgsMatrix<T> A, BlockMap bm, int blockOffset, int rowBegin, int rowEnd, SolveState st,
                                                               const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
     {
         // Latency: the block's rows, its system and the done flag are loaded
         // together, and the first slab's row bounds are in flight before the
         // early-exit test resolves (the test reads them, so they are issued
         // first); the chain to the first gather is three loads deep.
-        const int blk = blockOffset + logicalBlock(blockIdx.x, gridDim.x);
-        const int4 range = bm.blocks[blk];
+        const int4 range = sweepRange(bm, blockOffset, logicalBlock(blockIdx.x, gridDim.x), rowBegin, rowEnd);
         const int sys = range.z;
         const int done = st.flags[0];
         int row = range.x + static_cast<int>(threadIdx.x) / LANES;
@@ -623,7 +650,7 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         if (systemActive<NCH>(st, range.z))
         {
             for (int base = range.x; base < range.y; base += kBlock / LANES)
@@ -669,7 +696,7 @@ namespace nsfem
         {
             part[k] = 0;
         }
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         for (int base = range.x; base < range.y; base += kBlock / LANES)
         {
             const int row = base + static_cast<int>(threadIdx.x) / LANES;
@@ -730,14 +757,16 @@ namespace nsfem
         const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
         // independent loads together; the first slab's row bounds in flight
         // before the early-exit test (which reads them)
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         const int sys = range.z;
         const int done = st.flags[0];
-        const bool c0 = range.x < color0End; // blocks never straddle colours
+        // colour-0 rows (a flat block may straddle the end of colour 0)
+        const bool anyC0 = range.x < color0End;
         int row = range.x + static_cast<int>(threadIdx.x) / LANES;
         bool valid = row < range.y;
-        int j0 = (c0 && valid) ? M.rowStart[row] : 0;
-        int j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+        bool c0 = valid && row < color0End;
+        int j0 = c0 ? M.rowStart[row] : 0;
+        int j1 = c0 ? M.rowStart[row + 1] : 0;
         if (done || j1 < j0)
         {
             return;
@@ -756,8 +785,9 @@ namespace nsfem
                 {
                     row = base + static_cast<int>(threadIdx.x) / LANES;
                     valid = row < range.y;
-                    j0 = (c0 && valid) ? M.rowStart[row] : 0;
-                    j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+                    c0 = valid && row < color0End;
+                    j0 = c0 ? M.rowStart[row] : 0;
+                    j1 = c0 ? M.rowStart[row + 1] : 0;
                 }
                 // the row's own operands load alongside the gathers
                 const bool lead = valid && groupLeader<LANES>();
@@ -784,8 +814,9 @@ namespace nsfem
                     }
                 }
                 P acc[NCH];
-                if (c0)
+                if (anyC0)
                 {
+                    // rows past colour 0 have j0 = j1: no entries
                     rowDot<P, LANES, NCH>(M, j0, j1, z, acc);
                 }
                 if (lead)
@@ -823,7 +854,7 @@ namespace nsfem
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         // independent loads together; the first slab's row bounds in flight
         // before the early-exit test (which reads them)
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         const int s = range.z;
         const int done = st.flags[0];
         int row = range.x + static_cast<int>(threadIdx.x) / LANES;
@@ -1000,12 +1031,11 @@ namespace nsfem
     }
 
     template <typename T, int NCH>
-    __device__ __forceinline__ void mcgsColorWide(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st, const T * __restrict__ b,
+    __device__ __forceinline__ void mcgsColorWide(McgsMatrix<T> A, BlockMap bm, int blockOffset, int rowBegin, int rowEnd, SolveState st, const T * __restrict__ b,
                                                   T * __restrict__ x, int lowerOnly)
     {
         constexpr int slab = kBlock / NCH;
-        const int blk = blockOffset + logicalBlock(blockIdx.x, gridDim.x);
-        const int4 range = bm.blocks[blk];
+        const int4 range = sweepRange(bm, blockOffset, logicalBlock(blockIdx.x, gridDim.x), rowBegin, rowEnd);
         const int sys = range.z;
         const int done = st.flags[0];
         const int c = static_cast<int>(threadIdx.x) % NCH;
@@ -1050,7 +1080,7 @@ namespace nsfem
         }
         const int c = static_cast<int>(threadIdx.x) % NCH;
         double part = 0;
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         if (systemActive<NCH>(st, range.z))
         {
             for (int base = range.x; base < range.y; base += kBlock / NCH)
@@ -1079,7 +1109,7 @@ namespace nsfem
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         const int c = static_cast<int>(threadIdx.x) % NCH;
         double bb = 0, rr = 0;
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         for (int base = range.x; base < range.y; base += kBlock / NCH)
         {
             const int row = base + static_cast<int>(threadIdx.x) / NCH;
@@ -1124,15 +1154,16 @@ namespace nsfem
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         const int sys = range.z;
         const int done = st.flags[0];
-        const bool c0 = range.x < color0End; // blocks never straddle colours
         const int c = static_cast<int>(threadIdx.x) % NCH;
         int row = range.x + static_cast<int>(threadIdx.x) / NCH;
         bool valid = row < range.y;
-        int j0 = (c0 && valid) ? M.rowStart[row] : 0;
-        int j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+        // colour-0 rows (a flat block may straddle the end of colour 0)
+        bool c0 = valid && row < color0End;
+        int j0 = c0 ? M.rowStart[row] : 0;
+        int j1 = c0 ? M.rowStart[row + 1] : 0;
         if (done || j1 < j0)
         {
             return;
@@ -1146,8 +1177,9 @@ namespace nsfem
                 {
                     row = base + static_cast<int>(threadIdx.x) / NCH;
                     valid = row < range.y;
-                    j0 = (c0 && valid) ? M.rowStart[row] : 0;
-                    j1 = (c0 && valid) ? M.rowStart[row + 1] : 0;
+                    c0 = valid && row < color0End;
+                    j0 = c0 ? M.rowStart[row] : 0;
+                    j1 = c0 ? M.rowStart[row + 1] : 0;
                 }
                 if (valid)
                 {
@@ -1180,7 +1212,7 @@ namespace nsfem
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         const int s = range.z;
         const int done = st.flags[0];
         const int c = static_cast<int>(threadIdx.x) % NCH;
@@ -1231,15 +1263,15 @@ namespace nsfem
         storeChannelPartial<NCH, NCH>(total, st.partials, range.w, 0);
     }
     template <typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, SolveState st, const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
+    __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, int rowBegin, int rowEnd, SolveState st, const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
     {
         if constexpr (kWide<NCH>)
         {
-            mcgsColorWide<T, NCH>(A, bm, blockOffset, st, b, x, lowerOnly);
+            mcgsColorWide<T, NCH>(A, bm, blockOffset, rowBegin, rowEnd, st, b, x, lowerOnly);
         }
         else
         {
-            mcgsColorNarrow<T, LANES, NCH>(A, bm, blockOffset, st, b, x, lowerOnly);
+            mcgsColorNarrow<T, LANES, NCH>(A, bm, blockOffset, rowBegin, rowEnd, st, b, x, lowerOnly);
         }
     }
 
@@ -1313,7 +1345,7 @@ namespace nsfem
         __shared__ double shAlpha[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         // the block's rows, its system and the done flag load together
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         const int s = range.z;
         if (lb == 0 && threadIdx.x == 0)
         {
@@ -1682,7 +1714,7 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
         {
 #pragma unroll
@@ -1718,7 +1750,7 @@ namespace nsfem
     __global__ void __launch_bounds__(kBlock) subtractMeanKernel(int n, BlockMap bm, T * __restrict__ v, const double * __restrict__ mean)
     {
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
-        const int4 range = bm.blocks[lb];
+        const int4 range = blockRange(bm, lb);
         const int s = range.z;
         for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
         {

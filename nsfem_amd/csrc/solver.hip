@@ -236,7 +236,9 @@ namespace nsfem
             {
                 color0End_ = g;
             }
+            colorRowStart_.push_back(g); // end of colour c
         }
+        colorRowStart_.insert(colorRowStart_.begin(), 0);
 
         // Global stripped matrix in colour-major order.
         std::vector<int> rowStart(n_ + 1, 0), rowSplit(n_), column;
@@ -466,6 +468,8 @@ namespace nsfem
         sweepBlocks_.upload(blocks.empty() ? std::vector<int4>{int4{0, 0, 0, 0}} : blocks);
         sweepBm_ = bm_;
         sweepBm_.blocks = sweepBlocks_.get();
+        sweepBm_.numBlocks = static_cast<int>(blocks.size());
+        sweepBm_.rowsPerBlock = rpb;
         destroyGraphs();
         syncLibraryStream();
     }
@@ -491,7 +495,23 @@ namespace nsfem
 
         std::vector<int4> blocks;
         std::vector<int> blockSystem;
-        buildBlocks(rpb, blocks, blockSystem, colorBlockStart_);
+        if (flat())
+        {
+            // one system: the row-wise kernels cut [0, n) evenly, whatever
+            // the colours (a block may straddle two)
+            for (int b0 = 0; b0 < n_; b0 += rpb)
+            {
+                blocks.push_back(int4{b0, std::min(n_, b0 + rpb), 0, -1});
+                blockSystem.push_back(0);
+            }
+            color0Blocks_ = (color0End_ + rpb - 1) / rpb;
+        }
+        else
+        {
+            std::vector<int> colorStart;
+            buildBlocks(rpb, blocks, blockSystem, colorStart);
+            color0Blocks_ = colorStart.size() > 1 ? colorStart[1] : 0;
+        }
         numBlocks_ = static_cast<int>(blocks.size());
 
         std::vector<int> sysBlockStart(numSystems_ + 1, 0), sysBlocks(numBlocks_);
@@ -514,7 +534,8 @@ namespace nsfem
         blocks_.upload(blocks.empty() ? std::vector<int4>{int4{0, 0, 0, 0}} : blocks);
         sysBlockStart_.upload(sysBlockStart);
         sysBlocks_.upload(sysBlocks.empty() ? std::vector<int>{0} : sysBlocks);
-        bm_ = BlockMap{blocks_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_, numBlocks_};
+        bm_ = BlockMap{blocks_.get(), sysBlockStart_.get(), sysBlocks_.get(), systemRows_.get(), numSystems_, numBlocks_,
+                       flat() ? n_ : 0, rpb};
 
         // reduction scratch sized by the block count; captured graphs hold
         // the old launch shapes and buffers
@@ -716,7 +737,10 @@ namespace nsfem
                     continue;
                 }
                 dispatchLC(sweepLanes_, nch, [&]<int L, int C>()
-                           { mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, sweepBm_, colorSweepBlockStart_[c], st, b, x, 0); });
+                           {
+                               mcgsColorKernel<T, L, C><<<cnt, kBlock, 0, s>>>(Av, sweepBm_, colorSweepBlockStart_[c], colorRowStart_[c],
+                                                                              colorRowStart_[c + 1], st, b, x, 0);
+                           });
             }
             if (iter % opt_.mseCheckInterval == 0)
             {
@@ -795,7 +819,10 @@ namespace nsfem
                 continue;
             }
             dispatchLC(sweepLanes_, nch, [&]<int L, int C>()
-                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, sweepBm_, colorSweepBlockStart_[c], st, r, z, 1); });
+                       {
+                           mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, sweepBm_, colorSweepBlockStart_[c], colorRowStart_[c],
+                                                                          colorRowStart_[c + 1], st, r, z, 1);
+                       });
         }
         // backward colours C-2 .. 1 on full rows (colour C-1 is unchanged)
         for (int c = numColors_ - 2; c >= 1; c--)
@@ -806,7 +833,10 @@ namespace nsfem
                 continue;
             }
             dispatchLC(sweepLanes_, nch, [&]<int L, int C>()
-                       { mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, sweepBm_, colorSweepBlockStart_[c], st, r, z, 0); });
+                       {
+                           mcgsColorKernel<P, L, C><<<cnt, kBlock, 0, s>>>(M, sweepBm_, colorSweepBlockStart_[c], colorRowStart_[c],
+                                                                          colorRowStart_[c + 1], st, r, z, 0);
+                       });
         }
     }
 
@@ -816,7 +846,7 @@ namespace nsfem
         dispatchLC(lanes_, opt_.numCh, [&]<int L, int C>()
                    {
                        pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_,
-                                                                              colorBlockStart_[1]);
+                                                                              color0Blocks_);
                        pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
                    });
     }
@@ -1200,7 +1230,7 @@ namespace nsfem
                     dispatchLC(lanes_, nch, [&]<int L, int C>()
                                {
                                    pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_,
-                                                                                  colorBlockStart_[1]);
+                                                                                  color0Blocks_);
                                });
                 };
             }

@@ -239,7 +239,10 @@ namespace nsfem
         std::vector<RowChunk> chunks_;     // colour-major
         double meanRow_ = 0;               // off-diagonal entries per row
         int rowsPerBlock_ = 0;
-        std::vector<int> colorBlockStart_; // numColors + 1
+        std::vector<int> colorRowStart_; // numColors + 1 (global colour-major order)
+        int color0Blocks_ = 0;           // main-map blocks holding colour-0 rows (leading)
+        // one system: flat workgroup maps (ranges computed in the kernels)
+        bool flat() const { return numSystems_ == 1; }
         struct ColorStats
         {
             double rows = 0, lowerNnz = 0, fullNnz = 0, lowerCols = 0, fullCols = 0;
