@@ -58,10 +58,10 @@ def initial_velocity(ctx, cond, seed):
     return u
 
 
-def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune):
+def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     import numpy as np
     from nsfem_amd import _lib, fem
-    ch = fem.Chorin(ctx, cfg)
+    ch = make(cfg) if make else fem.Chorin(ctx, cfg)
     ch.set_time_step(tau)
     t0 = time.perf_counter()
     shape = ch.autotune(10) if autotune else ch.shape()
@@ -91,7 +91,10 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune):
            "shape": {"velocity_lanes": shape[0], "velocity_rows_per_block": shape[1], "pressure_lanes": shape[2],
                      "pressure_rows_per_block": shape[3], "velocity_sweep_lanes": shape[4],
                      "velocity_sweep_rows_per_block": shape[5], "pressure_sweep_lanes": shape[6],
-                     "pressure_sweep_rows_per_block": shape[7], "autotuned": bool(autotune), "autotune_s": tune_s}}
+                     "pressure_sweep_rows_per_block": shape[7],
+                     "velocity_direction_update_rho_rows": list(shape[8:11]),
+                     "pressure_direction_update_rho_rows": list(shape[11:14]),
+                     "autotuned": bool(autotune), "autotune_s": tune_s}}
     log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
         f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
     return ch, res, (u1, p1), (u, p)
@@ -405,7 +408,8 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--shape", default="0,0,0,0",
                     help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default), "
-                         "optionally followed by the same four for the colour sweeps")
+                         "optionally followed by the same four for the colour sweeps and then by the velocity and "
+                         "pressure direction,update,rho rows (14 numbers, as the JSON records them)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
     ap.add_argument("--kernels-only", action="store_true",
@@ -457,9 +461,18 @@ def main():
     u0 = initial_velocity(ctx, cond, 1)
     target = 1e-10
     shape = [int(v) for v in args.shape.split(",")]
-    if len(shape) not in (4, 8):
-        raise SystemExit("--shape takes 4 or 8 numbers")
-    vl, vr, pl, pr, vsl, vsr, psl, psr = shape + [0] * (8 - len(shape))
+    if len(shape) not in (4, 8, 14):
+        raise SystemExit("--shape takes 4, 8 or 14 numbers")
+    vl, vr, pl, pr, vsl, vsr, psl, psr = (shape + [0] * 4)[:8]
+    kind_rows = shape[8:14] if len(shape) == 14 else None
+
+    def make_chorin(cfg):
+        ch = fem.Chorin(ctx, cfg)
+        if kind_rows is not None:
+            for which, rows in (("velocity", kind_rows[0:3]), ("pressure", kind_rows[3:6])):
+                for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho"), rows):
+                    ch.set_solver_option(key, v, which)
+        return ch
 
     def solvers(method):
         return (fem.SolverConfig(method, 5000, target, 1, vl, vr, vsl, vsr),
@@ -473,7 +486,7 @@ def main():
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
                "kernels": {}}
         for name in paths:
-            ch = fem.Chorin(ctx, {"fp64": cfg64, "mixed": cfgmx}[name])
+            ch = make_chorin({"fp64": cfg64, "mixed": cfgmx}[name])
             ch.upload_velocity(u0)
             out["kernels"][name] = kernel_table(ch, None, args.kernel_reps)
             out[f"{name}_shape"] = ch.shape()
@@ -490,7 +503,7 @@ def main():
     results, kernels, firsts, lasts = {}, {}, {}, {}
     for name in paths:
         cfg = {"fp64": cfg64, "mixed": cfgmx}[name]
-        ch, r, first, last = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps, not args.no_autotune)
+        ch, r, first, last = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps, not args.no_autotune, make_chorin)
         kernels[name] = {} if args.no_kernels else kernel_table(ch, r)
         results[name], firsts[name], lasts[name] = r, first, last
         del ch

@@ -143,11 +143,15 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *   "innerTol"     pcg_mixed with mixedRestart: inner relative tolerance.
  *   "graphs"       0: launch every iteration from the host instead of
  *                  replaying captured graphs of iteration chunks (default 1).
- *   "fusedAlpha"   pcg: 1 finishes the step length alpha inside the update
- *                  kernel (one launch less per iteration; every update
- *                  workgroup re-reads the direction partial sums of its
- *                  system; 1 and 2 channels), 0 (default) or -1 in a one-block
- *                  finalize kernel. */
+ *   "fusedReductions" pcg on one system (no batch): 1 or -1 (default) lets
+ *                  the direction kernel finish the previous iteration's rho
+ *                  and convergence test and the update kernel the step
+ *                  length, every workgroup from the producers' partial sums,
+ *                  instead of two one-workgroup finalize kernels per
+ *                  iteration; 0 keeps the finalize kernels.
+ *   "rowsDirection", "rowsUpdate", "rowsRho"
+ *                  fused reductions: rows per workgroup of those kernels
+ *                  (0: the main shape's); autotune sets them. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events
@@ -178,6 +182,9 @@ int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock);
 int nsfem_solver_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock);
 int nsfem_solver_reshape_sweeps(nsfem_solver * s, int lanes, int rowsPerBlock);
 int nsfem_solver_sweep_shape(const nsfem_solver * s, int * lanes, int * rowsPerBlock);
+/* Rows per workgroup of the direction, update and rho kernels (the main
+ * shape's unless fused reductions give them their own). */
+int nsfem_solver_kind_rows(const nsfem_solver * s, int * direction, int * update, int * rho);
 int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock);
 int nsfem_solver_coloring(const nsfem_solver * s, int system, int * coloring, int * partitionStart);
 
@@ -318,10 +325,11 @@ int nsfem_chorin_download_wait(nsfem_chorin * ch, void * velocity, void * pressu
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);
-/* Launch shapes of the step's solvers, shape[8] = {velocity lanes, velocity
- * rows per workgroup, pressure lanes, pressure rows per workgroup, then the
- * same four for the colour sweeps}; autotune as nsfem_solver_autotune for
- * both solvers (the state u, p is kept). */
+/* Launch shapes of the step's solvers, shape[14] = {velocity lanes, velocity
+ * rows per workgroup, pressure lanes, pressure rows per workgroup, the same
+ * four for the colour sweeps, then velocity direction / update / rho rows
+ * and pressure direction / update / rho rows}; autotune as
+ * nsfem_solver_autotune for both solvers (the state u, p is kept). */
 int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
 int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
 /* nsfem_solver_set_option for the step's solvers: which = 0 velocity,

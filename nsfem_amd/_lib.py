@@ -130,6 +130,7 @@ _SIGNATURES = {
     "nsfem_solver_shape": [_vp, _c_int_p, _c_int_p],
     "nsfem_solver_reshape_sweeps": [_vp, ctypes.c_int, ctypes.c_int],
     "nsfem_solver_sweep_shape": [_vp, _c_int_p, _c_int_p],
+    "nsfem_solver_kind_rows": [_vp, _c_int_p, _c_int_p, _c_int_p],
     "nsfem_solver_autotune": [_vp, ctypes.c_int, _c_int_p, _c_int_p],
     "nsfem_solver_coloring": [_vp, ctypes.c_int, _vp, _vp],
     "nsfem_mcgs_coloring": [ctypes.c_int, _vp, _vp, _vp, _vp, _c_int_p],

@@ -85,10 +85,17 @@ namespace
         {
             solver.setUseGraphs(value != 0);
         }
-        else if (k == "fusedAlpha")
+        else if (k == "fusedReductions")
         {
-            if (value != -1 && value != 0 && value != 1) throw std::invalid_argument("fusedAlpha must be -1, 0 or 1");
-            solver.setFusedAlpha(static_cast<int>(value));
+            if (value != -1 && value != 0 && value != 1) throw std::invalid_argument("fusedReductions must be -1, 0 or 1");
+            solver.setFusedReductions(static_cast<int>(value));
+        }
+        else if (k == "rowsDirection" || k == "rowsUpdate" || k == "rowsRho")
+        {
+            if (!(value >= 0 && value <= 1 << 20)) throw std::invalid_argument(k + " must be in [0, 2^20]");
+            const int v = static_cast<int>(value);
+            solver.setKindRows(k == "rowsDirection" ? v : solver.kindRows(0), k == "rowsUpdate" ? v : solver.kindRows(1),
+                               k == "rowsRho" ? v : solver.kindRows(2));
         }
         else
         {
@@ -688,6 +695,16 @@ extern "C"
             if (rowsPerBlock) *rowsPerBlock = s->core->sweepRowsPerBlock(); });
     }
 
+    int nsfem_solver_kind_rows(const nsfem_solver * s, int * direction, int * update, int * rho)
+    {
+        return guarded([&]
+                       {
+            need(s, "solver");
+            if (direction) *direction = s->core->kindRows(0);
+            if (update) *update = s->core->kindRows(1);
+            if (rho) *rho = s->core->kindRows(2); });
+    }
+
     int nsfem_solver_autotune(nsfem_solver * s, int reps, int * lanes, int * rowsPerBlock)
     {
         return guarded([&]
@@ -1185,9 +1202,10 @@ extern "C"
     {
         const McgsSolver & v = ch.core->velocitySolver();
         const McgsSolver & p = ch.core->pressureSolver();
-        const int s[8] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
-                          v.sweepLanes(), v.sweepRowsPerBlock(), p.sweepLanes(), p.sweepRowsPerBlock()};
-        std::copy(s, s + 8, shape);
+        const int s[14] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
+                           v.sweepLanes(), v.sweepRowsPerBlock(), p.sweepLanes(), p.sweepRowsPerBlock(),
+                           v.kindRows(0), v.kindRows(1), v.kindRows(2), p.kindRows(0), p.kindRows(1), p.kindRows(2)};
+        std::copy(s, s + 14, shape);
     }
 
     int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape)

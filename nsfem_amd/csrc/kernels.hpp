@@ -409,9 +409,12 @@ namespace nsfem
         double * mse;   // RMS residual (gs mode)
         int * active;   // 1 while the channel iterates
         int * iters;    // iterations done / iteration index at convergence
-        int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen
+        int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen, [3] done next
         double * partials;  // blocks * 2 * NCH
         double * partials2; // blocks * NCH: |r|^2 of the update, read with rho's
+        double * partials3; // blocks * NCH: p.q of the direction kernel (fused reductions)
+        double * rhoNext;   // fused reductions: rho and activity decided by the
+        int * activeNext;   //   direction kernel, taken over by the update kernel
         double tol2;        // relative tolerance squared (pcg) or target RMS (gs)
     };
 
@@ -848,9 +851,11 @@ namespace nsfem
     // z is the preconditioned vector (type P), p and q are of the Krylov type T.
     template <typename T, typename P, int LANES, int NCH>
     __device__ __forceinline__ void pcgDirectionNarrow(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
-                                                                 T * __restrict__ p, T * __restrict__ q)
+                                                       T * __restrict__ p, T * __restrict__ q, int fused, int rhoParts, int updParts)
     {
         __shared__ double sm[NCH * kWaves];
+        __shared__ double shBeta[NCH];
+        __shared__ int shAct[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         // independent loads together; the first slab's row bounds in flight
         // before the early-exit test (which reads them)
@@ -871,14 +876,15 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        if (systemActive<NCH>(st, s))
+        const bool go = fused ? fusedRhoStep<NCH>(st, rhoParts, updParts, lb, sm, shBeta, shAct) : systemActive<NCH>(st, s);
+        if (go)
         {
             const bool first = st.flags[1] != 0;
             T beta[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                beta[c] = static_cast<T>(st.beta[s * NCH + c]);
+                beta[c] = static_cast<T>(fused ? shBeta[c] : st.beta[s * NCH + c]);
             }
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
@@ -941,7 +947,7 @@ namespace nsfem
             }
         }
         blockSum<NCH>(part, sm);
-        storePartial<NCH>(part, st.partials, range.w);
+        storePartial<NCH>(part, fused ? st.partials3 : st.partials, range.w);
     }
 
     // ------------------------------------------------------------------
@@ -1028,6 +1034,93 @@ namespace nsfem
         {
             partials[static_cast<size_t>(slot) * K + offset + threadIdx.x] = total;
         }
+    }
+
+    // Sum over count partial slots of channel threadIdx.x % NCH (slot i,
+    // channel c at parts[i*NCH + c]); the total of channel c is returned in
+    // thread c (c < NCH).  Same order in every block.
+    template <int NCH>
+    __device__ __forceinline__ double channelPartialSum(const double * __restrict__ parts, int count, double * sm)
+    {
+        constexpr int G = kBlock / NCH; // threads per channel
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        double acc = 0;
+#pragma unroll 4
+        for (int i = static_cast<int>(threadIdx.x) / NCH; i < count; i += G)
+        {
+            acc += parts[static_cast<size_t>(i) * NCH + c];
+        }
+        return channelBlockSum<NCH>(acc, sm);
+    }
+
+    // Fused reductions (one system, in the direction kernel's prologue): the
+    // previous iteration's rho = r.z (rho kernel partials) and |r|^2 (update
+    // kernel partials, not in the first iteration) are finished by every
+    // block in the same order, so all blocks hold the same convergence
+    // decision and the same beta, as pcgRhoFinalizeKernel would compute them.
+    // Block 0 records rho and the decision for the update kernel (rhoNext,
+    // activeNext, flags[3]) and rr, the iteration count and non-finite
+    // results for the host; no block of this launch reads those.  Returns
+    // whether any channel goes on; beta and activity per channel in LDS.
+    template <int NCH>
+    __device__ __forceinline__ bool fusedRhoStep(const SolveState & st, int rhoParts, int updParts, int lb, double * sm,
+                                                 double * shBeta, int * shAct)
+    {
+        const int first = st.flags[1];
+        const double rho = channelPartialSum<NCH>(st.partials, rhoParts, sm);
+        const double rr = first ? 0.0 : channelPartialSum<NCH>(st.partials2, updParts, sm);
+        const int c = static_cast<int>(threadIdx.x);
+        if (c < NCH)
+        {
+            const int was = st.active[c];
+            int act = was;
+            int bad = 0;
+            if (was && !first)
+            {
+                if (!(rr == rr) || rr > 1e300)
+                {
+                    act = 0;
+                    bad = 1;
+                }
+                else if (rr <= st.tol2 * st.ref[c])
+                {
+                    act = 0;
+                }
+            }
+            const double rhoOld = st.rho[c];
+            shBeta[c] = (first || rhoOld == 0) ? 0.0 : rho / rhoOld;
+            shAct[c] = act;
+            if (act && !(rho == rho))
+            {
+                bad = 1;
+            }
+            if (lb == 0)
+            {
+                st.rhoNext[c] = rho;
+                st.activeNext[c] = act;
+                if (was && !first)
+                {
+                    st.rr[c] = rr;
+                    st.iters[c] = st.iters[c] + 1;
+                }
+                if (bad)
+                {
+                    st.flags[2] = 1;
+                }
+            }
+        }
+        __syncthreads();
+        int any = 0;
+#pragma unroll
+        for (int k = 0; k < NCH; k++)
+        {
+            any |= shAct[k];
+        }
+        if (lb == 0 && threadIdx.x == 0)
+        {
+            st.flags[3] = any ? 0 : 1;
+        }
+        return any != 0;
     }
 
     template <typename T, int NCH>
@@ -1208,9 +1301,11 @@ namespace nsfem
 
     template <typename T, typename P, int NCH>
     __device__ __forceinline__ void pcgDirectionWide(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
-                                                     T * __restrict__ p, T * __restrict__ q)
+                                                     T * __restrict__ p, T * __restrict__ q, int fused, int rhoParts, int updParts)
     {
         __shared__ double sm[NCH * kWaves];
+        __shared__ double shBeta[NCH];
+        __shared__ int shAct[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         const int4 range = blockRange(bm, lb);
         const int s = range.z;
@@ -1225,10 +1320,11 @@ namespace nsfem
             return;
         }
         double part = 0;
-        if (systemActive<NCH>(st, s))
+        const bool go = fused ? fusedRhoStep<NCH>(st, rhoParts, updParts, lb, sm, shBeta, shAct) : systemActive<NCH>(st, s);
+        if (go)
         {
             const bool first = st.flags[1] != 0;
-            const T beta = static_cast<T>(st.beta[s * NCH + c]);
+            const T beta = static_cast<T>(fused ? shBeta[c] : st.beta[s * NCH + c]);
             for (int base = range.x; base < range.y; base += kBlock / NCH)
             {
                 if (base != range.x)
@@ -1260,7 +1356,7 @@ namespace nsfem
             }
         }
         const double total = channelBlockSum<NCH>(part, sm);
-        storeChannelPartial<NCH, NCH>(total, st.partials, range.w, 0);
+        storeChannelPartial<NCH, NCH>(total, fused ? st.partials3 : st.partials, range.w, 0);
     }
     template <typename T, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) mcgsColorKernel(McgsMatrix<T> A, BlockMap bm, int blockOffset, int rowBegin, int rowEnd, SolveState st, const T * __restrict__ b, T * __restrict__ x, int lowerOnly)
@@ -1315,43 +1411,64 @@ namespace nsfem
     }
 
     template <typename T, typename P, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z, T * __restrict__ p, T * __restrict__ q)
+    __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z, T * __restrict__ p, T * __restrict__ q,
+                                                                 int fused, int rhoParts, int updParts)
     {
         if constexpr (kWide<NCH>)
         {
-            pcgDirectionWide<T, P, NCH>(A, bm, st, z, p, q);
+            pcgDirectionWide<T, P, NCH>(A, bm, st, z, p, q, fused, rhoParts, updParts);
         }
         else
         {
-            pcgDirectionNarrow<T, P, LANES, NCH>(A, bm, st, z, p, q);
+            pcgDirectionNarrow<T, P, LANES, NCH>(A, bm, st, z, p, q, fused, rhoParts, updParts);
         }
     }
 
     // x += alpha p;  r -= alpha q;  z = r / d on colour 0 (preconditioner
     // type P; rP = (P) r when P is narrower than T); partial sums of |r|^2.
-    // fusedAlpha = 0: alpha comes from pcgDirectionFinalizeKernel.
-    // fusedAlpha = 1: the block finishes p.q itself from the direction
-    // kernel's partials of its system (every block of a system sums them in
-    // the same order, so all hold the same alpha = rho / p.q) while its first
-    // elements' operands are already loading: one launch less per iteration.
-    // (1 and 2 channels only.)
+    // fused = 0: alpha from pcgDirectionFinalizeKernel, activity from the
+    // solve state.  fused = 1 (one system, fused reductions): every block
+    // finishes p.q from the direction kernel's dirParts partials itself (same
+    // order in every block, so all hold the same alpha = rho / p.q) while its
+    // first elements' operands are already loading, with rho and activity as
+    // the direction kernel decided them; block 0 makes that decision the
+    // solve state for the kernels that follow.
     template <typename T, typename P, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(int n, BlockMap bm, SolveState st, const T * __restrict__ p,
                                                               const T * __restrict__ q, T * __restrict__ x, T * r, P * rP,
                                                               P * __restrict__ z, const P * __restrict__ invDiagP, int color0End,
-                                                              int fusedAlpha)
+                                                              int fused, int dirParts)
     {
         __shared__ double sm[NCH * kWaves];
         __shared__ double shAlpha[NCH];
+        __shared__ int shAct[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
         // the block's rows, its system and the done flag load together
         const int4 range = blockRange(bm, lb);
         const int s = range.z;
-        if (lb == 0 && threadIdx.x == 0)
+        if (fused)
         {
-            st.flags[1] = 0; // the direction kernel before has used p = z
+            // block 0 makes the direction kernel's decision the solve state
+            // (also when it stopped the solve); no block of this launch reads
+            // what it writes: the others test flags[3], not flags[0], and
+            // read activeNext and rhoNext
+            const int t = static_cast<int>(threadIdx.x);
+            if (lb == 0 && t < NCH)
+            {
+                st.active[t] = st.activeNext[t];
+                st.rho[t] = st.rhoNext[t];
+            }
+            if (lb == 0 && t == 0)
+            {
+                st.flags[0] = st.flags[3];
+                st.flags[1] = 0; // the direction kernel before has used p = z
+            }
+            if (st.flags[3])
+            {
+                return;
+            }
         }
-        if (st.flags[0])
+        else if (st.flags[0])
         {
             return;
         }
@@ -1359,17 +1476,57 @@ namespace nsfem
         // stride (kBlock) is a multiple of NCH, so a thread keeps one channel
         static_assert(kBlock % NCH == 0, "channels must divide the block");
         const int c = static_cast<int>(threadIdx.x) % NCH;
-        double sum = 0;
-        if (systemActive<NCH>(st, s))
+        const long i0 = static_cast<long>(range.x) * NCH + threadIdx.x;
+        const long end = static_cast<long>(range.y) * NCH;
+        const int row0 = static_cast<int>(i0 / NCH);
+        bool go;
+        int act;
+        T alpha;
+        // the first element's operands go out before alpha is known
+        T r0 = T(0), x0 = T(0), p0 = T(0), q0 = T(0);
+        P d0 = P(0);
+        if (fused)
         {
-            const int act = st.active[s * NCH + c];
-            const long i0 = static_cast<long>(range.x) * NCH + threadIdx.x;
-            const long end = static_cast<long>(range.y) * NCH;
-            // the first element's operands go out before alpha is known
-            T r0 = T(0), x0 = T(0), p0 = T(0), q0 = T(0);
-            P d0 = P(0);
-            const int row0 = static_cast<int>(i0 / NCH);
             if (i0 < end)
+            {
+                r0 = r[i0];
+                x0 = x[i0];
+                p0 = p[i0];
+                q0 = q[i0];
+                if (row0 < color0End)
+                {
+                    d0 = invDiagP[row0];
+                }
+            }
+            const double pq = channelPartialSum<NCH>(st.partials3, dirParts, sm);
+            const int t = static_cast<int>(threadIdx.x);
+            if (t < NCH)
+            {
+                const int a = st.activeNext[t];
+                shAct[t] = a;
+                shAlpha[t] = (a && pq != 0) ? st.rhoNext[t] / pq : 0.0;
+            }
+            __syncthreads();
+            int any = 0;
+#pragma unroll
+            for (int k = 0; k < NCH; k++)
+            {
+                any |= shAct[k];
+            }
+            go = any != 0;
+            act = shAct[c];
+            alpha = static_cast<T>(shAlpha[c]);
+        }
+        else
+        {
+            if (lb == 0 && threadIdx.x == 0)
+            {
+                st.flags[1] = 0; // the direction kernel before has used p = z
+            }
+            go = systemActive<NCH>(st, s);
+            act = st.active[s * NCH + c];
+            alpha = static_cast<T>(st.alpha[s * NCH + c]);
+            if (go && i0 < end)
             {
                 r0 = r[i0];
                 if (act)
@@ -1383,52 +1540,10 @@ namespace nsfem
                     d0 = invDiagP[row0];
                 }
             }
-            T alpha;
-            if (!kWide<NCH> && fusedAlpha)
-            {
-                double rho[NCH];
-#pragma unroll
-                for (int k = 0; k < NCH; k++)
-                {
-                    rho[k] = st.rho[s * NCH + k];
-                }
-                int j0 = 0, j1 = bm.numBlocks;
-                if (bm.numSystems > 1)
-                {
-                    j0 = bm.sysBlockStart[s];
-                    j1 = bm.sysBlockStart[s + 1];
-                }
-                double pq[NCH];
-#pragma unroll
-                for (int k = 0; k < NCH; k++)
-                {
-                    pq[k] = 0;
-                }
-#pragma unroll 8
-                for (int j = j0 + static_cast<int>(threadIdx.x); j < j1; j += kBlock)
-                {
-#pragma unroll
-                    for (int k = 0; k < NCH; k++)
-                    {
-                        pq[k] += st.partials[static_cast<size_t>(j) * NCH + k];
-                    }
-                }
-                blockSum<NCH>(pq, sm);
-                if (threadIdx.x == 0)
-                {
-#pragma unroll
-                    for (int k = 0; k < NCH; k++)
-                    {
-                        shAlpha[k] = (st.active[s * NCH + k] && pq[k] != 0) ? rho[k] / pq[k] : 0.0;
-                    }
-                }
-                __syncthreads();
-                alpha = static_cast<T>(shAlpha[c]);
-            }
-            else
-            {
-                alpha = static_cast<T>(st.alpha[s * NCH + c]);
-            }
+        }
+        double sum = 0;
+        if (go)
+        {
             auto finish = [&](long i, int row, T ri, T xi, T pi, T qi, P di)
             {
                 if (act)
@@ -1653,6 +1768,50 @@ namespace nsfem
                 }
             }
             if (withUpdate && !anyActive)
+            {
+                st.flags[0] = 1;
+            }
+        }
+    }
+
+    // Fused reductions: the convergence test of a solve's last update (no
+    // direction kernel follows to make it): rr, the iteration count and the
+    // stop flag as pcgRhoFinalizeKernel records them (one system).
+    template <int NCH>
+    __global__ void __launch_bounds__(kFinBlock) pcgUpdateCheckKernel(BlockMap bm, SolveState st, double * totals)
+    {
+        __shared__ double sm[NCH * kFinWaves];
+        if (st.flags[0])
+        {
+            return;
+        }
+        systemTotals<NCH>(bm, st.partials2, totals, sm);
+        if (threadIdx.x == 0)
+        {
+            int any = 0;
+            for (int k = 0; k < NCH; k++)
+            {
+                if (!st.active[k])
+                {
+                    continue;
+                }
+                const double rr = totals[k];
+                st.rr[k] = rr;
+                st.iters[k] = st.iters[k] + 1;
+                if (!(rr == rr) || rr > 1e300)
+                {
+                    st.flags[2] = 1;
+                    st.active[k] = 0;
+                    continue;
+                }
+                if (rr <= st.tol2 * st.ref[k])
+                {
+                    st.active[k] = 0;
+                    continue;
+                }
+                any = 1;
+            }
+            if (!any)
             {
                 st.flags[0] = 1;
             }

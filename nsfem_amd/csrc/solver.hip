@@ -129,10 +129,11 @@ namespace nsfem
     void DeviceState::init(int numBlocks, int numSystems, int numCh)
     {
         const int K = numSystems * numCh;
-        scal.resize(static_cast<size_t>(7) * K);
-        ints.resize(static_cast<size_t>(2) * K + 4);
+        scal.resize(static_cast<size_t>(8) * K);
+        ints.resize(static_cast<size_t>(3) * K + 4);
         partials.resize(static_cast<size_t>(std::max(1, numBlocks)) * 2 * numCh);
         partials2.resize(static_cast<size_t>(std::max(1, numBlocks)) * numCh);
+        partials3.resize(static_cast<size_t>(std::max(1, numBlocks)) * numCh);
         totals.resize(static_cast<size_t>(numSystems) * 2 * numCh);
         scal.zero();
         ints.zero();
@@ -152,8 +153,11 @@ namespace nsfem
         s.active = ints.get();
         s.iters = ints.get() + K;
         s.flags = ints.get() + 2 * K;
+        s.activeNext = ints.get() + 2 * K + 4;
+        s.rhoNext = d + 7 * K;
         s.partials = partials.get();
         s.partials2 = partials2.get();
+        s.partials3 = partials3.get();
         s.tol2 = tol2;
         return s;
     }
@@ -539,11 +543,8 @@ namespace nsfem
 
         // reduction scratch sized by the block count; captured graphs hold
         // the old launch shapes and buffers
-        inner_.init(numBlocks_, numSystems_, opt_.numCh);
-        if (opt_.precision == SolverPrecision::Mixed)
-        {
-            outer_.init(numBlocks_, numSystems_, opt_.numCh);
-        }
+        rowsDir_ = rowsUpd_ = rowsRho_ = 0;
+        setupKindMaps();
         // the sweeps take the same shape until reshapeSweeps
         reshapeSweeps(lanes_, rowsPerBlock_);
     }
@@ -562,20 +563,81 @@ namespace nsfem
         return (need + slab - 1) / slab * slab;
     }
 
-    void McgsSolver::setFusedAlpha(int mode)
+    void McgsSolver::setFusedReductions(int mode)
     {
         if (mode < -1 || mode > 1)
         {
-            throw std::invalid_argument("fusedAlpha must be -1 (auto), 0 or 1");
+            throw std::invalid_argument("fusedReductions must be -1 (auto), 0 or 1");
         }
         NSFEM_HIP(hipStreamSynchronize(stream_));
-        fusedAlphaMode_ = mode;
+        fusedMode_ = mode;
+        setupKindMaps();
         destroyGraphs();
     }
 
-    bool McgsSolver::fusedAlpha() const
+    void McgsSolver::setKindRows(int direction, int update, int rho)
     {
-        return fusedAlphaMode_ == 1 && opt_.numCh <= 2;
+        if (direction < 0 || update < 0 || rho < 0)
+        {
+            throw std::invalid_argument("rows per workgroup must be >= 0");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        // the direction and rho kernels work in row slabs of their lanes
+        const int slab = kBlock / lanes_;
+        auto slabs = [&](int r) { return r == 0 ? 0 : std::max(slab, r / slab * slab); };
+        rowsDir_ = slabs(direction);
+        rowsUpd_ = update;
+        rowsRho_ = slabs(rho);
+        setupKindMaps();
+        destroyGraphs();
+    }
+
+    int McgsSolver::kindRows(int kind) const
+    {
+        if (!fused())
+        {
+            return rowsPerBlock_;
+        }
+        switch (kind)
+        {
+        case 0: return bmDir_.rowsPerBlock;
+        case 1: return bmUpd_.rowsPerBlock;
+        case 2: return bmRho_.rowsPerBlock;
+        default: throw std::invalid_argument("kind must be 0 (direction), 1 (update) or 2 (rho)");
+        }
+    }
+
+    void McgsSolver::setupKindMaps()
+    {
+        int maxBlocks = numBlocks_;
+        if (fused())
+        {
+            auto flatMap = [&](int rows, int & nb)
+            {
+                BlockMap m = bm_;
+                m.rowsPerBlock = rows > 0 ? rows : rowsPerBlock_;
+                nb = (n_ + m.rowsPerBlock - 1) / m.rowsPerBlock;
+                m.numBlocks = nb;
+                maxBlocks = std::max(maxBlocks, nb);
+                return m;
+            };
+            bmDir_ = flatMap(rowsDir_, nbDir_);
+            bmUpd_ = flatMap(rowsUpd_, nbUpd_);
+            bmRho_ = flatMap(rowsRho_, nbRho_);
+            color0BlocksRho_ = (color0End_ + bmRho_.rowsPerBlock - 1) / bmRho_.rowsPerBlock;
+        }
+        else
+        {
+            bmDir_ = bmUpd_ = bmRho_ = bm_;
+            nbDir_ = nbUpd_ = nbRho_ = numBlocks_;
+            color0BlocksRho_ = color0Blocks_;
+        }
+        // reduction scratch sized by the largest block count
+        inner_.init(maxBlocks, numSystems_, opt_.numCh);
+        if (opt_.precision == SolverPrecision::Mixed)
+        {
+            outer_.init(maxBlocks, numSystems_, opt_.numCh);
+        }
     }
 
     double McgsSolver::iterationTime(int reps)
@@ -642,6 +704,31 @@ namespace nsfem
             }
         }
         reshapeSweeps(sweepLanes, sweepRows);
+        // 3. fused reductions: the direction, update and rho kernels on row
+        // counts of their own (every block of the next kernel re-sums their
+        // partials, so fewer, larger blocks can pay)
+        if (fused())
+        {
+            int rows[3] = {0, 0, 0};
+            for (int kind = 0; kind < 3; kind++)
+            {
+                int keep = rows[kind];
+                for (int mult : {2, 4, 8, 16})
+                {
+                    int trial[3] = {rows[0], rows[1], rows[2]};
+                    trial[kind] = mult * rowsPerBlock_;
+                    setKindRows(trial[0], trial[1], trial[2]);
+                    const double t = iterationMs();
+                    if (t < best)
+                    {
+                        best = t;
+                        keep = trial[kind];
+                    }
+                }
+                rows[kind] = keep;
+                setKindRows(rows[0], rows[1], rows[2]);
+            }
+        }
         return {bestLanes, bestRows};
     }
 
@@ -843,11 +930,16 @@ namespace nsfem
     template <typename T, typename P>
     void McgsSolver::launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate)
     {
+        // fused reductions: the next direction kernel finishes rho
+        const bool fin = !fused();
         dispatchLC(lanes_, opt_.numCh, [&]<int L, int C>()
                    {
-                       pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_,
-                                                                              color0Blocks_);
-                       pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
+                       pcgRhoKernel<P, T, L, C><<<nbRho_, kBlock, 0, s>>>(v.M, bmRho_, st, v.rP, v.r, v.z, color0End_,
+                                                                          color0BlocksRho_);
+                       if (fin)
+                       {
+                           pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
+                       }
                    });
     }
 
@@ -856,19 +948,19 @@ namespace nsfem
     {
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
-        const int fused = fusedAlpha() ? 1 : 0;
+        const int fz = fused() ? 1 : 0;
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
-                       pcgDirectionKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.z, v.p, v.q);
-                       if (!fused)
+                       pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, nbRho_, nbUpd_);
+                       if (!fz)
                        {
                            pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
                        }
                    });
         dispatchC(nch, [&]<int C>()
                   {
-                      pcgUpdateKernel<T, P, C><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                            color0End_, fused);
+                      pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                        color0End_, fz, nbDir_);
                   });
         // the update's convergence test is finished together with rho
         sgsSweeps<P>(s, v.M, st, v.rP, v.z);
@@ -976,7 +1068,11 @@ namespace nsfem
             }
             slot = prev;
         }
-        (void)ds;
+        if (fused())
+        {
+            // the last update's convergence test (no direction kernel follows)
+            dispatchC(opt_.numCh, [&]<int C>() { pcgUpdateCheckKernel<C><<<1, kFinBlock, 0, s>>>(bmUpd_, st, ds.totals.get()); });
+        }
         if (usePrediction)
         {
             predictedIters = launched; // only sizes the chunks of the next solve
@@ -1159,6 +1255,7 @@ namespace nsfem
             constexpr bool same = std::is_same_v<T, P>;
             const double tsT = sizeof(T), tsP = sizeof(P);
             const PcgOperands<T, P> v = operands<T, P>();
+            const int fz = fused() ? 1 : 0;
             // finite work vectors (a fresh solver's memory is arbitrary)
             const long nv = static_cast<long>(n_) * nch;
             const int dtT = std::is_same_v<T, double> ? F64 : F32;
@@ -1187,7 +1284,10 @@ namespace nsfem
                 launch = [=, this]()
                 {
                     dispatchLC(lanes_, nch, [&]<int L, int C>()
-                               { pcgDirectionKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.z, v.p, v.q); });
+                               {
+                                   pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, nbRho_,
+                                                                                            nbUpd_);
+                               });
                 };
             }
             else if (which == "sweep")
@@ -1229,8 +1329,8 @@ namespace nsfem
                 {
                     dispatchLC(lanes_, nch, [&]<int L, int C>()
                                {
-                                   pcgRhoKernel<P, T, L, C><<<numBlocks_, kBlock, 0, s>>>(v.M, bm_, st, v.rP, v.r, v.z, color0End_,
-                                                                                  color0Blocks_);
+                                   pcgRhoKernel<P, T, L, C><<<nbRho_, kBlock, 0, s>>>(v.M, bmRho_, st, v.rP, v.r, v.z, color0End_,
+                                                                                  color0BlocksRho_);
                                });
                 };
             }
@@ -1239,21 +1339,21 @@ namespace nsfem
                 // x, r read and written, p, q read; rP written (P narrower
                 // than T); z = r / d on colour 0
                 bytes = 6.0 * nch * n * tsT + (same ? 0.0 : nch * n * tsP) + n0 * tsP + nch * n0 * tsP;
-                const int fused = fusedAlpha() ? 1 : 0;
+
                 launch = [=, this]()
                 {
                     dispatchC(nch, [&]<int C>()
                               {
-                                  pcgUpdateKernel<T, P, C><<<numBlocks_, kBlock, 0, s>>>(n_, bm_, st, v.p, v.q, v.x, v.r, v.rP, v.z,
-                                                                                        v.M.invDiag, color0End_, fused);
+                                  pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z,
+                                                                                    v.M.invDiag, color0End_, fz, nbDir_);
                               });
                 };
             }
             else if (which == "finalize")
             {
-                // one of the per-iteration one-block finalize kernels (two,
-                // one when the update kernel finishes alpha itself)
-                launches = fusedAlpha() ? 1 : 2;
+                // one of the per-iteration one-block finalize kernels (two;
+                // none with fused reductions)
+                launches = fz ? 0 : 2;
                 bytes = static_cast<double>(numBlocks_) * nch * 8.0;
                 launch = [=, this]()
                 { dispatchC(nch, [&]<int C>() { pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals); }); };
@@ -1288,8 +1388,16 @@ namespace nsfem
         // A state in which every kernel does its full work: all channels
         // active, not the first iteration, zero step lengths.
         fillInt(K, 1, st.active, s);
+        fillInt(K, 1, st.activeNext, s);
         fillInt(4, 0, st.flags, s);
+        // finite partial sums: a fused direction kernel reads the rho and
+        // update partials before any launch of this replay has written them
+        for (DeviceArray<double> * a : {&inner_.partials, &inner_.partials2, &inner_.partials3})
+        {
+            fill(F64, static_cast<long>(a->size()), 0.0, a->get(), s);
+        }
         fill(F64, 3L * K, 0.0, st.rho, s); // rho, alpha, beta
+        fill(F64, 1L * K, 0.0, st.rhoNext, s);
         fill(F64, 1L * K, 1.0, st.ref, s);
         launch();
         NSFEM_HIP(hipGetLastError());

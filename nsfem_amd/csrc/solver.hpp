@@ -84,8 +84,8 @@ namespace nsfem
     struct DeviceState
     {
         DeviceArray<double> scal;    // 8 arrays of S*NCH doubles
-        DeviceArray<int> ints;       // active, iters (S*NCH each), flags[4]
-        DeviceArray<double> partials, partials2;
+        DeviceArray<int> ints;       // active, iters (S*NCH each), flags[4], activeNext
+        DeviceArray<double> partials, partials2, partials3;
         DeviceArray<double> totals;
         SolveState view(int k, double tol2);
         void init(int numBlocks, int numSystems, int numCh);
@@ -150,12 +150,19 @@ namespace nsfem
 
         hipStream_t stream() const { return stream_; }
         void setUseGraphs(bool on) { useGraphs_ = on; }
-        // Step length alpha finished inside the update kernel (1) or by its
-        // own one-block finalize kernel (0, default; -1 picks the same).
-        // Every update block then re-reads all direction partials of its
-        // system: at 500k rows that costs about what the launch saves.
-        void setFusedAlpha(int mode);
-        bool fusedAlpha() const;
+        // Fused reductions (one system): the direction kernel finishes the
+        // previous rho and convergence test and the update kernel the step
+        // length, each block from the producers' partial sums, instead of
+        // two one-block finalize kernels per iteration.  mode 1 or -1
+        // (default): on for single systems; 0: off.
+        void setFusedReductions(int mode);
+        bool fused() const { return flat() && fusedMode_ != 0; }
+        // Rows per workgroup of the direction, update and rho kernels of the
+        // fused mode (0: the main shape's); the sums of their partials are
+        // redone by every block of the next kernel, so fewer, larger blocks
+        // pay there.  reshape resets them.
+        void setKindRows(int direction, int update, int rho);
+        int kindRows(int kind) const; // 0 direction, 1 update, 2 rho
         // Fewest rows per workgroup of the default and autotuned shapes.
         int minRowsPerBlock() const;
 
@@ -185,8 +192,9 @@ namespace nsfem
         // Times one graph-replayed PCG iteration for lanes {2, 4, 8, 16} x
         // {1, 2, 4, 8} row slabs per workgroup (wide tiles: their own lanes x
         // {1, 2, 4, 8} times minRowsPerBlock()) and keeps the fastest shape,
-        // then the sweeps' own shape over lanes x {1, 2, 4} slabs; returns
-        // (lanes, rowsPerBlock).
+        // then the sweeps' own shape over lanes x {1, 2, 4} slabs, then (fused
+        // reductions) {2, 4, 8, 16} times the rows for the direction, update
+        // and rho kernels in turn; returns (lanes, rowsPerBlock).
         // Overwrites the internal work vectors (not the io vectors).
         std::pair<int, int> autotune(int reps);
 
@@ -249,7 +257,12 @@ namespace nsfem
         };
         std::vector<ColorStats> colorStats_;
         int numBlocks_ = 0;
-        int fusedAlphaMode_ = 0;
+        int fusedMode_ = -1;
+        // per-kernel-kind maps: all bm_ unless fused (flat, own row counts)
+        int rowsDir_ = 0, rowsUpd_ = 0, rowsRho_ = 0;
+        BlockMap bmDir_{}, bmUpd_{}, bmRho_{};
+        int nbDir_ = 0, nbUpd_ = 0, nbRho_ = 0, color0BlocksRho_ = 0;
+        void setupKindMaps();
         // the colour sweeps' workgroup map
         int sweepLanes_ = 8, sweepRows_ = 0;
         std::vector<int> colorSweepBlockStart_;

@@ -290,6 +290,12 @@ class AbstractSolver:
     def reshapeSweeps(self, lanes=0, rowsPerBlock=0):
         call("nsfem_solver_reshape_sweeps", self._h, int(lanes), int(rowsPerBlock))
 
+    def kindRows(self):
+        """Rows per workgroup of the (direction, update, rho) kernels."""
+        a, b, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        call("nsfem_solver_kind_rows", self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(d))
+        return a.value, b.value, d.value
+
     def sweepShape(self):
         """(lanes per row, rows per workgroup) of the colour sweeps."""
         a, b = ctypes.c_int(), ctypes.c_int()

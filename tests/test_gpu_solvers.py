@@ -236,14 +236,14 @@ def test_kernel_timing_hooks_report_per_launch_figures(octx_h025):
     s, _ = _solve("pcg", m, b, max_iters=2000, target=1e-10)
     C = s.numColors()
     nnz = m.nnz
-    # alpha by its own finalize kernel by default, or inside the update kernel
+    # one system: the reductions are fused by default (no finalize launches)
     for fused in (-1, 0, 1):
-        s.setOption("fusedAlpha", fused)
-        expect = {"direction": 1, "sweep": 2 * C - 3, "rho": 1, "update": 1, "finalize": 1 if fused == 1 else 2}
+        s.setOption("fusedReductions", fused)
+        expect = {"direction": 1, "sweep": 2 * C - 3, "rho": 1, "update": 1, "finalize": 2 if fused == 0 else 0}
         for k, launches in expect.items():
             ms, by, per_it = s.timeKernel(k, reps=3)
             assert ms > 0 and by > 0 and per_it == launches, (k, fused)
-    s.setOption("fusedAlpha", -1)
+    s.setOption("fusedReductions", -1)
     # a direction launch streams the whole off-diagonal matrix once
     assert s.timeKernel("direction", reps=1)[1] >= (nnz - m.shape[0]) * 12
     s.getRhs().upload(b)
@@ -321,19 +321,20 @@ def test_reshape_and_autotune_keep_results(octx_h025):
         s.reshape(3, 0)
 
 
-@pytest.mark.parametrize("name,nch", [("pcg", 1), ("pcg", 2), ("pcg_mixed", 1), ("pcg_mixed", 2)])
-def test_fused_alpha_matches_finalize_kernel(octx_h025, name, nch):
-    """alpha finished inside the update kernel (every block sums its system's
-    p.q partials) or by the one-block finalize kernel: the two sums differ
-    only in order, so both converge in the same iterations (+-1) to the same
-    solution, and the fused mode repeats bit for bit."""
+@pytest.mark.parametrize("name,nch", [("pcg", 1), ("pcg", 2), ("pcg_mixed", 1), ("pcg_mixed", 2), ("pcg", 8)])
+def test_fused_reductions_match_finalize_kernels(octx_h025, name, nch):
+    """Fused reductions (the direction kernel finishes rho and the
+    convergence test, the update kernel the step length, every block from
+    the partial sums) against the one-block finalize kernels: the sums
+    differ only in order, so both converge in the same iterations (+-1) to
+    the same solution, and the fused mode repeats bit for bit."""
     m = octx_h025["Kp_int"]
     n = m.shape[0]
     rng = np.random.default_rng(35)
     b = rng.uniform(-1, 1, nch * n)
     runs = {}
     for fused in (0, 1, 1):
-        s, x = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10, fusedAlpha=fused)
+        s, x = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10, fusedReductions=fused)
         its = s.getLastIterations()
         if fused in runs:
             assert its == runs[fused][1] and np.array_equal(x, runs[fused][0])
@@ -341,17 +342,51 @@ def test_fused_alpha_matches_finalize_kernel(octx_h025, name, nch):
         for c in range(nch):
             bc = b[c * n:(c + 1) * n]
             assert np.linalg.norm(bc - m @ x[c * n:(c + 1) * n]) <= 1.0001e-10 * np.linalg.norm(bc)
+            assert s.getLastRelRes(c) <= 1e-10
     (x0, it0), (x1, it1) = runs[0], runs[1]
     assert abs(it0 - it1) <= 1
     assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
 
 
-def test_fused_alpha_batch_of_uneven_systems(oracle):
-    """Fused alpha per system in a batch: each block sums only its own
-    system's partials, and every system converges as with the finalize
-    kernel (pcg: in the oracle's iterations +-1; pcg_mixed needs more
-    iterations than FP64 PCG at 1e-11, so it is compared with its own
-    unfused run)."""
+@pytest.mark.parametrize("fused", [0, 1])
+def test_iteration_cap_records_the_last_update(octx_h025, fused):
+    """A solve that stops at maxIters: the iteration count is the cap and the
+    reported relative residual is that of the returned x (the last update's
+    convergence test is recorded also when no direction kernel follows)."""
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(40)
+    b = rng.uniform(-1, 1, m.shape[0])
+    for cap in (1, 2, 17):
+        s, x = _solve("pcg", m, b, max_iters=cap, target=1e-12, fusedReductions=fused)
+        assert s.getLastIterations() == cap
+        rel = np.linalg.norm(b - m @ x) / np.linalg.norm(b)
+        assert abs(s.getLastRelRes() - rel) <= 1e-9 * rel
+
+
+def test_fused_reduction_row_counts(octx_h025):
+    """Own rows per workgroup for the direction, update and rho kernels:
+    results repeat within the iteration tolerance, the shape is reported."""
+    m = octx_h025["Kp_int"]
+    rng = np.random.default_rng(41)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x0 = _solve("pcg_mixed", m, b, max_iters=3000, target=1e-10)
+    it0 = s.getLastIterations()
+    L, R = s.shape()
+    for rows in ((4 * R, 2 * R, 8 * R), (R, 16 * R, 2 * R), (0, 0, 0)):
+        for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho"), rows):
+            s.setOption(key, v)
+        assert s.kindRows() == tuple(v if v else R for v in rows)
+        s.getRhs().upload(b)
+        s.getSol().memsetZero()
+        s.solve()
+        x = s.getSol().download()
+        assert abs(s.getLastIterations() - it0) <= 1
+        assert np.linalg.norm(x - x0) <= 1e-9 * np.linalg.norm(x0)
+
+
+def test_fused_reductions_option_on_a_batch(oracle):
+    """Batches keep the finalize kernels (their workgroup maps are per
+    system); the option changes nothing there."""
     from nsfem_amd import cu, fem, mesher
     mats = []
     for k, n in enumerate([600, 2500, 1300]):
@@ -360,24 +395,20 @@ def test_fused_alpha_batch_of_uneven_systems(oracle):
         mats.append(ctx.scipy("pressureStiffnessInternal"))
     rng = np.random.default_rng(36)
     bs = [rng.uniform(-1, 1, m.shape[0]) for m in mats]
-    for name in ("pcg", "pcg_mixed"):
-        its = {}
-        for fused in (0, 1):
-            s = cu.createBatchSolver(name, 1, mats, 3000, 1e-11, 1)
-            s.setOption("fusedAlpha", fused)
-            s.getRhs().upload(np.concatenate(bs))
-            s.getSol().memsetZero()
-            s.solve()
-            x = s.getSol().download()
-            its[fused] = [s.getLastIterations(k) for k in range(len(mats))]
-            for k, (m, b) in enumerate(zip(mats, bs)):
-                o = s.system_offsets[k]
-                xk = x[o:o + m.shape[0]]
-                assert np.linalg.norm(b - m @ xk) <= 1.0001e-11 * np.linalg.norm(b)
-                if name == "pcg":
-                    xo, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-11, 3000)
-                    assert abs(its[fused][k] - it) <= 1, (fused, k)
-        assert all(abs(a - b) <= 1 for a, b in zip(its[0], its[1])), its
+    xs = {}
+    for fused in (0, 1):
+        s = cu.createBatchSolver("pcg", 1, mats, 3000, 1e-11, 1)
+        s.setOption("fusedReductions", fused)
+        s.getRhs().upload(np.concatenate(bs))
+        s.getSol().memsetZero()
+        s.solve()
+        xs[fused] = s.getSol().download()
+        for k, (m, b) in enumerate(zip(mats, bs)):
+            o = s.system_offsets[k]
+            assert np.linalg.norm(b - m @ xs[fused][o:o + m.shape[0]]) <= 1.0001e-11 * np.linalg.norm(b)
+            xo, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-11, 3000)
+            assert abs(s.getLastIterations(k) - it) <= 1, (fused, k)
+    assert np.array_equal(xs[0], xs[1])
 
 
 @pytest.mark.parametrize("name", ["pcg", "pcg_mixed"])
