@@ -358,7 +358,7 @@ namespace nsfem
         {
             bytes = nnz * (ts + 4) + (N + 1) * 4 + 4 * N * ts;
             launch = [&]()
-            { csrMul(A_.view(cfg_.dtype), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.velocity.lanes, s); };
+            { csrMul(A_.view(cfg_.dtype), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, 0, s); };
         }
         else
         {
@@ -438,8 +438,9 @@ namespace nsfem
         assembleConvection(u_.get(), A_.values.get(), nullptr, s);
         NSFEM_HIP(hipEventRecord(ev_[1], s));
 
-        // Tentative velocity: M a = A u;  u -= tau a
-        csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, cfg_.velocity.lanes, s);
+        // Tentative velocity: M a = A u;  u -= tau a (the product's lanes per
+        // row follow its mean row length, not the solver shapes)
+        csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, 0, s);
         fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
         vSolver_->solve();
         st.mseTentX = maxMse(*vSolver_, 0);
