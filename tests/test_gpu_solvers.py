@@ -480,3 +480,39 @@ def test_wide_channel_pure_neumann_batch(octx_h025):
     for c in range(nch):
         assert abs(x[c].mean()) <= 1e-12 * np.max(np.abs(x[c]))
         assert np.linalg.norm(b[c] - m @ x[c]) <= 1.0001e-8 * np.linalg.norm(b[c])
+
+
+def test_batch_of_uneven_systems_with_wide_tiles():
+    """A block-diagonal batch of uneven systems, each with 4 right-hand sides
+    (one lane per channel, per-system workgroup table): every column of every
+    system equals its own one-system solve."""
+    from nsfem_amd import cu, fem, mesher
+    mats = []
+    for k, n in enumerate([700, 1800]):
+        m = mesher.channel_mesh(n, seed=200 + k)
+        ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), fem.DfgConditions(0.01, 1.0, 1.0, 4.0))
+        mats.append(ctx.scipy("pressureStiffnessInternal"))
+    nch = 4
+    rng = np.random.default_rng(42)
+    bs = [rng.uniform(-1, 1, nch * m.shape[0]) for m in mats]
+    for name in ("pcg", "pcg_mixed"):
+        s = cu.createBatchSolver(name, nch, mats, 3000, 1e-10, 1)
+        # io layout: channel-planar over the stacked systems
+        total = sum(m.shape[0] for m in mats)
+        rhs = np.zeros(nch * total)
+        for k, (m, b) in enumerate(zip(mats, bs)):
+            o = s.system_offsets[k]
+            for c in range(nch):
+                rhs[c * total + o:c * total + o + m.shape[0]] = b[c * m.shape[0]:(c + 1) * m.shape[0]]
+        s.getRhs().upload(rhs)
+        s.getSol().memsetZero()
+        s.solve()
+        x = s.getSol().download()
+        for k, (m, b) in enumerate(zip(mats, bs)):
+            o, n = s.system_offsets[k], m.shape[0]
+            s1, x1 = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10)
+            for c in range(nch):
+                xc = x[c * total + o:c * total + o + n]
+                bc = b[c * n:(c + 1) * n]
+                assert np.linalg.norm(bc - m @ xc) <= 1.0001e-10 * np.linalg.norm(bc), (name, k, c)
+                assert np.linalg.norm(xc - x1[c * n:(c + 1) * n]) <= 1e-8 * np.linalg.norm(xc), (name, k, c)
