@@ -853,7 +853,7 @@ namespace nsfem
     __device__ __forceinline__ void pcgDirectionNarrow(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
                                                        T * __restrict__ p, T * __restrict__ q, int fused, int rhoParts, int updParts)
     {
-        __shared__ double sm[NCH * kWaves];
+        __shared__ double sm[2 * NCH * kWaves];
         __shared__ double shBeta[NCH];
         __shared__ int shAct[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
@@ -1026,6 +1026,40 @@ namespace nsfem
         return s;
     }
 
+    // channelBlockSum of two values at once (one pair of barriers); sm holds
+    // 2 * NCH * kWaves doubles.  Totals in threads c < NCH.
+    template <int NCH>
+    __device__ __forceinline__ void channelBlockSum2(double & a, double & b, double * sm)
+    {
+#pragma unroll
+        for (int off = 32; off >= NCH; off >>= 1)
+        {
+            a += __shfl_xor(a, off, 64);
+            b += __shfl_xor(b, off, 64);
+        }
+        const int wave = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        if (lane < NCH)
+        {
+            sm[lane * kWaves + wave] = a;
+            sm[(NCH + lane) * kWaves + wave] = b;
+        }
+        __syncthreads();
+        double sa = 0, sb = 0;
+        if (static_cast<int>(threadIdx.x) < NCH)
+        {
+#pragma unroll
+            for (int w = 0; w < kWaves; w++)
+            {
+                sa += sm[threadIdx.x * kWaves + w];
+                sb += sm[(NCH + threadIdx.x) * kWaves + w];
+            }
+        }
+        __syncthreads();
+        a = sa;
+        b = sb;
+    }
+
     // partials[slot*K + offset + c] = total of channel c (threads c < NCH).
     template <int NCH, int K>
     __device__ __forceinline__ void storeChannelPartial(double total, double * partials, int slot, int offset)
@@ -1066,13 +1100,35 @@ namespace nsfem
     __device__ __forceinline__ bool fusedRhoStep(const SolveState & st, int rhoParts, int updParts, int lb, double * sm,
                                                  double * shBeta, int * shAct)
     {
+        // the flags and per-channel state load with the partial sums; the two
+        // partial loops run back to back and finish under one pair of
+        // barriers (the |r|^2 sum is unused in the first iteration)
+        const int t = static_cast<int>(threadIdx.x);
         const int first = st.flags[1];
-        const double rho = channelPartialSum<NCH>(st.partials, rhoParts, sm);
-        const double rr = first ? 0.0 : channelPartialSum<NCH>(st.partials2, updParts, sm);
-        const int c = static_cast<int>(threadIdx.x);
-        if (c < NCH)
+        int was = 0;
+        double ref = 0, rhoOld = 0;
+        if (t < NCH)
         {
-            const int was = st.active[c];
+            was = st.active[t];
+            ref = st.ref[t];
+            rhoOld = st.rho[t];
+        }
+        constexpr int G = kBlock / NCH; // threads per channel
+        const int c = t % NCH;
+        double rho = 0, rr = 0;
+#pragma unroll 4
+        for (int i = t / NCH; i < rhoParts; i += G)
+        {
+            rho += st.partials[static_cast<size_t>(i) * NCH + c];
+        }
+#pragma unroll 4
+        for (int i = t / NCH; i < updParts; i += G)
+        {
+            rr += st.partials2[static_cast<size_t>(i) * NCH + c];
+        }
+        channelBlockSum2<NCH>(rho, rr, sm);
+        if (t < NCH)
+        {
             int act = was;
             int bad = 0;
             if (was && !first)
@@ -1082,26 +1138,25 @@ namespace nsfem
                     act = 0;
                     bad = 1;
                 }
-                else if (rr <= st.tol2 * st.ref[c])
+                else if (rr <= st.tol2 * ref)
                 {
                     act = 0;
                 }
             }
-            const double rhoOld = st.rho[c];
-            shBeta[c] = (first || rhoOld == 0) ? 0.0 : rho / rhoOld;
-            shAct[c] = act;
+            shBeta[t] = (first || rhoOld == 0) ? 0.0 : rho / rhoOld;
+            shAct[t] = act;
             if (act && !(rho == rho))
             {
                 bad = 1;
             }
             if (lb == 0)
             {
-                st.rhoNext[c] = rho;
-                st.activeNext[c] = act;
+                st.rhoNext[t] = rho;
+                st.activeNext[t] = act;
                 if (was && !first)
                 {
-                    st.rr[c] = rr;
-                    st.iters[c] = st.iters[c] + 1;
+                    st.rr[t] = rr;
+                    st.iters[t] = st.iters[t] + 1;
                 }
                 if (bad)
                 {
@@ -1116,7 +1171,7 @@ namespace nsfem
         {
             any |= shAct[k];
         }
-        if (lb == 0 && threadIdx.x == 0)
+        if (lb == 0 && t == 0)
         {
             st.flags[3] = any ? 0 : 1;
         }
@@ -1303,7 +1358,7 @@ namespace nsfem
     __device__ __forceinline__ void pcgDirectionWide(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z,
                                                      T * __restrict__ p, T * __restrict__ q, int fused, int rhoParts, int updParts)
     {
-        __shared__ double sm[NCH * kWaves];
+        __shared__ double sm[2 * NCH * kWaves];
         __shared__ double shBeta[NCH];
         __shared__ int shAct[NCH];
         const int lb = logicalBlock(blockIdx.x, gridDim.x);
