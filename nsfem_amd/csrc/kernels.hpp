@@ -419,6 +419,16 @@ namespace nsfem
     };
 
 
+    // Registers of the fused reductions' loads (fusedRhoLoad / fusedRhoFinish).
+    constexpr int kFusedPre = 8;
+    template <int NCH>
+    struct FusedRhoLoads
+    {
+        int first, was;
+        double ref, rhoOld;
+        double rho[kFusedPre], rr[kFusedPre];
+    };
+
     // Finishes a per-system reduction in a one-block finalize kernel of
     // kFinBlock threads: out[s*K + k] is the sum over the blocks of system s
     // of their partials.  Producers store a block's partial at its slot (the
@@ -876,15 +886,22 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        const bool go = fused ? fusedRhoStep<NCH>(st, rhoParts, updParts, lb, sm, shBeta, shAct) : systemActive<NCH>(st, s);
-        if (go)
+        // fused: the partial sums load now, with the first slab's rows, and
+        // are finished after its gathers (beta is needed only to combine)
+        FusedRhoLoads<NCH> fl;
+        bool pending = fused != 0;
+        if (pending)
+        {
+            fusedRhoLoad<NCH>(st, rhoParts, updParts, fl);
+        }
+        if (pending || systemActive<NCH>(st, s))
         {
             const bool first = st.flags[1] != 0;
             T beta[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                beta[c] = static_cast<T>(fused ? shBeta[c] : st.beta[s * NCH + c]);
+                beta[c] = pending ? T(0) : static_cast<T>(st.beta[s * NCH + c]);
             }
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
@@ -920,6 +937,20 @@ namespace nsfem
                 }
                 T acc[NCH];
                 rowDot<T, LANES, NCH>(A, j0, j1, z, acc);
+                if (pending)
+                {
+                    // uniform: every thread is in the first slab here
+                    pending = false;
+                    if (!fusedRhoFinish<NCH>(st, fl, rhoParts, updParts, lb, sm, shBeta, shAct))
+                    {
+                        break;
+                    }
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        beta[c] = static_cast<T>(shBeta[c]);
+                    }
+                }
                 if (lead)
                 {
 #pragma unroll
@@ -1096,39 +1127,72 @@ namespace nsfem
     // activeNext, flags[3]) and rr, the iteration count and non-finite
     // results for the host; no block of this launch reads those.  Returns
     // whether any channel goes on; beta and activity per channel in LDS.
+    // Fused reductions, first half (fusedRhoLoad): the flags, the per-channel
+    // state and up to kFusedPre partial sums per thread of each set load into
+    // registers, so that they are in flight with the kernel's first row loads.
     template <int NCH>
-    __device__ __forceinline__ bool fusedRhoStep(const SolveState & st, int rhoParts, int updParts, int lb, double * sm,
-                                                 double * shBeta, int * shAct)
+    __device__ __forceinline__ void fusedRhoLoad(const SolveState & st, int rhoParts, int updParts, FusedRhoLoads<NCH> & f)
     {
-        // the flags and per-channel state load with the partial sums; the two
-        // partial loops run back to back and finish under one pair of
-        // barriers (the |r|^2 sum is unused in the first iteration)
         const int t = static_cast<int>(threadIdx.x);
-        const int first = st.flags[1];
-        int was = 0;
-        double ref = 0, rhoOld = 0;
+        f.first = st.flags[1];
+        f.was = 0;
+        f.ref = 0;
+        f.rhoOld = 0;
         if (t < NCH)
         {
-            was = st.active[t];
-            ref = st.ref[t];
-            rhoOld = st.rho[t];
+            f.was = st.active[t];
+            f.ref = st.ref[t];
+            f.rhoOld = st.rho[t];
         }
         constexpr int G = kBlock / NCH; // threads per channel
         const int c = t % NCH;
+        const int i0 = t / NCH;
+#pragma unroll
+        for (int k = 0; k < kFusedPre; k++)
+        {
+            const int i = i0 + k * G;
+            f.rho[k] = i < rhoParts ? st.partials[static_cast<size_t>(i) * NCH + c] : 0.0;
+            f.rr[k] = i < updParts ? st.partials2[static_cast<size_t>(i) * NCH + c] : 0.0;
+        }
+    }
+
+    // Second half: finish the previous iteration's rho = r.z (rho kernel
+    // partials) and |r|^2 (update kernel partials, unused in the first
+    // iteration).  Every block sums them in the same order, so all blocks
+    // hold the same convergence decision and the same beta, as
+    // pcgRhoFinalizeKernel would compute them.  Block 0 records rho and the
+    // decision for the update kernel (rhoNext, activeNext, flags[3]) and rr,
+    // the iteration count and non-finite results for the host; no block of
+    // this launch reads those.  Returns whether any channel goes on; beta and
+    // activity per channel in LDS.
+    template <int NCH>
+    __device__ __forceinline__ bool fusedRhoFinish(const SolveState & st, const FusedRhoLoads<NCH> & f, int rhoParts, int updParts,
+                                                   int lb, double * sm, double * shBeta, int * shAct)
+    {
+        const int t = static_cast<int>(threadIdx.x);
+        constexpr int G = kBlock / NCH;
+        const int c = t % NCH;
         double rho = 0, rr = 0;
-#pragma unroll 4
-        for (int i = t / NCH; i < rhoParts; i += G)
+#pragma unroll
+        for (int k = 0; k < kFusedPre; k++)
+        {
+            rho += f.rho[k];
+            rr += f.rr[k];
+        }
+        // beyond the preloaded slots (more than kFusedPre * G partials)
+        for (int i = t / NCH + kFusedPre * G; i < rhoParts; i += G)
         {
             rho += st.partials[static_cast<size_t>(i) * NCH + c];
         }
-#pragma unroll 4
-        for (int i = t / NCH; i < updParts; i += G)
+        for (int i = t / NCH + kFusedPre * G; i < updParts; i += G)
         {
             rr += st.partials2[static_cast<size_t>(i) * NCH + c];
         }
         channelBlockSum2<NCH>(rho, rr, sm);
+        const int first = f.first;
         if (t < NCH)
         {
+            const int was = f.was;
             int act = was;
             int bad = 0;
             if (was && !first)
@@ -1138,12 +1202,12 @@ namespace nsfem
                     act = 0;
                     bad = 1;
                 }
-                else if (rr <= st.tol2 * ref)
+                else if (rr <= st.tol2 * f.ref)
                 {
                     act = 0;
                 }
             }
-            shBeta[t] = (first || rhoOld == 0) ? 0.0 : rho / rhoOld;
+            shBeta[t] = (first || f.rhoOld == 0) ? 0.0 : rho / f.rhoOld;
             shAct[t] = act;
             if (act && !(rho == rho))
             {
@@ -1176,6 +1240,15 @@ namespace nsfem
             st.flags[3] = any ? 0 : 1;
         }
         return any != 0;
+    }
+
+    template <int NCH>
+    __device__ __forceinline__ bool fusedRhoStep(const SolveState & st, int rhoParts, int updParts, int lb, double * sm,
+                                                 double * shBeta, int * shAct)
+    {
+        FusedRhoLoads<NCH> f;
+        fusedRhoLoad<NCH>(st, rhoParts, updParts, f);
+        return fusedRhoFinish<NCH>(st, f, rhoParts, updParts, lb, sm, shBeta, shAct);
     }
 
     template <typename T, int NCH>
