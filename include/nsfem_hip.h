@@ -143,15 +143,15 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *   "innerTol"     pcg_mixed with mixedRestart: inner relative tolerance.
  *   "graphs"       0: launch every iteration from the host instead of
  *                  replaying captured graphs of iteration chunks (default 1).
- *   "fusedReductions" pcg on one system (no batch): 1 or -1 (default) lets
- *                  the direction kernel finish the previous iteration's rho
- *                  and convergence test and the update kernel the step
- *                  length, every workgroup from the producers' partial sums,
- *                  instead of two one-workgroup finalize kernels per
- *                  iteration; 0 keeps the finalize kernels.
+ *   "fusedReductions" pcg: 1 or -1 (default) lets the direction kernel
+ *                  finish the previous iteration's rho and convergence test
+ *                  and the update kernel the step length, every workgroup
+ *                  from its system's partial sums, instead of two
+ *                  one-workgroup finalize kernels per iteration; 0 keeps the
+ *                  finalize kernels.
  *   "rowsDirection", "rowsUpdate", "rowsRho"
- *                  fused reductions: rows per workgroup of those kernels
- *                  (0: the main shape's); autotune sets them. */
+ *                  fused reductions on one system: rows per workgroup of
+ *                  those kernels (0: the main shape's); autotune sets them. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events

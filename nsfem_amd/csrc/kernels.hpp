@@ -409,7 +409,7 @@ namespace nsfem
         double * mse;   // RMS residual (gs mode)
         int * active;   // 1 while the channel iterates
         int * iters;    // iterations done / iteration index at convergence
-        int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen, [3] done next
+        int * flags;    // [0] done, [1] first (p = z), [2] non-finite seen, [3] unused
         double * partials;  // blocks * 2 * NCH
         double * partials2; // blocks * NCH: |r|^2 of the update, read with rho's
         double * partials3; // blocks * NCH: p.q of the direction kernel (fused reductions)
@@ -418,6 +418,38 @@ namespace nsfem
         double tol2;        // relative tolerance squared (pcg) or target RMS (gs)
     };
 
+
+    // Fused reductions: the partial slots of a block's system (one system: all
+    // of them; a batch: the system's contiguous slot range) and whether the
+    // block records the system's decision (its first block).
+    struct FusedSlice
+    {
+        int s;        // system
+        int base;     // first partial slot of the system
+        int rhoParts; // rho kernel partials of the system
+        int updParts; // update kernel partials of the system
+        bool leader;
+    };
+
+    __device__ __forceinline__ FusedSlice fusedSlice(const BlockMap & bm, int4 range, int lb, int rhoParts, int updParts)
+    {
+        FusedSlice f;
+        f.s = range.z;
+        if (bm.flatRows > 0)
+        {
+            f.base = 0;
+            f.rhoParts = rhoParts;
+            f.updParts = updParts;
+            f.leader = lb == 0;
+        }
+        else
+        {
+            f.base = bm.sysBlockStart[f.s];
+            f.rhoParts = f.updParts = bm.sysBlockStart[f.s + 1] - f.base;
+            f.leader = range.w == f.base;
+        }
+        return f;
+    }
 
     // Registers of the fused reductions' loads (fusedRhoLoad / fusedRhoFinish).
     constexpr int kFusedPre = 8;
@@ -889,10 +921,12 @@ namespace nsfem
         // fused: the partial sums load now, with the first slab's rows, and
         // are finished after its gathers (beta is needed only to combine)
         FusedRhoLoads<NCH> fl;
+        FusedSlice sl;
         bool pending = fused != 0;
         if (pending)
         {
-            fusedRhoLoad<NCH>(st, rhoParts, updParts, fl);
+            sl = fusedSlice(bm, range, lb, rhoParts, updParts);
+            fusedRhoLoad<NCH>(st, sl, fl);
         }
         if (pending || systemActive<NCH>(st, s))
         {
@@ -941,7 +975,7 @@ namespace nsfem
                 {
                     // uniform: every thread is in the first slab here
                     pending = false;
-                    if (!fusedRhoFinish<NCH>(st, fl, rhoParts, updParts, lb, sm, shBeta, shAct))
+                    if (!fusedRhoFinish<NCH>(st, fl, sl, sm, shBeta, shAct))
                     {
                         break;
                     }
@@ -1118,20 +1152,11 @@ namespace nsfem
         return channelBlockSum<NCH>(acc, sm);
     }
 
-    // Fused reductions (one system, in the direction kernel's prologue): the
-    // previous iteration's rho = r.z (rho kernel partials) and |r|^2 (update
-    // kernel partials, not in the first iteration) are finished by every
-    // block in the same order, so all blocks hold the same convergence
-    // decision and the same beta, as pcgRhoFinalizeKernel would compute them.
-    // Block 0 records rho and the decision for the update kernel (rhoNext,
-    // activeNext, flags[3]) and rr, the iteration count and non-finite
-    // results for the host; no block of this launch reads those.  Returns
-    // whether any channel goes on; beta and activity per channel in LDS.
-    // Fused reductions, first half (fusedRhoLoad): the flags, the per-channel
-    // state and up to kFusedPre partial sums per thread of each set load into
+    // Fused reductions, first half: the flags, the system's per-channel state
+    // and up to kFusedPre partial sums per thread of each set load into
     // registers, so that they are in flight with the kernel's first row loads.
     template <int NCH>
-    __device__ __forceinline__ void fusedRhoLoad(const SolveState & st, int rhoParts, int updParts, FusedRhoLoads<NCH> & f)
+    __device__ __forceinline__ void fusedRhoLoad(const SolveState & st, const FusedSlice & sl, FusedRhoLoads<NCH> & f)
     {
         const int t = static_cast<int>(threadIdx.x);
         f.first = st.flags[1];
@@ -1140,34 +1165,38 @@ namespace nsfem
         f.rhoOld = 0;
         if (t < NCH)
         {
-            f.was = st.active[t];
-            f.ref = st.ref[t];
-            f.rhoOld = st.rho[t];
+            const int k = sl.s * NCH + t;
+            f.was = st.active[k];
+            f.ref = st.ref[k];
+            f.rhoOld = st.rho[k];
         }
         constexpr int G = kBlock / NCH; // threads per channel
         const int c = t % NCH;
         const int i0 = t / NCH;
+        const double * pr = st.partials + static_cast<size_t>(sl.base) * NCH;
+        const double * pu = st.partials2 + static_cast<size_t>(sl.base) * NCH;
 #pragma unroll
         for (int k = 0; k < kFusedPre; k++)
         {
             const int i = i0 + k * G;
-            f.rho[k] = i < rhoParts ? st.partials[static_cast<size_t>(i) * NCH + c] : 0.0;
-            f.rr[k] = i < updParts ? st.partials2[static_cast<size_t>(i) * NCH + c] : 0.0;
+            f.rho[k] = i < sl.rhoParts ? pr[static_cast<size_t>(i) * NCH + c] : 0.0;
+            f.rr[k] = i < sl.updParts ? pu[static_cast<size_t>(i) * NCH + c] : 0.0;
         }
     }
 
     // Second half: finish the previous iteration's rho = r.z (rho kernel
     // partials) and |r|^2 (update kernel partials, unused in the first
-    // iteration).  Every block sums them in the same order, so all blocks
-    // hold the same convergence decision and the same beta, as
-    // pcgRhoFinalizeKernel would compute them.  Block 0 records rho and the
-    // decision for the update kernel (rhoNext, activeNext, flags[3]) and rr,
-    // the iteration count and non-finite results for the host; no block of
-    // this launch reads those.  Returns whether any channel goes on; beta and
-    // activity per channel in LDS.
+    // iteration) of the block's system.  Every block of the system sums them
+    // in the same order, so all hold the same convergence decision and the
+    // same beta, as pcgRhoFinalizeKernel would compute them.  The system's
+    // first block records rho and the decision for the update kernel
+    // (rhoNext, activeNext) and rr, the iteration count and non-finite
+    // results for the host; no block of this launch reads those.  Returns
+    // whether any channel of the system goes on; beta and activity per
+    // channel in LDS.
     template <int NCH>
-    __device__ __forceinline__ bool fusedRhoFinish(const SolveState & st, const FusedRhoLoads<NCH> & f, int rhoParts, int updParts,
-                                                   int lb, double * sm, double * shBeta, int * shAct)
+    __device__ __forceinline__ bool fusedRhoFinish(const SolveState & st, const FusedRhoLoads<NCH> & f, const FusedSlice & sl,
+                                                   double * sm, double * shBeta, int * shAct)
     {
         const int t = static_cast<int>(threadIdx.x);
         constexpr int G = kBlock / NCH;
@@ -1180,13 +1209,15 @@ namespace nsfem
             rr += f.rr[k];
         }
         // beyond the preloaded slots (more than kFusedPre * G partials)
-        for (int i = t / NCH + kFusedPre * G; i < rhoParts; i += G)
+        const double * pr = st.partials + static_cast<size_t>(sl.base) * NCH;
+        const double * pu = st.partials2 + static_cast<size_t>(sl.base) * NCH;
+        for (int i = t / NCH + kFusedPre * G; i < sl.rhoParts; i += G)
         {
-            rho += st.partials[static_cast<size_t>(i) * NCH + c];
+            rho += pr[static_cast<size_t>(i) * NCH + c];
         }
-        for (int i = t / NCH + kFusedPre * G; i < updParts; i += G)
+        for (int i = t / NCH + kFusedPre * G; i < sl.updParts; i += G)
         {
-            rr += st.partials2[static_cast<size_t>(i) * NCH + c];
+            rr += pu[static_cast<size_t>(i) * NCH + c];
         }
         channelBlockSum2<NCH>(rho, rr, sm);
         const int first = f.first;
@@ -1213,14 +1244,15 @@ namespace nsfem
             {
                 bad = 1;
             }
-            if (lb == 0)
+            if (sl.leader)
             {
-                st.rhoNext[t] = rho;
-                st.activeNext[t] = act;
+                const int k = sl.s * NCH + t;
+                st.rhoNext[k] = rho;
+                st.activeNext[k] = act;
                 if (was && !first)
                 {
-                    st.rr[t] = rr;
-                    st.iters[t] = st.iters[t] + 1;
+                    st.rr[k] = rr;
+                    st.iters[k] = st.iters[k] + 1;
                 }
                 if (bad)
                 {
@@ -1235,20 +1267,16 @@ namespace nsfem
         {
             any |= shAct[k];
         }
-        if (lb == 0 && t == 0)
-        {
-            st.flags[3] = any ? 0 : 1;
-        }
         return any != 0;
     }
 
     template <int NCH>
-    __device__ __forceinline__ bool fusedRhoStep(const SolveState & st, int rhoParts, int updParts, int lb, double * sm,
-                                                 double * shBeta, int * shAct)
+    __device__ __forceinline__ bool fusedRhoStep(const SolveState & st, const FusedSlice & sl, double * sm, double * shBeta,
+                                                 int * shAct)
     {
         FusedRhoLoads<NCH> f;
-        fusedRhoLoad<NCH>(st, rhoParts, updParts, f);
-        return fusedRhoFinish<NCH>(st, f, rhoParts, updParts, lb, sm, shBeta, shAct);
+        fusedRhoLoad<NCH>(st, sl, f);
+        return fusedRhoFinish<NCH>(st, f, sl, sm, shBeta, shAct);
     }
 
     template <typename T, int NCH>
@@ -1448,7 +1476,8 @@ namespace nsfem
             return;
         }
         double part = 0;
-        const bool go = fused ? fusedRhoStep<NCH>(st, rhoParts, updParts, lb, sm, shBeta, shAct) : systemActive<NCH>(st, s);
+        const bool go = fused ? fusedRhoStep<NCH>(st, fusedSlice(bm, range, lb, rhoParts, updParts), sm, shBeta, shAct)
+                              : systemActive<NCH>(st, s);
         if (go)
         {
             const bool first = st.flags[1] != 0;
@@ -1555,12 +1584,12 @@ namespace nsfem
     // x += alpha p;  r -= alpha q;  z = r / d on colour 0 (preconditioner
     // type P; rP = (P) r when P is narrower than T); partial sums of |r|^2.
     // fused = 0: alpha from pcgDirectionFinalizeKernel, activity from the
-    // solve state.  fused = 1 (one system, fused reductions): every block
-    // finishes p.q from the direction kernel's dirParts partials itself (same
-    // order in every block, so all hold the same alpha = rho / p.q) while its
-    // first elements' operands are already loading, with rho and activity as
-    // the direction kernel decided them; block 0 makes that decision the
-    // solve state for the kernels that follow.
+    // solve state.  fused = 1 (fused reductions): every block finishes p.q of
+    // its system from the direction kernel's partials (one system: dirParts of
+    // them) itself (same order in every block of the system, so all hold the
+    // same alpha = rho / p.q) while its first elements' operands are already
+    // loading, with rho and activity as the direction kernel decided them;
+    // block 0 makes those decisions the solve state for the kernels that follow.
     template <typename T, typename P, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgUpdateKernel(int n, BlockMap bm, SolveState st, const T * __restrict__ p,
                                                               const T * __restrict__ q, T * __restrict__ x, T * r, P * rP,
@@ -1576,22 +1605,36 @@ namespace nsfem
         const int s = range.z;
         if (fused)
         {
-            // block 0 makes the direction kernel's decision the solve state
-            // (also when it stopped the solve); no block of this launch reads
-            // what it writes: the others test flags[3], not flags[0], and
-            // read activeNext and rhoNext
+            // block 0 makes the direction kernel's decisions (every system)
+            // the solve state, also when they stop the solve; no block of
+            // this launch reads what it writes: the blocks test their own
+            // system's activeNext and read rhoNext
             const int t = static_cast<int>(threadIdx.x);
-            if (lb == 0 && t < NCH)
+            if (lb == 0)
             {
-                st.active[t] = st.activeNext[t];
-                st.rho[t] = st.rhoNext[t];
+                const int K = bm.numSystems * NCH;
+                int any = 0;
+                for (int k = t; k < K; k += kBlock)
+                {
+                    const int a = st.activeNext[k];
+                    st.active[k] = a;
+                    st.rho[k] = st.rhoNext[k];
+                    any |= a;
+                }
+                any = __syncthreads_or(any);
+                if (t == 0)
+                {
+                    st.flags[0] = any ? 0 : 1;
+                    st.flags[1] = 0; // the direction kernel before has used p = z
+                }
             }
-            if (lb == 0 && t == 0)
+            int sysAny = 0;
+#pragma unroll
+            for (int k = 0; k < NCH; k++)
             {
-                st.flags[0] = st.flags[3];
-                st.flags[1] = 0; // the direction kernel before has used p = z
+                sysAny |= st.activeNext[s * NCH + k];
             }
-            if (st.flags[3])
+            if (!sysAny)
             {
                 return;
             }
@@ -1626,13 +1669,14 @@ namespace nsfem
                     d0 = invDiagP[row0];
                 }
             }
-            const double pq = channelPartialSum<NCH>(st.partials3, dirParts, sm);
+            const FusedSlice sl = fusedSlice(bm, range, lb, dirParts, dirParts);
+            const double pq = channelPartialSum<NCH>(st.partials3 + static_cast<size_t>(sl.base) * NCH, sl.rhoParts, sm);
             const int t = static_cast<int>(threadIdx.x);
             if (t < NCH)
             {
-                const int a = st.activeNext[t];
+                const int a = st.activeNext[s * NCH + t];
                 shAct[t] = a;
-                shAlpha[t] = (a && pq != 0) ? st.rhoNext[t] / pq : 0.0;
+                shAlpha[t] = (a && pq != 0) ? st.rhoNext[s * NCH + t] / pq : 0.0;
             }
             __syncthreads();
             int any = 0;
@@ -1904,7 +1948,7 @@ namespace nsfem
 
     // Fused reductions: the convergence test of a solve's last update (no
     // direction kernel follows to make it): rr, the iteration count and the
-    // stop flag as pcgRhoFinalizeKernel records them (one system).
+    // stop flag as pcgRhoFinalizeKernel records them.
     template <int NCH>
     __global__ void __launch_bounds__(kFinBlock) pcgUpdateCheckKernel(BlockMap bm, SolveState st, double * totals)
     {
@@ -1917,7 +1961,7 @@ namespace nsfem
         if (threadIdx.x == 0)
         {
             int any = 0;
-            for (int k = 0; k < NCH; k++)
+            for (int k = 0; k < bm.numSystems * NCH; k++)
             {
                 if (!st.active[k])
                 {

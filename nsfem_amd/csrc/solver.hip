@@ -594,7 +594,7 @@ namespace nsfem
 
     int McgsSolver::kindRows(int kind) const
     {
-        if (!fused())
+        if (!(fused() && flat()))
         {
             return rowsPerBlock_;
         }
@@ -610,7 +610,7 @@ namespace nsfem
     void McgsSolver::setupKindMaps()
     {
         int maxBlocks = numBlocks_;
-        if (fused())
+        if (fused() && flat())
         {
             auto flatMap = [&](int rows, int & nb)
             {
@@ -704,10 +704,10 @@ namespace nsfem
             }
         }
         reshapeSweeps(sweepLanes, sweepRows);
-        // 3. fused reductions: the direction, update and rho kernels on row
-        // counts of their own (every block of the next kernel re-sums their
-        // partials, so fewer, larger blocks can pay)
-        if (fused())
+        // 3. fused reductions on one system: the direction, update and rho
+        // kernels on row counts of their own (every block of the next kernel
+        // re-sums their partials, so fewer, larger blocks can pay)
+        if (fused() && flat())
         {
             int rows[3] = {0, 0, 0};
             for (int kind = 0; kind < 3; kind++)

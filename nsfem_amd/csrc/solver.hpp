@@ -150,17 +150,16 @@ namespace nsfem
 
         hipStream_t stream() const { return stream_; }
         void setUseGraphs(bool on) { useGraphs_ = on; }
-        // Fused reductions (one system): the direction kernel finishes the
-        // previous rho and convergence test and the update kernel the step
-        // length, each block from the producers' partial sums, instead of
-        // two one-block finalize kernels per iteration.  mode 1 or -1
-        // (default): on for single systems; 0: off.
+        // Fused reductions: the direction kernel finishes the previous rho and
+        // convergence test and the update kernel the step length, each block
+        // from its system's partial sums, instead of two one-block finalize
+        // kernels per iteration.  mode 1 or -1 (default): on; 0: off.
         void setFusedReductions(int mode);
-        bool fused() const { return flat() && fusedMode_ != 0; }
+        bool fused() const { return fusedMode_ != 0; }
         // Rows per workgroup of the direction, update and rho kernels of the
-        // fused mode (0: the main shape's); the sums of their partials are
-        // redone by every block of the next kernel, so fewer, larger blocks
-        // pay there.  reshape resets them.
+        // fused mode on one system (0: the main shape's); the sums of their
+        // partials are redone by every block of the next kernel, so fewer,
+        // larger blocks pay there.  reshape resets them.
         void setKindRows(int direction, int update, int rho);
         int kindRows(int kind) const; // 0 direction, 1 update, 2 rho
         // Fewest rows per workgroup of the default and autotuned shapes.

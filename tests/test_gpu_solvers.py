@@ -385,8 +385,10 @@ def test_fused_reduction_row_counts(octx_h025):
 
 
 def test_fused_reductions_option_on_a_batch(oracle):
-    """Batches keep the finalize kernels (their workgroup maps are per
-    system); the option changes nothing there."""
+    """Fused reductions on a batch: every workgroup sums its own system's
+    partials and each system's first workgroup records its decision; every
+    system converges as with the finalize kernels (the oracle's iterations
+    +-1) to the same solution."""
     from nsfem_amd import cu, fem, mesher
     mats = []
     for k, n in enumerate([600, 2500, 1300]):
@@ -408,7 +410,8 @@ def test_fused_reductions_option_on_a_batch(oracle):
             assert np.linalg.norm(b - m @ xs[fused][o:o + m.shape[0]]) <= 1.0001e-11 * np.linalg.norm(b)
             xo, rel, it = oracle.pcg_solve(m, b, np.zeros_like(b), 1e-11, 3000)
             assert abs(s.getLastIterations(k) - it) <= 1, (fused, k)
-    assert np.array_equal(xs[0], xs[1])
+            assert s.getLastRelRes(0, k) <= 1e-11
+    assert np.linalg.norm(xs[1] - xs[0]) <= 1e-8 * np.linalg.norm(xs[0])
 
 
 @pytest.mark.parametrize("name", ["pcg", "pcg_mixed"])
