@@ -167,13 +167,13 @@ int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 int nsfem_solver_time_kernel(nsfem_solver * s, const char * which, int reps, double * avgMs, double * bytes,
                              double * launchesPerIteration);
 int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
-/* Launch shape of the solver kernels: lanes per matrix row (2, 4, 8, 16; 0
+/* Launch shape of the solver kernels: lanes per matrix row (1, 2, 4, 8, 16; 0
  * picks from the mean row length) and rows per workgroup (0: default).
  * reshape rebuilds the workgroup map only (colouring and matrices kept) and
  * gives the colour sweeps the same shape; reshape_sweeps then sets the
  * sweeps' own (they write no partial sums).  autotune times one PCG
  * iteration replayed from a captured graph (pcg solvers, reps iterations)
- * for lanes {2, 4, 8, 16} x {1, 2, 4, 8} row slabs, keeps the fastest, then
+ * for lanes {1, 2, 4, 8, 16} x {1, 2, 4, 8} row slabs, keeps the fastest, then
  * tries the sweeps on lanes x {1, 2, 4} slabs of their own; it overwrites the
  * internal work vectors, not rhs/sol.  Solvers of 4 to 32 channels run one
  * lane per channel (lanes = numCh) and keep at most about 16k workgroups.
@@ -277,7 +277,7 @@ typedef struct nsfem_solver_config
     int maxIterations;
     double targetMse;
     int mseCheckInterval;
-    int lanes;        /* lanes per matrix row: 2, 4, 8, 16 (0: from the mean row length) */
+    int lanes;        /* lanes per matrix row: 1, 2, 4, 8, 16 (0: from the mean row length) */
     int rowsPerBlock; /* rows per workgroup of the solver kernels (0: default) */
     int sweepLanes;   /* the colour sweeps' own shape (0, 0: as lanes, rowsPerBlock) */
     int sweepRowsPerBlock;

@@ -39,11 +39,12 @@ namespace nsfem
             };
             switch (lanes)
             {
+            case 1: byC.template operator()<1>(); break;
             case 2: byC.template operator()<2>(); break;
             case 4: byC.template operator()<4>(); break;
             case 8: byC.template operator()<8>(); break;
             case 16: byC.template operator()<16>(); break;
-            default: throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
+            default: throw std::invalid_argument("solver lanes must be 1, 2, 4, 8 or 16");
             }
         }
 
@@ -198,9 +199,9 @@ namespace nsfem
             partitions_.push_back(layouts.back().partitionStart);
         }
         meanRow_ = n_ > 0 ? nnzOff / n_ : 0.0;
-        if (opt_.lanes != 0 && opt_.lanes != 2 && opt_.lanes != 4 && opt_.lanes != 8 && opt_.lanes != 16)
+        if (opt_.lanes != 0 && opt_.lanes != 1 && opt_.lanes != 2 && opt_.lanes != 4 && opt_.lanes != 8 && opt_.lanes != 16)
         {
-            throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
+            throw std::invalid_argument("solver lanes must be 1, 2, 4, 8 or 16");
         }
 
         // Colour-major global order: colour c, then system s, then the
@@ -419,9 +420,9 @@ namespace nsfem
 
     std::pair<int, int> McgsSolver::resolveShape(int lanes, int rowsPerBlock, bool sweeps) const
     {
-        if (lanes != 0 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && !(opt_.numCh > 2 && lanes == opt_.numCh))
+        if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && !(opt_.numCh > 2 && lanes == opt_.numCh))
         {
-            throw std::invalid_argument("solver lanes must be 2, 4, 8 or 16");
+            throw std::invalid_argument("solver lanes must be 1, 2, 4, 8 or 16");
         }
         if (rowsPerBlock < 0)
         {
@@ -664,7 +665,7 @@ namespace nsfem
         // (Eager replays when graphs are off, e.g. under a kernel tracer.)
         const char * kind = useGraphs_ ? "iteration_graph" : "iteration";
         auto iterationMs = [&]() { return timeKernel(kind, reps, nullptr, nullptr); };
-        const std::vector<int> laneChoices = opt_.numCh > 2 ? std::vector<int>{opt_.numCh} : std::vector<int>{2, 4, 8, 16};
+        const std::vector<int> laneChoices = opt_.numCh > 2 ? std::vector<int>{opt_.numCh} : std::vector<int>{1, 2, 4, 8, 16};
         // 1. one shape for every kernel
         double best = 0;
         int bestLanes = lanes_, bestRows = rowsPerBlock_;

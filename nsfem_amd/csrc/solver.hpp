@@ -175,7 +175,7 @@ namespace nsfem
         // Sum over one PCG iteration's launches of their timed durations (ms).
         double iterationTime(int reps);
 
-        // Launch shape: lanes per matrix row (2, 4, 8, 16; 0 picks from the mean
+        // Launch shape: lanes per matrix row (1, 2, 4, 8, 16; 0 picks from the mean
         // row length; wide channel tiles always run one lane per channel) and
         // rows per workgroup (0: four row slabs, for wide tiles at least
         // minRowsPerBlock()).  Rebuilds the block map only; the colouring and
@@ -188,7 +188,7 @@ namespace nsfem
         void reshapeSweeps(int lanes, int rowsPerBlock);
         int sweepLanes() const { return sweepLanes_; }
         int sweepRowsPerBlock() const { return sweepRows_; }
-        // Times one graph-replayed PCG iteration for lanes {2, 4, 8, 16} x
+        // Times one graph-replayed PCG iteration for lanes {1, 2, 4, 8, 16} x
         // {1, 2, 4, 8} row slabs per workgroup (wide tiles: their own lanes x
         // {1, 2, 4, 8} times minRowsPerBlock()) and keeps the fastest shape,
         // then the sweeps' own shape over lanes x {1, 2, 4} slabs, then (fused

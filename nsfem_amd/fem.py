@@ -229,7 +229,7 @@ class SolverConfig:
     maxIterations: int = 200
     targetMse: float = 1e-6
     mseCheckInterval: int = 1
-    lanes: int = 0         # lanes per matrix row (2, 4, 8, 16); 0 picks from the mean row length
+    lanes: int = 0         # lanes per matrix row (1, 2, 4, 8, 16); 0 picks from the mean row length
     rowsPerBlock: int = 0  # rows per workgroup of the solver kernels; 0 is the default
     sweepLanes: int = 0    # the colour sweeps' own shape; 0, 0: as lanes, rowsPerBlock
     sweepRowsPerBlock: int = 0

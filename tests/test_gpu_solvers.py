@@ -285,10 +285,10 @@ def test_reshape_and_autotune_keep_results(octx_h025):
     b = rng.uniform(-1, 1, m.shape[0])
     s, x0 = _solve("pcg", m, b, max_iters=3000, target=1e-11)
     it0 = s.getLastIterations()
-    for lanes, rows in ((2, 128), (4, 64), (8, 32), (16, 16), (8, 256), (0, 0)):
+    for lanes, rows in ((1, 256), (2, 128), (4, 64), (8, 32), (16, 16), (8, 256), (0, 0)):
         s.reshape(lanes, rows)
         L, R = s.shape()
-        assert L in (2, 4, 8, 16) and R % (256 // L) == 0 and R >= 256 // L
+        assert L in (1, 2, 4, 8, 16) and R % (256 // L) == 0 and R >= 256 // L
         assert s.sweepShape() == (L, R)
         s.getRhs().upload(b)
         s.getSol().memsetZero()
@@ -311,7 +311,7 @@ def test_reshape_and_autotune_keep_results(octx_h025):
     L, R = s.autotune(reps=2)
     assert (L, R) == s.shape()
     SL, SR = s.sweepShape()
-    assert SL in (2, 4, 8, 16) and SR % (256 // SL) == 0
+    assert SL in (1, 2, 4, 8, 16) and SR % (256 // SL) == 0
     s.getRhs().upload(b)
     s.getSol().memsetZero()
     s.solve()
