@@ -94,6 +94,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
                      "pressure_sweep_rows_per_block": shape[7],
                      "velocity_direction_update_rho_rows": list(shape[8:11]),
                      "pressure_direction_update_rho_rows": list(shape[11:14]),
+                     "velocity_pressure_direction_lanes": list(shape[14:16]),
                      "autotuned": bool(autotune), "autotune_s": tune_s}}
     log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
         f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
@@ -409,7 +410,8 @@ def main():
     ap.add_argument("--shape", default="0,0,0,0",
                     help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default), "
                          "optionally followed by the same four for the colour sweeps and then by the velocity and "
-                         "pressure direction,update,rho rows (14 numbers, as the JSON records them)")
+                         "pressure direction,update,rho rows and the velocity and pressure direction lanes "
+                         "(16 numbers, as the JSON records them)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
     ap.add_argument("--kernels-only", action="store_true",
@@ -461,10 +463,11 @@ def main():
     u0 = initial_velocity(ctx, cond, 1)
     target = 1e-10
     shape = [int(v) for v in args.shape.split(",")]
-    if len(shape) not in (4, 8, 14):
-        raise SystemExit("--shape takes 4, 8 or 14 numbers")
+    if len(shape) not in (4, 8, 14, 16):
+        raise SystemExit("--shape takes 4, 8, 14 or 16 numbers")
     vl, vr, pl, pr, vsl, vsr, psl, psr = (shape + [0] * 4)[:8]
-    kind_rows = shape[8:14] if len(shape) == 14 else None
+    kind_rows = shape[8:14] if len(shape) >= 14 else None
+    dir_lanes = shape[14:16] if len(shape) == 16 else None
 
     def make_chorin(cfg):
         ch = fem.Chorin(ctx, cfg)
@@ -472,6 +475,9 @@ def main():
             for which, rows in (("velocity", kind_rows[0:3]), ("pressure", kind_rows[3:6])):
                 for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho"), rows):
                     ch.set_solver_option(key, v, which)
+        if dir_lanes is not None:
+            for which, v in zip(("velocity", "pressure"), dir_lanes):
+                ch.set_solver_option("lanesDirection", v, which)
         return ch
 
     def solvers(method):

@@ -151,7 +151,9 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  finalize kernels.
  *   "rowsDirection", "rowsUpdate", "rowsRho"
  *                  fused reductions on one system: rows per workgroup of
- *                  those kernels (0: the main shape's); autotune sets them. */
+ *                  those kernels (0: the main shape's); autotune sets them.
+ *   "lanesDirection" lanes per row of the direction kernel alone (0: the
+ *                  main shape's; 1 and 2 channels); autotune sets it. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events
@@ -325,11 +327,12 @@ int nsfem_chorin_download_wait(nsfem_chorin * ch, void * velocity, void * pressu
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);
-/* Launch shapes of the step's solvers, shape[14] = {velocity lanes, velocity
+/* Launch shapes of the step's solvers, shape[16] = {velocity lanes, velocity
  * rows per workgroup, pressure lanes, pressure rows per workgroup, the same
- * four for the colour sweeps, then velocity direction / update / rho rows
- * and pressure direction / update / rho rows}; autotune as
- * nsfem_solver_autotune for both solvers (the state u, p is kept). */
+ * four for the colour sweeps, velocity direction / update / rho rows,
+ * pressure direction / update / rho rows, then the velocity and pressure
+ * direction kernels' lanes}; autotune as nsfem_solver_autotune for both
+ * solvers (the state u, p is kept). */
 int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
 int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
 /* nsfem_solver_set_option for the step's solvers: which = 0 velocity,

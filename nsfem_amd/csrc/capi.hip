@@ -90,6 +90,12 @@ namespace
             if (value != -1 && value != 0 && value != 1) throw std::invalid_argument("fusedReductions must be -1, 0 or 1");
             solver.setFusedReductions(static_cast<int>(value));
         }
+        else if (k == "lanesDirection")
+        {
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+                throw std::invalid_argument("lanesDirection must be 0, 1, 2, 4, 8 or 16");
+            solver.setDirectionLanes(static_cast<int>(value));
+        }
         else if (k == "rowsDirection" || k == "rowsUpdate" || k == "rowsRho")
         {
             if (!(value >= 0 && value <= 1 << 20)) throw std::invalid_argument(k + " must be in [0, 2^20]");
@@ -1202,10 +1208,11 @@ extern "C"
     {
         const McgsSolver & v = ch.core->velocitySolver();
         const McgsSolver & p = ch.core->pressureSolver();
-        const int s[14] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
+        const int s[16] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
                            v.sweepLanes(), v.sweepRowsPerBlock(), p.sweepLanes(), p.sweepRowsPerBlock(),
-                           v.kindRows(0), v.kindRows(1), v.kindRows(2), p.kindRows(0), p.kindRows(1), p.kindRows(2)};
-        std::copy(s, s + 14, shape);
+                           v.kindRows(0), v.kindRows(1), v.kindRows(2), p.kindRows(0), p.kindRows(1), p.kindRows(2),
+                           v.directionLanes(), p.directionLanes()};
+        std::copy(s, s + 16, shape);
     }
 
     int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape)

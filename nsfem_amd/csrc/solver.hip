@@ -545,6 +545,7 @@ namespace nsfem
         // reduction scratch sized by the block count; captured graphs hold
         // the old launch shapes and buffers
         rowsDir_ = rowsUpd_ = rowsRho_ = 0;
+        lanesDir_ = 0;
         setupKindMaps();
         // the sweeps take the same shape until reshapeSweeps
         reshapeSweeps(lanes_, rowsPerBlock_);
@@ -590,6 +591,17 @@ namespace nsfem
         rowsUpd_ = update;
         rowsRho_ = slabs(rho);
         setupKindMaps();
+        destroyGraphs();
+    }
+
+    void McgsSolver::setDirectionLanes(int lanes)
+    {
+        if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+        {
+            throw std::invalid_argument("direction lanes must be 0, 1, 2, 4, 8 or 16");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        lanesDir_ = lanes;
         destroyGraphs();
     }
 
@@ -729,6 +741,27 @@ namespace nsfem
                 rows[kind] = keep;
                 setKindRows(rows[0], rows[1], rows[2]);
             }
+        }
+        // 4. the direction kernel's own lanes per row (its rows are whole
+        // slabs of any lane count up to 16 once they are multiples of 256)
+        if (opt_.numCh <= 2)
+        {
+            int keep = 0;
+            for (int lanes : {1, 2, 4, 8, 16})
+            {
+                if (lanes == lanes_)
+                {
+                    continue;
+                }
+                setDirectionLanes(lanes);
+                const double t = iterationMs();
+                if (t < best)
+                {
+                    best = t;
+                    keep = lanes;
+                }
+            }
+            setDirectionLanes(keep);
         }
         return {bestLanes, bestRows};
     }
@@ -950,7 +983,7 @@ namespace nsfem
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
         const int fz = fused() ? 1 : 0;
-        dispatchLC(lanes_, nch, [&]<int L, int C>()
+        dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                    {
                        pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, nbRho_, nbUpd_);
                        if (!fz)
@@ -1284,7 +1317,7 @@ namespace nsfem
                 bytes = nnzOff * (tsT + 4.0) + (n + 1) * 4.0 + n * tsT + nch * (n * tsP + 4.0 * n * tsT);
                 launch = [=, this]()
                 {
-                    dispatchLC(lanes_, nch, [&]<int L, int C>()
+                    dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                                {
                                    pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, nbRho_,
                                                                                             nbUpd_);

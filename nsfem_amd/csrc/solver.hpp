@@ -162,6 +162,10 @@ namespace nsfem
         // larger blocks pay there.  reshape resets them.
         void setKindRows(int direction, int update, int rho);
         int kindRows(int kind) const; // 0 direction, 1 update, 2 rho
+        // Lanes per row of the direction kernel alone (0: the main shape's;
+        // 1 and 2 channels).  reshape resets it.
+        void setDirectionLanes(int lanes);
+        int directionLanes() const { return lanesDir_ > 0 && opt_.numCh <= 2 ? lanesDir_ : lanes_; }
         // Fewest rows per workgroup of the default and autotuned shapes.
         int minRowsPerBlock() const;
 
@@ -258,7 +262,7 @@ namespace nsfem
         int numBlocks_ = 0;
         int fusedMode_ = -1;
         // per-kernel-kind maps: all bm_ unless fused (flat, own row counts)
-        int rowsDir_ = 0, rowsUpd_ = 0, rowsRho_ = 0;
+        int rowsDir_ = 0, rowsUpd_ = 0, rowsRho_ = 0, lanesDir_ = 0;
         BlockMap bmDir_{}, bmUpd_{}, bmRho_{};
         int nbDir_ = 0, nbUpd_ = 0, nbRho_ = 0, color0BlocksRho_ = 0;
         void setupKindMaps();

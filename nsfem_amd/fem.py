@@ -313,16 +313,17 @@ class Chorin:
     def shape(self):
         """(velocity lanes, velocity rows per workgroup, pressure lanes,
         pressure rows per workgroup, the same four for the colour sweeps, then
-        velocity direction / update / rho rows and pressure direction /
-        update / rho rows) of the solver kernels."""
-        out = (ctypes.c_int * 14)()
+        velocity direction / update / rho rows, pressure direction / update /
+        rho rows, then the velocity and pressure direction kernels' lanes) of
+        the solver kernels."""
+        out = (ctypes.c_int * 16)()
         call("nsfem_chorin_shape", self._h, out)
         return tuple(out)
 
     def autotune(self, reps=10):
         """Autotunes both solvers' launch shapes (the state is kept); returns
         shape()."""
-        out = (ctypes.c_int * 14)()
+        out = (ctypes.c_int * 16)()
         call("nsfem_chorin_autotune", self._h, int(reps), out)
         return tuple(out)
 

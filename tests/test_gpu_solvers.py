@@ -519,3 +519,23 @@ def test_batch_of_uneven_systems_with_wide_tiles():
                 bc = b[c * n:(c + 1) * n]
                 assert np.linalg.norm(bc - m @ xc) <= 1.0001e-10 * np.linalg.norm(bc), (name, k, c)
                 assert np.linalg.norm(xc - x1[c * n:(c + 1) * n]) <= 1e-8 * np.linalg.norm(xc), (name, k, c)
+
+
+@pytest.mark.parametrize("nch", [1, 2])
+def test_direction_kernel_lanes(octx_h025, nch):
+    """The direction kernel on its own lanes per row: same iterations (+-1)
+    and solutions as the main shape's."""
+    m = octx_h025["Kp_int"]
+    n = m.shape[0]
+    rng = np.random.default_rng(43)
+    b = rng.uniform(-1, 1, nch * n)
+    s, x0 = _solve("pcg", m, b, nch=nch, max_iters=3000, target=1e-10)
+    it0 = s.getLastIterations()
+    for lanes in (1, 2, 4, 8, 16, 0):
+        s.setOption("lanesDirection", lanes)
+        s.getRhs().upload(b)
+        s.getSol().memsetZero()
+        s.solve()
+        x = s.getSol().download()
+        assert abs(s.getLastIterations() - it0) <= 1, lanes
+        assert np.linalg.norm(x - x0) <= 1e-9 * np.linalg.norm(x0), lanes
