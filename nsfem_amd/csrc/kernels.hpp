@@ -1603,13 +1603,55 @@ namespace nsfem
         // the block's rows, its system and the done flag load together
         const int4 range = blockRange(bm, lb);
         const int s = range.z;
+        // flat over the block's interleaved elements, one per thread: the
+        // stride (kBlock) is a multiple of NCH, so a thread keeps one channel
+        static_assert(kBlock % NCH == 0, "channels must divide the block");
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        const long i0 = static_cast<long>(range.x) * NCH + threadIdx.x;
+        const long end = static_cast<long>(range.y) * NCH;
+        const int row0 = static_cast<int>(i0 / NCH);
+        bool go;
+        int act;
+        T alpha;
+        // the first element's operands go out before alpha is known
+        T r0 = T(0), x0 = T(0), p0 = T(0), q0 = T(0);
+        P d0 = P(0);
         if (fused)
         {
+            // every load goes out first: the system's next activity and rho,
+            // the first element's operands and the direction kernel's partials
+            // of the system (the stop test comes after)
+            const int t = static_cast<int>(threadIdx.x);
+            int sysAny = 0;
+#pragma unroll
+            for (int k = 0; k < NCH; k++)
+            {
+                sysAny |= st.activeNext[s * NCH + k];
+            }
+            int aNext = 0;
+            double rhoNext = 0;
+            if (t < NCH)
+            {
+                aNext = st.activeNext[s * NCH + t];
+                rhoNext = st.rhoNext[s * NCH + t];
+            }
+            if (i0 < end)
+            {
+                r0 = r[i0];
+                x0 = x[i0];
+                p0 = p[i0];
+                q0 = q[i0];
+                if (row0 < color0End)
+                {
+                    d0 = invDiagP[row0];
+                }
+            }
+            const FusedSlice sl = fusedSlice(bm, range, lb, dirParts, dirParts);
+            const double pq = channelPartialSum<NCH>(st.partials3 + static_cast<size_t>(sl.base) * NCH, sl.rhoParts, sm);
             // block 0 makes the direction kernel's decisions (every system)
             // the solve state, also when they stop the solve; no block of
             // this launch reads what it writes: the blocks test their own
             // system's activeNext and read rhoNext
-            const int t = static_cast<int>(threadIdx.x);
             if (lb == 0)
             {
                 const int K = bm.numSystems * NCH;
@@ -1628,69 +1670,26 @@ namespace nsfem
                     st.flags[1] = 0; // the direction kernel before has used p = z
                 }
             }
-            int sysAny = 0;
-#pragma unroll
-            for (int k = 0; k < NCH; k++)
-            {
-                sysAny |= st.activeNext[s * NCH + k];
-            }
             if (!sysAny)
             {
                 return;
             }
-        }
-        else if (st.flags[0])
-        {
-            return;
-        }
-        // flat over the block's interleaved elements, one per thread: the
-        // stride (kBlock) is a multiple of NCH, so a thread keeps one channel
-        static_assert(kBlock % NCH == 0, "channels must divide the block");
-        const int c = static_cast<int>(threadIdx.x) % NCH;
-        const long i0 = static_cast<long>(range.x) * NCH + threadIdx.x;
-        const long end = static_cast<long>(range.y) * NCH;
-        const int row0 = static_cast<int>(i0 / NCH);
-        bool go;
-        int act;
-        T alpha;
-        // the first element's operands go out before alpha is known
-        T r0 = T(0), x0 = T(0), p0 = T(0), q0 = T(0);
-        P d0 = P(0);
-        if (fused)
-        {
-            if (i0 < end)
-            {
-                r0 = r[i0];
-                x0 = x[i0];
-                p0 = p[i0];
-                q0 = q[i0];
-                if (row0 < color0End)
-                {
-                    d0 = invDiagP[row0];
-                }
-            }
-            const FusedSlice sl = fusedSlice(bm, range, lb, dirParts, dirParts);
-            const double pq = channelPartialSum<NCH>(st.partials3 + static_cast<size_t>(sl.base) * NCH, sl.rhoParts, sm);
-            const int t = static_cast<int>(threadIdx.x);
             if (t < NCH)
             {
-                const int a = st.activeNext[s * NCH + t];
-                shAct[t] = a;
-                shAlpha[t] = (a && pq != 0) ? st.rhoNext[s * NCH + t] / pq : 0.0;
+                shAct[t] = aNext;
+                shAlpha[t] = (aNext && pq != 0) ? rhoNext / pq : 0.0;
             }
             __syncthreads();
-            int any = 0;
-#pragma unroll
-            for (int k = 0; k < NCH; k++)
-            {
-                any |= shAct[k];
-            }
-            go = any != 0;
+            go = true;
             act = shAct[c];
             alpha = static_cast<T>(shAlpha[c]);
         }
         else
         {
+            if (st.flags[0])
+            {
+                return;
+            }
             if (lb == 0 && threadIdx.x == 0)
             {
                 st.flags[1] = 0; // the direction kernel before has used p = z
