@@ -812,6 +812,8 @@ namespace nsfem
         bool c0 = valid && row < color0End;
         int j0 = c0 ? M.rowStart[row] : 0;
         int j1 = c0 ? M.rowStart[row + 1] : 0;
+        // the system's activity loads with the flag and the row bounds
+        const bool act = systemActive<NCH>(st, sys);
         if (done || j1 < j0)
         {
             return;
@@ -822,7 +824,7 @@ namespace nsfem
         {
             part[c] = 0;
         }
-        if (systemActive<NCH>(st, sys))
+        if (act)
         {
             for (int base = range.x; base < range.y; base += kBlock / LANES)
             {
@@ -1413,12 +1415,13 @@ namespace nsfem
         bool c0 = valid && row < color0End;
         int j0 = c0 ? M.rowStart[row] : 0;
         int j1 = c0 ? M.rowStart[row + 1] : 0;
+        const bool act = systemActive<NCH>(st, sys);
         if (done || j1 < j0)
         {
             return;
         }
         double part = 0;
-        if (systemActive<NCH>(st, sys))
+        if (act)
         {
             for (int base = range.x; base < range.y; base += kBlock / NCH)
             {
