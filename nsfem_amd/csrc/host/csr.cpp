@@ -314,4 +314,70 @@ namespace nsfem
         L.stripped.rowStart[n] = static_cast<int>(L.stripped.column.size());
         return L;
     }
+    std::vector<int> localityRank(const std::vector<int> & rowStart, const std::vector<int> & column, int n)
+    {
+        std::vector<int> degree(n);
+        for (int i = 0; i < n; i++)
+        {
+            degree[i] = rowStart[i + 1] - rowStart[i];
+        }
+        std::vector<int> rank(n, -1), queue;
+        queue.reserve(n);
+        std::vector<int> mark(n, -1);
+        std::vector<int> nbrs;
+        // breadth-first from `start`, marking with `tag`; appends to `out`
+        auto bfs = [&](int start, int tag, std::vector<int> & out)
+        {
+            size_t head = out.size();
+            out.push_back(start);
+            mark[start] = tag;
+            while (head < out.size())
+            {
+                const int u = out[head++];
+                nbrs.clear();
+                for (int j = rowStart[u]; j < rowStart[u + 1]; j++)
+                {
+                    const int v = column[j];
+                    if (v != u && mark[v] != tag)
+                    {
+                        mark[v] = tag;
+                        nbrs.push_back(v);
+                    }
+                }
+                std::sort(nbrs.begin(), nbrs.end(), [&](int a, int b) { return degree[a] != degree[b] ? degree[a] < degree[b] : a < b; });
+                out.insert(out.end(), nbrs.begin(), nbrs.end());
+            }
+        };
+        int next = 0;
+        std::vector<int> probe;
+        for (int seed = 0; seed < n; seed++)
+        {
+            if (rank[seed] >= 0)
+            {
+                continue;
+            }
+            // a far node of this component: the last one reached from the
+            // component's minimum-degree node
+            probe.clear();
+            bfs(seed, -2 - seed, probe);
+            int start = seed;
+            for (int v : probe)
+            {
+                if (degree[v] < degree[start])
+                {
+                    start = v;
+                }
+            }
+            probe.clear();
+            bfs(start, -2 - n - seed, probe);
+            start = probe.back();
+            const size_t before = queue.size();
+            bfs(start, n + seed, queue);
+            for (size_t k = before; k < queue.size(); k++)
+            {
+                rank[queue[k]] = next++;
+            }
+        }
+        return rank;
+    }
 } // namespace nsfem

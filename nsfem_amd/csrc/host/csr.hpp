@@ -62,4 +62,11 @@ namespace nsfem
     };
 
     McgsLayout buildMcgsLayout(const CsrD & m);
+
+    // Locality rank of every row of a square pattern: its position in a
+    // breadth-first (Cuthill-McKee) order of the row graph, each component
+    // started from a far node (the last one reached from a minimum-degree
+    // node) and neighbours taken by increasing degree.  Rows that are close
+    // in the graph get close ranks.
+    std::vector<int> localityRank(const std::vector<int> & rowStart, const std::vector<int> & column, int n);
 } // namespace nsfem

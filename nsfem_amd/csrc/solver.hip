@@ -204,15 +204,29 @@ namespace nsfem
             throw std::invalid_argument("solver lanes must be 1, 2, 4, 8 or 16");
         }
 
-        // Colour-major global order: colour c, then system s, then the
-        // system's own colour order.  Each (colour, system) range of rows is
-        // one chunk; workgroups never straddle chunks.
+        // Colour-major global order: colour c, then system s, then inside the
+        // (colour, system) chunk the rows by locality rank (rows of one colour
+        // are independent, so their order changes no result beyond rounding;
+        // neighbouring rows then gather neighbouring entries of the other
+        // colours).  Each chunk is one range of rows; workgroups never
+        // straddle chunks.  The exported colouring keeps the reference's order.
         std::vector<int> perm(n_);
         std::vector<std::vector<int>> gpos(numSystems_);
+        std::vector<std::vector<int>> key(numSystems_);
         for (int s = 0; s < numSystems_; s++)
         {
             gpos[s].resize(layouts[s].n);
+            if (opt_.localityOrder)
+            {
+                const std::vector<int> rank = localityRank(systems[s]->rowStart, systems[s]->column, layouts[s].n);
+                key[s].resize(layouts[s].n);
+                for (int l = 0; l < layouts[s].n; l++)
+                {
+                    key[s][l] = rank[layouts[s].coloring[l]];
+                }
+            }
         }
+        std::vector<int> order;
         chunks_.clear();
         int g = 0;
         for (int c = 0; c < numColors_; c++)
@@ -226,7 +240,17 @@ namespace nsfem
                 }
                 const int l0 = L.partitionStart[c], l1 = L.partitionStart[c + 1];
                 const int g0 = g;
+                order.resize(l1 - l0);
                 for (int l = l0; l < l1; l++)
+                {
+                    order[l - l0] = l;
+                }
+                if (opt_.localityOrder)
+                {
+                    const std::vector<int> & k = key[s];
+                    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return k[a] < k[b]; });
+                }
+                for (int l : order)
                 {
                     gpos[s][l] = g;
                     perm[g] = sysOffset_[s] + L.coloring[l];
@@ -252,6 +276,7 @@ namespace nsfem
         values.reserve(static_cast<size_t>(nnzOff));
         {
             std::vector<std::pair<int, int>> owner(n_); // global row -> (system, local permuted row)
+            std::vector<std::pair<int, double>> e;     // row-part sort buffer
             for (int s = 0; s < numSystems_; s++)
             {
                 for (int l = 0; l < layouts[s].n; l++)
@@ -272,6 +297,27 @@ namespace nsfem
                 rowSplit[r] = rowStart[r] + (L.rowSplit[l] - L.stripped.rowStart[l]);
                 diag[r] = L.diag[l];
                 invDiag[r] = L.invDiag[l];
+                if (opt_.localityOrder)
+                {
+                    // ascending columns inside the lower-colour and the
+                    // upper part of the row (the split stays where it is)
+                    auto sortPart = [&](int j0, int j1)
+                    {
+                        e.clear();
+                        for (int j = j0; j < j1; j++)
+                        {
+                            e.emplace_back(column[j], values[j]);
+                        }
+                        std::sort(e.begin(), e.end(), [](const auto & a, const auto & b) { return a.first < b.first; });
+                        for (int j = j0; j < j1; j++)
+                        {
+                            column[j] = e[j - j0].first;
+                            values[j] = e[j - j0].second;
+                        }
+                    };
+                    sortPart(rowStart[r], rowSplit[r]);
+                    sortPart(rowSplit[r], static_cast<int>(column.size()));
+                }
             }
             rowStart[n_] = static_cast<int>(column.size());
         }

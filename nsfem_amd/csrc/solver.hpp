@@ -53,6 +53,7 @@ namespace nsfem
         int rowsPerBlock = 0;    // 0: 4 row slabs per block
         int sweepLanes = 0;      // colour sweeps' own shape (0: as above)
         int sweepRowsPerBlock = 0;
+        bool localityOrder = true; // rows inside each colour by locality rank
         int checkChunk = 16;     // iterations between host convergence polls
         // Mixed precision.  Default: FP64 Krylov vectors, matrix products and
         // reductions, the symmetric GS preconditioner in FP32 (its sweeps on
