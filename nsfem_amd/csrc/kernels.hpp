@@ -574,30 +574,43 @@ namespace nsfem
         {
             acc[c] = T(0);
         }
-        // two strided entries per step: their loads, then their gathers, go
-        // out together (the terms are added in the same order as one by one)
-        for (int j = j0 + lane; j < j1; j += 2 * LANES)
+        // U strided entries per step: their loads, then their gathers, go out
+        // together (the terms are added in the same order as one by one).
+        // Four for short rows on 1-2 lanes, two where a lane sees few entries.
+        constexpr int U = LANES <= 2 ? 4 : 2;
+        for (int j = j0 + lane; j < j1; j += U * LANES)
         {
-            const int jb = j + LANES;
-            const bool hasB = jb < j1;
-            const T a0 = A.values[j];
-            const int c0 = A.column[j];
-            const T a1 = hasB ? A.values[jb] : T(0);
-            const int c1 = hasB ? A.column[jb] : c0;
-            T x0[NCH], x1[NCH];
+            T a[U];
+            int col[U];
+            bool has[U];
 #pragma unroll
-            for (int c = 0; c < NCH; c++)
+            for (int u = 0; u < U; u++)
             {
-                x0[c] = static_cast<T>(v[static_cast<long>(c0) * NCH + c]);
-                x1[c] = static_cast<T>(v[static_cast<long>(c1) * NCH + c]);
+                const int ju = j + u * LANES;
+                has[u] = ju < j1;
+                a[u] = has[u] ? A.values[ju] : T(0);
+                col[u] = has[u] ? A.column[ju] : 0;
+            }
+            T xv[U][NCH];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    xv[u][c] = has[u] ? static_cast<T>(v[static_cast<long>(col[u]) * NCH + c]) : T(0);
+                }
             }
 #pragma unroll
-            for (int c = 0; c < NCH; c++)
+            for (int u = 0; u < U; u++)
             {
-                acc[c] = fmaT(a0, x0[c], acc[c]);
-                if (hasB)
+                if (has[u])
                 {
-                    acc[c] = fmaT(a1, x1[c], acc[c]);
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        acc[c] = fmaT(a[u], xv[u][c], acc[c]);
+                    }
                 }
             }
         }
