@@ -95,6 +95,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
                      "velocity_direction_update_rho_rows": list(shape[8:11]),
                      "pressure_direction_update_rho_rows": list(shape[11:14]),
                      "velocity_pressure_direction_lanes": list(shape[14:16]),
+                     "velocity_pressure_fused_reductions": list(shape[16:18]),
                      "autotuned": bool(autotune), "autotune_s": tune_s}}
     log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
         f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
@@ -415,8 +416,8 @@ def main():
     ap.add_argument("--shape", default="0,0,0,0",
                     help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default), "
                          "optionally followed by the same four for the colour sweeps and then by the velocity and "
-                         "pressure direction,update,rho rows and the velocity and pressure direction lanes "
-                         "(16 numbers, as the JSON records them)")
+                         "pressure direction,update,rho rows, the velocity and pressure direction lanes and "
+                         "whether each fuses its reductions (18 numbers, as the JSON records them)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
     ap.add_argument("--kernels-only", action="store_true",
@@ -468,14 +469,18 @@ def main():
     u0 = initial_velocity(ctx, cond, 1)
     target = 1e-10
     shape = [int(v) for v in args.shape.split(",")]
-    if len(shape) not in (4, 8, 14, 16):
-        raise SystemExit("--shape takes 4, 8, 14 or 16 numbers")
+    if len(shape) not in (4, 8, 14, 16, 18):
+        raise SystemExit("--shape takes 4, 8, 14, 16 or 18 numbers")
     vl, vr, pl, pr, vsl, vsr, psl, psr = (shape + [0] * 4)[:8]
     kind_rows = shape[8:14] if len(shape) >= 14 else None
-    dir_lanes = shape[14:16] if len(shape) == 16 else None
+    dir_lanes = shape[14:16] if len(shape) >= 16 else None
+    fused_modes = shape[16:18] if len(shape) == 18 else None
 
     def make_chorin(cfg):
         ch = fem.Chorin(ctx, cfg)
+        if fused_modes is not None:
+            for which, v in zip(("velocity", "pressure"), fused_modes):
+                ch.set_solver_option("fusedReductions", v, which)
         if kind_rows is not None:
             for which, rows in (("velocity", kind_rows[0:3]), ("pressure", kind_rows[3:6])):
                 for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho"), rows):

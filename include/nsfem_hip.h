@@ -176,8 +176,11 @@ int nsfem_solver_num_colors(const nsfem_solver * s, int system, int * out);
  * sweeps' own (they write no partial sums).  autotune times one PCG
  * iteration replayed from a captured graph (pcg solvers, reps iterations)
  * for lanes {1, 2, 4, 8, 16} x {1, 2, 4, 8} row slabs, keeps the fastest, then
- * tries the sweeps on lanes x {1, 2, 4} slabs of their own; it overwrites the
- * internal work vectors, not rhs/sol.  Solvers of 4 to 32 channels run one
+ * tries the sweeps on lanes x {1, 2, 4} slabs of their own, then (fused
+ * reductions) the direction / update / rho rows and the direction lanes, and
+ * last, unless fusedReductions was set, the finalize kernels against the
+ * fused reductions; it overwrites the internal work vectors, not rhs/sol.
+ * Solvers of 4 to 32 channels run one
  * lane per channel (lanes = numCh) and keep at most about 16k workgroups.
  * No reference counterpart (the reference's launch shapes are fixed). */
 int nsfem_solver_reshape(nsfem_solver * s, int lanes, int rowsPerBlock);
@@ -327,12 +330,13 @@ int nsfem_chorin_download_wait(nsfem_chorin * ch, void * velocity, void * pressu
 int nsfem_chorin_upload_velocity(nsfem_chorin * ch, const void * velocity);
 /* u = 0 + Dirichlet values, p = 0, pressure warm start cleared. */
 int nsfem_chorin_reset(nsfem_chorin * ch);
-/* Launch shapes of the step's solvers, shape[16] = {velocity lanes, velocity
+/* Launch shapes of the step's solvers, shape[18] = {velocity lanes, velocity
  * rows per workgroup, pressure lanes, pressure rows per workgroup, the same
  * four for the colour sweeps, velocity direction / update / rho rows,
- * pressure direction / update / rho rows, then the velocity and pressure
- * direction kernels' lanes}; autotune as nsfem_solver_autotune for both
- * solvers (the state u, p is kept). */
+ * pressure direction / update / rho rows, the velocity and pressure
+ * direction kernels' lanes, then whether each solver fuses its reductions};
+ * autotune as nsfem_solver_autotune for both solvers (the state u, p is
+ * kept). */
 int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
 int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
 /* nsfem_solver_set_option for the step's solvers: which = 0 velocity,

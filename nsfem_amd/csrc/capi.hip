@@ -1208,11 +1208,11 @@ extern "C"
     {
         const McgsSolver & v = ch.core->velocitySolver();
         const McgsSolver & p = ch.core->pressureSolver();
-        const int s[16] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
+        const int s[18] = {v.lanes(), v.rowsPerBlock(), p.lanes(), p.rowsPerBlock(),
                            v.sweepLanes(), v.sweepRowsPerBlock(), p.sweepLanes(), p.sweepRowsPerBlock(),
                            v.kindRows(0), v.kindRows(1), v.kindRows(2), p.kindRows(0), p.kindRows(1), p.kindRows(2),
-                           v.directionLanes(), p.directionLanes()};
-        std::copy(s, s + 16, shape);
+                           v.directionLanes(), p.directionLanes(), v.fused() ? 1 : 0, p.fused() ? 1 : 0};
+        std::copy(s, s + 18, shape);
     }
 
     int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape)

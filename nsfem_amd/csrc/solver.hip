@@ -809,6 +809,39 @@ namespace nsfem
             }
             setDirectionLanes(keep);
         }
+        // 5. fused reductions against the finalize kernels (on a large system
+        // the fused mode needs large direction workgroups to keep its partial
+        // sums few; the finalize kernels let every kernel keep the main shape)
+        if (fused() && fusedMode_ == -1)
+        {
+            const int keepDir = lanesDir_;
+            const int keepRows[3] = {rowsDir_, rowsUpd_, rowsRho_};
+            fusedMode_ = 0;
+            setupKindMaps();
+            destroyGraphs();
+            double unfused = 0;
+            int unfusedLanes = 0;
+            for (int lanes : {0, 1, 2, 4, 8, 16})
+            {
+                setDirectionLanes(lanes);
+                const double t = iterationMs();
+                if (unfused == 0 || t < unfused)
+                {
+                    unfused = t;
+                    unfusedLanes = lanes;
+                }
+            }
+            if (unfused < best)
+            {
+                setDirectionLanes(unfusedLanes);
+            }
+            else
+            {
+                fusedMode_ = -1;
+                lanesDir_ = keepDir;
+                setKindRows(keepRows[0], keepRows[1], keepRows[2]);
+            }
+        }
         return {bestLanes, bestRows};
     }
 

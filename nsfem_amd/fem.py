@@ -314,16 +314,16 @@ class Chorin:
         """(velocity lanes, velocity rows per workgroup, pressure lanes,
         pressure rows per workgroup, the same four for the colour sweeps, then
         velocity direction / update / rho rows, pressure direction / update /
-        rho rows, then the velocity and pressure direction kernels' lanes) of
-        the solver kernels."""
-        out = (ctypes.c_int * 16)()
+        rho rows, the velocity and pressure direction kernels' lanes, then
+        whether each solver fuses its reductions) of the solver kernels."""
+        out = (ctypes.c_int * 18)()
         call("nsfem_chorin_shape", self._h, out)
         return tuple(out)
 
     def autotune(self, reps=10):
         """Autotunes both solvers' launch shapes (the state is kept); returns
         shape()."""
-        out = (ctypes.c_int * 16)()
+        out = (ctypes.c_int * 18)()
         call("nsfem_chorin_autotune", self._h, int(reps), out)
         return tuple(out)
 
