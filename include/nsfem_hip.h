@@ -153,7 +153,12 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  fused reductions on one system: rows per workgroup of
  *                  those kernels (0: the main shape's); autotune sets them.
  *   "lanesDirection" lanes per row of the direction kernel alone (0: the
- *                  main shape's; 1 and 2 channels); autotune sets it. */
+ *                  main shape's; 1 and 2 channels); autotune sets it.
+ *   "regionDirection" fused reductions on one system: 1 (default) orders
+ *                  the direction kernel's workgroups region-major, so the
+ *                  workgroups of one XCD take the same stretch (one eighth
+ *                  of the rows' locality order) of every colour and gather
+ *                  from one eighth of the vector; 0: rows in colour order. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events

@@ -368,6 +368,10 @@ namespace nsfem
         // is b; the kernels compute the range instead of loading it.
         int flatRows;
         int rowsPerBlock;
+        // flat maps only, optional: block b is rows rowTable[b] instead
+        // (same slot b); the region-major direction map, where the blocks
+        // running on one XCD take the same stretch of every colour
+        const int2 * rowTable;
     };
 
     // The rows, system and partial slot of block lb.
@@ -375,6 +379,11 @@ namespace nsfem
     {
         if (bm.flatRows > 0)
         {
+            if (bm.rowTable)
+            {
+                const int2 t = bm.rowTable[lb];
+                return make_int4(t.x, t.y, 0, lb);
+            }
             const int r0 = lb * bm.rowsPerBlock;
             return make_int4(r0, min(bm.flatRows, r0 + bm.rowsPerBlock), 0, lb);
         }

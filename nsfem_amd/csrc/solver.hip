@@ -250,6 +250,25 @@ namespace nsfem
                     const std::vector<int> & k = key[s];
                     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return k[a] < k[b]; });
                 }
+                if (numSystems_ == 1)
+                {
+                    // eight stretches of the colour: by locality rank
+                    // (rank < j n / 8), else by count
+                    const long m = static_cast<long>(order.size());
+                    for (int j = 0; j < 8; j++)
+                    {
+                        long at = m * j / 8;
+                        if (opt_.localityOrder)
+                        {
+                            const std::vector<int> & k = key[s];
+                            const long bound = static_cast<long>(L.n) * j / 8;
+                            at = std::partition_point(order.begin(), order.end(), [&](int a) { return k[a] < bound; }) -
+                                 order.begin();
+                        }
+                        colorRegionStart_.push_back(g0 + static_cast<int>(at));
+                    }
+                    colorRegionStart_.push_back(g0 + static_cast<int>(m));
+                }
                 for (int l : order)
                 {
                     gpos[s][l] = g;
@@ -651,6 +670,18 @@ namespace nsfem
         destroyGraphs();
     }
 
+    void McgsSolver::setRegionedDirection(int on)
+    {
+        if (on != 0 && on != 1)
+        {
+            throw std::invalid_argument("regionDirection must be 0 or 1");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        regionDir_ = on;
+        setupKindMaps();
+        destroyGraphs();
+    }
+
     int McgsSolver::kindRows(int kind) const
     {
         if (!(fused() && flat()))
@@ -683,6 +714,30 @@ namespace nsfem
             bmDir_ = flatMap(rowsDir_, nbDir_);
             bmUpd_ = flatMap(rowsUpd_, nbUpd_);
             bmRho_ = flatMap(rowsRho_, nbRho_);
+            if (regionDir_ && n_ > 0)
+            {
+                // stretch k of every colour, then stretch k + 1: the XCD
+                // that runs a contiguous range of blocks (logicalBlock)
+                // gathers from one stretch of the locality order
+                const int rpb = bmDir_.rowsPerBlock;
+                std::vector<int2> t;
+                for (int k = 0; k < 8; k++)
+                {
+                    for (int c = 0; c < numColors_; c++)
+                    {
+                        const int r0 = colorRegionStart_[9 * c + k], r1 = colorRegionStart_[9 * c + k + 1];
+                        for (int b0 = r0; b0 < r1; b0 += rpb)
+                        {
+                            t.push_back(int2{b0, std::min(r1, b0 + rpb)});
+                        }
+                    }
+                }
+                dirTable_.upload(t);
+                nbDir_ = static_cast<int>(t.size());
+                bmDir_.numBlocks = nbDir_;
+                bmDir_.rowTable = dirTable_.get();
+                maxBlocks = std::max(maxBlocks, nbDir_);
+            }
             color0BlocksRho_ = (color0End_ + bmRho_.rowsPerBlock - 1) / bmRho_.rowsPerBlock;
         }
         else

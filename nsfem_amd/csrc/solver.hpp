@@ -167,6 +167,13 @@ namespace nsfem
         // 1 and 2 channels).  reshape resets it.
         void setDirectionLanes(int lanes);
         int directionLanes() const { return lanesDir_ > 0 && opt_.numCh <= 2 ? lanesDir_ : lanes_; }
+        // Region-major direction map (fused reductions, one system): the
+        // rows of every colour are cut into eight stretches of the locality
+        // order, and the workgroups one XCD runs take stretch k of every
+        // colour, so that XCD gathers from one eighth of the vector.
+        // 1 on (default), 0 off; reshape keeps it.
+        void setRegionedDirection(int on);
+        int regionedDirection() const { return regionDir_; }
         // Fewest rows per workgroup of the default and autotuned shapes.
         int minRowsPerBlock() const;
 
@@ -252,6 +259,11 @@ namespace nsfem
         double meanRow_ = 0;               // off-diagonal entries per row
         int rowsPerBlock_ = 0;
         std::vector<int> colorRowStart_; // numColors + 1 (global colour-major order)
+        // one system: colour c's rows split into 8 stretches of the locality
+        // order, [colorRegionStart_[9c + k], colorRegionStart_[9c + k + 1])
+        std::vector<int> colorRegionStart_;
+        int regionDir_ = 1;
+        DeviceArray<int2> dirTable_;
         int color0Blocks_ = 0;           // main-map blocks holding colour-0 rows (leading)
         // one system: flat workgroup maps (ranges computed in the kernels)
         bool flat() const { return numSystems_ == 1; }

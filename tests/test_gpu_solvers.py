@@ -549,3 +549,32 @@ def test_direction_kernel_lanes(octx_h025, nch):
         x = s.getSol().download()
         assert abs(s.getLastIterations() - it0) <= 1, lanes
         assert np.linalg.norm(x - x0) <= 1e-9 * np.linalg.norm(x0), lanes
+
+
+@pytest.mark.parametrize("name,nch", [("pcg", 1), ("pcg", 2), ("pcg_mixed", 2), ("pcg", 4)])
+def test_region_major_direction_map(octx_h025, name, nch):
+    """The direction kernel on the region-major map (each XCD's workgroups
+    take one stretch of every colour; the default): same iterations (+-1) and
+    solutions as the row-order map, also with larger direction workgroups, and the
+    repeated solve is bit-identical (its partial sums keep a fixed order)."""
+    m = octx_h025["Kp_int"]
+    n = m.shape[0]
+    rng = np.random.default_rng(47)
+    b = rng.uniform(-1, 1, nch * n)
+    s, x0 = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10, regionDirection=0)
+    it0 = s.getLastIterations()
+    for rows in (0, 512):
+        s.setOption("rowsDirection", rows)
+        s.setOption("regionDirection", 1)
+        xs = []
+        for _ in range(2):
+            s.getRhs().upload(b)
+            s.getSol().memsetZero()
+            s.solve()
+            xs.append(s.getSol().download())
+            assert abs(s.getLastIterations() - it0) <= 1, rows
+        assert np.array_equal(xs[0], xs[1]), rows
+        assert np.linalg.norm(xs[0] - x0) <= 1e-9 * np.linalg.norm(x0), rows
+        res = np.linalg.norm(b.reshape(nch, n).T - m @ xs[0].reshape(nch, n).T, axis=0)
+        assert np.all(res <= 1.01e-10 * np.linalg.norm(b.reshape(nch, n), axis=1)), rows
+        s.setOption("regionDirection", 0)
