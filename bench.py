@@ -127,6 +127,17 @@ def kernel_table(ch, res, reps=30, with_iteration=True):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_chorin_500k_mixed.md")
 
 
+def copy_peak_gbps(mib=2048, reps=10):
+    """Achievable streaming bandwidth on this device (GB/s, read + write): a
+    2 GiB device-to-device copy, far beyond the 256 MB Infinity Cache,
+    repeated back to back and timed with HIP events (nsfem_copy_bandwidth)."""
+    import ctypes
+    from nsfem_amd import _lib
+    v = ctypes.c_double()
+    _lib.call("nsfem_copy_bandwidth", ctypes.c_size_t(mib << 20), int(reps), ctypes.byref(v))
+    return v.value / 1e9
+
+
 def measured_traffic(kind):
     """HBM bytes per launch of a mixed-path kernel kind, from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if
@@ -551,10 +562,13 @@ def main():
         dom = kernmx[dom_name]
         peak = 8000.0
         traffic = measured_traffic(dom_name) if head == "mixed" and args.nodes == 500_000 else None
+        copy = copy_peak_gbps()
+        log(f"achievable copy bandwidth: {copy:.0f} GB/s")
         roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": peak, "unit": "GB/s",
                     "frac": dom["GBps"] / peak, "traffic": traffic, "bytes_per_launch": dom["bytes"],
                     "avg_launch_ms": dom["avg_ms"],
-                    "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None}
+                    "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
+                    "achievable_copy": copy, "frac_of_achievable": dom["GBps"] / copy}
     out = {
         "metric": "Chorin steps/s @500k p-DOFs, mixed vs FP64 GPU, same residual; SpMV % HBM peak",
         "value": rmx["steps_per_s"],

@@ -80,6 +80,13 @@ int nsfem_csr_mul(nsfem_csr * m, int numCh, const void * x, long ldx, void * y, 
  * ---------------------------------------------------------------------- */
 int nsfem_axpy(int dtype, long n, double alpha, const void * src, void * dst); /* dst += alpha src */
 int nsfem_scale(int dtype, long n, double alpha, void * dst);
+/* Achievable streaming bandwidth of the device: a device-to-device copy of
+ * `bytes` (rounded down to 16; use more than the 256 MB Infinity Cache for
+ * the HBM figure) repeated `reps` times back to back, timed with HIP events
+ * on the library stream; *bytesPerSecond = read + write bytes per second.
+ * Allocates and frees its two buffers.  No reference counterpart
+ * (measurement hook). */
+int nsfem_copy_bandwidth(size_t bytes, int reps, double * bytesPerSecond);
 int nsfem_geam(int dtype, long n, double alpha, const void * a, double beta, const void * b, void * c);
 int nsfem_dot(int dtype, long n, const void * x, const void * y, double * out);
 int nsfem_nrm2(int dtype, long n, const void * x, double * out);

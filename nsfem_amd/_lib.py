@@ -83,6 +83,7 @@ _SIGNATURES = {
     "nsfem_last_error": [],
     "nsfem_version": [_c_int_p, _c_int_p],
     "nsfem_synchronize": [],
+    "nsfem_copy_bandwidth": [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
     "nsfem_malloc": [ctypes.c_size_t, ctypes.POINTER(_vp)],
     "nsfem_free": [_vp],
     "nsfem_upload": [_vp, _vp, ctypes.c_size_t],

@@ -376,6 +376,15 @@ extern "C"
             syncLibraryStream(); });
     }
 
+    int nsfem_copy_bandwidth(size_t bytes, int reps, double * bytesPerSecond)
+    {
+        return guarded([&]
+                       {
+            need(bytesPerSecond, "bytesPerSecond");
+            *bytesPerSecond = copyBandwidth(bytes, reps, libraryStream());
+            syncLibraryStream(); });
+    }
+
     int nsfem_geam(int dtype, long n, double alpha, const void * a, double beta, const void * b, void * c)
     {
         return guarded([&]

@@ -221,6 +221,16 @@ namespace nsfem
         }
     }
 
+    // dst = src over n 16-byte elements, one per thread and grid-stride
+    // (the achievable streaming bandwidth of the device).
+    static __global__ void streamCopyKernel(long n, const double2 * __restrict__ src, double2 * __restrict__ dst)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            dst[i] = src[i];
+        }
+    }
+
     template <typename T>
     __global__ void fillKernel(long n, T value, T * __restrict__ dst)
     {

@@ -162,6 +162,39 @@ namespace nsfem
                     (scaleKernel<double><<<g, 256, 0, s>>>(n, alpha, static_cast<double *>(dst))));
     }
 
+    double copyBandwidth(size_t bytes, int reps, hipStream_t s)
+    {
+        const long n = static_cast<long>(bytes / 16);
+        if (n < 1 || reps < 1)
+        {
+            throw std::invalid_argument("copy bandwidth: at least 16 bytes and one repetition");
+        }
+        DeviceArray<double2> src(n), dst(n);
+        NSFEM_HIP(hipMemsetAsync(src.get(), 0, n * sizeof(double2), s));
+        const long blocks = std::min<long>((n + 255) / 256, 1L << 30);
+        const int g = static_cast<int>(blocks);
+        streamCopyKernel<<<g, 256, 0, s>>>(n, src.get(), dst.get());
+        NSFEM_HIP(hipGetLastError());
+        NSFEM_HIP(hipStreamSynchronize(s));
+        // the repetitions queue behind a 20 ms delay kernel and run back to back
+        delayKernel<<<1, 64, 0, s>>>(2000000);
+        hipEvent_t e0, e1;
+        NSFEM_HIP(hipEventCreate(&e0));
+        NSFEM_HIP(hipEventCreate(&e1));
+        NSFEM_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; i++)
+        {
+            streamCopyKernel<<<g, 256, 0, s>>>(n, src.get(), dst.get());
+        }
+        NSFEM_HIP(hipEventRecord(e1, s));
+        NSFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        NSFEM_HIP(hipEventDestroy(e0));
+        NSFEM_HIP(hipEventDestroy(e1));
+        return 2.0 * static_cast<double>(n) * sizeof(double2) * reps / (static_cast<double>(ms) * 1e-3);
+    }
+
     void geam(int dt, long n, double alpha, const void * a, double beta, const void * b, void * c, hipStream_t s)
     {
         if (n <= 0)

@@ -49,4 +49,8 @@ namespace nsfem
 
     // Grid size for grid-stride element-wise kernels.
     int elementwiseGrid(long n);
+    // Achievable streaming bandwidth: a device-to-device copy of bytes
+    // (rounded down to 16) timed with HIP events on s over reps back-to-back
+    // launches; returns read + write bytes per second.
+    double copyBandwidth(size_t bytes, int reps, hipStream_t s);
 } // namespace nsfem

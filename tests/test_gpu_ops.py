@@ -116,3 +116,16 @@ def test_scatter_gather_and_copies():
     assert np.array_equal(other.download(), ref)
     other.memsetZero()
     assert not other.download().any()
+
+
+def test_copy_bandwidth_hook():
+    """The achievable-bandwidth hook copies, times and frees: a plausible
+    rate for a 64 MiB copy (above 100 GB/s, below 20 TB/s read + write), and
+    bad sizes are rejected."""
+    import ctypes
+    from nsfem_amd import _lib
+    v = ctypes.c_double()
+    _lib.call("nsfem_copy_bandwidth", ctypes.c_size_t(64 << 20), 3, ctypes.byref(v))
+    assert 1e11 < v.value < 2e13
+    with pytest.raises(Exception):
+        _lib.call("nsfem_copy_bandwidth", ctypes.c_size_t(8), 3, ctypes.byref(v))
