@@ -441,8 +441,7 @@ namespace nsfem
         // Tentative velocity: M a = A u;  u -= tau a (the product's lanes per
         // row follow its mean row length, not the solver shapes)
         csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, 0, s);
-        fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
-        vSolver_->solve();
+        vSolver_->solve(true); // cold start
         st.mseTentX = maxMse(*vSolver_, 0);
         st.mseTentY = maxMse(*vSolver_, 1);
         st.relTent = maxRel(*vSolver_);
@@ -468,8 +467,7 @@ namespace nsfem
 
         // Projection: M a = PVD p;  u -= tau a
         csrMul(pvd_.view(dt), 1, p_.get(), 0, vSolver_->ioRhs(), 0, 1.0, 0.0, nullptr, 2 * N_, 0, s);
-        fill(dt, twoN, 0.0, vSolver_->ioSol(), s);
-        vSolver_->solve();
+        vSolver_->solve(true); // cold start
         st.mseFinalX = maxMse(*vSolver_, 0);
         st.mseFinalY = maxMse(*vSolver_, 1);
         st.relFinal = maxRel(*vSolver_);

@@ -1163,12 +1163,13 @@ namespace nsfem
         ChunkGraph & g = chunkGraph<T, P>();
         int & predictedIters = predicted<T, P>();
         // Chunks of up to checkChunk iterations, shorter for solves expected
-        // to take few iterations (at least 8 chunks per solve, at least 4
-        // iterations per chunk), so the host polls keep pace.
+        // to take few iterations (at least 8 chunks per solve, at least 2
+        // iterations per chunk), so the host polls keep pace and little runs
+        // past convergence.
         int chunk = std::max(1, opt_.checkChunk);
         if (usePrediction)
         {
-            while (chunk > 4 && 8 * chunk > std::max(predictedIters, 1))
+            while (chunk > 2 && 8 * chunk > std::max(predictedIters, 1))
             {
                 chunk /= 2;
             }
@@ -1312,12 +1313,17 @@ namespace nsfem
         NSFEM_HIP(hipStreamSynchronize(s));
     }
 
-    void McgsSolver::solve()
+    void McgsSolver::solve(bool zeroInitial)
     {
         hipStream_t s = stream_;
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
         lastOuter_ = 0;
+        const bool pcgZero = zeroInitial && opt_.method == SolverMethod::Pcg && !(opt_.precision == SolverPrecision::Mixed && opt_.restart);
+        if (zeroInitial && !pcgZero)
+        {
+            fill(opt_.precision == SolverPrecision::F32 ? F32 : F64, static_cast<long>(n_) * nch, 0.0, ioSol_.get(), s);
+        }
         if (opt_.method == SolverMethod::GaussSeidel)
         {
             if (opt_.precision == SolverPrecision::F64)
@@ -1340,8 +1346,11 @@ namespace nsfem
         if (opt_.precision == SolverPrecision::F32)
         {
             permute(F32, F32, n_, nch, perm_.get(), ioRhs_.get(), b32_.get(), false, s);
-            permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
-            runPcg<float, float>(s, operands<float, float>(), st, false, nullptr, opt_.maxIters, true, true);
+            if (!pcgZero)
+            {
+                permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
+            }
+            runPcg<float, float>(s, operands<float, float>(), st, pcgZero, nullptr, opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
             if (opt_.projectMean)
             {
@@ -1354,14 +1363,17 @@ namespace nsfem
             // FP64 Krylov vectors and products; the preconditioner in FP64
             // (plain) or FP32 (mixed)
             permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
-            permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            if (!pcgZero)
+            {
+                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            }
             if (opt_.precision == SolverPrecision::F64)
             {
-                runPcg<double, double>(s, operands<double, double>(), st, false, nullptr, opt_.maxIters, true, true);
+                runPcg<double, double>(s, operands<double, double>(), st, pcgZero, nullptr, opt_.maxIters, true, true);
             }
             else
             {
-                runPcg<double, float>(s, operands<double, float>(), st, false, nullptr, opt_.maxIters, true, true);
+                runPcg<double, float>(s, operands<double, float>(), st, pcgZero, nullptr, opt_.maxIters, true, true);
             }
             readStats(s, inner_, false, 0);
             if (opt_.projectMean)

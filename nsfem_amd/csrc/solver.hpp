@@ -130,7 +130,9 @@ namespace nsfem
         const std::vector<int> & partitionStart(int s) const { return partitions_[s]; }
 
         // Runs the configured method on the current io vectors (blocking).
-        void solve();
+        // zeroInitial: the initial guess is zero whatever the io solution
+        // vector holds (pcg: the first residual is b, no matrix product).
+        void solve(bool zeroInitial = false);
         // Enqueues the solve without host synchronisation where possible.
         void solveAsync(hipStream_t stream);
 
