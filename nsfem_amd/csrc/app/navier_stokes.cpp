@@ -1,6 +1,6 @@
 // ./navier_stokes <config> <msh file> <output dir>
 //
-// Host driver of the GPU Chorin path with the reference's command line and
+// Host program of the GPU Chorin path with the reference's command line and
 // config keys (cpp/execs/NavierStokes/src/navierStokes.cpp:20-83,
 // src/nsConfig.cpp:8-45, utils ConfigParser key = value with '#' comments).
 // Frames are written as linalg .bin vectors (velocity_<j>.bin = [ux; uy],
