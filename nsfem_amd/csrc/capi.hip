@@ -90,6 +90,11 @@ namespace
             if (value != -1 && value != 0 && value != 1) throw std::invalid_argument("fusedReductions must be -1, 0 or 1");
             solver.setFusedReductions(static_cast<int>(value));
         }
+        else if (k == "fuseUpdateSweep")
+        {
+            if (value != 0 && value != 1) throw std::invalid_argument("fuseUpdateSweep must be 0 or 1");
+            solver.setFusedUpdateSweep(static_cast<int>(value));
+        }
         else if (k == "regionDirection")
         {
             if (value != 0 && value != 1) throw std::invalid_argument("regionDirection must be 0 or 1");

@@ -580,12 +580,14 @@ namespace nsfem
         return a != 0;
     }
 
-    // Lane-group sum over j in [j0, j1) of val[j] * v[col[j]*NCH + c], in the
-    // matrix type T (v may be stored narrower, e.g. an FP32 preconditioned
-    // vector under an FP64 matrix).  Solver vectors interleave their
-    // channels, so one gathered column is one contiguous access.
-    template <typename T, int LANES, int NCH, typename V>
-    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const V * __restrict__ v, T (&acc)[NCH])
+    // Lane-group sum over j in [j0, j1) of val[j] * gather(col[j], c), in the
+    // matrix type T.  rowDot gathers v[col*NCH + c] (v may be stored
+    // narrower, e.g. an FP32 preconditioned vector under an FP64 matrix;
+    // solver vectors interleave their channels, so one gathered column is one
+    // contiguous access); the fused update-and-sweep kernel computes the
+    // gathered values instead.
+    template <typename T, int LANES, int NCH, typename G>
+    __device__ __forceinline__ void rowDotG(const McgsMatrix<T> & A, int j0, int j1, G gather, T (&acc)[NCH])
     {
         const int lane = threadIdx.x & (LANES - 1);
 #pragma unroll
@@ -617,7 +619,7 @@ namespace nsfem
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    xv[u][c] = has[u] ? static_cast<T>(v[static_cast<long>(col[u]) * NCH + c]) : T(0);
+                    xv[u][c] = has[u] ? gather(col[u], c) : T(0);
                 }
             }
 #pragma unroll
@@ -638,6 +640,13 @@ namespace nsfem
         {
             acc[c] = laneGroupSum<LANES>(acc[c]);
         }
+    }
+
+    template <typename T, int LANES, int NCH, typename V>
+    __device__ __forceinline__ void rowDot(const McgsMatrix<T> & A, int j0, int j1, const V * __restrict__ v, T (&acc)[NCH])
+    {
+        rowDotG<T, LANES, NCH>(A, j0, j1, [v](int col, int c) { return static_cast<T>(v[static_cast<long>(col) * NCH + c]); },
+                               acc);
     }
 
     template <int LANES>
@@ -828,7 +837,8 @@ namespace nsfem
     // that every XCD's range of logical blocks gets its share.
     template <typename P, typename T, int LANES, int NCH>
     __device__ __forceinline__ void pcgRhoNarrow(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP,
-                                                           const T * r, P * __restrict__ z, int color0End, int color0Blocks)
+                                                           T * r, P * __restrict__ z, int color0End, int color0Blocks,
+                                                           const T * __restrict__ q, int r0Fix)
     {
         __shared__ double sm[NCH * kWaves];
         const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
@@ -846,6 +856,16 @@ namespace nsfem
         int j1 = c0 ? M.rowStart[row + 1] : 0;
         // the system's activity loads with the flag and the row bounds
         const bool act = systemActive<NCH>(st, sys);
+        // r0Fix: r -= alpha q on colour 0 is still to be stored (the fused
+        // update-and-sweep kernel leaves it), per active channel
+        bool fixC[NCH];
+        T alphaC[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            fixC[c] = r0Fix && st.active[sys * NCH + c];
+            alphaC[c] = fixC[c] ? static_cast<T>(st.alpha[sys * NCH + c]) : T(0);
+        }
         if (done || j1 < j0)
         {
             return;
@@ -888,8 +908,22 @@ namespace nsfem
                     for (int c = 0; c < NCH; c++)
                     {
                         const long i = static_cast<long>(row) * NCH + c;
-                        own[c] = c0 ? rP[i] : z[i];
-                        rr[c] = static_cast<double>(r[i]);
+                        T ri = r[i];
+                        if (c0 && fixC[c])
+                        {
+                            ri = fmaT(-alphaC[c], q[i], ri);
+                            r[i] = ri;
+                        }
+                        // with P == T, rP is r: its colour-0 value is ri
+                        if constexpr (std::is_same_v<T, P>)
+                        {
+                            own[c] = c0 ? ri : z[i];
+                        }
+                        else
+                        {
+                            own[c] = c0 ? rP[i] : z[i];
+                        }
+                        rr[c] = static_cast<double>(ri);
                     }
                 }
                 P acc[NCH];
@@ -1435,6 +1469,7 @@ namespace nsfem
     __device__ __forceinline__ void pcgRhoWide(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP, const T * r,
                                                P * __restrict__ z, int color0End, int color0Blocks)
     {
+        // (wide tiles never take the fused update and sweep: no r0Fix here)
         __shared__ double sm[NCH * kWaves];
         const int lb = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, color0Blocks);
         const int4 range = blockRange(bm, lb);
@@ -1589,8 +1624,12 @@ namespace nsfem
         }
     }
 
+    // r0Fix = 1 (1-2 channels, after pcgUpdateSweepKernel): the colour-0
+    // residuals r -= alpha q are stored here first (alpha and the channels'
+    // activity from the solve state).
     template <typename P, typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP, const T * r, P * __restrict__ z, int color0End, int color0Blocks)
+    __global__ void __launch_bounds__(kBlock) pcgRhoKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const P * rP, T * r, P * __restrict__ z, int color0End, int color0Blocks,
+                                                           const T * __restrict__ q, int r0Fix)
     {
         if constexpr (kWide<NCH>)
         {
@@ -1598,7 +1637,7 @@ namespace nsfem
         }
         else
         {
-            pcgRhoNarrow<P, T, LANES, NCH>(M, bm, st, rP, r, z, color0End, color0Blocks);
+            pcgRhoNarrow<P, T, LANES, NCH>(M, bm, st, rP, r, z, color0End, color0Blocks, q, r0Fix);
         }
     }
 
@@ -1807,6 +1846,228 @@ namespace nsfem
             blockSum<NCH>(part, sm);
             storePartial<NCH>(part, st.partials2, range.w);
         }
+    }
+
+    // The update kernel fused with the first forward sweep (one system, flat
+    // map, fused reductions, 1-2 channels): the update of every row and, on
+    // the rows of colour 1, their forward symmetric-GS sweep.  That sweep's
+    // lower part gathers colour-0 rows only; their z = (P)(r - alpha q) / d is
+    // computed from r and q as this launch finds them, with the arithmetic of
+    // the stored z, so the sweep equals update-then-sweep bit for bit.  This
+    // launch therefore leaves the colour-0 residuals (r, and rP when it is the
+    // same array) unstored; the rho kernel of the same iteration stores them
+    // (pcgRhoKernel r0Fix).  Workgroups holding colour-1 rows are interleaved
+    // evenly among the others, so every XCD gets its share of sweep work.
+    template <typename T, typename P, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) pcgUpdateSweepKernel(McgsMatrix<P> M, BlockMap bm, SolveState st, const T * __restrict__ p,
+                                                                   const T * __restrict__ q, T * __restrict__ x, T * r, P * rP,
+                                                                   P * __restrict__ z, int color0End, int c1Begin, int c1End,
+                                                                   int specialBegin, int specialCount, int dirParts)
+    {
+        static_assert(!kWide<NCH>, "the fused update and sweep takes 1 or 2 channels");
+        static_assert(kBlock % NCH == 0, "channels must divide the block");
+        __shared__ double sm[NCH * kWaves];
+        __shared__ double shAlpha[NCH];
+        __shared__ int shAct[NCH];
+        const int k = interleaveFirst(logicalBlock(blockIdx.x, gridDim.x), gridDim.x, specialCount);
+        const int lb = k < specialCount ? specialBegin + k : (k - specialCount < specialBegin ? k - specialCount : k);
+        const int4 range = blockRange(bm, lb);
+        const int s = range.z;
+        const int t = static_cast<int>(threadIdx.x);
+        // the block's colour-1 rows; the first slab's row bounds go out with
+        // the prologue's loads
+        const int s0 = max(range.x, c1Begin), s1 = min(range.y, c1End);
+        int srow = s0 + t / LANES;
+        bool svalid = srow < s1;
+        int j0 = svalid ? M.rowStart[srow] : 0;
+        int j1 = svalid ? M.rowSplit[srow] : 0;
+        int sysAny = 0;
+#pragma unroll
+        for (int kk = 0; kk < NCH; kk++)
+        {
+            sysAny |= st.activeNext[s * NCH + kk];
+        }
+        int aNext = 0;
+        double rhoNext = 0;
+        if (t < NCH)
+        {
+            aNext = st.activeNext[s * NCH + t];
+            rhoNext = st.rhoNext[s * NCH + t];
+        }
+        const FusedSlice sl = fusedSlice(bm, range, lb, dirParts, dirParts);
+        const double pq = channelPartialSum<NCH>(st.partials3 + static_cast<size_t>(sl.base) * NCH, sl.rhoParts, sm);
+        // block 0 makes the direction kernel's decisions the solve state (as
+        // pcgUpdateKernel); no block of this launch reads what it writes
+        if (lb == 0)
+        {
+            const int K = bm.numSystems * NCH;
+            int any = 0;
+            for (int kk = t; kk < K; kk += kBlock)
+            {
+                const int a = st.activeNext[kk];
+                st.active[kk] = a;
+                st.rho[kk] = st.rhoNext[kk];
+                any |= a;
+            }
+            any = __syncthreads_or(any);
+            if (t == 0)
+            {
+                st.flags[0] = any ? 0 : 1;
+                st.flags[1] = 0;
+            }
+        }
+        if (!sysAny | (j1 < j0))
+        {
+            return;
+        }
+        if (t < NCH)
+        {
+            shAct[t] = aNext;
+            const double a = (aNext && pq != 0) ? rhoNext / pq : 0.0;
+            shAlpha[t] = a;
+            if (sl.leader)
+            {
+                st.alpha[s * NCH + t] = a; // for the rho kernel's colour-0 residuals
+            }
+        }
+        __syncthreads();
+        double part[NCH];
+#pragma unroll
+        for (int cc = 0; cc < NCH; cc++)
+        {
+            part[cc] = 0;
+        }
+        {
+            // 1. the update of the rows outside colour 1, one element per thread
+            const int c = t % NCH;
+            const int act = shAct[c];
+            const T alpha = static_cast<T>(shAlpha[c]);
+            double sum = 0;
+            const long end = static_cast<long>(range.y) * NCH;
+            for (long i = static_cast<long>(range.x) * NCH + t; i < end; i += kBlock)
+            {
+                const int row = static_cast<int>(i / NCH);
+                if (row >= c1Begin && row < c1End)
+                {
+                    continue;
+                }
+                T ri = r[i];
+                if (act)
+                {
+                    const T xi = x[i], pi = p[i], qi = q[i];
+                    x[i] = fmaT(alpha, pi, xi);
+                    ri = fmaT(-alpha, qi, ri);
+                }
+                const P rp = static_cast<P>(ri);
+                if (row < color0End)
+                {
+                    z[i] = rp * M.invDiag[row];
+                    if constexpr (!std::is_same_v<T, P>)
+                    {
+                        if (act)
+                        {
+                            rP[i] = rp; // a separate array: not read by this launch
+                        }
+                    }
+                }
+                else if (act)
+                {
+                    r[i] = ri;
+                    if constexpr (!std::is_same_v<T, P>)
+                    {
+                        rP[i] = rp;
+                    }
+                }
+                sum = fma(static_cast<double>(ri), static_cast<double>(ri), sum);
+            }
+            part[c] = sum;
+        }
+        if (s0 < s1)
+        {
+            // 2. colour-1 rows: the update and the forward sweep, LANES lanes
+            // per row
+            int actC[NCH];
+            T alphaC[NCH];
+#pragma unroll
+            for (int cc = 0; cc < NCH; cc++)
+            {
+                actC[cc] = shAct[cc];
+                alphaC[cc] = static_cast<T>(shAlpha[cc]);
+            }
+            auto zColour0 = [&](int col, int cc) -> P
+            {
+                const long jj = static_cast<long>(col) * NCH + cc;
+                T rj = r[jj];
+                if (actC[cc])
+                {
+                    rj = fmaT(-alphaC[cc], q[jj], rj);
+                }
+                return static_cast<P>(rj) * M.invDiag[col];
+            };
+            for (int base = s0; base < s1; base += kBlock / LANES)
+            {
+                if (base != s0)
+                {
+                    srow = base + t / LANES;
+                    svalid = srow < s1;
+                    j0 = svalid ? M.rowStart[srow] : 0;
+                    j1 = svalid ? M.rowSplit[srow] : 0;
+                }
+                const bool lead = svalid && groupLeader<LANES>();
+                P inv = P(0);
+                T ri[NCH], xi[NCH], pi[NCH], qi[NCH];
+#pragma unroll
+                for (int cc = 0; cc < NCH; cc++)
+                {
+                    ri[cc] = xi[cc] = pi[cc] = qi[cc] = T(0);
+                }
+                if (lead)
+                {
+                    inv = M.invDiag[srow];
+#pragma unroll
+                    for (int cc = 0; cc < NCH; cc++)
+                    {
+                        const long i = static_cast<long>(srow) * NCH + cc;
+                        ri[cc] = r[i];
+                        if (actC[cc])
+                        {
+                            xi[cc] = x[i];
+                            pi[cc] = p[i];
+                            qi[cc] = q[i];
+                        }
+                    }
+                }
+                P acc[NCH];
+                rowDotG<P, LANES, NCH>(M, j0, j1, zColour0, acc);
+                if (lead)
+                {
+#pragma unroll
+                    for (int cc = 0; cc < NCH; cc++)
+                    {
+                        const long i = static_cast<long>(srow) * NCH + cc;
+                        T rn = ri[cc];
+                        if (actC[cc])
+                        {
+                            x[i] = fmaT(alphaC[cc], pi[cc], xi[cc]);
+                            rn = fmaT(-alphaC[cc], qi[cc], rn);
+                            r[i] = rn;
+                        }
+                        const P rp = static_cast<P>(rn);
+                        if constexpr (!std::is_same_v<T, P>)
+                        {
+                            if (actC[cc])
+                            {
+                                rP[i] = rp;
+                            }
+                        }
+                        z[i] = (rp - acc[cc]) * inv;
+                        part[cc] = fma(static_cast<double>(rn), static_cast<double>(rn), part[cc]);
+                    }
+                }
+            }
+        }
+        blockSum<NCH>(part, sm);
+        storePartial<NCH>(part, st.partials2, range.w);
     }
 
     // Per-system RMS residuals of gsResidualKernel, stop test and flags.

@@ -1062,11 +1062,12 @@ namespace nsfem
     }
 
     template <typename P>
-    void McgsSolver::sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z)
+    void McgsSolver::sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z, int firstForward)
     {
         const int nch = opt_.numCh;
-        // forward colours 1 .. C-1 on the lower part (colour 0 was set to r/d)
-        for (int c = 1; c < numColors_; c++)
+        // forward colours firstForward .. C-1 on the lower part (colour 0 was
+        // set to r/d; colour 1 may have been swept with the update)
+        for (int c = firstForward; c < numColors_; c++)
         {
             const int cnt = colorSweepBlockStart_[c + 1] - colorSweepBlockStart_[c];
             if (cnt == 0)
@@ -1103,12 +1104,43 @@ namespace nsfem
         dispatchLC(lanes_, opt_.numCh, [&]<int L, int C>()
                    {
                        pcgRhoKernel<P, T, L, C><<<nbRho_, kBlock, 0, s>>>(v.M, bmRho_, st, v.rP, v.r, v.z, color0End_,
-                                                                          color0BlocksRho_);
+                                                                          color0BlocksRho_, v.q, withUpdate && mergedUpdate() ? 1 : 0);
                        if (fin)
                        {
                            pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
                        }
                    });
+    }
+
+    template <typename T, typename P>
+    void McgsSolver::launchUpdateSweep(hipStream_t s, const PcgOperands<T, P> & v, SolveState st)
+    {
+        // colour 1 = rows [colorRowStart_[1], colorRowStart_[2]); the update
+        // map's workgroups that hold them are interleaved among the others
+        const int rpb = bmUpd_.rowsPerBlock;
+        const int c1Begin = colorRowStart_[1], c1End = colorRowStart_[2];
+        const int spBegin = c1Begin / rpb;
+        const int spCount = c1End > c1Begin ? (c1End + rpb - 1) / rpb - spBegin : 0;
+        dispatchLC(sweepLanes_, opt_.numCh, [&]<int L, int C>()
+                   {
+                       if constexpr (!kWide<C>)
+                       {
+                           pcgUpdateSweepKernel<T, P, L, C><<<nbUpd_, kBlock, 0, s>>>(v.M, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z,
+                                                                                     color0End_, c1Begin, c1End, spBegin, spCount,
+                                                                                     nbDir_);
+                       }
+                   });
+    }
+
+    void McgsSolver::setFusedUpdateSweep(int on)
+    {
+        if (on != 0 && on != 1)
+        {
+            throw std::invalid_argument("fuseUpdateSweep must be 0 or 1");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        mergeMode_ = on;
+        destroyGraphs();
     }
 
     template <typename T, typename P>
@@ -1125,13 +1157,20 @@ namespace nsfem
                            pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
                        }
                    });
-        dispatchC(nch, [&]<int C>()
-                  {
-                      pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                        color0End_, fz, nbDir_);
-                  });
+        if (mergedUpdate())
+        {
+            launchUpdateSweep<T, P>(s, v, st);
+        }
+        else
+        {
+            dispatchC(nch, [&]<int C>()
+                      {
+                          pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                            color0End_, fz, nbDir_);
+                      });
+        }
         // the update's convergence test is finished together with rho
-        sgsSweeps<P>(s, v.M, st, v.rP, v.z);
+        sgsSweeps<P>(s, v.M, st, v.rP, v.z, mergedUpdate() ? 2 : 1);
         launchRho<T, P>(s, v, st, totals, true);
     }
 
@@ -1473,10 +1512,12 @@ namespace nsfem
             else if (which == "sweep")
             {
                 // the colour sweeps of one preconditioner application:
-                // forward colours 1 .. C-1 (lower part), backward C-2 .. 1
+                // forward colours 1 .. C-1 (lower part; from 2 when colour 1
+                // is swept with the update), backward C-2 .. 1
+                const int firstForward = mergedUpdate() ? 2 : 1;
                 double total = 0;
                 int count = 0;
-                for (int c = 1; c < numColors_; c++)
+                for (int c = firstForward; c < numColors_; c++)
                 {
                     if (colorSweepBlockStart_[c + 1] > colorSweepBlockStart_[c])
                     {
@@ -1492,25 +1533,28 @@ namespace nsfem
                         count++;
                     }
                 }
-                if (count == 0)
+                if (numColors_ < 2)
                 {
                     throw std::invalid_argument("the system has a single colour: no sweep launches");
                 }
                 launches = count;
-                bytes = total / count;
-                launch = [=, this]() { sgsSweeps<P>(s, v.M, st, v.rP, v.z); };
+                bytes = count > 0 ? total / count : 0.0;
+                launch = [=, this]() { sgsSweeps<P>(s, v.M, st, v.rP, v.z, firstForward); };
             }
             else if (which == "rho")
             {
                 // backward colour 0 (full rows) fused with r.z over all rows
+                // (after the fused update and sweep: also r -= alpha q on
+                // colour 0, q read and r written there)
+                const int fix = mergedUpdate() ? 1 : 0;
                 bytes = (numColors_ > 0 ? sweepBytes(0, false) : 0.0) + nch * n * tsT + nch * (n - n0) * tsP -
-                        (same ? nch * n0 * tsP : 0.0);
+                        (same ? nch * n0 * tsP : 0.0) + fix * 2.0 * nch * n0 * tsT;
                 launch = [=, this]()
                 {
                     dispatchLC(lanes_, nch, [&]<int L, int C>()
                                {
                                    pcgRhoKernel<P, T, L, C><<<nbRho_, kBlock, 0, s>>>(v.M, bmRho_, st, v.rP, v.r, v.z, color0End_,
-                                                                                  color0BlocksRho_);
+                                                                                  color0BlocksRho_, v.q, fix);
                                });
                 };
             }
@@ -1519,15 +1563,28 @@ namespace nsfem
                 // x, r read and written, p, q read; rP written (P narrower
                 // than T); z = r / d on colour 0
                 bytes = 6.0 * nch * n * tsT + (same ? 0.0 : nch * n * tsP) + n0 * tsP + nch * n0 * tsP;
-
-                launch = [=, this]()
+                if (mergedUpdate())
                 {
-                    dispatchC(nch, [&]<int C>()
-                              {
-                                  pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z,
-                                                                                    v.M.invDiag, color0End_, fz, nbDir_);
-                              });
-                };
+                    // with the forward sweep of colour 1: its lower part, its
+                    // diagonal and z written; the colour-0 columns it
+                    // gathers read r and q and their diagonal once; colour 0's
+                    // r is written by the rho kernel instead
+                    const ColorStats & c1 = colorStats_[1];
+                    bytes += c1.lowerNnz * (tsP + 4.0) + (c1.rows + 1) * 4.0 + c1.rows * 4.0 + c1.rows * tsP +
+                             nch * c1.rows * tsP + c1.lowerCols * (2.0 * nch * tsT + tsP) - nch * n0 * tsT;
+                    launch = [=, this]() { launchUpdateSweep<T, P>(s, v, st); };
+                }
+                else
+                {
+                    launch = [=, this]()
+                    {
+                        dispatchC(nch, [&]<int C>()
+                                  {
+                                      pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP,
+                                                                                        v.z, v.M.invDiag, color0End_, fz, nbDir_);
+                                  });
+                    };
+                }
             }
             else if (which == "finalize")
             {

@@ -175,6 +175,18 @@ namespace nsfem
         // colour, so that XCD gathers from one eighth of the vector.
         // 1 on (default), 0 off; reshape keeps it.
         void setRegionedDirection(int on);
+        // Update kernel fused with the first forward sweep (fused reductions,
+        // one system, 1-2 channels, two colours or more): 1 on (default), 0
+        // off.  One launch fewer per iteration.  The update workgroups that
+        // hold colour-1 rows sweep them in slabs of kBlock / sweep lanes
+        // rows, so it applies only while that is at most two slabs (more
+        // leave the launch waiting on those workgroups).
+        void setFusedUpdateSweep(int on);
+        bool mergedUpdate() const
+        {
+            return mergeMode_ && fused() && flat() && opt_.numCh <= 2 && numColors_ >= 2 &&
+                   bmUpd_.rowsPerBlock * sweepLanes_ <= 2 * kBlock;
+        }
         int regionedDirection() const { return regionDir_; }
         // Fewest rows per workgroup of the default and autotuned shapes.
         int minRowsPerBlock() const;
@@ -236,9 +248,11 @@ namespace nsfem
         template <typename T, typename P>
         void pcgIteration(hipStream_t s, const PcgOperands<T, P> & v, SolveState st);
         template <typename P>
-        void sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z);
+        void sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z, int firstForward = 1);
         template <typename T, typename P>
         void launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate);
+        template <typename T, typename P>
+        void launchUpdateSweep(hipStream_t s, const PcgOperands<T, P> & v, SolveState st);
         void runMixedRestart(hipStream_t s);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
         template <typename T>
@@ -265,6 +279,7 @@ namespace nsfem
         // order, [colorRegionStart_[9c + k], colorRegionStart_[9c + k + 1])
         std::vector<int> colorRegionStart_;
         int regionDir_ = 1;
+        int mergeMode_ = 1;
         DeviceArray<int2> dirTable_;
         int color0Blocks_ = 0;           // main-map blocks holding colour-0 rows (leading)
         // one system: flat workgroup maps (ranges computed in the kernels)

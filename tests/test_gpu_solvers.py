@@ -228,18 +228,20 @@ def test_pure_neumann_laplacian_with_mean_projection(oracle, octx_h025, name):
 
 def test_kernel_timing_hooks_report_per_launch_figures(octx_h025):
     """The measurement hook replays one iteration's launches: positive times
-    and byte counts, 2C-3 sweep launches per iteration, and the solver still
-    solves afterwards."""
+    and byte counts, 2C-3 sweep launches per iteration (2C-4 when the update
+    kernel also sweeps colour 1: fused reductions on one system), and the
+    solver still solves afterwards."""
     m = octx_h025["Kp_int"]
     rng = np.random.default_rng(32)
     b = rng.uniform(-1, 1, m.shape[0])
-    s, _ = _solve("pcg", m, b, max_iters=2000, target=1e-10)
+    s, _ = _solve("pcg", m, b, max_iters=2000, target=1e-10, rowsUpdate=64)
     C = s.numColors()
     nnz = m.nnz
     # one system: the reductions are fused by default (no finalize launches)
     for fused in (-1, 0, 1):
         s.setOption("fusedReductions", fused)
-        expect = {"direction": 1, "sweep": 2 * C - 3, "rho": 1, "update": 1, "finalize": 2 if fused == 0 else 0}
+        expect = {"direction": 1, "sweep": 2 * C - 3 - (fused != 0), "rho": 1, "update": 1,
+                  "finalize": 2 if fused == 0 else 0}
         for k, launches in expect.items():
             ms, by, per_it = s.timeKernel(k, reps=3)
             assert ms > 0 and by > 0 and per_it == launches, (k, fused)
@@ -578,3 +580,41 @@ def test_region_major_direction_map(octx_h025, name, nch):
         res = np.linalg.norm(b.reshape(nch, n).T - m @ xs[0].reshape(nch, n).T, axis=0)
         assert np.all(res <= 1.01e-10 * np.linalg.norm(b.reshape(nch, n), axis=1)), rows
         s.setOption("regionDirection", 0)
+
+
+@pytest.mark.parametrize("name,nch", [("pcg", 1), ("pcg", 2), ("pcg_mixed", 1), ("pcg_mixed", 2)])
+def test_update_fused_with_first_sweep(octx_h025, name, nch):
+    """The update kernel fused with the colour-1 forward sweep (the default
+    on one system): same iterations (+-1) and solutions as the separate
+    launches, with several update workgroup sizes, and the repeated solve is
+    bit-identical; with the iteration count fixed (an unreachable target) the
+    two give bit-identical solutions: the sweep computes its colour-0 values
+    with the arithmetic of the stored ones."""
+    m = octx_h025["Kp_int"]
+    n = m.shape[0]
+    rng = np.random.default_rng(53)
+    b = rng.uniform(-1, 1, nch * n)
+    fixed = []
+    for fz in (0, 1):
+        sf, xf = _solve(name, m, b, nch=nch, max_iters=40, target=1e-30, rowsUpdate=64, fuseUpdateSweep=fz)
+        assert sf.getLastIterations() == 40
+        assert sf.timeKernel("sweep", reps=1)[2] == 2 * sf.numColors() - 3 - fz
+        fixed.append(xf)
+    assert np.array_equal(fixed[0], fixed[1])
+    s, x0 = _solve(name, m, b, nch=nch, max_iters=3000, target=1e-10, fuseUpdateSweep=0)
+    it0 = s.getLastIterations()
+    for rows in (0, 64, 1024):
+        s.setOption("rowsUpdate", rows)
+        s.setOption("fuseUpdateSweep", 1)
+        xs = []
+        for _ in range(2):
+            s.getRhs().upload(b)
+            s.getSol().memsetZero()
+            s.solve()
+            xs.append(s.getSol().download())
+            assert abs(s.getLastIterations() - it0) <= 1, rows
+        assert np.array_equal(xs[0], xs[1]), rows
+        assert np.linalg.norm(xs[0] - x0) <= 1e-9 * np.linalg.norm(x0), rows
+        res = np.linalg.norm(b.reshape(nch, n).T - m @ xs[0].reshape(nch, n).T, axis=0)
+        assert np.all(res <= 1.01e-10 * np.linalg.norm(b.reshape(nch, n), axis=1)), rows
+        s.setOption("fuseUpdateSweep", 0)
