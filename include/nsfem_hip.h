@@ -165,14 +165,22 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  the direction kernel's workgroups region-major, so the
  *                  workgroups of one XCD take the same stretch (one eighth
  *                  of the rows' locality order) of every colour and gather
- *                  from one eighth of the vector; 0: rows in colour order. */
+ *                  from one eighth of the vector; 0: rows in colour order.
+ *   "fuseUpdateSweep" fused reductions on one system, 1-2 channels: 1
+ *                  (default) runs the update kernel and the colour-1 forward
+ *                  sweep as one launch (the sweep computes its colour-0
+ *                  values from r and q; bit-identical results) while the
+ *                  update workgroups hold at most two sweep slabs of rows
+ *                  (rows per update workgroup x sweep lanes <= 512); 0: two
+ *                  launches. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events
  * bracket GPU execution only), replaying the launches of one PCG iteration: "direction" (A z fused with the direction update and
  * p.q), "sweep" (the colour sweeps of one preconditioner application:
- * forward colours 1..C-1 on the lower part, backward C-2..1), "rho"
- * (backward colour 0 fused with r.z), "update" (x, r, colour-0 z, r.r),
+ * forward colours 1..C-1 on the lower part, from 2 with fuseUpdateSweep,
+ * backward C-2..1), "rho" (backward colour 0 fused with r.z), "update" (x,
+ * r, colour-0 z, r.r; with fuseUpdateSweep also the colour-1 sweep),
  * "finalize" (a one-block reduction finalize).  avgMs and bytes
  * (algorithmic, x read once) are per kernel launch, averaged over the
  * launches; launchesPerIteration (optional) is how many launches of that
