@@ -251,3 +251,57 @@ def test_full_size_channel_step_properties():
     dp = np.linalg.norm(pm - p1) / np.linalg.norm(p1)
     print(f"500k: mixed vs FP64 after two steps: velocity {du:.2e}, pressure {dp:.2e}")
     assert du <= 1e-9 and dp <= 1e-6
+
+
+def test_cpu_sized_channel_workload_matches_oracle(oracle):
+    """The CPU-sized channel workload (seeded mesh with the cylinder, about
+    5k P1 / 20k P2 DOFs, peak inflow 1.5, Re 20, dt 1e-3, up to 200 steps).
+    At this time step the explicit scheme is past its stability bound on this
+    mesh: the oracle's velocity settles to about 1.85 by step 40, then grows
+    (3 at step 50, 21 at step 60) and overflows near step 74.  The FP64 GPU
+    path follows the oracle step by step to 1e-10 relative (both solving to
+    1e-13) through the first 40 steps, the mixed path to its own bounds over
+    10 steps, and the GPU run stops with the reference's non-finite error
+    ("Tentative acceleration is bad") before step 200, at a step where the
+    oracle's own state has already grown past 1e20."""
+    from nsfem_amd import _lib, fem, mesher
+    m = mesher.channel_mesh(5000, seed=20241103)
+    peak = 1.5
+    nu = (2.0 / 3.0) * peak * 2 * m["radius"] / 20.0
+    tri = {"xy": m["xy"].astype(np.float64), "triangles": m["triangles"],
+           "border": oracle.border_from_edges(m["triangles"], m["border"]), "groups": m["groups"]}
+    octx = oracle.build_context(tri, nu, peak, channel_height=m["height"], profile_scale=4.0)
+    cond = fem.DfgConditions(viscosity=nu, peakVelocity=peak, channelHeight=m["height"], profileScale=4.0)
+    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), cond)
+    assert (ctx.numPressureNodes, ctx.numVelocityNodes) == (octx["V"], octx["N"])
+    tau = 1e-3
+    orc = oracle.ChorinOracle(octx, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    ch = fem.Chorin(ctx, _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1)))
+    mx = fem.Chorin(ctx, _cfg(("pcg_mixed", 2000, 1e-10, 1), ("pcg_mixed", 8000, 1e-10, 1)))
+    ch.set_time_step(tau)
+    mx.set_time_step(tau)
+    for k in range(40):
+        ch.step()
+        uo, po = orc.step()
+        u, p = ch.download()
+        assert _rel(u, uo) <= 1e-10 and _rel(p, po) <= 1e-10, k
+        if k < 10:
+            mx.step()
+        if k == 9:
+            um, pm = mx.download()
+            eu, ep = _rel(um, uo), _rel(pm, po)
+            print(f"5k channel, 10 steps: mixed vs oracle velocity {eu:.2e}, pressure {ep:.2e}")
+            assert eu < 1e-8 and ep < 1e-7
+    stop = None
+    with np.errstate(all="ignore"):
+        for k in range(40, 200):
+            try:
+                ch.step()
+            except _lib.NsfemNonFinite as e:
+                assert "Tentative acceleration is bad" in str(e)
+                stop = k
+                break
+            uo, po = orc.step()
+    print(f"5k channel: the GPU run stops at step {stop + 1 if stop is not None else None}, "
+          f"the oracle's velocity then {np.abs(uo).max():.3g}")
+    assert stop is not None and np.abs(uo).max() > 1e20
