@@ -7,7 +7,10 @@ and bench.py's CPU-baseline leg import it; the product never does.
 Each function cites the reference code it restates (paths relative to
 kosyonikolov/NsFemOptim).  How each one is pinned (known answers, closed
 forms, invariants) is listed in its tests; parity status:
-  parse_gmsh, create_mesh           pinned: mid-edge invariant, shipped meshes' counts, Euler characteristic
+  parse_gmsh                        pinned: shipped meshes' counts, Euler characteristic
+  create_mesh                       pinned: bit-exact against the reference's own createMesh
+                                            (concreteMesh.cpp + Element sources built into oracle/_ref/
+                                            by oracle/Makefile) on the three shipped meshes
   dirichlet_nodes                   pinned: boundary geometry, profile values
   local matrices                    pinned: closed forms (P1 cotangent stiffness, P2 mass 1/180 pattern,
                                             row sums, Kronecker property, exactness of the rule)
@@ -17,7 +20,10 @@ forms, invariants) is listed in its tests; parity status:
   sl_greedy_coloring (C)            pinned: valid colouring, smallest-last and greedy properties
   gs_sweeps / gs_solve              pinned: sequential GS in colour order (gaussSeidelCustomOrder)
   pcg_solve                         pinned: dense solve (numpy.linalg) to 1e-10
-  chorin_run                        pinned: Poiseuille steady state on a hole-free channel
+  ChorinOracle.step                 pinned: the projection removes most of the tentative velocity's
+                                            discrete divergence on the internal pressure nodes (a wrong sign
+                                            or 1/tau scaling of the pressure correction does not); the same
+                                            step with sparse direct solves to 1e-10
 """
 
 import ctypes
@@ -196,6 +202,34 @@ def create_mesh(tri, order):
         ia, ib = int(T[e, a]), int(T[e, b])
         border.append((e, side, g, ia, mid[(min(ia, ib), max(ia, ib))], ib))
     return {"xy": allxy, "elements": elements, "border": np.array(border, dtype=np.int64), "groups": tri["groups"]}
+
+
+REF_LIB = os.path.join(HERE, "_ref", "libref_mesh.so")
+
+
+def ref_create_mesh(tri, order):
+    """The reference's own createMesh (concreteMesh.cpp:78-293 with el::P1 /
+    el::P2), built from its sources into oracle/_ref/ by oracle/Makefile, on
+    the parsed triangle mesh.  Returns dict(xy float32, elements, border) or
+    None when the reference build is absent (it needs the reference tree)."""
+    if not os.path.exists(REF_LIB):
+        return None
+    lib = ctypes.CDLL(REF_LIB)
+    xy = np.ascontiguousarray(tri["xy"], dtype=np.float32)
+    T = np.ascontiguousarray(tri["triangles"], dtype=np.int32)
+    B = np.ascontiguousarray(tri["border"], dtype=np.int32).reshape(-1, 3)
+    E, nb = len(T), len(B)
+    per = 3 if order == 1 else 6
+    cap = len(xy) + 3 * E
+    oxy = np.zeros((cap, 2), dtype=np.float32)
+    oel = np.zeros((E, per), dtype=np.int32)
+    obd = np.zeros((nb, 3 + order + 1), dtype=np.int32)
+    n = ctypes.c_int(0)
+    rc = lib.ref_create_mesh(ctypes.c_int(order), ctypes.c_int(len(xy)), _p(xy), ctypes.c_int(E), _p(T), ctypes.c_int(nb), _p(B),
+                             ctypes.c_int(len(tri["groups"])), ctypes.c_int(cap), _p(oxy), _p(oel), _p(obd), ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"reference createMesh failed ({rc})")
+    return {"xy": oxy[:n.value].copy(), "elements": oel.astype(np.int64), "border": obd.astype(np.int64)}
 
 
 def dirichlet_nodes(mesh, group_names, value_fn):
@@ -647,6 +681,7 @@ class ChorinOracle:
         a = self._solve_velocity(rhs)
         self.u -= tau * a
         self.impose()
+        self.u_tentative = self.u.copy()
         d = -(1.0 / tau) * (ctx["VPD"] @ self.u)
         if self.pres[0] == "gs":
             self.p_int, _, _ = gs_solve(None, d[ctx["int_p"]], self.p_int, self.pres[1], self.pres[2], self.pres[3], self.Lp)

@@ -107,3 +107,21 @@ def test_generated_mesh_setup_matches_oracle(oracle):
         assert np.max(np.abs(val - o[oname].data)) <= 1e-13 * np.max(np.abs(o[oname].data))
     ids, vals = p.dirichlet("vx")
     assert np.array_equal(ids, o["dvx"][0]) and np.array_equal(vals, o["dvx"][1])
+
+
+@pytest.mark.parametrize("name", ["c3", "h_0.025", "h_0.01"])
+def test_numbering_matches_reference_build(oracle, name):
+    """The library's P1/P2 element arrays and node coordinates equal the
+    reference's own createMesh (built from its sources into oracle/_ref/)."""
+    tri = oracle.parse_gmsh(mesh_path(name))
+    ref2 = oracle.ref_create_mesh(tri, 2)
+    if ref2 is None:
+        pytest.skip("reference build oracle/_ref/libref_mesh.so absent (needs the reference sources)")
+    ref1 = oracle.ref_create_mesh(tri, 1)
+    p = _product(name)
+    xy, el = p.velocity_mesh()
+    assert np.array_equal(el, ref2["elements"])
+    assert np.array_equal(xy.astype(np.float32), ref2["xy"])
+    xy1, el1 = p.pressure_mesh()
+    assert np.array_equal(el1, ref1["elements"])
+    assert np.array_equal(xy1.astype(np.float32), ref1["xy"])

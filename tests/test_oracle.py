@@ -306,3 +306,78 @@ def test_pcg_on_consistent_singular_system(oracle, h025):
     assert rel <= 1e-10
     xp = np.linalg.lstsq(m.toarray(), b, rcond=None)[0]
     assert np.linalg.norm((x - x.mean()) - (xp - xp.mean())) <= 1e-7 * np.linalg.norm(xp)
+
+
+@pytest.mark.parametrize("name", ["c3", "h_0.025", "h_0.01"])
+def test_create_mesh_matches_reference_build(oracle, name):
+    """P1/P2 numbering, node coordinates and border elements of the oracle
+    equal the reference's own createMesh (compiled from its sources into
+    oracle/_ref/) bit for bit."""
+    tri = oracle.parse_gmsh(mesh_path(name))
+    for order in (1, 2):
+        ref = oracle.ref_create_mesh(tri, order)
+        if ref is None:
+            pytest.skip("reference build oracle/_ref/libref_mesh.so absent (needs the reference sources)")
+        mine = oracle.create_mesh(tri, order)
+        assert np.array_equal(mine["elements"], ref["elements"])
+        assert np.array_equal(mine["border"], ref["border"])
+        # the reference computes mid-edge nodes in float: the oracle's exact
+        # double midpoints round to them
+        assert np.array_equal(mine["xy"].astype(np.float32), ref["xy"])
+
+
+def _divergence_ratios(oracle, ctx, measure_vpd, steps, tau=1e-3):
+    orc = oracle.ChorinOracle(ctx, tau)
+    ip = ctx["int_p"]
+    out = []
+    for _ in range(steps):
+        u, _ = orc.step()
+        out.append(np.linalg.norm((measure_vpd @ u)[ip]) / np.linalg.norm((measure_vpd @ orc.u_tentative)[ip]))
+    return out
+
+
+@pytest.mark.parametrize("name", ["c3", "h_0.025"])
+def test_projection_reduces_divergence(oracle, name):
+    """The pressure correction must remove most of the tentative velocity's
+    discrete divergence on the internal pressure nodes (the P1 pressure
+    Laplacian is an approximate projection: about 0.25-0.4 of it remains).
+    The same measure on copies of the oracle with a mistake planted in its
+    operators shows that the check discriminates: the correction's sign
+    flipped makes the divergence grow (~1.9x), a pressure mis-scaled by tau
+    leaves it unchanged (~1.0)."""
+    ctx = oracle.build_context(oracle.parse_gmsh(mesh_path(name)), 1e-3, 1.0)
+    vpd = ctx["VPD"]
+    ok = _divergence_ratios(oracle, ctx, vpd, 4)
+    assert max(ok) <= 0.5, ok
+    flipped = dict(ctx)
+    flipped["PVD"] = -ctx["PVD"]
+    assert min(_divergence_ratios(oracle, flipped, vpd, 2)) > 1.5
+    misscaled = dict(ctx)
+    misscaled["VPD"] = ctx["VPD"] * 1e-3  # right-hand side -(1/tau) VPD u scaled by tau
+    assert min(_divergence_ratios(oracle, misscaled, vpd, 2)) > 0.9
+
+
+def test_tentative_step_closed_form_on_linear_field(oracle, octx_h025):
+    """For u = (x, -y) and nu = 0 the convective acceleration (u . grad) u =
+    (x, y) lies in P2 and its Galerkin integrand (degree 3) is exact under
+    the degree-4 rule, so M^-1 C(u) u is its nodal interpolant and the
+    tentative velocity is u - tau (x, y) at every node off the Dirichlet
+    boundary.  Pins the convection's sign, the tau scaling and the planar
+    [ux; uy] layout of the step."""
+    ctx = dict(octx_h025)
+    ctx["viscosity"] = 0.0
+    N = ctx["N"]
+    xy = ctx["vm"]["xy"]
+    ux, uy = xy[:, 0].copy(), -xy[:, 1]
+    ctx["dvx"] = (ctx["dvx"][0], ux[ctx["dvx"][0]])
+    ctx["dvy"] = (ctx["dvy"][0], uy[ctx["dvy"][0]])
+    tau = 1e-3
+    orc = oracle.ChorinOracle(ctx, tau, vel=("pcg", 2000, 1e-14, 1))
+    orc.u[:] = np.concatenate([ux, uy])
+    orc.step()
+    ut = orc.u_tentative
+    free = np.ones(N, bool)
+    free[ctx["dvx"][0]] = False
+    assert free.sum() > 0.5 * N
+    assert np.max(np.abs(ut[:N] - (ux - tau * xy[:, 0]))[free]) <= 1e-12
+    assert np.max(np.abs(ut[N:] - (uy - tau * xy[:, 1]))[free]) <= 1e-12
