@@ -7,8 +7,9 @@ Workload: seeded channel 5.4 x 1 with a hole (r = 0.12 at (0.5, 0.49)),
 Re = 100 with a parabolic inflow of peak 1.5, convection reassembled every
 step into the fixed CSR pattern, a seeded 1e-3 perturbation of the
 inflow-initialised field.  The requested tau = 5e-4 violates the explicit
-scheme's diffusion bound (tau nu lambda_max(M^-1 K) <= 2) at this mesh size,
-so tau is clamped to 1 / (nu lambda_max); both values are reported.
+scheme's diffusion bound (tau nu lambda_max(M^-1 K) <= 2) at this mesh size:
+the library's time-step guard (clamp policy) runs it at 1 / lambda_max(M^-1
+nu K); both values are reported.
 
 Prints one JSON line.  Progress goes to stderr."""
 
@@ -63,6 +64,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     from nsfem_amd import _lib, fem
     ch = make(cfg) if make else fem.Chorin(ctx, cfg)
     ch.set_time_step(tau)
+    tau_info = ch.time_step_info()
     t0 = time.perf_counter()
     shape = ch.autotune(10) if autotune else ch.shape()
     tune_s = time.perf_counter() - t0 if autotune else 0.0
@@ -86,6 +88,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
            "iters": {"tentative": agg["itersTent"], "pressure": agg["itersPressure"], "final": agg["itersFinal"]},
            "outer": {"tentative": agg["outerTent"], "pressure": agg["outerPressure"], "final": agg["outerFinal"]},
            "max_relres": max(max(s["relTent"], s["relPressure"], s["relFinal"]) for s in stats),
+           "time_step": tau_info,
            "ms": {"convection": agg["msConvection"], "tentative": agg["msTentative"], "pressure": agg["msPressure"],
                   "final": agg["msFinal"]},
            "shape": {"velocity_lanes": shape[0], "velocity_rows_per_block": shape[1], "pressure_lanes": shape[2],
@@ -211,8 +214,6 @@ def run_batch(args):
     from nsfem_amd import F64, fem
     t_setup = time.perf_counter()
     ctx, info = build_batch(args.batch, args.seed)
-    lam = fem.estimate_lambda_max(ctx)  # the batch K carries each instance's viscosity
-    tau = min(args.tau, 1.0 / lam)
     rng = np.random.default_rng(1)
     N = ctx.numVelocityNodes
     u0 = rng.uniform(-1, 1, 2 * N) * 1e-3
@@ -223,8 +224,10 @@ def run_batch(args):
     target = 1e-10
     res = {}
     for name, method in (("fp64", "pcg"), ("mixed", "pcg_mixed")):
-        cfg = fem.ChorinConfig(fem.SolverConfig(method, 5000, target, 1), fem.SolverConfig(method, 50000, target, 1), dtype=F64)
-        ch, r, _, _ = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps, not args.no_autotune)
+        cfg = fem.ChorinConfig(fem.SolverConfig(method, 5000, target, 1), fem.SolverConfig(method, 50000, target, 1), dtype=F64,
+                               timeStepPolicy=fem.TAU_CLAMP)
+        # the guard clamps tau to the batch's limit (the tightest instance's)
+        ch, r, _, _ = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune)
         r["kernels"] = {} if args.no_kernels else kernel_table(ch, r, args.kernel_reps)
         del ch
         r["instance_steps_per_s"] = r["steps_per_s"] * args.batch
@@ -238,7 +241,8 @@ def run_batch(args):
         "data": "synthetic: seeded channel meshes and seeded initial perturbation",
         "config": {"workload": f"chorin_step_batch_{args.batch}_uneven", "instances": args.batch,
                    "pressure_dofs_total": ctx.numPressureNodes, "velocity_dofs_per_component_total": N,
-                   "tau_used": tau, "lambda_max_nu": lam, "solver_rtol": target, "seed": args.seed,
+                   "tau_requested": args.tau, "tau_used": res["mixed"]["time_step"]["tauUsed"],
+                   "lambda_max_nu": res["mixed"]["time_step"]["lambdaNu"], "solver_rtol": target, "seed": args.seed,
                    "instances_detail": info},
         "fp64": res["fp64"], "mixed": res["mixed"],
         "mixed_over_fp64": res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], "setup_s": setup_s,
@@ -506,8 +510,8 @@ def main():
                 fem.SolverConfig(method, 50000, target, 1, pl, pr, psl, psr))
 
     eager = 1 if args.eager else 0
-    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager)
-    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager)
+    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP)
+    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP)
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     if args.kernels_only:
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
@@ -520,17 +524,16 @@ def main():
             del ch
         print(json.dumps(out))
         return 0
-    lam = fem.estimate_lambda_max(ctx, graphs=not args.eager)
-    tau_stable = 1.0 / (cond.viscosity * lam)
-    tau = min(args.tau, tau_stable)
-    log(f"lambda_max(M^-1 K) ~ {lam:.4g}; tau requested {args.tau:g}, used {tau:.4g} "
-        f"(tau nu lambda = {tau * cond.viscosity * lam:.3f})")
     setup_s = time.perf_counter() - t_setup
 
     results, kernels, firsts, lasts = {}, {}, {}, {}
     for name in paths:
         cfg = {"fp64": cfg64, "mixed": cfgmx}[name]
-        ch, r, first, last = run_path(name, ctx, cfg, tau, u0, args.warmup, args.steps, not args.no_autotune, make_chorin)
+        # the library's guard clamps the requested tau to 1 / lambda_max(M^-1 nu K)
+        ch, r, first, last = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune, make_chorin)
+        ti = r["time_step"]
+        log(f"{name}: tau requested {ti['tauRequested']:g}, used {ti['tauUsed']:.4g} "
+            f"(lambda_max(M^-1 nu K) = {ti['lambdaNu']:.4g})")
         kernels[name] = {} if args.no_kernels else kernel_table(ch, r)
         results[name], firsts[name], lasts[name] = r, first, last
         del ch
@@ -542,6 +545,9 @@ def main():
     if "fp64" in results and "mixed" in results:
         errors["mixed_vs_fp64_after_steps"] = {"velocity": rel(lasts["mixed"][0], lasts["fp64"][0]),
                                                "pressure": rel(lasts["mixed"][1], lasts["fp64"][1])}
+    head = "mixed" if "mixed" in results else paths[0]
+    tau = results[head]["time_step"]["tauUsed"]
+    lam = results[head]["time_step"]["lambdaNu"] / cond.viscosity
     cpu = None
     if not args.no_cpu:
         cpu_rate, (uo, po) = cpu_baseline(ctx, cond, tau, u0)
@@ -551,7 +557,6 @@ def main():
                "sample": "1 Chorin step of the same 500k-DOF workload from the same initial state: FP64 numpy/scipy "
                          "oracle, PCG-MCGS to 1e-10 for all three solves, single thread"}
     log(f"errors: {errors}")
-    head = "mixed" if "mixed" in results else paths[0]
     rmx, kernmx = results[head], kernels[head]
     r64 = results.get("fp64")
 

@@ -9,29 +9,52 @@
 //   cu::Sparse                 include/cu/sparse.h
 //   cu::AbstractSolver         include/cu/solvers/abstractSolver.h:8-32
 //   cu::createSolver           include/cu/solvers/solverFactory.h:13-14
-//   linalg::CsrMatrix<F>       cpp/libs/LinAlg/include/linalg/csrMatrix.h:13-23 (data members)
+// Host matrices are the caller's own type: anything with the data members of
+// the reference's linalg::CsrMatrix<F> (cpp/libs/LinAlg/include/linalg/
+// csrMatrix.h:13-23: rows, cols, values, column, rowStart) binds, so the
+// header sits next to the reference's linalg headers without redefining them.
 // Handles (Blas, Sparse) carry no state: the library owns one stream.
 #pragma once
 
 #include <cmath>
+#include <concepts>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../nsfem_hip.h"
 
-namespace linalg
+namespace nsfem
 {
+    // A host CSR matrix: int32 rows, cols, 0-based column indices and
+    // rowStart[rows + 1], contiguous values.
+    template <typename M>
+    concept CsrMatrixLike = requires(const M & m) {
+        { m.rows } -> std::convertible_to<int>;
+        { m.cols } -> std::convertible_to<int>;
+        { m.values.data() };
+        { m.values.size() } -> std::convertible_to<size_t>;
+        { m.column.data() } -> std::convertible_to<const int *>;
+        { m.rowStart.data() } -> std::convertible_to<const int *>;
+    };
+
+    template <CsrMatrixLike M>
+    using CsrValueType = std::remove_cvref_t<decltype(*std::declval<const M &>().values.data())>;
+
+    // The reference's binary CSR file (cpp/libs/LinAlg/src/io.cpp:49-84) as
+    // read by include/nsfem/linalg_io.hpp; a CsrMatrixLike.
     template <typename F>
-    struct CsrMatrix
+    struct HostCsr
     {
         int rows = 0, cols = 0;
         std::vector<F> values;
         std::vector<int> column;
         std::vector<int> rowStart; // rows + 1
     };
-} // namespace linalg
+} // namespace nsfem
 
 namespace cu
 {
@@ -177,8 +200,14 @@ namespace cu
         nsfem_csr * h = nullptr;
         vec<F> values; // view of the device values, mutable in place
 
-        explicit csrT(const linalg::CsrMatrix<F> & m) : rows(m.rows), cols(m.cols)
+        template <nsfem::CsrMatrixLike M>
+            requires std::same_as<nsfem::CsrValueType<M>, F>
+        explicit csrT(const M & m) : rows(m.rows), cols(m.cols)
         {
+            if (m.rowStart.size() != static_cast<size_t>(m.rows) + 1 || m.column.size() != m.values.size())
+            {
+                throw std::invalid_argument("CSR matrix: rowStart must have rows + 1 entries and column one per value");
+            }
             check(nsfem_csr_create(m.rows, m.cols, static_cast<int>(m.values.size()), m.rowStart.data(), m.column.data(), m.values.data(),
                                    dtypeOf<F>(), &h));
             void * p = nullptr;
@@ -284,23 +313,23 @@ namespace cu
         void setMseCheckInterval(const int k) override { check(nsfem_solver_set_mse_check_interval(h, k)); }
     };
 
-    // name: "gs", "pcg" ("dss" is rejected, as is an unknown name).
-    inline std::unique_ptr<AbstractSolver> createSolver(const std::string & name, const int numCh, const linalg::CsrMatrix<float> & m,
-                                                        const int maxIters, const float targetMse, const int mseCheckInterval)
+    // name: "gs", "pcg" ("dss" is rejected, as is an unknown name) for FP32
+    // matrices (the reference's), "gs", "pcg", "pcg_mixed" for FP64 ones; the
+    // solver's io vectors have the matrix's value type.
+    template <nsfem::CsrMatrixLike M>
+        requires std::same_as<nsfem::CsrValueType<M>, float> || std::same_as<nsfem::CsrValueType<M>, double>
+    std::unique_ptr<AbstractSolverT<nsfem::CsrValueType<M>>> createSolver(const std::string & name, const int numCh, const M & m,
+                                                                          const int maxIters, const nsfem::CsrValueType<M> targetMse,
+                                                                          const int mseCheckInterval)
     {
+        using F = nsfem::CsrValueType<M>;
+        if (m.rowStart.size() != static_cast<size_t>(m.rows) + 1 || m.column.size() != m.values.size())
+        {
+            throw std::invalid_argument("CSR matrix: rowStart must have rows + 1 entries and column one per value");
+        }
         nsfem_solver * h = nullptr;
         check(nsfem_solver_create(name.c_str(), numCh, m.rows, static_cast<int>(m.values.size()), m.rowStart.data(), m.column.data(),
-                                  m.values.data(), NSFEM_F32, maxIters, targetMse, mseCheckInterval, &h));
-        return std::make_unique<HipSolver<float>>(h, numCh, static_cast<size_t>(m.rows));
-    }
-
-    // FP64 solvers: "gs", "pcg", "pcg_mixed".
-    inline std::unique_ptr<AbstractSolverT<double>> createSolver(const std::string & name, const int numCh, const linalg::CsrMatrix<double> & m,
-                                                                 const int maxIters, const double targetMse, const int mseCheckInterval)
-    {
-        nsfem_solver * h = nullptr;
-        check(nsfem_solver_create(name.c_str(), numCh, m.rows, static_cast<int>(m.values.size()), m.rowStart.data(), m.column.data(),
-                                  m.values.data(), NSFEM_F64, maxIters, targetMse, mseCheckInterval, &h));
-        return std::make_unique<HipSolver<double>>(h, numCh, static_cast<size_t>(m.rows));
+                                  m.values.data(), dtypeOf<F>(), maxIters, targetMse, mseCheckInterval, &h));
+        return std::make_unique<HipSolver<F>>(h, numCh, static_cast<size_t>(m.rows));
     }
 } // namespace cu

@@ -1,6 +1,8 @@
 // The reference's binary vector / CSR format (cpp/libs/LinAlg/src/io.cpp:49-148):
 // int32 dtypeId (0 = f32, 1 = f64), then for a vector int32 n and the data,
-// for a CSR int32 rows, cols, nnz, then values, column, rowStart.
+// for a CSR int32 rows, cols, nnz, then values, column, rowStart.  In
+// namespace nsfem, so a translation unit may also include the reference's own
+// linalg/io.h.
 #pragma once
 
 #include <cstdint>
@@ -9,13 +11,15 @@
 #include <string>
 #include <vector>
 
-namespace linalg
+#include "cu.hpp"
+
+namespace nsfem
 {
     template <typename F>
     constexpr int32_t dtypeId() { return sizeof(F) == 8 ? 1 : 0; }
 
     template <typename F>
-    void write(const std::string & fileName, const std::vector<F> & v)
+    void writeVec(const std::string & fileName, const std::vector<F> & v)
     {
         std::ofstream f(fileName, std::ios::binary);
         if (!f.is_open())
@@ -54,15 +58,7 @@ namespace linalg
         return v;
     }
     template <typename F>
-    struct CsrFile
-    {
-        int32_t rows = 0, cols = 0;
-        std::vector<F> values;
-        std::vector<int32_t> column, rowStart;
-    };
-
-    template <typename F>
-    CsrFile<F> readCsr(const std::string & fileName)
+    HostCsr<F> readCsr(const std::string & fileName)
     {
         std::ifstream f(fileName, std::ios::binary);
         if (!f.is_open())
@@ -70,7 +66,7 @@ namespace linalg
             throw std::runtime_error("Failed to open input file " + fileName);
         }
         int32_t id = -1, nnz = 0;
-        CsrFile<F> m;
+        HostCsr<F> m;
         if (!f.read(reinterpret_cast<char *>(&id), 4) || id != dtypeId<F>())
         {
             throw std::runtime_error("Failed to read CSR matrix: Bad dtype");
@@ -99,4 +95,4 @@ namespace linalg
         }
         return m;
     }
-} // namespace linalg
+} // namespace nsfem

@@ -317,7 +317,19 @@ typedef struct nsfem_chorin_config
     int eagerLaunches; /* 1: launch every solver iteration from the host instead
                         * of replaying captured graphs (for tools that cannot
                         * trace graph launches); 0: graphs */
+    int timeStepPolicy; /* NSFEM_TAU_REJECT (0, default), NSFEM_TAU_CLAMP or
+                         * NSFEM_TAU_UNCHECKED; see nsfem_chorin_set_time_step */
 } nsfem_chorin_config;
+
+/* Time-step policies.  The step is explicit (forward Euler) in diffusion and
+ * convection (chorinCuda.cu:263-278) and stays bounded only while
+ * tau nu lambda_max(M^-1 K) <= 2; the limit enforced is half of that,
+ * tau_max = 1 / lambda_max(M^-1 nu K), estimated once per Chorin object by
+ * power iteration on the GPU (the largest over the instances of a batch).
+ * The reference checks nothing (NSFEM_TAU_UNCHECKED). */
+#define NSFEM_TAU_REJECT 0
+#define NSFEM_TAU_CLAMP 1
+#define NSFEM_TAU_UNCHECKED 2
 
 typedef struct nsfem_step_stats
 {
@@ -326,12 +338,22 @@ typedef struct nsfem_step_stats
     double msConvection, msTentative, msPressure, msFinal, msTotal;
     int itersTent, itersPressure, itersFinal;
     int outerTent, outerPressure, outerFinal;
+    double tauRequested, tauUsed; /* the step asked for and the step taken */
 } nsfem_step_stats;
 
 typedef struct nsfem_chorin nsfem_chorin;
 int nsfem_chorin_create(const nsfem_context * c, const nsfem_chorin_config * cfg, nsfem_chorin ** out);
 int nsfem_chorin_destroy(nsfem_chorin * ch);
+/* Replaces the fixed tau = maxT / ceil(maxT / tau0) of chorinCuda.cu:221-222.
+ * Under NSFEM_TAU_REJECT a tau above tau_max fails with NSFEM_EINVAL (the
+ * state is kept; the previous step stays in force); NSFEM_TAU_CLAMP uses
+ * tau_max instead; NSFEM_TAU_UNCHECKED takes any tau > 0.  A step taken
+ * without a set_time_step call uses 1e-3 under the same policy. */
 int nsfem_chorin_set_time_step(nsfem_chorin * ch, double tau);
+/* The last requested and the used time step, lambda_max(M^-1 nu K) and
+ * tau_max = 1 / lambda_max (any pointer may be NULL; asking for lambdaNu or
+ * tauMax runs the estimate if it has not run).  No reference counterpart. */
+int nsfem_chorin_time_step_info(nsfem_chorin * ch, double * tauRequested, double * tauUsed, double * lambdaNu, double * tauMax);
 /* One projection step; stats may be NULL.  NSFEM_ENONFINITE on a bad
  * velocity solve ("Tentative/Final acceleration is bad", chorinCuda.cu:270-273, 349-352). */
 int nsfem_chorin_step(nsfem_chorin * ch, nsfem_step_stats * stats);

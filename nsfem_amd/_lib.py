@@ -18,6 +18,9 @@ NSFEM_ENONFINITE = 4
 
 F32 = 0
 F64 = 1
+TAU_REJECT = 0     # time-step policies (nsfem_chorin_config.timeStepPolicy)
+TAU_CLAMP = 1
+TAU_UNCHECKED = 2
 NSFEM_CONTEXT_GPU_ASSEMBLY = 1
 
 MAT_VELOCITY_MASS = 0
@@ -65,14 +68,15 @@ class SolverConfig(ctypes.Structure):
 
 class ChorinConfigC(ctypes.Structure):
     _fields_ = [("velocitySolver", SolverConfig), ("pressureSolver", SolverConfig), ("dtype", ctypes.c_int),
-                ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int)]
+                ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int), ("timeStepPolicy", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "mseTentX", "mseTentY", "msePressure", "mseFinalX", "mseFinalY", "relTent", "relPressure", "relFinal",
         "msConvection", "msTentative", "msPressure", "msFinal", "msTotal")] + \
-        [(n, ctypes.c_int) for n in ("itersTent", "itersPressure", "itersFinal", "outerTent", "outerPressure", "outerFinal")]
+        [(n, ctypes.c_int) for n in ("itersTent", "itersPressure", "itersFinal", "outerTent", "outerPressure", "outerFinal")] + \
+        [(n, ctypes.c_double) for n in ("tauRequested", "tauUsed")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -160,6 +164,7 @@ _SIGNATURES = {
     "nsfem_chorin_create": [_vp, ctypes.POINTER(ChorinConfigC), ctypes.POINTER(_vp)],
     "nsfem_chorin_destroy": [_vp],
     "nsfem_chorin_set_time_step": [_vp, ctypes.c_double],
+    "nsfem_chorin_time_step_info": [_vp, _c_double_p, _c_double_p, _c_double_p, _c_double_p],
     "nsfem_chorin_step": [_vp, ctypes.POINTER(StepStats)],
     "nsfem_chorin_state": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "nsfem_chorin_download": [_vp, _vp, _vp],

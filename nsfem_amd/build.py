@@ -75,7 +75,7 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     # C++ host tools over the C ABI (the reference's host language)
     for app in APPS:
         out = os.path.join(HERE, app)
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", out + ".tmp", os.path.join(CSRC, "app", app + ".cpp"),
+        _run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-o", out + ".tmp", os.path.join(CSRC, "app", app + ".cpp"),
               "-L" + HERE, "-lnsfem_hip", "-Wl,-rpath,$ORIGIN"], verbose)
         os.replace(out + ".tmp", out)
     return LIB

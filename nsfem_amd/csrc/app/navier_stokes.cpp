@@ -104,10 +104,43 @@ int main(int argc, char ** argv)
                              static_cast<int>(cfg.num("chorinCuda.pressureSolver.mseCheckInterval", 1))};
         cc.dtype = f32 ? NSFEM_F32 : NSFEM_F64;
         cc.innerTol = cfg.num("hip.innerTol", 1e-4);
+        // what a time step past the explicit scheme's stability limit does
+        const std::string policy = cfg.str("hip.timeStepPolicy", "reject");
+        if (policy == "reject")
+        {
+            cc.timeStepPolicy = NSFEM_TAU_REJECT;
+        }
+        else if (policy == "clamp")
+        {
+            cc.timeStepPolicy = NSFEM_TAU_CLAMP;
+        }
+        else if (policy == "unchecked")
+        {
+            cc.timeStepPolicy = NSFEM_TAU_UNCHECKED;
+        }
+        else
+        {
+            std::cerr << "hip.timeStepPolicy must be reject, clamp or unchecked\n";
+            return 1;
+        }
         nsfem_chorin * ch = nullptr;
         cu::check(nsfem_chorin_create(ctx, &cc, &ch));
 
-        const double maxT = cfg.num("maxT", 1.0), tau0 = cfg.num("tau", 1e-4);
+        // tau = maxT / ceil(maxT / tau0) (chorinCuda.cu:221-222); under the
+        // clamp policy tau0 is first lowered to the stability limit, so the
+        // run still ends at maxT
+        const double maxT = cfg.num("maxT", 1.0);
+        double tau0 = cfg.num("tau", 1e-4);
+        if (cc.timeStepPolicy == NSFEM_TAU_CLAMP)
+        {
+            double tauMax = 0;
+            cu::check(nsfem_chorin_time_step_info(ch, nullptr, nullptr, nullptr, &tauMax));
+            if (tau0 > tauMax)
+            {
+                std::printf("time step %g is past the stability limit %g: clamped\n", tau0, tauMax);
+                tau0 = tauMax;
+            }
+        }
         const int numTimeSteps = static_cast<int>(std::ceil(maxT / tau0));
         const double tau = maxT / numTimeSteps;
         cu::check(nsfem_chorin_set_time_step(ch, tau));
@@ -132,14 +165,14 @@ int main(int argc, char ** argv)
                 if (f32)
                 {
                     cu::check(nsfem_chorin_download(ch, uf.data(), pf.data()));
-                    linalg::write(dir + "/velocity_" + std::to_string(frame) + ".bin", uf);
-                    linalg::write(dir + "/pressure_" + std::to_string(frame) + ".bin", pf);
+                    nsfem::writeVec(dir + "/velocity_" + std::to_string(frame) + ".bin", uf);
+                    nsfem::writeVec(dir + "/pressure_" + std::to_string(frame) + ".bin", pf);
                 }
                 else
                 {
                     cu::check(nsfem_chorin_download(ch, u.data(), p.data()));
-                    linalg::write(dir + "/velocity_" + std::to_string(frame) + ".bin", u);
-                    linalg::write(dir + "/pressure_" + std::to_string(frame) + ".bin", p);
+                    nsfem::writeVec(dir + "/velocity_" + std::to_string(frame) + ".bin", u);
+                    nsfem::writeVec(dir + "/pressure_" + std::to_string(frame) + ".bin", p);
                 }
                 frame++;
             }

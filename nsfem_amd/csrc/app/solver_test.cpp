@@ -24,16 +24,10 @@ int main(int argc, char ** argv)
     try
     {
         const std::string algo = argv[1];
-        const auto f = linalg::readCsr<float>(argv[2]);
-        const auto rhs = linalg::readVec<float>(argv[3]);
+        const nsfem::HostCsr<float> m = nsfem::readCsr<float>(argv[2]);
+        const auto rhs = nsfem::readVec<float>(argv[3]);
         const int maxIters = argc > 4 ? std::stoi(argv[4]) : 100;
         const float target = argc > 5 ? std::stof(argv[5]) : 1e-9f;
-        linalg::CsrMatrix<float> m;
-        m.rows = f.rows;
-        m.cols = f.cols;
-        m.values = f.values;
-        m.column = f.column;
-        m.rowStart = f.rowStart;
         if (m.rows != m.cols)
         {
             std::cerr << "Only square matrices supported for now\n";
@@ -99,7 +93,7 @@ int main(int argc, char ** argv)
                 std::cout << "Final MSE = " << std::sqrt(err / n) << " iterations = " << solver->getLastIterations() << "\n";
             }
         }
-        linalg::write(std::string(argv[3]) + ".sol.bin", x);
+        nsfem::writeVec(std::string(argv[3]) + ".sol.bin", x);
     }
     catch (const std::exception & e)
     {

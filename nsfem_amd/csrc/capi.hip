@@ -1111,6 +1111,11 @@ extern "C"
             cc.dtype = cfg->dtype;
             cc.innerTol = cfg->innerTol > 0 ? cfg->innerTol : 1e-4;
             cc.graphs = cfg->eagerLaunches == 0;
+            if (cfg->timeStepPolicy < NSFEM_TAU_REJECT || cfg->timeStepPolicy > NSFEM_TAU_UNCHECKED)
+            {
+                throw std::invalid_argument("timeStepPolicy must be NSFEM_TAU_REJECT, NSFEM_TAU_CLAMP or NSFEM_TAU_UNCHECKED");
+            }
+            cc.timeStepPolicy = static_cast<TimeStepPolicy>(cfg->timeStepPolicy);
             auto h = std::make_unique<nsfem_chorin>();
             h->core = std::make_unique<ChorinGpu>(c->ctx, cc);
             *out = h.release(); });
@@ -1131,6 +1136,17 @@ extern "C"
             need(ch, "chorin");
             if (!(tau > 0)) throw std::invalid_argument("time step must be positive");
             ch->core->setTimeStep(tau); });
+    }
+
+    int nsfem_chorin_time_step_info(nsfem_chorin * ch, double * tauRequested, double * tauUsed, double * lambdaNu, double * tauMax)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            if (tauRequested) *tauRequested = ch->core->requestedTimeStep();
+            if (tauUsed) *tauUsed = ch->core->timeStep();
+            if (lambdaNu) *lambdaNu = ch->core->diffusionLambda();
+            if (tauMax) *tauMax = ch->core->stableTimeStep(); });
     }
 
     int nsfem_chorin_step(nsfem_chorin * ch, nsfem_step_stats * stats)
@@ -1161,6 +1177,8 @@ extern "C"
                 stats->outerTent = s.outerTent;
                 stats->outerPressure = s.outerPressure;
                 stats->outerFinal = s.outerFinal;
+                stats->tauRequested = s.tauRequested;
+                stats->tauUsed = s.tauUsed;
             } });
     }
 

@@ -2,6 +2,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 
@@ -293,6 +295,108 @@ namespace nsfem
         }
     }
 
+    double ChorinGpu::diffusionLambda()
+    {
+        if (lambdaNu_ > 0)
+        {
+            return lambdaNu_;
+        }
+        const hipStream_t s = libraryStream();
+        const int dt = cfg_.dtype;
+        const size_t n2 = 2 * static_cast<size_t>(N_);
+        // two independent seeded starts, one per solver channel
+        std::vector<double> h(n2);
+        uint64_t st = 0x9E3779B97F4A7C15ull;
+        for (auto & v : h)
+        {
+            st ^= st << 13;
+            st ^= st >> 7;
+            st ^= st << 17;
+            v = static_cast<double>(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        }
+        DeviceArray<char> x(n2 * tsize_), w(n2 * tsize_);
+        if (dt == F64)
+        {
+            NSFEM_HIP(hipMemcpyAsync(x.get(), h.data(), n2 * 8, hipMemcpyHostToDevice, s));
+        }
+        else
+        {
+            std::vector<float> f(h.begin(), h.end());
+            NSFEM_HIP(hipMemcpyAsync(x.get(), f.data(), n2 * 4, hipMemcpyHostToDevice, s));
+        }
+        const CsrView K = nuK_.view(dt);
+        void * rhs = vSolver_->ioRhs();
+        void * y = vSolver_->ioSol();
+        double lam = 0;
+        for (int it = 0; it < std::max(1, cfg_.lambdaIterations); it++)
+        {
+            // x is scaled to unit RMS per channel, so absolute ("gs") and
+            // relative ("pcg") solver targets both apply
+            double nrm[2];
+            dot(dt, N_, 2, x.get(), N_, x.get(), N_, nrm, s);
+            for (int c = 0; c < 2; c++)
+            {
+                if (!(nrm[c] > 0) || !std::isfinite(nrm[c]))
+                {
+                    throw NonFiniteError("lambda_max estimate: the power iteration vector vanished");
+                }
+                scale(dt, N_, std::sqrt(N_ / nrm[c]), static_cast<char *>(x.get()) + c * tsize_ * N_, s);
+            }
+            csrMul(K, 2, x.get(), N_, rhs, N_, 1.0, 0.0, nullptr, N_, 0, s);
+            vSolver_->solve(true);
+            csrMul(K, 2, y, N_, w.get(), N_, 1.0, 0.0, nullptr, N_, 0, s);
+            double yKy[2], yMy[2];
+            dot(dt, N_, 2, y, N_, w.get(), N_, yKy, s);
+            dot(dt, N_, 2, y, N_, rhs, N_, yMy, s); // M y = nu K x
+            lam = 0;
+            for (int c = 0; c < 2; c++)
+            {
+                if (yMy[c] > 0)
+                {
+                    lam = std::max(lam, yKy[c] / yMy[c]);
+                }
+            }
+            NSFEM_HIP(hipMemcpyAsync(x.get(), y, n2 * tsize_, hipMemcpyDeviceToDevice, s));
+        }
+        syncLibraryStream();
+        if (!(lam > 0) || !std::isfinite(lam))
+        {
+            throw NonFiniteError("lambda_max estimate failed (non-positive or non-finite)");
+        }
+        lambdaNu_ = lam;
+        return lambdaNu_;
+    }
+
+    void ChorinGpu::setTimeStep(double tau)
+    {
+        if (!(tau > 0) || !std::isfinite(tau))
+        {
+            throw std::invalid_argument("time step must be positive");
+        }
+        double used = tau;
+        if (cfg_.timeStepPolicy != TimeStepPolicy::Unchecked)
+        {
+            const double tmax = stableTimeStep();
+            if (tau > tmax)
+            {
+                if (cfg_.timeStepPolicy != TimeStepPolicy::Clamp)
+                {
+                    // rejected: the previous step stays in force
+                    char msg[320];
+                    std::snprintf(msg, sizeof msg,
+                                  "time step %.6g exceeds the explicit scheme's stability limit %.6g (tau nu lambda_max(M^-1 K) = "
+                                  "%.3f > 1); use a smaller step or the clamp policy",
+                                  tau, tmax, tau * diffusionLambda());
+                    throw std::invalid_argument(msg);
+                }
+                used = tmax;
+            }
+        }
+        tauRequested_ = tau;
+        tau_ = used;
+        tauChecked_ = true;
+    }
+
     void ChorinGpu::imposeDirichlet(hipStream_t s) { scatter(cfg_.dtype, nDir_, dirIds_.get(), dirValues_.get(), u_.get(), s); }
 
     void ChorinGpu::resetState()
@@ -428,6 +532,10 @@ namespace nsfem
 
     void ChorinGpu::step(ChorinStepStats * stats)
     {
+        if (!tauChecked_)
+        {
+            setTimeStep(tau_); // the default step goes through the policy too
+        }
         const hipStream_t s = libraryStream();
         const int dt = cfg_.dtype;
         const long twoN = 2L * N_;
@@ -491,6 +599,8 @@ namespace nsfem
         st.msPressure = ms[2];
         st.msFinal = ms[3];
         st.msTotal = ms[0] + ms[1] + ms[2] + ms[3];
+        st.tauRequested = tauRequested_;
+        st.tauUsed = tau_;
         if (stats)
         {
             *stats = st;

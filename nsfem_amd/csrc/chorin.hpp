@@ -37,6 +37,18 @@ namespace nsfem
         int sweepRowsPerBlock = 0;
     };
 
+    // What a time step beyond the explicit scheme's stability limit does.
+    // The step is forward Euler in diffusion and convection
+    // (chorinCuda.cu:263-278): it stays bounded only while tau nu
+    // lambda_max(M^-1 K) <= 2.  The limit used here is half that,
+    // tau_max = 1 / lambda_max(M^-1 nu K), leaving room for convection.
+    enum class TimeStepPolicy : int
+    {
+        Reject = 0,   // an unstable tau is an error (std::invalid_argument)
+        Clamp = 1,    // an unstable tau is replaced by tau_max
+        Unchecked = 2 // any tau runs (the reference's behaviour)
+    };
+
     struct ChorinConfig
     {
         ChorinSolverConfig velocity;
@@ -44,6 +56,8 @@ namespace nsfem
         int dtype = F64;        // state and matrix precision (F32 = the reference's precision)
         double innerTol = 1e-4; // pcg_mixed inner tolerance
         bool graphs = true;     // solver iterations replayed as captured graphs
+        TimeStepPolicy timeStepPolicy = TimeStepPolicy::Reject;
+        int lambdaIterations = 40; // power iterations of the lambda_max estimate
     };
 
     struct ChorinStepStats
@@ -53,6 +67,7 @@ namespace nsfem
         int itersTent, itersPressure, itersFinal;
         int outerTent, outerPressure, outerFinal;
         double msConvection, msTentative, msPressure, msFinal, msTotal;
+        double tauRequested, tauUsed;
     };
 
     class ChorinGpu
@@ -61,8 +76,19 @@ namespace nsfem
         ChorinGpu(const ChorinContext & ctx, const ChorinConfig & cfg);
         ~ChorinGpu();
 
-        void setTimeStep(double tau) { tau_ = tau; }
+        // Sets the time step under the configured policy: the requested value,
+        // or tau_max (Clamp), or an std::invalid_argument (Reject, the
+        // previous step stays in force) when it exceeds the stability limit.
+        void setTimeStep(double tau);
         double timeStep() const { return tau_; }
+        double requestedTimeStep() const { return tauRequested_; }
+        // lambda_max(M^-1 nu K) (the largest over the instances of a batch),
+        // estimated on first use by power iteration with the step's velocity
+        // mass solver: y = M^-1 nu K x, Rayleigh quotient y.(nu K y) / y.(M y)
+        // with M y = nu K x, x = y normalised.  Overwrites the velocity
+        // solver's io vectors, not the state.
+        double diffusionLambda();
+        double stableTimeStep() { return 1.0 / diffusionLambda(); }
         void step(ChorinStepStats * stats);
 
         // Convection assembly alone: A = nu K + C(u) (and C if cOut != nullptr).
@@ -105,6 +131,9 @@ namespace nsfem
         ChorinConfig cfg_;
         int N_ = 0, V_ = 0, E_ = 0;
         double tau_ = 1e-3;
+        double tauRequested_ = 1e-3;
+        bool tauChecked_ = false;
+        double lambdaNu_ = -1;
         double nu_ = 0;
         size_t tsize_ = 8;
 

@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from . import _lib
-from ._lib import F32, F64, call
+from ._lib import F32, F64, TAU_CLAMP, TAU_REJECT, TAU_UNCHECKED, call  # noqa: F401
 
 _NP = {F32: np.float32, F64: np.float64}
 
@@ -242,6 +242,7 @@ class ChorinConfig:
     dtype: int = F64
     innerTol: float = 1e-4
     eagerLaunches: int = 0  # 1: no graph replay of solver iterations (for tracing tools)
+    timeStepPolicy: int = 0  # TAU_REJECT: an unstable step is an error; TAU_CLAMP, TAU_UNCHECKED
 
 
 class Chorin:
@@ -257,7 +258,7 @@ class Chorin:
                                                  v.rowsPerBlock, v.sweepLanes, v.sweepRowsPerBlock),
                                _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
                                                  p.rowsPerBlock, p.sweepLanes, p.sweepRowsPerBlock),
-                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches)
+                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy)
         self._h = ctypes.c_void_p()
         call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
         self.dtype = _NP[cfg.dtype]
@@ -265,7 +266,16 @@ class Chorin:
         self.V = ctx.numPressureNodes
 
     def set_time_step(self, tau):
+        """Under TAU_REJECT a step above the stability limit raises
+        NsfemError (the previous step stays); TAU_CLAMP takes the limit."""
         call("nsfem_chorin_set_time_step", self._h, float(tau))
+
+    def time_step_info(self):
+        """dict(tauRequested, tauUsed, lambdaNu = lambda_max(M^-1 nu K),
+        tauMax = 1 / lambdaNu); runs the estimate on first use."""
+        v = [ctypes.c_double() for _ in range(4)]
+        call("nsfem_chorin_time_step_info", self._h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("tauRequested", "tauUsed", "lambdaNu", "tauMax"), (x.value for x in v)))
 
     def step(self):
         st = _lib.StepStats()
@@ -362,11 +372,15 @@ class TimeStepSolution:
 def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t: float, cfg: ChorinConfig,
                     frame_step: int = 1, verbose: bool = False):
     """solveNsChorinCuda: numTimeSteps = ceil(maxT / tau0), tau = maxT / numTimeSteps,
-    numTimeSteps + 1 steps from u = Dirichlet values; every frame_step-th state
+    numTimeSteps + 1 steps from u = Dirichlet values (under TAU_CLAMP tau0 is
+    first lowered to the stability limit); every frame_step-th state
     is kept (the frames the reference renders, navierStokes.cpp:109), each
     copied to the host while the following steps run."""
     ctx = ChorinContext(mesh, cond)
     ch = Chorin(ctx, cfg)
+    if cfg.timeStepPolicy == TAU_CLAMP:
+        # lowered to the stability limit first, so the run still ends at max_t
+        tau0 = min(tau0, ch.time_step_info()["tauMax"])
     n_steps = int(math.ceil(max_t / tau0))
     tau = max_t / n_steps
     ch.set_time_step(tau)

@@ -19,9 +19,9 @@ def _ctx(name="c3", visc=1e-3, peak=1.0):
     return fem.ChorinContext(fem.TriangleMesh.load_gmsh(mesh_path(name)), fem.DfgConditions(visc, peak))
 
 
-def _cfg(vel, pres, dtype=1, inner=1e-4):
+def _cfg(vel, pres, dtype=1, inner=1e-4, policy=0):
     from nsfem_amd import fem
-    return fem.ChorinConfig(fem.SolverConfig(*vel), fem.SolverConfig(*pres), dtype=dtype, innerTol=inner)
+    return fem.ChorinConfig(fem.SolverConfig(*vel), fem.SolverConfig(*pres), dtype=dtype, innerTol=inner, timeStepPolicy=policy)
 
 
 def _rel(a, b):
@@ -180,11 +180,12 @@ def test_batched_instances_match_single_runs():
         meshes.append(mesher.to_triangle_mesh(m))
         conds.append(fem.DfgConditions(nu, peak, 1.0, 4.0))
     cfg = _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1))
-    tau = 2e-4
     batch = fem.ChorinContext.batch(meshes, conds)
     assert batch.num_systems() == 3
     vo, po = batch.system_offsets()
     bch = fem.Chorin(batch, cfg)
+    # the batch's limit is the tightest instance's, so every single run accepts it
+    tau = min(2e-4, bch.time_step_info()["tauMax"])
     bch.set_time_step(tau)
     singles = []
     for m, c in zip(meshes, conds):
@@ -208,8 +209,9 @@ def test_batched_instances_match_single_runs():
 
 def test_full_size_channel_step_properties():
     """The 500k-pressure-DOF channel workload (seeded mesh with the cylinder,
-    Re 100, the stable time step 1.937e-5 of that mesh, seeded 1e-3
-    perturbation), two steps from the same state, checked through properties
+    Re 100, the requested 5e-4 clamped by the stability guard to about
+    1.94e-5 on that mesh, seeded 1e-3 perturbation), two steps from the same
+    state, checked through properties
     that hold at any size: two FP64 runs are bit-identical (every reduction has
     a fixed order); the Dirichlet values are imposed exactly; every solve meets
     its 1e-10 relative residual; the mixed path (FP32 preconditioner) agrees
@@ -233,11 +235,16 @@ def test_full_size_channel_step_properties():
     u0[idy + N] = valy
     out = {}
     for method in ("pcg", "pcg_mixed", "pcg"):
-        ch = fem.Chorin(ctx, _cfg((method, 5000, 1e-10, 1), (method, 50000, 1e-10, 1)))
-        ch.set_time_step(1.937088709631264e-05)
+        ch = fem.Chorin(ctx, _cfg((method, 5000, 1e-10, 1), (method, 50000, 1e-10, 1), policy=fem.TAU_CLAMP))
+        # the requested 5e-4 is past the stability limit: the guard clamps it
+        ch.set_time_step(5e-4)
+        info = ch.time_step_info()
+        assert info["tauRequested"] == 5e-4 and info["tauUsed"] == info["tauMax"]
+        assert 1.8e-5 < info["tauUsed"] < 2.1e-5, info
         ch.upload_velocity(u0)
         for _ in range(2):
             st = ch.step()
+            assert st["tauUsed"] == info["tauUsed"] and st["tauRequested"] == 5e-4
             assert st["relTent"] <= 1e-10 and st["relPressure"] <= 1e-10 and st["relFinal"] <= 1e-10, method
         u, p = ch.download()
         assert np.isfinite(u).all() and np.isfinite(p).all()
@@ -255,15 +262,18 @@ def test_full_size_channel_step_properties():
 
 def test_cpu_sized_channel_workload_matches_oracle(oracle):
     """The CPU-sized channel workload (seeded mesh with the cylinder, about
-    5k P1 / 20k P2 DOFs, peak inflow 1.5, Re 20, dt 1e-3, up to 200 steps).
-    At this time step the explicit scheme is past its stability bound on this
-    mesh: the oracle's velocity settles to about 1.85 by step 40, then grows
-    (3 at step 50, 21 at step 60) and overflows near step 74.  The FP64 GPU
-    path follows the oracle step by step to 1e-10 relative (both solving to
-    1e-13) through the first 40 steps, the mixed path to its own bounds over
-    10 steps, and the GPU run stops with the reference's non-finite error
-    ("Tentative acceleration is bad") before step 200, at a step where the
-    oracle's own state has already grown past 1e20."""
+    5k P1 / 20k P2 DOFs, peak inflow 1.5, Re 20, dt 1e-3, 200 steps).
+
+    The requested dt is past the explicit scheme's stability limit on this
+    mesh, and the product's guard decides: the default policy rejects it
+    (the state is untouched), the clamp policy runs all 200 steps at
+    tau_max, matching the oracle stepped at the same tau to 1e-10 (FP64) and
+    the mixed path to its own bounds, and staying bounded.  Unchecked, the
+    reference's behaviour, follows the oracle through the instability: the
+    oracle's velocity settles to about 1.85 by step 40, then grows and
+    overflows near step 74; the GPU run matches it to 1e-10 for 40 steps and
+    stops with the reference's non-finite error ("Tentative acceleration is
+    bad") at a step where the oracle's state is past 1e20."""
     from nsfem_amd import _lib, fem, mesher
     m = mesher.channel_mesh(5000, seed=20241103)
     peak = 1.5
@@ -275,23 +285,59 @@ def test_cpu_sized_channel_workload_matches_oracle(oracle):
     ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), cond)
     assert (ctx.numPressureNodes, ctx.numVelocityNodes) == (octx["V"], octx["N"])
     tau = 1e-3
-    orc = oracle.ChorinOracle(octx, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
-    ch = fem.Chorin(ctx, _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1)))
-    mx = fem.Chorin(ctx, _cfg(("pcg_mixed", 2000, 1e-10, 1), ("pcg_mixed", 8000, 1e-10, 1)))
+    fp64 = (("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1))
+    mixed = (("pcg_mixed", 2000, 1e-10, 1), ("pcg_mixed", 8000, 1e-10, 1))
+
+    # default policy: rejected, nothing changes
+    rj = fem.Chorin(ctx, _cfg(*fp64))
+    with pytest.raises(_lib.NsfemInvalidArgument, match="stability limit"):
+        rj.set_time_step(tau)
+    with pytest.raises(_lib.NsfemInvalidArgument, match="stability limit"):
+        rj.step()  # the default 1e-3 goes through the same check
+    tau_max = rj.time_step_info()["tauMax"]
+    assert tau_max < tau
+    # the estimate against a direct generalized eigensolve of M^-1 K
+    import scipy.sparse.linalg as spla
+    lam = spla.eigsh(octx["K"].tocsc(), k=1, M=octx["M"].tocsc(), which="LA", return_eigenvectors=False)[0]
+    assert 0.9 <= 1.0 / (tau_max * nu * lam) <= 1.0 + 1e-9, (tau_max, nu * lam)
+    u_rest, _ = rj.download()
+    del rj
+
+    # clamp policy: 200 steps at tau_max against the oracle at that tau
+    ch = fem.Chorin(ctx, _cfg(*fp64, policy=fem.TAU_CLAMP))
+    mx = fem.Chorin(ctx, _cfg(*mixed, policy=fem.TAU_CLAMP))
     ch.set_time_step(tau)
     mx.set_time_step(tau)
-    for k in range(40):
-        ch.step()
-        uo, po = orc.step()
-        u, p = ch.download()
-        assert _rel(u, uo) <= 1e-10 and _rel(p, po) <= 1e-10, k
+    used = ch.time_step_info()["tauUsed"]
+    assert used == tau_max and abs(mx.time_step_info()["tauUsed"] / used - 1) < 1e-6
+    orc = oracle.ChorinOracle(octx, used, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    for k in range(200):
+        st = ch.step()
+        assert st["tauRequested"] == tau and st["tauUsed"] == used
+        if k < 20:
+            uo, po = orc.step()
+            u, p = ch.download()
+            assert _rel(u, uo) <= 1e-10 and _rel(p, po) <= 1e-10, k
         if k < 10:
             mx.step()
         if k == 9:
             um, pm = mx.download()
             eu, ep = _rel(um, uo), _rel(pm, po)
-            print(f"5k channel, 10 steps: mixed vs oracle velocity {eu:.2e}, pressure {ep:.2e}")
+            print(f"5k channel, clamped tau {used:.4g}, 10 steps: mixed vs oracle velocity {eu:.2e}, pressure {ep:.2e}")
             assert eu < 1e-8 and ep < 1e-7
+    u, p = ch.download()
+    assert np.isfinite(u).all() and np.abs(u).max() < 10 * peak
+    del ch, mx
+
+    # unchecked (the reference's behaviour): follows the oracle into the blow-up
+    orc = oracle.ChorinOracle(octx, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    ch = fem.Chorin(ctx, _cfg(*fp64, policy=fem.TAU_UNCHECKED))
+    ch.set_time_step(tau)
+    for k in range(40):
+        ch.step()
+        uo, po = orc.step()
+        u, p = ch.download()
+        assert _rel(u, uo) <= 1e-10 and _rel(p, po) <= 1e-10, k
     stop = None
     with np.errstate(all="ignore"):
         for k in range(40, 200):
@@ -302,6 +348,6 @@ def test_cpu_sized_channel_workload_matches_oracle(oracle):
                 stop = k
                 break
             uo, po = orc.step()
-    print(f"5k channel: the GPU run stops at step {stop + 1 if stop is not None else None}, "
+    print(f"5k channel, unchecked: the GPU run stops at step {stop + 1 if stop is not None else None}, "
           f"the oracle's velocity then {np.abs(uo).max():.3g}")
     assert stop is not None and np.abs(uo).max() > 1e20
