@@ -90,6 +90,14 @@ int nsfem_copy_bandwidth(size_t bytes, int reps, double * bytesPerSecond);
 int nsfem_geam(int dtype, long n, double alpha, const void * a, double beta, const void * b, void * c);
 int nsfem_dot(int dtype, long n, const void * x, const void * y, double * out);
 int nsfem_nrm2(int dtype, long n, const void * x, double * out);
+/* nsfem_dot without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and result as nsfem_dot, bit for bit; its
+ * scratch (2 * 2048 + 2 doubles: one partial sum per workgroup, at most 2048
+ * workgroups, so any n fits; n < 0 is rejected) is a fixed device array the
+ * library holds on each GPU until the process ends.  Calls from several host
+ * threads are serialised by a lock; blocking, like nsfem_dot. */
+int nsfem_dot_noalloc(int dtype, long n, const void * x, const void * y, double * out);
 int nsfem_scatter(int dtype, int nnz, const int * idx, const void * values, void * dense);
 int nsfem_gather(int dtype, int nnz, const void * dense, const int * idx, void * values);
 int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * dst);

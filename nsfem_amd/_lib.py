@@ -105,6 +105,7 @@ _SIGNATURES = {
     "nsfem_scale": [ctypes.c_int, ctypes.c_long, ctypes.c_double, _vp],
     "nsfem_geam": [ctypes.c_int, ctypes.c_long, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp],
     "nsfem_dot": [ctypes.c_int, ctypes.c_long, _vp, _vp, _c_double_p],
+    "nsfem_dot_noalloc": [ctypes.c_int, ctypes.c_long, _vp, _vp, _c_double_p],
     "nsfem_nrm2": [ctypes.c_int, ctypes.c_long, _vp, _c_double_p],
     "nsfem_scatter": [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp],
     "nsfem_gather": [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp],

@@ -406,6 +406,19 @@ extern "C"
             dot(dtype, n, 1, x, n, y, n, out, libraryStream()); });
     }
 
+    int nsfem_dot_noalloc(int dtype, long n, const void * x, const void * y, double * out)
+    {
+        return guarded([&]
+                       {
+            need(out, "out");
+            if (n > 0)
+            {
+                need(x, "x");
+                need(y, "y");
+            }
+            dotNoAlloc(dtype, n, x, y, out, libraryStream()); });
+    }
+
     int nsfem_nrm2(int dtype, long n, const void * x, double * out)
     {
         return guarded([&]

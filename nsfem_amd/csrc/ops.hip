@@ -319,11 +319,57 @@ namespace nsfem
             static DotScratch * d = []
             {
                 auto * p = new DotScratch;
-                p->partials.resize(2048 * 2);
+                p->partials.resize(2 * kDotMaxBlocks);
                 p->out.resize(2);
                 return p;
             }();
             return *d;
+        }
+    } // namespace
+
+    namespace
+    {
+        // Launches the two dot-product kernels: partials[g * K], then out[K]
+        // (device); returns g.
+        void dotLaunch(int dt, long n, int K, const void * x, long ldx, const void * y, long ldy, double * partials, double * devOut,
+                       hipStream_t s)
+        {
+            const int g = static_cast<int>(std::max(1L, std::min((n + kBlock - 1) / kBlock, static_cast<long>(kDotMaxBlocks))));
+            if (dt == F64)
+            {
+                if (K == 1)
+                {
+                    dotKernel<double, 1><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
+                                                             partials);
+                }
+                else
+                {
+                    dotKernel<double, 2><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
+                                                             partials);
+                }
+            }
+            else
+            {
+                if (K == 1)
+                {
+                    dotKernel<float, 1><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
+                                                            partials);
+                }
+                else
+                {
+                    dotKernel<float, 2><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
+                                                            partials);
+                }
+            }
+            if (K == 1)
+            {
+                dotFinalizeKernel<1><<<1, kBlock, 0, s>>>(g, partials, devOut);
+            }
+            else
+            {
+                dotFinalizeKernel<2><<<1, kBlock, 0, s>>>(g, partials, devOut);
+            }
+            NSFEM_HIP(hipGetLastError());
         }
     } // namespace
 
@@ -344,43 +390,75 @@ namespace nsfem
         }
         auto & d = dotScratch();
         std::lock_guard<std::mutex> lock(d.mutex);
-        const int g = static_cast<int>(std::max(1L, std::min((n + kBlock - 1) / kBlock, 2048L)));
-        if (dt == F64)
-        {
-            if (K == 1)
-            {
-                dotKernel<double, 1><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
-                                                         d.partials.get());
-            }
-            else
-            {
-                dotKernel<double, 2><<<g, kBlock, 0, s>>>(n, static_cast<const double *>(x), ldx, static_cast<const double *>(y), ldy,
-                                                         d.partials.get());
-            }
-        }
-        else
-        {
-            if (K == 1)
-            {
-                dotKernel<float, 1><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
-                                                        d.partials.get());
-            }
-            else
-            {
-                dotKernel<float, 2><<<g, kBlock, 0, s>>>(n, static_cast<const float *>(x), ldx, static_cast<const float *>(y), ldy,
-                                                        d.partials.get());
-            }
-        }
-        if (K == 1)
-        {
-            dotFinalizeKernel<1><<<1, kBlock, 0, s>>>(g, d.partials.get(), d.out.get());
-        }
-        else
-        {
-            dotFinalizeKernel<2><<<1, kBlock, 0, s>>>(g, d.partials.get(), d.out.get());
-        }
-        NSFEM_HIP(hipGetLastError());
+        dotLaunch(dt, n, K, x, ldx, y, ldy, d.partials.get(), d.out.get(), s);
         NSFEM_HIP(hipMemcpyAsync(out, d.out.get(), K * sizeof(double), hipMemcpyDeviceToHost, s));
+        NSFEM_HIP(hipStreamSynchronize(s));
+    }
+
+    // ------------------------------------------------------------------
+    // Allocation-free dot product.  Its scratch (the first kernel's
+    // per-workgroup partial sums and the result slot) is this file-scope
+    // device array: one copy per GPU, loaded with the library's code object
+    // and held until the process ends.  In host code the array's name is
+    // not a device address; the address comes from hipGetSymbolAddress, once
+    // per device with that device current, and only that address is used.
+    // Nothing in it needs clearing: every partial the finalize kernel reads
+    // is written by the same call's first kernel, and the result slot by the
+    // finalize kernel.
+    // ------------------------------------------------------------------
+    static __device__ __align__(256) double scratch[kNoAllocDotScratchDoubles];
+
+    namespace
+    {
+        constexpr int kMaxDevices = 64;
+        std::mutex & noAllocMutex()
+        {
+            static std::mutex m;
+            return m;
+        }
+
+        double * noAllocScratch()
+        {
+            int dev = -1;
+            NSFEM_HIP(hipGetDevice(&dev));
+            if (dev < 0 || dev >= kMaxDevices)
+            {
+                throw std::runtime_error("allocation-free dot: device index " + std::to_string(dev) + " out of range");
+            }
+            static double * ptrs[kMaxDevices] = {};
+            if (!ptrs[dev])
+            {
+                void * p = nullptr;
+                const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+                if (rc != hipSuccess || p == nullptr)
+                {
+                    throw DeviceError(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorName(rc));
+                }
+                ptrs[dev] = static_cast<double *>(p);
+            }
+            return ptrs[dev];
+        }
+    } // namespace
+
+    void dotNoAlloc(int dt, long n, const void * x, const void * y, double * out, hipStream_t s)
+    {
+        checkDtype(dt);
+        if (n < 0)
+        {
+            throw std::invalid_argument("dot: n must be non-negative");
+        }
+        if (n == 0)
+        {
+            *out = 0;
+            return;
+        }
+        // calls from several host threads would share the array: one at a time
+        std::lock_guard<std::mutex> lock(noAllocMutex());
+        double * base = noAllocScratch();
+        double * partials = base;
+        double * result = base + 2 * kDotMaxBlocks;
+        dotLaunch(dt, n, 1, x, n, y, n, partials, result, s);
+        NSFEM_HIP(hipMemcpyAsync(out, result, sizeof(double), hipMemcpyDeviceToHost, s));
         NSFEM_HIP(hipStreamSynchronize(s));
     }
 } // namespace nsfem

@@ -45,7 +45,16 @@ namespace nsfem
     void permute(int srcDt, int dstDt, int n, int nch, const int * perm, const void * src, void * dst, bool inverse, hipStream_t s);
 
     // Blocking FP64-accumulated dot product(s): out[k] = x_k . y_k, k < K (1 or 2).
+    // Partial sums of at most kDotMaxBlocks workgroups, summed in workgroup
+    // order (deterministic); the scratch is a library buffer allocated on
+    // first use.
+    constexpr int kDotMaxBlocks = 2048;
     void dot(int dt, long n, int K, const void * x, long ldx, const void * y, long ldy, double * out, hipStream_t s);
+    // The same single dot product (the same kernels, bit-identical result)
+    // with its scratch in a fixed file-scope device array instead of
+    // allocated memory: no allocation, ever.
+    constexpr int kNoAllocDotScratchDoubles = 2 * kDotMaxBlocks + 2;
+    void dotNoAlloc(int dt, long n, const void * x, const void * y, double * out, hipStream_t s);
 
     // Grid size for grid-stride element-wise kernels.
     int elementwiseGrid(long n);
