@@ -180,25 +180,17 @@ def cpu_baseline(ctx, cond, tau, u0):
 
 
 def build_batch(count, seed):
-    """Independent channel-with-hole instances of uneven sizes, one block-
-    diagonal context: P1 pressure DOFs log-uniform in [10k, 250k], hole
-    radius uniform in [0.08, 0.16], inflow peak uniform in [0.5, 2.0],
-    Re uniform in [20, 100], all drawn from one seed."""
-    import numpy as np
+    """Independent channel-with-hole instances of uneven sizes (the seeded
+    spec of mesher.uneven_instances), one block-diagonal context."""
     from nsfem_amd import fem, mesher
-    rng = np.random.default_rng(seed)
-    meshes, conds, info = [], [], []
     t0 = time.perf_counter()
-    for i in range(count):
-        nodes = int(round(math.exp(rng.uniform(math.log(10_000), math.log(250_000)))))
-        radius = float(rng.uniform(0.08, 0.16))
-        peak = float(rng.uniform(0.5, 2.0))
-        re = float(rng.uniform(20.0, 100.0))
-        m = mesher.channel_mesh(nodes, radius=radius, seed=seed + 1 + i)
-        nu = (2.0 / 3.0) * peak * 2 * radius / re
+    inst = mesher.uneven_instances(count, seed)
+    meshes, conds, info = [], [], []
+    for d in inst:
+        m = d["mesh"]
         meshes.append(mesher.to_triangle_mesh(m))
-        conds.append(fem.DfgConditions(viscosity=nu, peakVelocity=peak, channelHeight=m["height"], profileScale=4.0))
-        info.append({"pressure_dofs": len(m["xy"]), "radius": radius, "peak": peak, "re": re})
+        conds.append(fem.DfgConditions(viscosity=d["viscosity"], peakVelocity=d["peak"], channelHeight=m["height"], profileScale=4.0))
+        info.append({"pressure_dofs": len(m["xy"]), "radius": d["radius"], "peak": d["peak"], "re": d["re"]})
     t1 = time.perf_counter()
     ctx = fem.ChorinContext.batch(meshes, conds)
     t2 = time.perf_counter()

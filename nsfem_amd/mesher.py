@@ -133,3 +133,22 @@ def dfg_channel(num_nodes, seed=20241103):
 def to_triangle_mesh(m):
     from .fem import TriangleMesh
     return TriangleMesh.from_arrays(m["xy"], m["triangles"], m["border"], m["groups"])
+
+
+def uneven_instances(count, seed):
+    """The batched workload: independent channel-with-hole instances of
+    uneven sizes, P1 pressure DOFs log-uniform in [10k, 250k], hole radius
+    uniform in [0.08, 0.16], inflow peak uniform in [0.5, 2.0], Re uniform in
+    [20, 100] (viscosity = 2/3 peak * diameter / Re), all drawn from one seed;
+    instance i is meshed with seed + 1 + i.  Returns a list of dicts (mesh,
+    radius, peak, re, viscosity)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nodes = int(round(math.exp(rng.uniform(math.log(10_000), math.log(250_000)))))
+        radius = float(rng.uniform(0.08, 0.16))
+        peak = float(rng.uniform(0.5, 2.0))
+        re = float(rng.uniform(20.0, 100.0))
+        m = channel_mesh(nodes, radius=radius, seed=seed + 1 + i)
+        out.append({"mesh": m, "radius": radius, "peak": peak, "re": re, "viscosity": (2.0 / 3.0) * peak * 2 * radius / re})
+    return out
