@@ -156,27 +156,64 @@ def measured_traffic(kind):
     return None
 
 
-def cpu_baseline(ctx, cond, tau, u0):
-    """The FP64 CPU oracle timed on one step of the same workload (single
-    thread, numpy/scipy), from the same initial state."""
+def host_cpu_info():
+    """Host cores usable by this process and the CPU model (/proc/cpuinfo's
+    model name, what lscpu reports)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": usable, "os_cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(ctx, cond, tau, u0, budget_s=45.0, max_steps=10):
+    """The CPU baseline: the FP64 C++ restatement of the step (oracle/
+    cpu_step.cpp; OpenMP over the usable host cores, or OMP_NUM_THREADS),
+    every solve SGS-PCG to 1e-10 like the GPU paths, from the same initial
+    state.  One warm-up step (its state is the reference for the GPU paths'
+    first step), then up to max_steps timed steps, stopping early once
+    budget_s has passed (at least 2)."""
     import numpy as np
     from oracle import nsfem_oracle as O
     t0 = time.perf_counter()
-    xyp, ep = ctx.pressure_mesh()
-    _, ev = ctx.velocity_mesh()
-    B, det = O.geometry(xyp, ep)
+    xyv, ev = ctx.velocity_mesh()
     octx = {"N": ctx.numVelocityNodes, "V": ctx.numPressureNodes, "M": ctx.scipy("velocityMass"),
             "K": ctx.scipy("velocityStiffness"), "Kp_int": ctx.scipy("pressureStiffnessInternal"),
             "VPD": ctx.scipy("velocityPressureDiv"), "PVD": ctx.scipy("pressureVelocityDiv"),
-            "vm": {"elements": ev.astype(np.int64)}, "B": B, "det": det, "dvx": ctx.dirichlet("vx"),
-            "dvy": ctx.dirichlet("vy"), "int_p": ctx.internal_pressure(), "viscosity": cond.viscosity}
-    orc = O.ChorinOracle(octx, tau, vel=("pcg", 2000, 1e-10, 1), pres=("pcg", 20000, 1e-10, 1))
-    orc.u = np.array(u0, dtype=np.float64)
+            "vm": {"elements": ev, "xy": xyv}, "dvx": ctx.dirichlet("vx"), "dvy": ctx.dirichlet("vy"),
+            "int_p": ctx.internal_pressure(), "viscosity": cond.viscosity}
+    info = host_cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["nproc"]
+    cpu = O.CpuStep(octx, tau, rtol=1e-10, threads=threads)
+    cpu.set_velocity(u0)
     t1 = time.perf_counter()
-    u, p = orc.step()
+    warm_iters = cpu.run(1)
+    first = cpu.state()
     t2 = time.perf_counter()
-    log(f"cpu oracle: set-up {t1 - t0:.1f}s, one step {t2 - t1:.1f}s")
-    return 1.0 / (t2 - t1), (u.copy(), p.copy())
+    iters, times = [], []
+    while len(times) < max_steps and (len(times) < 2 or sum(times) < budget_s):
+        t = time.perf_counter()
+        iters.append(cpu.run(1)[0].tolist())
+        times.append(time.perf_counter() - t)
+    steps = len(times)
+    rate = steps / sum(times)
+    log(f"cpu baseline ({cpu.threads} threads, {info['model']}): set-up {t1 - t0:.1f}s, warm-up step {t2 - t1:.1f}s "
+        f"(iterations {warm_iters[0].tolist()}), {steps} steps at {1.0 / rate:.2f} s/step")
+    res = {"value": rate, "unit": "steps/s", "cores": cpu.threads, "kind": "port",
+           "sample": f"{steps} Chorin steps of the same 500k-DOF workload after 1 warm-up step, from the same initial state: "
+                     f"FP64 C++ restatement (oracle/cpu_step.cpp), OpenMP on {cpu.threads} threads, every solve SGS-PCG "
+                     f"to 1e-10 relative",
+           "seconds_per_step": 1.0 / rate, "iterations_per_step": iters, "host": info}
+    return res, first
 
 
 def build_batch(count, seed):
@@ -439,6 +476,8 @@ def main():
                          "this many rows, RHS tile width autotuned (4, 8, 16, 32)")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the 500k workload: this many uneven instances (10k-250k pressure DOFs) in one batch")
+    ap.add_argument("--cpu-budget", type=float, default=45.0,
+                    help="seconds of timed CPU-baseline steps (at most 10 steps, at least 2)")
     ap.add_argument("--eager", action="store_true",
                     help="launch solver iterations from the host (no graph replay; for kernel tracing tools)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -542,12 +581,9 @@ def main():
     lam = results[head]["time_step"]["lambdaNu"] / cond.viscosity
     cpu = None
     if not args.no_cpu:
-        cpu_rate, (uo, po) = cpu_baseline(ctx, cond, tau, u0)
+        cpu, (uo, po) = cpu_baseline(ctx, cond, tau, u0, budget_s=args.cpu_budget)
         for name in results:
             errors[f"{name}_vs_oracle_step1"] = {"velocity": rel(firsts[name][0], uo), "pressure": rel(firsts[name][1], po)}
-        cpu = {"value": cpu_rate, "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": "1 Chorin step of the same 500k-DOF workload from the same initial state: FP64 numpy/scipy "
-                         "oracle, PCG-MCGS to 1e-10 for all three solves, single thread"}
     log(f"errors: {errors}")
     rmx, kernmx = results[head], kernels[head]
     r64 = results.get("fp64")

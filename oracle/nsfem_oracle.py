@@ -694,3 +694,76 @@ class ChorinOracle:
         self.u -= tau * a
         self.impose()
         return self.u, self.p
+
+
+# ---------------------------------------------------------------------------
+# The CPU baseline: the same step in multithreaded C++ (oracle/cpu_step.cpp)
+# ---------------------------------------------------------------------------
+
+CPU_LIB = os.path.join(HERE, "_build", "libcpu_step.so")
+
+
+class CpuStep:
+    """FP64 C++/OpenMP Chorin step over a context dict with the keys of
+    build_context (M, K, Kp_int, VPD, PVD, vm (elements, xy), dvx, dvy,
+    int_p, viscosity, N, V): every solve SGS-PCG to rtol."""
+
+    def __init__(self, ctx, tau, rtol=1e-10, threads=0):
+        if not os.path.exists(CPU_LIB):
+            raise RuntimeError(f"{CPU_LIB} is not built (make -C oracle)")
+        self.lib = ctypes.CDLL(CPU_LIB)
+        self.lib.cpu_step_error.restype = ctypes.c_char_p
+        self.N, self.V = int(ctx["N"]), int(ctx["V"])
+        keep = []
+
+        def csr(m):
+            m = m.tocsr()
+            arrs = (np.ascontiguousarray(m.indptr, np.int32), np.ascontiguousarray(m.indices, np.int32),
+                    np.ascontiguousarray(m.data, np.float64))
+            keep.extend(arrs)
+            return [_p(a) for a in arrs]
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return _p(a)
+
+        el = ctx["vm"]["elements"]
+        h = ctypes.c_void_p()
+        rc = self.lib.cpu_step_create(
+            ctypes.c_int(self.N), ctypes.c_int(self.V), ctypes.c_int(len(el)), *csr(ctx["M"]), *csr(ctx["K"]),
+            ctypes.c_double(ctx["viscosity"]), ctypes.c_int(len(ctx["int_p"])), *csr(ctx["Kp_int"]), *csr(ctx["VPD"]),
+            *csr(ctx["PVD"]), arr(el, np.int32), arr(ctx["vm"]["xy"], np.float64), ctypes.c_int(len(ctx["dvx"][0])),
+            arr(ctx["dvx"][0], np.int32), arr(ctx["dvx"][1], np.float64), ctypes.c_int(len(ctx["dvy"][0])),
+            arr(ctx["dvy"][0], np.int32), arr(ctx["dvy"][1], np.float64), arr(ctx["int_p"], np.int32), ctypes.c_double(tau),
+            ctypes.c_double(rtol), ctypes.c_int(threads), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError("cpu_step_create failed")
+        self.h = h
+
+    @property
+    def threads(self):
+        return int(self.lib.cpu_step_threads())
+
+    def set_velocity(self, u):
+        u = np.ascontiguousarray(u, np.float64)
+        self.lib.cpu_step_set_velocity(self.h, _p(u))
+
+    def run(self, steps=1):
+        """Returns the iteration counts (steps x [tentative, pressure, final])."""
+        it = np.zeros((steps, 3), np.int32)
+        if self.lib.cpu_step_run(self.h, ctypes.c_int(steps), _p(it)) != 0:
+            raise RuntimeError("cpu step failed: " + self.lib.cpu_step_error(self.h).decode())
+        return it
+
+    def state(self):
+        u = np.empty(2 * self.N)
+        p = np.empty(self.V)
+        self.lib.cpu_step_state(self.h, _p(u), _p(p))
+        return u, p
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.cpu_step_destroy(h)
+            self.h = None

@@ -381,3 +381,20 @@ def test_tentative_step_closed_form_on_linear_field(oracle, octx_h025):
     assert free.sum() > 0.5 * N
     assert np.max(np.abs(ut[:N] - (ux - tau * xy[:, 0]))[free]) <= 1e-12
     assert np.max(np.abs(ut[N:] - (uy - tau * xy[:, 1]))[free]) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["c3", "h_0.025"])
+def test_cpu_baseline_step_matches_oracle(oracle, name):
+    """The multithreaded C++ baseline (the bench's CPU figure) steps like the
+    numpy oracle: 3 steps, every solve to 1e-13, velocity and pressure to
+    1e-10 relative."""
+    ctx = oracle.build_context(oracle.parse_gmsh(mesh_path(name)), 1e-3, 1.0)
+    orc = oracle.ChorinOracle(ctx, 1e-3, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    cpu = oracle.CpuStep(ctx, 1e-3, rtol=1e-13, threads=4)
+    for k in range(3):
+        uo, po = orc.step()
+        it = cpu.run(1)
+        u, p = cpu.state()
+        assert np.max(np.abs(u - uo)) <= 1e-10 * np.max(np.abs(uo)), k
+        assert np.max(np.abs(p - po)) <= 1e-10 * np.max(np.abs(po)), k
+        assert it[0, 1] > 0
