@@ -69,15 +69,21 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     shape = ch.autotune(10) if autotune else ch.shape()
     tune_s = time.perf_counter() - t0 if autotune else 0.0
     ch.upload_velocity(u0)
+    log(f"  {name}: autotuned in {tune_s:.1f} s; first step")
     first = ch.step()
     u1, p1 = ch.download()
     for _ in range(warmup):
         ch.step()
+    log(f"  {name}: warm-up done")
     _lib.call("nsfem_synchronize")
-    stats = []
+    stats, walls = [], []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
+        t = time.perf_counter()
         stats.append(ch.step())
+        walls.append(1e3 * (time.perf_counter() - t))
+        if k % 5 == 4:
+            log(f"  {name}: step {k + 1}/{steps}, {walls[-1]:.1f} ms")
     _lib.call("nsfem_synchronize")
     dt = time.perf_counter() - t0
     u, p = ch.download()
@@ -89,6 +95,9 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
            "outer": {"tentative": agg["outerTent"], "pressure": agg["outerPressure"], "final": agg["outerFinal"]},
            "max_relres": max(max(s["relTent"], s["relPressure"], s["relFinal"]) for s in stats),
            "time_step": tau_info,
+           "captured_steps": int(sum(s["capturedStep"] for s in stats)),
+           "step_wall_ms": {"min": min(walls), "median": float(np.median(walls)), "max": max(walls)},
+           "step_graph": ch.step_graph_info(),
            "ms": {"convection": agg["msConvection"], "tentative": agg["msTentative"], "pressure": agg["msPressure"],
                   "final": agg["msFinal"]},
            "shape": {"velocity_lanes": shape[0], "velocity_rows_per_block": shape[1], "pressure_lanes": shape[2],
@@ -254,7 +263,7 @@ def run_batch(args):
     res = {}
     for name, method in (("fp64", "pcg"), ("mixed", "pcg_mixed")):
         cfg = fem.ChorinConfig(fem.SolverConfig(method, 5000, target, 1), fem.SolverConfig(method, 50000, target, 1), dtype=F64,
-                               timeStepPolicy=fem.TAU_CLAMP)
+                               timeStepPolicy=fem.TAU_CLAMP, stepGraph=1 if args.step_graph else 0)
         # the guard clamps tau to the batch's limit (the tightest instance's)
         ch, r, _, _ = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune)
         r["kernels"] = {} if args.no_kernels else kernel_table(ch, r, args.kernel_reps)
@@ -463,6 +472,8 @@ def main():
                          "pressure direction,update,rho rows, the velocity and pressure direction lanes and "
                          "whether each fuses its reductions (18 numbers, as the JSON records them)")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
+    ap.add_argument("--step-graph", action="store_true",
+                    help="each step as one captured graph with device-side convergence (after a first regular step)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default (or --shape) launch shapes")
     ap.add_argument("--kernels-only", action="store_true",
                     help="no time steps: build the workload and time each kernel kind (--kernel-reps launches); "
@@ -541,8 +552,9 @@ def main():
                 fem.SolverConfig(method, 50000, target, 1, pl, pr, psl, psr))
 
     eager = 1 if args.eager else 0
-    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP)
-    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP)
+    sg = 1 if args.step_graph else 0
+    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg)
+    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg)
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     if args.kernels_only:
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},

@@ -327,6 +327,14 @@ typedef struct nsfem_chorin_config
                         * trace graph launches); 0: graphs */
     int timeStepPolicy; /* NSFEM_TAU_REJECT (0, default), NSFEM_TAU_CLAMP or
                          * NSFEM_TAU_UNCHECKED; see nsfem_chorin_set_time_step */
+    int stepGraph;      /* 1: each step (after a first regular one) is one captured
+                         * graph: a device snapshot of the state, convection, the
+                         * three solves with iteration budgets from the last
+                         * counts (their kernels idle once converged, so
+                         * convergence is decided on the device), the updates and
+                         * impositions; a solve out of budget rolls the step back
+                         * to the snapshot and reruns it regularly.  pcg solvers
+                         * without mixedRestart.  No reference counterpart. */
 } nsfem_chorin_config;
 
 /* Time-step policies.  The step is explicit (forward Euler) in diffusion and
@@ -347,6 +355,7 @@ typedef struct nsfem_step_stats
     int itersTent, itersPressure, itersFinal;
     int outerTent, outerPressure, outerFinal;
     double tauRequested, tauUsed; /* the step asked for and the step taken */
+    int capturedStep;             /* 1: ran as one captured graph (phase times 0) */
 } nsfem_step_stats;
 
 typedef struct nsfem_chorin nsfem_chorin;
@@ -362,6 +371,15 @@ int nsfem_chorin_set_time_step(nsfem_chorin * ch, double tau);
  * tau_max = 1 / lambda_max (any pointer may be NULL; asking for lambdaNu or
  * tauMax runs the estimate if it has not run).  No reference counterpart. */
 int nsfem_chorin_time_step_info(nsfem_chorin * ch, double * tauRequested, double * tauUsed, double * lambdaNu, double * tauMax);
+/* Captured-step bookkeeping: graphs built (and the host seconds spent
+ * capturing and instantiating them) and steps rolled back so far, and the
+ * current iteration budgets (tentative, pressure, final; 0 before the first
+ * step).  Any pointer may be NULL.  No reference counterpart. */
+int nsfem_chorin_step_graph_info(const nsfem_chorin * ch, int * builds, double * buildSeconds, int * rollbacks, int * budgets);
+/* Sets the iteration budgets (tentative, pressure, final; each > 0) of the
+ * next captured step, e.g. from counts the caller knows; later steps adapt
+ * them from their own counts again.  No reference counterpart. */
+int nsfem_chorin_set_step_graph_budgets(nsfem_chorin * ch, int tentative, int pressure, int final);
 /* One projection step; stats may be NULL.  NSFEM_ENONFINITE on a bad
  * velocity solve ("Tentative/Final acceleration is bad", chorinCuda.cu:270-273, 349-352). */
 int nsfem_chorin_step(nsfem_chorin * ch, nsfem_step_stats * stats);

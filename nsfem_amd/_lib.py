@@ -68,7 +68,8 @@ class SolverConfig(ctypes.Structure):
 
 class ChorinConfigC(ctypes.Structure):
     _fields_ = [("velocitySolver", SolverConfig), ("pressureSolver", SolverConfig), ("dtype", ctypes.c_int),
-                ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int), ("timeStepPolicy", ctypes.c_int)]
+                ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int), ("timeStepPolicy", ctypes.c_int),
+                ("stepGraph", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -76,7 +77,7 @@ class StepStats(ctypes.Structure):
         "mseTentX", "mseTentY", "msePressure", "mseFinalX", "mseFinalY", "relTent", "relPressure", "relFinal",
         "msConvection", "msTentative", "msPressure", "msFinal", "msTotal")] + \
         [(n, ctypes.c_int) for n in ("itersTent", "itersPressure", "itersFinal", "outerTent", "outerPressure", "outerFinal")] + \
-        [(n, ctypes.c_double) for n in ("tauRequested", "tauUsed")]
+        [(n, ctypes.c_double) for n in ("tauRequested", "tauUsed")] + [("capturedStep", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -166,6 +167,9 @@ _SIGNATURES = {
     "nsfem_chorin_destroy": [_vp],
     "nsfem_chorin_set_time_step": [_vp, ctypes.c_double],
     "nsfem_chorin_time_step_info": [_vp, _c_double_p, _c_double_p, _c_double_p, _c_double_p],
+    "nsfem_chorin_set_step_graph_budgets": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "nsfem_chorin_step_graph_info": [_vp, ctypes.POINTER(ctypes.c_int), _c_double_p, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)],
     "nsfem_chorin_step": [_vp, ctypes.POINTER(StepStats)],
     "nsfem_chorin_state": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "nsfem_chorin_download": [_vp, _vp, _vp],

@@ -1129,6 +1129,22 @@ extern "C"
                 throw std::invalid_argument("timeStepPolicy must be NSFEM_TAU_REJECT, NSFEM_TAU_CLAMP or NSFEM_TAU_UNCHECKED");
             }
             cc.timeStepPolicy = static_cast<TimeStepPolicy>(cfg->timeStepPolicy);
+            if (cfg->stepGraph != 0 && cfg->stepGraph != 1)
+            {
+                throw std::invalid_argument("stepGraph must be 0 or 1");
+            }
+            cc.stepGraph = cfg->stepGraph == 1;
+            if (cc.stepGraph)
+            {
+                for (const auto * sc : {&cc.velocity, &cc.pressure})
+                {
+                    const SolverOptions o = parseSolverName(sc->method);
+                    if (o.method != SolverMethod::Pcg)
+                    {
+                        throw std::invalid_argument("stepGraph needs pcg or pcg_mixed solvers");
+                    }
+                }
+            }
             auto h = std::make_unique<nsfem_chorin>();
             h->core = std::make_unique<ChorinGpu>(c->ctx, cc);
             *out = h.release(); });
@@ -1149,6 +1165,28 @@ extern "C"
             need(ch, "chorin");
             if (!(tau > 0)) throw std::invalid_argument("time step must be positive");
             ch->core->setTimeStep(tau); });
+    }
+
+    int nsfem_chorin_step_graph_info(const nsfem_chorin * ch, int * builds, double * buildSeconds, int * rollbacks, int * budgets)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            if (builds) *builds = ch->core->stepGraphBuilds();
+            if (buildSeconds) *buildSeconds = ch->core->stepGraphBuildSeconds();
+            if (rollbacks) *rollbacks = ch->core->stepGraphRollbacks();
+            if (budgets)
+            {
+                for (int k = 0; k < 3; k++) budgets[k] = ch->core->stepGraphBudget(k);
+            } });
+    }
+
+    int nsfem_chorin_set_step_graph_budgets(nsfem_chorin * ch, int tentative, int pressure, int final)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            ch->core->setStepGraphBudgets(tentative, pressure, final); });
     }
 
     int nsfem_chorin_time_step_info(nsfem_chorin * ch, double * tauRequested, double * tauUsed, double * lambdaNu, double * tauMax)
@@ -1192,6 +1230,7 @@ extern "C"
                 stats->outerFinal = s.outerFinal;
                 stats->tauRequested = s.tauRequested;
                 stats->tauUsed = s.tauUsed;
+                stats->capturedStep = s.capturedStep;
             } });
     }
 

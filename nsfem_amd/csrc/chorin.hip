@@ -1,6 +1,7 @@
 #include "chorin.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -217,6 +218,7 @@ namespace nsfem
 
     ChorinGpu::~ChorinGpu()
     {
+        destroyStepGraph();
         if (dlPending_)
         {
             (void)hipEventSynchronize(copyDone_);
@@ -327,33 +329,53 @@ namespace nsfem
         const CsrView K = nuK_.view(dt);
         void * rhs = vSolver_->ioRhs();
         void * y = vSolver_->ioSol();
+        // Each instance of a batch is normalised and measured on its own rows
+        // (K and M are block diagonal): normalised over the whole batch, the
+        // instances with the smaller lambda would decay geometrically towards
+        // the FP32 underflow of a mixed or FP32 preconditioner.
+        const int S = vSolver_->numSystems();
+        std::vector<int> off(S + 1);
+        for (int k = 0; k < S; k++)
+        {
+            off[k] = vSolver_->systemOffset(k);
+        }
+        off[S] = N_;
+        auto slice = [&](void * v, int k, int c) { return static_cast<char *>(v) + tsize_ * (static_cast<size_t>(c) * N_ + off[k]); };
         double lam = 0;
         for (int it = 0; it < std::max(1, cfg_.lambdaIterations); it++)
         {
-            // x is scaled to unit RMS per channel, so absolute ("gs") and
-            // relative ("pcg") solver targets both apply
-            double nrm[2];
-            dot(dt, N_, 2, x.get(), N_, x.get(), N_, nrm, s);
-            for (int c = 0; c < 2; c++)
+            // x scaled to unit RMS per instance and channel, so absolute
+            // ("gs") and relative ("pcg") solver targets both apply
+            for (int k = 0; k < S; k++)
             {
-                if (!(nrm[c] > 0) || !std::isfinite(nrm[c]))
+                const long n = off[k + 1] - off[k];
+                for (int c = 0; c < 2; c++)
                 {
-                    throw NonFiniteError("lambda_max estimate: the power iteration vector vanished");
+                    double nrm = 0;
+                    dot(dt, n, 1, slice(x.get(), k, c), n, slice(x.get(), k, c), n, &nrm, s);
+                    if (!(nrm > 0) || !std::isfinite(nrm))
+                    {
+                        throw NonFiniteError("lambda_max estimate: the power iteration vector vanished");
+                    }
+                    scale(dt, n, std::sqrt(n / nrm), slice(x.get(), k, c), s);
                 }
-                scale(dt, N_, std::sqrt(N_ / nrm[c]), static_cast<char *>(x.get()) + c * tsize_ * N_, s);
             }
             csrMul(K, 2, x.get(), N_, rhs, N_, 1.0, 0.0, nullptr, N_, 0, s);
             vSolver_->solve(true);
             csrMul(K, 2, y, N_, w.get(), N_, 1.0, 0.0, nullptr, N_, 0, s);
-            double yKy[2], yMy[2];
-            dot(dt, N_, 2, y, N_, w.get(), N_, yKy, s);
-            dot(dt, N_, 2, y, N_, rhs, N_, yMy, s); // M y = nu K x
             lam = 0;
-            for (int c = 0; c < 2; c++)
+            for (int k = 0; k < S; k++)
             {
-                if (yMy[c] > 0)
+                const long n = off[k + 1] - off[k];
+                for (int c = 0; c < 2; c++)
                 {
-                    lam = std::max(lam, yKy[c] / yMy[c]);
+                    double yKy = 0, yMy = 0;
+                    dot(dt, n, 1, slice(y, k, c), n, slice(w.get(), k, c), n, &yKy, s);
+                    dot(dt, n, 1, slice(y, k, c), n, slice(rhs, k, c), n, &yMy, s); // M y = nu K x
+                    if (yMy > 0)
+                    {
+                        lam = std::max(lam, yKy / yMy);
+                    }
                 }
             }
             NSFEM_HIP(hipMemcpyAsync(x.get(), y, n2 * tsize_, hipMemcpyDeviceToDevice, s));
@@ -530,12 +552,8 @@ namespace nsfem
         }
     } // namespace
 
-    void ChorinGpu::step(ChorinStepStats * stats)
+    void ChorinGpu::stepRegular(ChorinStepStats * stats)
     {
-        if (!tauChecked_)
-        {
-            setTimeStep(tau_); // the default step goes through the policy too
-        }
         const hipStream_t s = libraryStream();
         const int dt = cfg_.dtype;
         const long twoN = 2L * N_;
@@ -601,6 +619,207 @@ namespace nsfem
         st.msTotal = ms[0] + ms[1] + ms[2] + ms[3];
         st.tauRequested = tauRequested_;
         st.tauUsed = tau_;
+        st.capturedStep = 0;
+        if (stats)
+        {
+            *stats = st;
+        }
+    }
+
+    void ChorinGpu::setStepGraph(bool on)
+    {
+        cfg_.stepGraph = on;
+        destroyStepGraph();
+        budget_[0] = budget_[1] = budget_[2] = 0;
+    }
+
+    void ChorinGpu::setStepGraphBudgets(int tent, int pres, int fin)
+    {
+        if (tent < 1 || pres < 1 || fin < 1)
+        {
+            throw std::invalid_argument("iteration budgets must be positive");
+        }
+        budget_[0] = tent;
+        budget_[1] = pres;
+        budget_[2] = fin;
+        destroyStepGraph();
+    }
+
+    void ChorinGpu::destroyStepGraph()
+    {
+        if (stepExec_)
+        {
+            (void)hipGraphExecDestroy(stepExec_);
+            stepExec_ = nullptr;
+        }
+    }
+
+    void ChorinGpu::setBudgets(int tent, int pres, int fin)
+    {
+        // A measured count c gets c + m iterations, m = max(3, c / 4) for the
+        // short velocity solves and max(16, c / 24) for the pressure solve.
+        // The graph is rebuilt when less than half a margin is left or more
+        // than two margins are idle (hysteresis: counts drift by a few
+        // iterations from step to step).
+        const int counts[3] = {tent, pres, fin};
+        const int minMargin[3] = {3, 16, 3};
+        const int frac[3] = {4, 24, 4};
+        bool rebuild = false;
+        for (int k = 0; k < 3; k++)
+        {
+            const int c = counts[k];
+            const int m = std::max(minMargin[k], c / frac[k]);
+            if (c + m / 2 > budget_[k] || budget_[k] > c + 2 * m + 8)
+            {
+                budget_[k] = c + m;
+                rebuild = true;
+            }
+        }
+        if (rebuild)
+        {
+            destroyStepGraph();
+        }
+    }
+
+    void ChorinGpu::buildStepGraph()
+    {
+        const hipStream_t s = libraryStream();
+        const int dt = cfg_.dtype;
+        const long twoN = 2L * N_;
+        const size_t pBytes = static_cast<size_t>(pSolver_->n()) * dtypeSize(pSolver_->ioDtype());
+        if (stepSnapU_.size() < static_cast<size_t>(twoN) * tsize_)
+        {
+            stepSnapU_.resize(static_cast<size_t>(twoN) * tsize_);
+        }
+        if (stepSnapP_.size() < pBytes)
+        {
+            stepSnapP_.resize(std::max<size_t>(pBytes, 8));
+        }
+        syncLibraryStream();
+        const auto t0 = std::chrono::steady_clock::now();
+        hipGraph_t graph = nullptr;
+        NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            // snapshot for a rollback
+            NSFEM_HIP(hipMemcpyAsync(stepSnapU_.get(), u_.get(), twoN * tsize_, hipMemcpyDeviceToDevice, s));
+            NSFEM_HIP(hipMemcpyAsync(stepSnapP_.get(), pSolver_->ioSol(), pBytes, hipMemcpyDeviceToDevice, s));
+            assembleConvection(u_.get(), A_.values.get(), nullptr, s);
+            csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, 0, s);
+            vSolver_->enqueueSolve(s, true, budget_[0]);
+            vSolver_->snapshotStats(s, 0);
+            axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+            imposeDirichlet(s);
+            csrMul(vpd_.view(dt), 1, u_.get(), 0, pSolver_->ioRhs(), 0, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
+            pSolver_->enqueueSolve(s, false, budget_[1]);
+            pSolver_->snapshotStats(s, 1);
+            scatter(dt, nIntP_, internalP_.get(), pSolver_->ioSol(), p_.get(), s);
+            csrMul(pvd_.view(dt), 1, p_.get(), 0, vSolver_->ioRhs(), 0, 1.0, 0.0, nullptr, 2 * N_, 0, s);
+            vSolver_->enqueueSolve(s, true, budget_[2]);
+            vSolver_->snapshotStats(s, 2);
+            axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+            imposeDirichlet(s);
+        }
+        catch (...)
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g)
+            {
+                (void)hipGraphDestroy(g);
+            }
+            throw;
+        }
+        NSFEM_HIP(hipStreamEndCapture(s, &graph));
+        const hipError_t rc = hipGraphInstantiate(&stepExec_, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        NSFEM_HIP(rc);
+        graphTau_ = tau_;
+        graphBuilds_++;
+        graphBuildSeconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    bool ChorinGpu::stepCaptured(ChorinStepStats * stats)
+    {
+        const hipStream_t s = libraryStream();
+        if (stepExec_ && graphTau_ != tau_)
+        {
+            destroyStepGraph(); // tau is a parameter of the captured kernels
+        }
+        if (!stepExec_)
+        {
+            buildStepGraph();
+        }
+        NSFEM_HIP(hipEventRecord(ev_[0], s));
+        NSFEM_HIP(hipGraphLaunch(stepExec_, s));
+        NSFEM_HIP(hipEventRecord(ev_[4], s));
+        NSFEM_HIP(hipEventSynchronize(ev_[4]));
+        ChorinStepStats st{};
+        const bool okT = vSolver_->collectStats(0);
+        st.mseTentX = maxMse(*vSolver_, 0);
+        st.mseTentY = maxMse(*vSolver_, 1);
+        st.relTent = maxRel(*vSolver_);
+        st.itersTent = maxIters(*vSolver_);
+        const bool badT = !std::isfinite(st.mseTentX) || !std::isfinite(st.mseTentY) || vSolver_->lastNonFinite();
+        const bool okP = pSolver_->collectStats(1);
+        st.msePressure = maxMse(*pSolver_, 0);
+        st.relPressure = maxRel(*pSolver_);
+        st.itersPressure = maxIters(*pSolver_);
+        const bool okF = vSolver_->collectStats(2);
+        st.mseFinalX = maxMse(*vSolver_, 0);
+        st.mseFinalY = maxMse(*vSolver_, 1);
+        st.relFinal = maxRel(*vSolver_);
+        st.itersFinal = maxIters(*vSolver_);
+        const bool badF = !std::isfinite(st.mseFinalX) || !std::isfinite(st.mseFinalY) || vSolver_->lastNonFinite();
+        if (badT || badF || !(okT && okP && okF))
+        {
+            // back to the state before the step (the regular path's state at
+            // its error, or the start of the rerun)
+            const size_t pBytes = static_cast<size_t>(pSolver_->n()) * dtypeSize(pSolver_->ioDtype());
+            NSFEM_HIP(hipMemcpyAsync(u_.get(), stepSnapU_.get(), 2 * static_cast<size_t>(N_) * tsize_, hipMemcpyDeviceToDevice, s));
+            NSFEM_HIP(hipMemcpyAsync(pSolver_->ioSol(), stepSnapP_.get(), pBytes, hipMemcpyDeviceToDevice, s));
+            syncLibraryStream();
+            if (badT)
+            {
+                throw NonFiniteError("Tentative acceleration is bad");
+            }
+            if (badF && okT && okP)
+            {
+                throw NonFiniteError("Final acceleration is bad");
+            }
+            graphRollbacks_++;
+            return false;
+        }
+        float ms = 0;
+        NSFEM_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[4]));
+        st.msTotal = ms;
+        st.tauRequested = tauRequested_;
+        st.tauUsed = tau_;
+        st.capturedStep = 1;
+        setBudgets(st.itersTent, st.itersPressure, st.itersFinal);
+        if (stats)
+        {
+            *stats = st;
+        }
+        return true;
+    }
+
+    void ChorinGpu::step(ChorinStepStats * stats)
+    {
+        if (!tauChecked_)
+        {
+            setTimeStep(tau_); // the default step goes through the policy too
+        }
+        if (cfg_.stepGraph && budget_[1] > 0 && stepCaptured(stats))
+        {
+            return;
+        }
+        ChorinStepStats st{};
+        stepRegular(&st);
+        if (cfg_.stepGraph)
+        {
+            setBudgets(st.itersTent, st.itersPressure, st.itersFinal);
+        }
         if (stats)
         {
             *stats = st;

@@ -58,6 +58,9 @@ namespace nsfem
         bool graphs = true;     // solver iterations replayed as captured graphs
         TimeStepPolicy timeStepPolicy = TimeStepPolicy::Reject;
         int lambdaIterations = 40; // power iterations of the lambda_max estimate
+        // The whole step as one captured graph (pcg solvers without
+        // mixedRestart); see ChorinGpu::step.
+        bool stepGraph = false;
     };
 
     struct ChorinStepStats
@@ -68,6 +71,7 @@ namespace nsfem
         int outerTent, outerPressure, outerFinal;
         double msConvection, msTentative, msPressure, msFinal, msTotal;
         double tauRequested, tauUsed;
+        int capturedStep; // 1: ran as one captured graph (phase times are 0)
     };
 
     class ChorinGpu
@@ -89,7 +93,30 @@ namespace nsfem
         // solver's io vectors, not the state.
         double diffusionLambda();
         double stableTimeStep() { return 1.0 / diffusionLambda(); }
+        // One projection step.  With the stepGraph option, after a first
+        // regular step that measures the iteration counts, the whole step -
+        // a device-side snapshot of the velocity and the pressure warm start,
+        // convection, the tentative solve, the pressure solve, the final
+        // solve, the updates and Dirichlet impositions - is one captured
+        // graph launched once per step: every solve runs a fixed budget of
+        // iterations (the last counts plus a margin) whose kernels do
+        // nothing once that system has converged, so convergence is decided
+        // on the device and the host reads the statistics only after the
+        // step.  A solve that ran out of budget (still iterating below
+        // maxIters) rolls the state back to the snapshot and the step runs
+        // again on the regular path; the graph is then rebuilt with budgets
+        // from the new counts (also when the budgets grow far above need, or
+        // tau changes).
         void step(ChorinStepStats * stats);
+        void setStepGraph(bool on);
+        bool stepGraph() const { return cfg_.stepGraph; }
+        int stepGraphBuilds() const { return graphBuilds_; }
+        int stepGraphRollbacks() const { return graphRollbacks_; }
+        double stepGraphBuildSeconds() const { return graphBuildSeconds_; }
+        int stepGraphBudget(int k) const { return budget_[k]; }
+        // Sets the iteration budgets of the next captured step (they adapt
+        // again from its counts).
+        void setStepGraphBudgets(int tent, int pres, int fin);
 
         // Convection assembly alone: A = nu K + C(u) (and C if cOut != nullptr).
         void assembleConvection(const void * u, void * aOut, void * cOut, hipStream_t s);
@@ -137,7 +164,19 @@ namespace nsfem
         double nu_ = 0;
         size_t tsize_ = 8;
 
+        void stepRegular(ChorinStepStats * stats);
+        bool stepCaptured(ChorinStepStats * stats);
+        void buildStepGraph();
+        void destroyStepGraph();
+        void setBudgets(int tent, int pres, int fin);
+
         DevCsr nuK_, A_, vpd_, pvd_;
+        hipGraphExec_t stepExec_ = nullptr;
+        double graphTau_ = -1;
+        int budget_[3] = {0, 0, 0}; // tentative, pressure, final iterations
+        int graphBuilds_ = 0, graphRollbacks_ = 0;
+        double graphBuildSeconds_ = 0;
+        DeviceArray<char> stepSnapU_, stepSnapP_;
         DeviceArray<int> elemNodes_, contribStart_, contribIndex_;
         DeviceArray<double> elemGeom_;
         DeviceArray<char> elemC_;

@@ -445,6 +445,9 @@ namespace nsfem
         lastMse_.assign(K, -1.0);
         lastRel_.assign(K, -1.0);
         lastIters_.assign(numSystems_, -1);
+        // statistics slots of captured solves (allocated here: nothing may be
+        // allocated while a stream is being captured)
+        statSlots_.resize(kStatSlots * (8 * static_cast<size_t>(K) * sizeof(double) + (3 * static_cast<size_t>(K) + 4) * sizeof(int)));
         syncLibraryStream();
     }
 
@@ -1438,6 +1441,138 @@ namespace nsfem
     }
 
     void McgsSolver::solveAsync(hipStream_t) { solve(); }
+
+    template <typename T, typename P>
+    void McgsSolver::enqueuePcg(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, bool zeroGuess, int budget)
+    {
+        const int nch = opt_.numCh;
+        double * totals = inner_.totals.get();
+        fillInt(4, 0, st.flags, s);
+        dispatchLC(lanes_, nch, [&]<int L, int C>()
+                   {
+                       pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                              color0End_, zeroGuess ? 1 : 0);
+                       pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, nullptr, totals);
+                   });
+        sgsSweeps<P>(s, v.M, st, v.rP, v.z);
+        launchRho<T, P>(s, v, st, totals, false);
+        for (int i = 0; i < budget; i++)
+        {
+            pcgIteration<T, P>(s, v, st);
+        }
+        if (fused())
+        {
+            // the last update's convergence test (no direction kernel follows)
+            dispatchC(nch, [&]<int C>() { pcgUpdateCheckKernel<C><<<1, kFinBlock, 0, s>>>(bmUpd_, st, totals); });
+        }
+        NSFEM_HIP(hipGetLastError());
+    }
+
+    void McgsSolver::enqueueSolve(hipStream_t s, bool zeroInitial, int budget)
+    {
+        if (opt_.method != SolverMethod::Pcg || (opt_.precision == SolverPrecision::Mixed && opt_.restart))
+        {
+            throw std::invalid_argument("captured solves need a pcg solver (pcg or pcg_mixed without mixedRestart)");
+        }
+        if (budget < 1)
+        {
+            throw std::invalid_argument("captured solve: the iteration budget must be positive");
+        }
+        budget = std::min(budget, opt_.maxIters);
+        const int nch = opt_.numCh;
+        const int K = numSystems_ * nch;
+        SolveState st = inner_.view(K, opt_.target * opt_.target);
+        if (opt_.precision == SolverPrecision::F32)
+        {
+            permute(F32, F32, n_, nch, perm_.get(), ioRhs_.get(), b32_.get(), false, s);
+            if (!zeroInitial)
+            {
+                permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
+            }
+            enqueuePcg<float, float>(s, operands<float, float>(), st, zeroInitial, budget);
+            if (opt_.projectMean)
+            {
+                projectMean<float>(s, x32_.get());
+            }
+            permute(F32, F32, n_, nch, perm_.get(), x32_.get(), ioSol_.get(), true, s);
+        }
+        else
+        {
+            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            if (!zeroInitial)
+            {
+                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            }
+            if (opt_.precision == SolverPrecision::F64)
+            {
+                enqueuePcg<double, double>(s, operands<double, double>(), st, zeroInitial, budget);
+            }
+            else
+            {
+                enqueuePcg<double, float>(s, operands<double, float>(), st, zeroInitial, budget);
+            }
+            if (opt_.projectMean)
+            {
+                projectMean<double>(s, x64_.get());
+            }
+            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+        }
+    }
+
+    void McgsSolver::snapshotStats(hipStream_t s, int slot)
+    {
+        if (slot < 0 || slot >= kStatSlots)
+        {
+            throw std::invalid_argument("statistics slot out of range");
+        }
+        const int K = numSystems_ * opt_.numCh;
+        const size_t scalBytes = 8 * static_cast<size_t>(K) * sizeof(double);
+        const size_t intBytes = (3 * static_cast<size_t>(K) + 4) * sizeof(int);
+        const size_t slotBytes = scalBytes + intBytes;
+        char * base = statSlots_.get() + slot * slotBytes;
+        NSFEM_HIP(hipMemcpyAsync(base, inner_.scal.get(), scalBytes, hipMemcpyDeviceToDevice, s));
+        NSFEM_HIP(hipMemcpyAsync(base + scalBytes, inner_.ints.get(), intBytes, hipMemcpyDeviceToDevice, s));
+    }
+
+    bool McgsSolver::collectStats(int slot)
+    {
+        const int K = numSystems_ * opt_.numCh;
+        const size_t scalBytes = 8 * static_cast<size_t>(K) * sizeof(double);
+        const size_t intBytes = (3 * static_cast<size_t>(K) + 4) * sizeof(int);
+        const size_t slotBytes = scalBytes + intBytes;
+        if (slot < 0 || slot >= kStatSlots || statSlots_.size() < kStatSlots * slotBytes)
+        {
+            throw std::invalid_argument("no statistics in that slot");
+        }
+        const char * base = statSlots_.get() + slot * slotBytes;
+        NSFEM_HIP(hipMemcpyAsync(hostScal_, base, 7 * K * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        NSFEM_HIP(hipMemcpyAsync(hostInts_, base + scalBytes, (2 * K + 4) * sizeof(int), hipMemcpyDeviceToHost, stream_));
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        const double * rr = hostScal_ + 3 * K;
+        const double * ref = hostScal_ + 5 * K;
+        const int * active = hostInts_;
+        const int * iters = hostInts_ + K;
+        lastNonFinite_ = hostInts_[2 * K + 2] != 0;
+        lastOuter_ = 0;
+        bool complete = true;
+        for (int sys = 0; sys < numSystems_; sys++)
+        {
+            int it = 0;
+            for (int c = 0; c < opt_.numCh; c++)
+            {
+                const int k = sys * opt_.numCh + c;
+                lastMse_[k] = std::sqrt(rr[k] / sysRows_[sys]);
+                lastRel_[k] = ref[k] > 0 ? std::sqrt(rr[k] / ref[k]) : 0.0;
+                it = std::max(it, iters[k]);
+                if (active[k] != 0 && iters[k] < opt_.maxIters)
+                {
+                    complete = false;
+                }
+            }
+            lastIters_[sys] = it;
+        }
+        return complete;
+    }
 
     double McgsSolver::timeKernel(const std::string & which, int reps, double * bytesPerLaunch, double * launchesPerIteration)
     {

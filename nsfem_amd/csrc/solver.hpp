@@ -136,6 +136,21 @@ namespace nsfem
         // Enqueues the solve without host synchronisation where possible.
         void solveAsync(hipStream_t stream);
 
+        // Captured solves (pcg methods without mixedRestart): enqueueSolve
+        // puts a whole solve on the stream - permutations, the PCG prologue
+        // and exactly `budget` iterations (capped at maxIters) whose kernels
+        // do nothing once the solve has converged - with no host
+        // synchronisation, so it can be captured into a graph.
+        // snapshotStats copies the statistics of the solve just enqueued to
+        // device slot `slot` (< kStatSlots); collectStats reads a slot after
+        // the stream has finished, sets the last-solve statistics and returns
+        // false if some system was still iterating when the budget ran out
+        // (below maxIters), i.e. the solve needs more iterations.
+        static constexpr int kStatSlots = 4;
+        void enqueueSolve(hipStream_t s, bool zeroInitial, int budget);
+        void snapshotStats(hipStream_t s, int slot);
+        bool collectStats(int slot);
+
         // Statistics of the last solve (per system s, channel c).
         double lastMse(int s, int c) const { return lastMse_[s * opt_.numCh + c]; }
         double lastRelRes(int s, int c) const { return lastRel_[s * opt_.numCh + c]; }
@@ -247,6 +262,8 @@ namespace nsfem
         int iterate(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, int maxIters, bool sync, bool usePrediction);
         template <typename T, typename P>
         void pcgIteration(hipStream_t s, const PcgOperands<T, P> & v, SolveState st);
+        template <typename T, typename P>
+        void enqueuePcg(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, bool zeroGuess, int budget);
         template <typename P>
         void sgsSweeps(hipStream_t s, const McgsMatrix<P> & M, SolveState st, const P * r, P * z, int firstForward = 1);
         template <typename T, typename P>
@@ -319,6 +336,7 @@ namespace nsfem
         DeviceArray<float> b32_, x32_, r32_, z32_, p32_, q32_;
 
         DeviceState inner_, outer_;
+        DeviceArray<char> statSlots_; // kStatSlots x (8K doubles, 3K + 4 ints)
         DeviceArray<double> means_;
         int * hostFlags_ = nullptr; // pinned
         int * pollRing_ = nullptr;  // pinned, two slots of the solve flags

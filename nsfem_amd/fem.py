@@ -243,6 +243,7 @@ class ChorinConfig:
     innerTol: float = 1e-4
     eagerLaunches: int = 0  # 1: no graph replay of solver iterations (for tracing tools)
     timeStepPolicy: int = 0  # TAU_REJECT: an unstable step is an error; TAU_CLAMP, TAU_UNCHECKED
+    stepGraph: int = 0  # 1: each step is one captured graph with device-side convergence (pcg solvers)
 
 
 class Chorin:
@@ -258,7 +259,7 @@ class Chorin:
                                                  v.rowsPerBlock, v.sweepLanes, v.sweepRowsPerBlock),
                                _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
                                                  p.rowsPerBlock, p.sweepLanes, p.sweepRowsPerBlock),
-                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy)
+                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy, cfg.stepGraph)
         self._h = ctypes.c_void_p()
         call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
         self.dtype = _NP[cfg.dtype]
@@ -269,6 +270,17 @@ class Chorin:
         """Under TAU_REJECT a step above the stability limit raises
         NsfemError (the previous step stays); TAU_CLAMP takes the limit."""
         call("nsfem_chorin_set_time_step", self._h, float(tau))
+
+    def step_graph_info(self):
+        """dict(builds, build_seconds, rollbacks, budgets) of the captured-step mode."""
+        b, r = ctypes.c_int(), ctypes.c_int()
+        sec = ctypes.c_double()
+        bud = (ctypes.c_int * 3)()
+        call("nsfem_chorin_step_graph_info", self._h, ctypes.byref(b), ctypes.byref(sec), ctypes.byref(r), bud)
+        return {"builds": b.value, "build_seconds": sec.value, "rollbacks": r.value, "budgets": list(bud)}
+
+    def set_step_graph_budgets(self, tentative, pressure, final):
+        call("nsfem_chorin_set_step_graph_budgets", self._h, int(tentative), int(pressure), int(final))
 
     def time_step_info(self):
         """dict(tauRequested, tauUsed, lambdaNu = lambda_max(M^-1 nu K),

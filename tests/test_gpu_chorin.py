@@ -351,3 +351,104 @@ def test_cpu_sized_channel_workload_matches_oracle(oracle):
     print(f"5k channel, unchecked: the GPU run stops at step {stop + 1 if stop is not None else None}, "
           f"the oracle's velocity then {np.abs(uo).max():.3g}")
     assert stop is not None and np.abs(uo).max() > 1e20
+
+
+def _run(ch, steps):
+    out = []
+    for _ in range(steps):
+        out.append(ch.step())
+    return out
+
+
+def test_captured_step_matches_regular_steps_and_oracle(oracle, octx_h025):
+    """stepGraph: after one regular step, every step is one captured graph
+    (device-side convergence, iteration budgets); it runs the regular path's
+    kernels in the same order, so the states equal the regular path's bit for
+    bit, and the FP64 oracle's to 1e-10."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    cfg = _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1))
+    reg = fem.Chorin(_ctx("h_0.025"), cfg)
+    cfg.stepGraph = 1
+    cap = fem.Chorin(_ctx("h_0.025"), cfg)
+    orc = oracle.ChorinOracle(octx_h025, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    for ch in (reg, cap):
+        ch.set_time_step(tau)
+    for k in range(6):
+        sr, sc = reg.step(), cap.step()
+        assert sc["capturedStep"] == (0 if k == 0 else 1), k
+        assert sc["relTent"] <= 1e-13 and sc["relPressure"] <= 1e-13 and sc["relFinal"] <= 1e-13
+        assert (sc["itersTent"], sc["itersPressure"], sc["itersFinal"]) == (sr["itersTent"], sr["itersPressure"], sr["itersFinal"])
+        ur, pr = reg.download()
+        uc, pc = cap.download()
+        assert np.array_equal(ur, uc) and np.array_equal(pr, pc), k
+        uo, po = orc.step()
+        assert _rel(uc, uo) <= 1e-10 and _rel(pc, po) <= 1e-10, k
+    info = cap.step_graph_info()
+    assert info["builds"] >= 1 and info["rollbacks"] == 0 and info["budgets"][1] >= sc["itersPressure"]
+
+
+def test_captured_step_rolls_back_when_a_budget_runs_out():
+    """A pressure budget far below what the step needs: the captured step
+    rolls back to its snapshot and reruns the step regularly, so the result
+    still equals the regular path's; the budgets then adapt from the counts
+    and the next step is captured again."""
+    from nsfem_amd import fem
+    cfg = _cfg(("pcg", 2000, 1e-12, 1), ("pcg", 8000, 1e-12, 1))
+    ctx = _ctx("h_0.025")
+    reg = fem.Chorin(ctx, cfg)
+    cfg.stepGraph = 1
+    cap = fem.Chorin(ctx, cfg)
+    for ch in (reg, cap):
+        ch.set_time_step(1e-3)
+        _run(ch, 3)
+    cap.set_step_graph_budgets(30, 10, 30)
+    sr, sc = reg.step(), cap.step()
+    assert sr["itersPressure"] > 10
+    assert sc["capturedStep"] == 0 and cap.step_graph_info()["rollbacks"] == 1
+    assert (sc["itersTent"], sc["itersPressure"], sc["itersFinal"]) == (sr["itersTent"], sr["itersPressure"], sr["itersFinal"])
+    ur, pr = reg.download()
+    uc, pc = cap.download()
+    assert np.array_equal(ur, uc) and np.array_equal(pr, pc)
+    assert cap.step_graph_info()["budgets"][1] > sr["itersPressure"]
+    sr, sc = reg.step(), cap.step()
+    assert sc["capturedStep"] == 1
+    ur, pr = reg.download()
+    uc, pc = cap.download()
+    assert np.array_equal(ur, uc) and np.array_equal(pr, pc)
+
+
+@pytest.mark.parametrize("method", ["pcg", "pcg_mixed"])
+def test_captured_batch_step_matches_regular(method):
+    """A block-diagonal batch of uneven instances stepped as one captured
+    graph per step (per-instance convergence on the device) equals the
+    regular batch step."""
+    from nsfem_amd import fem, mesher
+    specs = [(1500, 0.10, 1.0, 0.02), (4000, 0.14, 0.6, 0.05), (2500, 0.08, 1.8, 0.015)]
+    meshes, conds = [], []
+    for k, (n, r, peak, nu) in enumerate(specs):
+        m = mesher.channel_mesh(n, radius=r, seed=300 + k)
+        meshes.append(mesher.to_triangle_mesh(m))
+        conds.append(fem.DfgConditions(nu, peak, m["height"], 4.0))
+    batch = fem.ChorinContext.batch(meshes, conds)
+    cfg = _cfg((method, 2000, 1e-10, 1), (method, 8000, 1e-10, 1), policy=fem.TAU_CLAMP)
+    reg = fem.Chorin(batch, cfg)
+    cfg.stepGraph = 1
+    cap = fem.Chorin(batch, cfg)
+    for ch in (reg, cap):
+        ch.set_time_step(5e-4)
+    for k in range(5):
+        sr, sc = reg.step(), cap.step()
+        assert sc["capturedStep"] == (0 if k == 0 else 1)
+        assert sc["relPressure"] <= 1e-10
+    ur, pr = reg.download()
+    uc, pc = cap.download()
+    assert np.array_equal(ur, uc) and np.array_equal(pr, pc)
+
+
+def test_captured_step_needs_pcg_solvers():
+    from nsfem_amd import _lib, fem
+    cfg = _cfg(("gs", 200, 1e-6, 1), ("pcg", 2000, 1e-10, 1))
+    cfg.stepGraph = 1
+    with pytest.raises(_lib.NsfemInvalidArgument, match="stepGraph"):
+        fem.Chorin(_ctx(), cfg)

@@ -220,3 +220,25 @@ def test_multi_rhs_4m_rows_32_columns(oracle):
         rel = np.linalg.norm(B[c] - K @ X[c]) / np.linalg.norm(B[c])
         assert rel <= 1e-8 * (1 + 1e-6), (c, rel)
         assert abs(X[c].mean()) <= 1e-12 * np.max(np.abs(X[c])), c
+
+
+@pytest.mark.timeout(120)
+@pytest.mark.parametrize("method", ["pcg", "pcg_mixed"])
+def test_batch_stability_limit_is_the_tightest_instance(method):
+    """The time-step guard on a batch whose instances' lambda_max(M^-1 nu K)
+    differ by a factor of about 400 (viscosities 100x apart on meshes 4x
+    apart): the batch's lambda is the largest instance's.  The power
+    iteration normalises every instance on its own, so the smaller ones do
+    not decay into the FP32 underflow of a mixed preconditioner."""
+    from nsfem_amd import fem, mesher
+    specs = [(6000, 0.05), (1500, 0.0005)]
+    meshes, conds, singles = [], [], []
+    cfg = fem.ChorinConfig(fem.SolverConfig(method, 2000, 1e-10, 1), fem.SolverConfig(method, 8000, 1e-10, 1))
+    for k, (n, nu) in enumerate(specs):
+        m = mesher.channel_mesh(n, seed=500 + k)
+        meshes.append(mesher.to_triangle_mesh(m))
+        conds.append(fem.DfgConditions(nu, 1.0, m["height"], 4.0))
+        singles.append(fem.Chorin(fem.ChorinContext(meshes[-1], conds[-1]), cfg).time_step_info()["lambdaNu"])
+    assert max(singles) / min(singles) > 100
+    lam = fem.Chorin(fem.ChorinContext.batch(meshes, conds), cfg).time_step_info()["lambdaNu"]
+    assert abs(lam / max(singles) - 1) < 0.05, (lam, singles)
