@@ -114,6 +114,15 @@ int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * d
  *                      symmetric GS preconditioner in FP32 (values must be
  *                      NSFEM_F64); option "mixedRestart" switches to FP64
  *                      iterative refinement around FP32 inner PCG.
+ *   name "bpcg"      : block PCG for a tile of numCh = 4, 8, 16 or 32
+ *                      right-hand sides of one system: one Krylov space for
+ *                      the tile with numCh x numCh step matrices (FP64 small
+ *                      Cholesky solves that drop a direction whose pivot falls
+ *                      below 1e-13 of the largest), the symmetric GS
+ *                      preconditioner; stop when every column has
+ *                      |b_c - A x_c| <= targetMse |b_c|.  FP64 values.
+ *   name "bpcg_mixed": bpcg with the preconditioner in FP32.
+ *                      No reference counterpart for either.
  *   name "dss"       : rejected with NSFEM_EINVAL (no direct solver here).
  * The compute precision is valueDtype ("gs"/"pcg").  Colouring is the
  * reference's smallest-last greedy colouring (graphs.cpp:64-176).
@@ -142,8 +151,9 @@ int nsfem_solver_last_mse(const nsfem_solver * s, int ch, double * out);
 int nsfem_solver_last_relres(const nsfem_solver * s, int system, int ch, double * out);
 int nsfem_solver_last_iterations(const nsfem_solver * s, int * out);
 int nsfem_solver_last_iterations_system(const nsfem_solver * s, int system, int * out);
-/* pcg_mixed with mixedRestart: refinement steps of the last solve; 0
- * otherwise.  No reference counterpart. */
+/* pcg_mixed with mixedRestart: refinement steps of the last solve; bpcg:
+ * restart and verification cycles after the first (each restarts from the
+ * true residual); 0 otherwise.  No reference counterpart. */
 int nsfem_solver_last_outer_iterations(const nsfem_solver * s, int * out);
 int nsfem_solver_set_max_iters(nsfem_solver * s, int n);
 int nsfem_solver_set_target_mse(nsfem_solver * s, double target);

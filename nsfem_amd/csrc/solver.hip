@@ -116,6 +116,15 @@ namespace nsfem
             o.method = SolverMethod::Pcg;
             o.precision = SolverPrecision::Mixed;
         }
+        else if (name == "bpcg")
+        {
+            o.method = SolverMethod::BlockPcg;
+        }
+        else if (name == "bpcg_mixed")
+        {
+            o.method = SolverMethod::BlockPcg;
+            o.precision = SolverPrecision::Mixed;
+        }
         else if (name == "dss")
         {
             throw std::invalid_argument("Solver 'dss' (cuDSS direct solve) is not available in this build; use 'gs', 'pcg' or 'pcg_mixed'");
@@ -180,6 +189,21 @@ namespace nsfem
         if (opt_.method == SolverMethod::GaussSeidel && opt_.precision == SolverPrecision::Mixed)
         {
             throw std::invalid_argument("Mixed precision needs the pcg method");
+        }
+        if (opt_.method == SolverMethod::BlockPcg)
+        {
+            if (opt_.numCh < 4)
+            {
+                throw std::invalid_argument("block PCG needs a tile of 4, 8, 16 or 32 right-hand sides");
+            }
+            if (systems.size() != 1)
+            {
+                throw std::invalid_argument("block PCG solves one system");
+            }
+            if (opt_.precision == SolverPrecision::F32)
+            {
+                throw std::invalid_argument("block PCG keeps FP64 Krylov vectors: use an FP64 matrix (bpcg or bpcg_mixed)");
+            }
         }
         stream_ = libraryStream();
         numSystems_ = static_cast<int>(systems.size());
@@ -448,6 +472,20 @@ namespace nsfem
         // statistics slots of captured solves (allocated here: nothing may be
         // allocated while a stream is being captured)
         statSlots_.resize(kStatSlots * (8 * static_cast<size_t>(K) * sizeof(double) + (3 * static_cast<size_t>(K) + 4) * sizeof(int)));
+        if (opt_.method == SolverMethod::BlockPcg)
+        {
+            // at most 512 workgroups, each a whole number of row slabs
+            const int nch = opt_.numCh;
+            const int slab = kBlock / nch;
+            int rows = ceilDiv(n_, 512);
+            rows = std::max(slab, ceilDiv(rows, slab) * slab);
+            bcgRows_ = rows;
+            bcgBlocks_ = std::max(1, ceilDiv(n_, rows));
+            bcgGram_.resize(static_cast<size_t>(bcgBlocks_) * nch * nch);
+            bcgNorm_.resize(static_cast<size_t>(bcgBlocks_) * 2 * nch);
+            bcgSmall_.resize(2 * static_cast<size_t>(nch) * nch + 2 * nch + 1);
+            bcgInts_.resize(3);
+        }
         syncLibraryStream();
     }
 
@@ -1355,12 +1393,190 @@ namespace nsfem
         NSFEM_HIP(hipStreamSynchronize(s));
     }
 
+    BcgState McgsSolver::bcgState(SolveState st)
+    {
+        const int nch = opt_.numCh;
+        BcgState b;
+        b.gramParts = bcgGram_.get();
+        b.normParts = bcgNorm_.get();
+        b.rho = bcgSmall_.get();
+        b.step = b.rho + nch * nch;
+        b.bb = b.step + nch * nch;
+        b.rr = b.bb + nch;
+        b.best = b.rr + nch;
+        b.flags = st.flags;
+        b.iters = bcgInts_.get();
+        b.tol2 = opt_.target * opt_.target;
+        b.n = n_;
+        b.blocks = bcgBlocks_;
+        b.rowsPerBlock = bcgRows_;
+        return b;
+    }
+
+    namespace
+    {
+        // f<C>() for the wide channel counts of block PCG
+        template <typename F>
+        void dispatchWide(int nch, F && f)
+        {
+            switch (nch)
+            {
+            case 4: f.template operator()<4>(); break;
+            case 8: f.template operator()<8>(); break;
+            case 16: f.template operator()<16>(); break;
+            case 32: f.template operator()<32>(); break;
+            default: throw std::invalid_argument("block PCG tiles have 4, 8, 16 or 32 columns");
+            }
+        }
+    } // namespace
+
+    template <typename P>
+    void McgsSolver::bcgIteration(hipStream_t s, const BcgState & b, SolveState st, const McgsMatrix<P> & M, P * rP, P * z)
+    {
+        const McgsMatrix<double> A = A64_.view(n_);
+        const int g = bcgBlocks_;
+        dispatchWide(opt_.numCh, [&]<int C>()
+                     {
+                         bcgProductKernel<double, C><<<g, kBlock, 0, s>>>(A, b, p64_.get(), q64_.get());
+                         bcgAlphaKernel<C><<<1, kBcgSmallBlock, 0, s>>>(b);
+                         bcgUpdateKernel<double, P, C><<<g, kBlock, 0, s>>>(b, M.invDiag, p64_.get(), q64_.get(), x64_.get(), r64_.get(),
+                                                                             rP, z, color0End_);
+                     });
+        sgsSweeps<P>(s, M, st, rP, z);
+        dispatchWide(opt_.numCh, [&]<int C>()
+                     {
+                         bcgRhoKernel<P, double, C><<<g, kBlock, 0, s>>>(M, b, rP, r64_.get(), z, color0End_);
+                         bcgBetaKernel<C><<<1, kBcgSmallBlock, 0, s>>>(b, 0);
+                         bcgDirectionKernel<double, P, C><<<g, kBlock, 0, s>>>(b, z, p64_.get(), 0);
+                     });
+    }
+
+    template <typename P>
+    void McgsSolver::runBlockPcg(hipStream_t s, const McgsMatrix<P> & M, P * rP, P * z, bool zeroGuess)
+    {
+        const int nch = opt_.numCh;
+        const int K = nch;
+        SolveState st = inner_.view(K, opt_.target * opt_.target);
+        const BcgState b = bcgState(st);
+        const McgsMatrix<double> A = A64_.view(n_);
+        const int g = bcgBlocks_;
+        fillInt(K, 1, st.active, s); // the sweeps run while the solve does
+        fillInt(3, 0, b.iters, s);
+        // cycles: a (re)start on the true residual B - A X, then iterations in
+        // chunks with lagged polls of the stop flag (as the single-RHS
+        // solver); the kernels after a stop exit at once.  A stop with a
+        // restart or verification request starts the next cycle.
+        constexpr int kMaxCycles = 200;
+        const int chunk = std::max(1, opt_.checkChunk);
+        int launched = 0;
+        int cycles = 0;
+        int hostFlags[4] = {0, 0, 0, 0};
+        for (;; cycles++)
+        {
+            fillInt(4, 0, st.flags, s);
+            fillInt(2, 0, b.iters + 1, s); // the cycle's rank flags
+            const int first = cycles == 0;
+            dispatchWide(nch, [&]<int C>()
+                         {
+                             bcgInitKernel<double, P, C><<<g, kBlock, 0, s>>>(A, b, b64_.get(), x64_.get(), r64_.get(), rP, z, M.invDiag,
+                                                                               color0End_, first && zeroGuess ? 1 : 0);
+                         });
+            sgsSweeps<P>(s, M, st, rP, z);
+            dispatchWide(nch, [&]<int C>()
+                         {
+                             bcgRhoKernel<P, double, C><<<g, kBlock, 0, s>>>(M, b, rP, r64_.get(), z, color0End_);
+                             bcgBetaKernel<C><<<1, kBcgSmallBlock, 0, s>>>(b, first ? 2 : 1);
+                             bcgDirectionKernel<double, P, C><<<g, kBlock, 0, s>>>(b, z, p64_.get(), 1);
+                         });
+            NSFEM_HIP(hipGetLastError());
+            int slot = 0;
+            bool pending[2] = {false, false};
+            bool stopped = false;
+            while (launched < opt_.maxIters)
+            {
+                const int k = std::min(chunk, opt_.maxIters - launched);
+                for (int i = 0; i < k; i++)
+                {
+                    bcgIteration<P>(s, b, st, M, rP, z);
+                }
+                NSFEM_HIP(hipGetLastError());
+                launched += k;
+                NSFEM_HIP(hipMemcpyAsync(pollRing_ + 4 * slot, st.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+                NSFEM_HIP(hipEventRecord(pollEvents_[slot], s));
+                pending[slot] = true;
+                const int prev = slot ^ 1;
+                if (pending[prev])
+                {
+                    NSFEM_HIP(hipEventSynchronize(pollEvents_[prev]));
+                    pending[prev] = false;
+                    if (pollRing_[4 * prev] != 0)
+                    {
+                        stopped = true;
+                        break;
+                    }
+                }
+                slot = prev;
+            }
+            NSFEM_HIP(hipMemcpyAsync(hostFlags, st.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+            NSFEM_HIP(hipStreamSynchronize(s));
+            (void)stopped;
+            const bool again = hostFlags[0] != 0 && hostFlags[2] == 0 && (hostFlags[3] == 1 || hostFlags[3] == 2);
+            if (!again || launched >= opt_.maxIters || cycles + 1 >= kMaxCycles)
+            {
+                break;
+            }
+        }
+        lastOuter_ = cycles; // restarts and verifications after the first cycle
+        // statistics: |b_c|^2, |r_c|^2, updates done, dropped directions, flags
+        std::vector<double> h(2 * nch);
+        int ints[2] = {0, 0};
+        int flags[4] = {0, 0, 0, 0};
+        NSFEM_HIP(hipMemcpyAsync(h.data(), b.bb, 2 * nch * sizeof(double), hipMemcpyDeviceToHost, s));
+        NSFEM_HIP(hipMemcpyAsync(ints, b.iters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        NSFEM_HIP(hipMemcpyAsync(flags, st.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        NSFEM_HIP(hipStreamSynchronize(s));
+        bool finite = true;
+        for (int c = 0; c < nch; c++)
+        {
+            const double bb = h[c], rr = h[nch + c];
+            finite = finite && std::isfinite(rr);
+            lastMse_[c] = std::sqrt(rr / n_);
+            lastRel_[c] = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+        }
+        lastIters_[0] = ints[0];
+        lastDropped_ = ints[1] != 0;
+        lastNonFinite_ = !finite || flags[2] != 0;
+    }
+
     void McgsSolver::solve(bool zeroInitial)
     {
         hipStream_t s = stream_;
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
         lastOuter_ = 0;
+        if (opt_.method == SolverMethod::BlockPcg)
+        {
+            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            if (!zeroInitial)
+            {
+                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            }
+            if (opt_.precision == SolverPrecision::F64)
+            {
+                runBlockPcg<double>(s, A64_.view(n_), r64_.get(), z64_.get(), zeroInitial);
+            }
+            else
+            {
+                runBlockPcg<float>(s, A32_.view(n_), r32_.get(), z32_.get(), zeroInitial);
+            }
+            if (opt_.projectMean)
+            {
+                projectMean<double>(s, x64_.get());
+            }
+            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            NSFEM_HIP(hipStreamSynchronize(s));
+            return;
+        }
         const bool pcgZero = zeroInitial && opt_.method == SolverMethod::Pcg && !(opt_.precision == SolverPrecision::Mixed && opt_.restart);
         if (zeroInitial && !pcgZero)
         {

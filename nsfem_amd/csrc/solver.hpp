@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "csr.hpp"
+#include "bcg_kernels.hpp"
 #include "kernels.hpp"
 #include "ops.hpp"
 
@@ -31,7 +32,8 @@ namespace nsfem
     enum class SolverMethod
     {
         GaussSeidel,
-        Pcg
+        Pcg,
+        BlockPcg // one Krylov space per tile of 4-32 right-hand sides
     };
 
     enum class SolverPrecision
@@ -157,6 +159,9 @@ namespace nsfem
         int lastIterations(int s) const { return lastIters_[s]; }
         int lastOuterIterations() const { return lastOuter_; }
         bool lastNonFinite() const { return lastNonFinite_; }
+        // block PCG: whether the last solve dropped a direction (a rank-
+        // deficient block)
+        bool lastDroppedDirection() const { return lastDropped_; }
 
         void setMaxIters(int n) { opt_.maxIters = n; }
         void setTarget(double t) { opt_.target = t; }
@@ -271,6 +276,11 @@ namespace nsfem
         template <typename T, typename P>
         void launchUpdateSweep(hipStream_t s, const PcgOperands<T, P> & v, SolveState st);
         void runMixedRestart(hipStream_t s);
+        template <typename P>
+        void runBlockPcg(hipStream_t s, const McgsMatrix<P> & M, P * rP, P * z, bool zeroGuess);
+        template <typename P>
+        void bcgIteration(hipStream_t s, const BcgState & b, SolveState st, const McgsMatrix<P> & M, P * rP, P * z);
+        BcgState bcgState(SolveState st);
         void readStats(hipStream_t s, DeviceState & st, bool gs, int launched);
         template <typename T>
         void projectMean(hipStream_t s, T * v);
@@ -337,6 +347,13 @@ namespace nsfem
 
         DeviceState inner_, outer_;
         DeviceArray<char> statSlots_; // kStatSlots x (8K doubles, 3K + 4 ints)
+        // block PCG: flat row partition of its own (few workgroups, so the
+        // NCH x NCH Gram partials stay small), Gram and norm partials, the
+        // small matrices, the iteration counter
+        int bcgRows_ = 0, bcgBlocks_ = 0;
+        DeviceArray<double> bcgGram_, bcgNorm_, bcgSmall_;
+        DeviceArray<int> bcgInts_;
+        bool lastDropped_ = false;
         DeviceArray<double> means_;
         int * hostFlags_ = nullptr; // pinned
         int * pollRing_ = nullptr;  // pinned, two slots of the solve flags

@@ -184,27 +184,44 @@ def test_multi_rhs_columns_match_oracle(oracle, width):
     assert abs(s.getLastIterations() - max(its)) <= 1, (s.getLastIterations(), its)
 
 
-@pytest.mark.slow
-@pytest.mark.timeout(900)
-def test_multi_rhs_4m_rows_32_columns(oracle):
-    """The multi-RHS configuration at full size: the pure-Neumann P1 Laplacian
-    of a seeded 4M-node channel mesh, 32 zero-mean uniform(-1, 1) right-hand
-    sides, pcg_mixed (FP32 preconditioner, FP64 Krylov vectors and
-    reductions) with mean projection, in tiles of 8 columns: every column's
-    true relative residual |b - A x| / |b| (FP64, on the host) is at most
-    1e-8 and its mean is zero."""
+@pytest.fixture(scope="module")
+def laplacian_4m(oracle):
+    """The pure-Neumann P1 Laplacian of a seeded 4M-node channel mesh and 32
+    zero-mean uniform(-1, 1) right-hand sides (rows x 32)."""
     import time
-    from nsfem_amd import cu, mesher
+    from nsfem_amd import mesher
     t0 = time.perf_counter()
     m = mesher.channel_mesh(4_000_000, seed=20241103)
     K = _p1_laplacian(oracle, m)
     del m
     n = K.shape[0]
-    assert 3_900_000 <= n <= 4_100_000
     rng = np.random.default_rng(1)
     B = rng.uniform(-1, 1, (32, n))
     B -= B.mean(axis=1, keepdims=True)
     print(f"4M multi-RHS: {n} rows, {K.nnz} nnz, set-up {time.perf_counter() - t0:.1f} s", flush=True)
+    return K, B
+
+
+def _check_columns(K, B, X, tol):
+    for c in range(B.shape[0]):
+        rel = np.linalg.norm(B[c] - K @ X[c]) / np.linalg.norm(B[c])
+        assert rel <= tol, (c, rel)
+        assert abs(X[c].mean()) <= 1e-12 * np.max(np.abs(X[c])), c
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_multi_rhs_4m_rows_32_columns(laplacian_4m):
+    """The multi-RHS configuration at full size with independent columns:
+    pcg_mixed (FP32 preconditioner, FP64 Krylov vectors and reductions) with
+    mean projection, in tiles of 8 columns: every column's true relative
+    residual |b - A x| / |b| (FP64, on the host) is at most 1e-8 and its
+    mean is zero."""
+    import time
+    from nsfem_amd import cu
+    K, B = laplacian_4m
+    n = K.shape[0]
+    assert 3_900_000 <= n <= 4_100_000
     width = 8
     s = cu.createSolver("pcg_mixed", width, K, 20000, 1e-8, 1)
     s.setOption("projectMean", 1)
@@ -216,10 +233,33 @@ def test_multi_rhs_4m_rows_32_columns(oracle):
         s.solve()
         X[first:first + width] = s.getSol().download().reshape(width, n)
         print(f"columns {first}-{first + width - 1}: {s.getLastIterations()} iterations, {time.perf_counter() - t:.1f} s", flush=True)
-    for c in range(32):
-        rel = np.linalg.norm(B[c] - K @ X[c]) / np.linalg.norm(B[c])
-        assert rel <= 1e-8 * (1 + 1e-6), (c, rel)
-        assert abs(X[c].mean()) <= 1e-12 * np.max(np.abs(X[c])), c
+    _check_columns(K, B, X, 1e-8 * (1 + 1e-6))
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_block_pcg_4m_rows_32_columns(laplacian_4m):
+    """The same 32 right-hand sides with block PCG (one Krylov space for all
+    32 columns, 32 x 32 step matrices; FP32 preconditioner, FP64 Krylov
+    vectors and reductions): every column's true relative residual at most
+    1e-8 (the solver stops on its recursive residual, which drifts from the
+    true one by rounding: allowed 1 % on top) and zero mean, in far fewer
+    iterations than a single column needs."""
+    import time
+    from nsfem_amd import cu
+    K, B = laplacian_4m
+    n = K.shape[0]
+    s = cu.createSolver("bpcg_mixed", 32, K, 20000, 1e-8, 1)
+    s.setOption("projectMean", 1)
+    t = time.perf_counter()
+    s.getRhs().upload(B.ravel())
+    s.getSol().memsetZero()
+    s.solve()
+    X = s.getSol().download().reshape(32, n)
+    its = s.getLastIterations()
+    print(f"block PCG, 32 columns: {its} iterations, {time.perf_counter() - t:.1f} s", flush=True)
+    _check_columns(K, B, X, 1.01e-8)
+    assert its < 2000
 
 
 @pytest.mark.timeout(120)
@@ -242,3 +282,86 @@ def test_batch_stability_limit_is_the_tightest_instance(method):
     assert max(singles) / min(singles) > 100
     lam = fem.Chorin(fem.ChorinContext.batch(meshes, conds), cfg).time_step_info()["lambdaNu"]
     assert abs(lam / max(singles) - 1) < 0.05, (lam, singles)
+
+
+
+def _laplacian_3000(oracle):
+    from nsfem_amd import mesher
+    return _p1_laplacian(oracle, mesher.channel_mesh(3000, seed=77))
+
+
+@pytest.mark.parametrize("method,width,tol", [("bpcg", 4, 1e-13), ("bpcg", 8, 1e-13), ("bpcg", 32, 1e-13),
+                                              ("bpcg_mixed", 8, 1e-10), ("bpcg_mixed", 16, 1e-10), ("bpcg_mixed", 32, 1e-10)])
+def test_block_pcg_matches_oracle(oracle, method, width, tol):
+    """Block PCG (one Krylov space for the tile, width x width step matrices,
+    the symmetric multicolour GS preconditioner; bpcg_mixed: the
+    preconditioner in FP32) on the pure-Neumann P1 Laplacian with mean
+    projection.  The solve ends on verified true residuals, so every column's
+    true relative residual (FP64, host) is at most tol (1 % for the
+    difference in summation order); the column equals the oracle's
+    single-column SGS-PCG solution (1e-13, made zero-mean) within what the
+    residuals allow (cond(A) ~ 1e4 times tol: 1e-9 at 1e-13, 1e-5 at 1e-10);
+    fewer iterations than the oracle's slowest column at the same
+    tolerance."""
+    from nsfem_amd import cu
+    K = _laplacian_3000(oracle)
+    n = K.shape[0]
+    rng = np.random.default_rng(9)
+    B = rng.uniform(-1, 1, (width, n))
+    B -= B.mean(axis=1, keepdims=True)
+    s = cu.createSolver(method, width, K, 5000, tol, 1)
+    s.setOption("projectMean", 1)
+    s.getRhs().upload(B.ravel())
+    s.getSol().memsetZero()
+    s.solve()
+    X = s.getSol().download().reshape(width, n)
+    layout = oracle.Mcgs(K)
+    its = []
+    bound = 1e-9 if tol <= 1e-13 else 1e-5
+    for c in range(width):
+        rel = np.linalg.norm(B[c] - K @ X[c]) / np.linalg.norm(B[c])
+        assert rel <= 1.01 * tol, (c, rel)
+        assert s.getLastRelRes(c) <= tol
+        its.append(oracle.pcg_solve(None, B[c], np.zeros(n), tol, 5000, layout)[2])
+        xo = oracle.pcg_solve(None, B[c], np.zeros(n), 1e-13, 5000, layout)[0]
+        xo -= xo.mean()
+        assert _rel(X[c], xo) <= bound, (c, _rel(X[c], xo))
+        assert abs(X[c].mean()) <= 1e-12 * np.max(np.abs(X[c]))
+    print(f"{method} x{width} at {tol:g}: {s.getLastIterations()} block iterations, {s.getLastOuterIterations()} more cycles; "
+          f"single columns {min(its)}-{max(its)}")
+    assert s.getLastIterations() < max(its)
+
+
+def test_block_pcg_rank_deficient_block(oracle):
+    """A tile whose right-hand sides are linearly dependent (column 1 equals
+    column 0, column 3 is column 0 + column 2): the Cholesky factorisations
+    drop the dependent directions and every column still converges, the
+    dependent columns to the matching combinations."""
+    from nsfem_amd import cu
+    K = _laplacian_3000(oracle)
+    n = K.shape[0]
+    rng = np.random.default_rng(10)
+    B = rng.uniform(-1, 1, (4, n))
+    B[1] = B[0]
+    B[3] = B[0] + B[2]
+    B -= B.mean(axis=1, keepdims=True)
+    s = cu.createSolver("bpcg", 4, K, 5000, 1e-12, 1)
+    s.setOption("projectMean", 1)
+    s.getRhs().upload(B.ravel())
+    s.getSol().memsetZero()
+    s.solve()
+    X = s.getSol().download().reshape(4, n)
+    for c in range(4):
+        rel = np.linalg.norm(B[c] - K @ X[c]) / np.linalg.norm(B[c])
+        assert rel <= 2e-12, (c, rel)
+    assert _rel(X[1], X[0]) <= 1e-9
+    assert _rel(X[3], X[0] + X[2]) <= 1e-9
+
+
+def test_block_pcg_rejects_unsupported_shapes(oracle):
+    from nsfem_amd import _lib, cu
+    K = _laplacian_3000(oracle)
+    with pytest.raises(_lib.NsfemInvalidArgument):
+        cu.createSolver("bpcg", 2, K, 100, 1e-8, 1)
+    with pytest.raises(_lib.NsfemInvalidArgument):
+        cu.createSolver("bpcg", 8, K, 100, 1e-8, 1, dtype=np.float32)
