@@ -17,18 +17,21 @@
 //             beta = rho^-1 rho'
 //   direction P = Z + P beta
 // The small solves are Cholesky factorisations in one workgroup with a
-// relative pivot threshold of 1e-13, deterministic like every reduction here
-// (fixed summation order).
+// relative pivot threshold (1e-13 for P^T A P; for rho, which is
+// symmetrised, one that matches the preconditioner's precision),
+// deterministic like every reduction here (fixed summation order).
 //
 // Breakdown and verification.  Near the FP64 floor a block's directions
 // become linearly dependent and the block recurrences lose accuracy (the
 // residual stalls or grows).  Right-hand sides that are themselves dependent
 // give a rank-deficient block from the start, which the dropped pivots
 // handle.  The beta kernel therefore stops the iteration (flags[0] = 1) and
-// asks the host for a restart (flags[3] = 1) when a factorisation keeps fewer
-// directions than the cycle's first one did (rank lost on the way) or the
-// largest relative residual grows 1000-fold over the best of the cycle; and
-// it asks for a verification (flags[3] = 2) when the recursive residuals have
+// asks the host for a restart (flags[3] = 1) when, after the cycle has cut
+// its largest relative residual 10-fold, a factorisation keeps fewer
+// directions than it did before (rank lost on the way; before such progress
+// the lower rank becomes the cycle's reference), or when the largest
+// relative residual grows 1000-fold over the best of the cycle; and it asks
+// for a verification (flags[3] = 2) when the recursive residuals have
 // converged.  Either way the host restarts from the
 // current X with the true residual B - A X; the solve ends when the true
 // residual of every column meets the tolerance (or at maxIters).
@@ -42,20 +45,31 @@ namespace nsfem
     {
         double * gramParts; // blocks * NCH * NCH
         double * normParts; // blocks * 2 * NCH: [bb | rr] per block
+        double * gramTotal; // kBcgSlices * NCH * NCH: slice sums of gramParts
+        double * normTotal; // kBcgSlices * 2 * NCH: slice sums of normParts
         double * rho;       // NCH * NCH
         double * step;      // NCH * NCH: alpha, then beta
         double * bb;        // NCH: |b_c|^2
         double * rr;        // NCH: |r_c|^2 after the last update
-        double * best;      // [0]: smallest max_c |r_c|^2 / |b_c|^2 of the cycle
+        double * best;      // [0]: smallest max_c |r_c|^2 / |b_c|^2 of the cycle, [1]: its value at the cycle's start
         int * flags;        // the solve flags ([0] stop, [3] restart 1 / verify 2), shared with the sweeps' state
-        int * iters;        // [0]: updates done; [1]: 1 when the cycle lost rank; [2]: the cycle's first rank (0: unset)
+        int * iters;        // [0]: updates done; [1]: unused; [2], [3]: the cycle's reference rank of
+                            // the alpha and the beta factorisation (0: unset); [4]: alpha's last rank
         double tol2;
+        // relative pivot threshold of the rho factorisation: rho = Z^T R
+        // carries the preconditioner's rounding (about 1e-7 relative with an
+        // FP32 preconditioner, where 1e-6 is used; 1e-13 with FP64)
+        double rhoThreshold;
         int n;
         int blocks;         // workgroups of the row-wise kernels
         int rowsPerBlock;   // a multiple of kBlock / NCH
     };
 
     constexpr int kBcgSmallBlock = 1024;
+    // The Gram partials of the row blocks are summed in kBcgSlices slices of
+    // blocks (b = slice mod kBcgSlices) by as many workgroups per group of 64
+    // entries; the small kernels add the slice sums in slice order.
+    constexpr int kBcgSlices = 16;
 
     // The value of lane `a` of this thread's NCH-lane group.
     template <int NCH, typename V>
@@ -151,7 +165,7 @@ namespace nsfem
     // the NCH columns of B (LDS); a dropped pivot's row of X is zero.
     // Returns (in every thread) the number of dropped pivots.
     template <int NCH>
-    __device__ int cholSolve(double * G, const double * B, double * X, double * piv, int * flag)
+    __device__ int cholSolve(double * G, const double * B, double * X, double * piv, int * flag, double relThreshold)
     {
         const int t = static_cast<int>(threadIdx.x);
         if (t == 0)
@@ -165,7 +179,7 @@ namespace nsfem
             *flag = 0;
         }
         __syncthreads();
-        const double thr = 1e-13 * piv[NCH];
+        const double thr = relThreshold * piv[NCH];
         for (int j = 0; j < NCH; j++)
         {
             if (t == 0)
@@ -222,20 +236,65 @@ namespace nsfem
         return *flag;
     }
 
-    // Sum of the Gram partials of all row blocks, in block order.
+    // Slice sums of the Gram partials (and, with norms, of the norm
+    // partials): workgroup (w, slice) sums entries [64 w, 64 w + 64) over the
+    // row blocks b = slice (mod kBcgSlices), four interleaved sub-slices per
+    // entry combined in a fixed order; the workgroups after the Gram ones sum
+    // the 2 NCH norm entries.
     template <int NCH>
-    __device__ __forceinline__ void gramSum(const BcgState & st, double * out)
+    __global__ void __launch_bounds__(kBlock) bcgGramReduceKernel(BcgState st, int norms)
     {
-        for (int e = static_cast<int>(threadIdx.x); e < NCH * NCH; e += blockDim.x)
+        __shared__ double sm[kBlock];
+        if (st.flags[0])
+        {
+            return;
+        }
+        constexpr int kGramGroups = (NCH * NCH + 63) / 64;
+        const int w = blockIdx.x / kBcgSlices;
+        const int slice = blockIdx.x % kBcgSlices;
+        const int t = static_cast<int>(threadIdx.x);
+        const int el = t & 63, sub = t >> 6;
+        const bool gram = w < kGramGroups;
+        if (!gram && !norms)
+        {
+            return;
+        }
+        const int e = gram ? w * 64 + el : el;
+        const int width = gram ? NCH * NCH : 2 * NCH;
+        const double * parts = gram ? st.gramParts : st.normParts;
+        double acc = 0;
+        if (e < width)
+        {
+            constexpr int stride = kBcgSlices * (kBlock / 64);
+            for (int b = slice + kBcgSlices * sub; b < st.blocks; b += stride)
+            {
+                acc += parts[static_cast<size_t>(b) * width + e];
+            }
+        }
+        sm[t] = acc;
+        __syncthreads();
+        if (sub == 0 && e < width)
         {
             double s = 0;
-            for (int b = 0; b < st.blocks; b++)
+#pragma unroll
+            for (int k = 0; k < kBlock / 64; k++)
             {
-                s += st.gramParts[static_cast<size_t>(b) * NCH * NCH + e];
+                s += sm[k * 64 + el];
             }
-            out[e] = s;
+            (gram ? st.gramTotal : st.normTotal)[static_cast<size_t>(slice) * width + e] = s;
         }
-        __syncthreads();
+    }
+
+    // Sum of the slice totals of entry e (width entries per slice).
+    __device__ __forceinline__ double sliceSum(const double * totals, int width, int e)
+    {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < kBcgSlices; k++)
+        {
+            s += totals[static_cast<size_t>(k) * width + e];
+        }
+        return s;
     }
 
     // alpha = (P^T Q)^-1 rho
@@ -248,26 +307,23 @@ namespace nsfem
         {
             return;
         }
-        gramSum<NCH>(st, G);
         for (int e = static_cast<int>(threadIdx.x); e < NCH * NCH; e += blockDim.x)
         {
+            G[e] = sliceSum(st.gramTotal, NCH * NCH, e);
             B[e] = st.rho[e];
         }
         __syncthreads();
-        const int kept = NCH - cholSolve<NCH>(G, B, X, piv, &dropped);
+        const int kept = NCH - cholSolve<NCH>(G, B, X, piv, &dropped, 1e-13);
         for (int e = static_cast<int>(threadIdx.x); e < NCH * NCH; e += blockDim.x)
         {
             st.step[e] = X[e];
         }
         if (threadIdx.x == 0)
         {
+            st.iters[4] = kept; // judged by the beta kernel after this update
             if (st.iters[2] == 0)
             {
-                st.iters[2] = kept; // the cycle's rank
-            }
-            else if (kept < st.iters[2])
-            {
-                st.iters[1] = 1; // rank lost on the way: restart after this update
+                st.iters[2] = kept; // the cycle's reference rank
             }
         }
     }
@@ -452,7 +508,16 @@ namespace nsfem
             return;
         }
         const int t = static_cast<int>(threadIdx.x);
-        gramSum<NCH>(st, Rn);
+        for (int e = t; e < NCH * NCH; e += blockDim.x)
+        {
+            X[e] = sliceSum(st.gramTotal, NCH * NCH, e);
+        }
+        __syncthreads();
+        for (int e = t; e < NCH * NCH; e += blockDim.x)
+        {
+            const int i = e / NCH, k = e % NCH;
+            Rn[e] = 0.5 * (X[e] + X[k * NCH + i]); // Z^T R is symmetric up to the preconditioner's rounding
+        }
         if (t == 0)
         {
             allDone = 1;
@@ -464,14 +529,11 @@ namespace nsfem
         {
             for (int c = 0; c < NCH; c++)
             {
-                double rr = 0, bb = 0;
-                for (int b = 0; b < st.blocks; b++)
+                const double rr = sliceSum(st.normTotal, 2 * NCH, NCH + c);
+                double bb = 0;
+                if (init == 2)
                 {
-                    rr += st.normParts[static_cast<size_t>(b) * 2 * NCH + NCH + c];
-                    if (init == 2)
-                    {
-                        bb += st.normParts[static_cast<size_t>(b) * 2 * NCH + c];
-                    }
+                    bb = sliceSum(st.normTotal, 2 * NCH, c);
                 }
                 if (init == 2)
                 {
@@ -526,15 +588,16 @@ namespace nsfem
             if (t == 0)
             {
                 st.best[0] = worst;
+                st.best[1] = worst;
             }
             return;
         }
         const double best = fmin(st.best[0], worst);
-        if (st.iters[1] != 0 || worst > 1e6 * best)
+        if (worst > 1e6 * best)
         {
             if (t == 0)
             {
-                st.flags[3] = 1;
+                st.flags[3] = 1; // the residual grew 1000-fold: restart
                 st.flags[0] = 1;
             }
             return;
@@ -544,16 +607,25 @@ namespace nsfem
             G[e] = st.rho[e];
         }
         __syncthreads();
-        const int kept = NCH - cholSolve<NCH>(G, Rn, X, piv, &dropped);
-        if (kept < st.iters[2])
+        const int kept = NCH - cholSolve<NCH>(G, Rn, X, piv, &dropped, st.rhoThreshold);
+        // rank lost after progress: restart; before it: the new reference
+        const bool progressed = best < 1e-2 * st.best[1];
+        const int keptA = st.iters[4];
+        const bool lost = keptA < st.iters[2] || (st.iters[3] != 0 && kept < st.iters[3]);
+        if (lost && progressed)
         {
             if (t == 0)
             {
-                st.iters[1] = 1;
                 st.flags[3] = 1;
                 st.flags[0] = 1;
             }
             return;
+        }
+        __syncthreads();
+        if (t == 0)
+        {
+            st.iters[2] = min(st.iters[2], keptA);
+            st.iters[3] = st.iters[3] == 0 ? kept : min(st.iters[3], kept);
         }
         for (int e = t; e < NCH * NCH; e += blockDim.x)
         {

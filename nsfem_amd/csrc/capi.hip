@@ -77,6 +77,11 @@ namespace
         {
             solver.setProjectMean(value != 0);
         }
+        else if (k == "blockPivot")
+        {
+            if (!(value > 0 && value < 1)) throw std::invalid_argument("blockPivot must be in (0, 1)");
+            solver.setBlockPivot(value);
+        }
         else if (k == "mixedRestart")
         {
             solver.setMixedRestart(value != 0);

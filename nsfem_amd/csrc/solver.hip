@@ -474,17 +474,18 @@ namespace nsfem
         statSlots_.resize(kStatSlots * (8 * static_cast<size_t>(K) * sizeof(double) + (3 * static_cast<size_t>(K) + 4) * sizeof(int)));
         if (opt_.method == SolverMethod::BlockPcg)
         {
-            // at most 512 workgroups, each a whole number of row slabs
+            // at most 2048 workgroups (eight per CU), each a whole number of
+            // row slabs; their Gram partials are summed by a reduction kernel
             const int nch = opt_.numCh;
             const int slab = kBlock / nch;
-            int rows = ceilDiv(n_, 512);
+            int rows = ceilDiv(n_, 2048);
             rows = std::max(slab, ceilDiv(rows, slab) * slab);
             bcgRows_ = rows;
             bcgBlocks_ = std::max(1, ceilDiv(n_, rows));
             bcgGram_.resize(static_cast<size_t>(bcgBlocks_) * nch * nch);
             bcgNorm_.resize(static_cast<size_t>(bcgBlocks_) * 2 * nch);
-            bcgSmall_.resize(2 * static_cast<size_t>(nch) * nch + 2 * nch + 1);
-            bcgInts_.resize(3);
+            bcgSmall_.resize((2 + kBcgSlices) * static_cast<size_t>(nch) * nch + (2 + 2 * kBcgSlices) * nch + 2);
+            bcgInts_.resize(5);
         }
         syncLibraryStream();
     }
@@ -1404,9 +1405,12 @@ namespace nsfem
         b.bb = b.step + nch * nch;
         b.rr = b.bb + nch;
         b.best = b.rr + nch;
+        b.gramTotal = b.best + 2;
+        b.normTotal = b.gramTotal + kBcgSlices * nch * nch;
         b.flags = st.flags;
         b.iters = bcgInts_.get();
         b.tol2 = opt_.target * opt_.target;
+        b.rhoThreshold = blockPivot_ > 0 ? blockPivot_ : (opt_.precision == SolverPrecision::Mixed ? 1e-10 : 1e-13);
         b.n = n_;
         b.blocks = bcgBlocks_;
         b.rowsPerBlock = bcgRows_;
@@ -1437,7 +1441,9 @@ namespace nsfem
         const int g = bcgBlocks_;
         dispatchWide(opt_.numCh, [&]<int C>()
                      {
+                         constexpr int red = (C * C + 63) / 64 + 1;
                          bcgProductKernel<double, C><<<g, kBlock, 0, s>>>(A, b, p64_.get(), q64_.get());
+                         bcgGramReduceKernel<C><<<(red - 1) * kBcgSlices, kBlock, 0, s>>>(b, 0);
                          bcgAlphaKernel<C><<<1, kBcgSmallBlock, 0, s>>>(b);
                          bcgUpdateKernel<double, P, C><<<g, kBlock, 0, s>>>(b, M.invDiag, p64_.get(), q64_.get(), x64_.get(), r64_.get(),
                                                                              rP, z, color0End_);
@@ -1445,7 +1451,9 @@ namespace nsfem
         sgsSweeps<P>(s, M, st, rP, z);
         dispatchWide(opt_.numCh, [&]<int C>()
                      {
+                         constexpr int red = (C * C + 63) / 64 + 1;
                          bcgRhoKernel<P, double, C><<<g, kBlock, 0, s>>>(M, b, rP, r64_.get(), z, color0End_);
+                         bcgGramReduceKernel<C><<<red * kBcgSlices, kBlock, 0, s>>>(b, 1);
                          bcgBetaKernel<C><<<1, kBcgSmallBlock, 0, s>>>(b, 0);
                          bcgDirectionKernel<double, P, C><<<g, kBlock, 0, s>>>(b, z, p64_.get(), 0);
                      });
@@ -1461,7 +1469,7 @@ namespace nsfem
         const McgsMatrix<double> A = A64_.view(n_);
         const int g = bcgBlocks_;
         fillInt(K, 1, st.active, s); // the sweeps run while the solve does
-        fillInt(3, 0, b.iters, s);
+        fillInt(5, 0, b.iters, s);
         // cycles: a (re)start on the true residual B - A X, then iterations in
         // chunks with lagged polls of the stop flag (as the single-RHS
         // solver); the kernels after a stop exit at once.  A stop with a
@@ -1473,8 +1481,15 @@ namespace nsfem
         int hostFlags[4] = {0, 0, 0, 0};
         for (;; cycles++)
         {
+            if (cycles > 0 && opt_.projectMean)
+            {
+                // verify (and restart from) the zero-mean solution that is
+                // returned: A 1 = 0 only up to rounding, so a mean removed
+                // after the check would change the residual
+                projectMean<double>(s, x64_.get());
+            }
             fillInt(4, 0, st.flags, s);
-            fillInt(2, 0, b.iters + 1, s); // the cycle's rank flags
+            fillInt(4, 0, b.iters + 1, s); // the cycle's rank bookkeeping
             const int first = cycles == 0;
             dispatchWide(nch, [&]<int C>()
                          {
@@ -1484,7 +1499,9 @@ namespace nsfem
             sgsSweeps<P>(s, M, st, rP, z);
             dispatchWide(nch, [&]<int C>()
                          {
+                             constexpr int red = (C * C + 63) / 64 + 1;
                              bcgRhoKernel<P, double, C><<<g, kBlock, 0, s>>>(M, b, rP, r64_.get(), z, color0End_);
+                             bcgGramReduceKernel<C><<<red * kBcgSlices, kBlock, 0, s>>>(b, 1);
                              bcgBetaKernel<C><<<1, kBcgSmallBlock, 0, s>>>(b, first ? 2 : 1);
                              bcgDirectionKernel<double, P, C><<<g, kBlock, 0, s>>>(b, z, p64_.get(), 1);
                          });
@@ -1517,9 +1534,12 @@ namespace nsfem
                 }
                 slot = prev;
             }
+            int done = 0;
             NSFEM_HIP(hipMemcpyAsync(hostFlags, st.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+            NSFEM_HIP(hipMemcpyAsync(&done, b.iters, sizeof(int), hipMemcpyDeviceToHost, s));
             NSFEM_HIP(hipStreamSynchronize(s));
             (void)stopped;
+            launched = done; // the budget counts updates, not the idle launches after a stop
             const bool again = hostFlags[0] != 0 && hostFlags[2] == 0 && (hostFlags[3] == 1 || hostFlags[3] == 2);
             if (!again || launched >= opt_.maxIters || cycles + 1 >= kMaxCycles)
             {

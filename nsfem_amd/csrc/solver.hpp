@@ -169,6 +169,9 @@ namespace nsfem
         void setInnerTol(double t) { opt_.innerTol = t; }
         void setMixedRestart(bool on) { opt_.restart = on; }
         void setProjectMean(bool on) { opt_.projectMean = on; }
+        // block PCG: relative pivot threshold of the rho = Z^T R
+        // factorisation (0: 1e-13 with an FP64 preconditioner, 1e-10 with FP32)
+        void setBlockPivot(double t) { blockPivot_ = t; }
         const SolverOptions & options() const { return opt_; }
 
         hipStream_t stream() const { return stream_; }
@@ -354,6 +357,7 @@ namespace nsfem
         DeviceArray<double> bcgGram_, bcgNorm_, bcgSmall_;
         DeviceArray<int> bcgInts_;
         bool lastDropped_ = false;
+        double blockPivot_ = 0;
         DeviceArray<double> means_;
         int * hostFlags_ = nullptr; // pinned
         int * pollRing_ = nullptr;  // pinned, two slots of the solve flags
