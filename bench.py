@@ -114,6 +114,39 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     return ch, res, (u1, p1), (u, p)
 
 
+def run_ensemble(args, ctx, cond, u0, target):
+    """B flows on the one mesh (instance b: inflow scale 1 - 0.4 b / (B - 1),
+    start u0 for b = 0, other seeds otherwise) stepped together: every
+    matrix is shared, the velocity solves are one pcg_mixed solve over 2B
+    columns and the pressure one solve over B columns (--ensemble-pressure:
+    pcg_mixed, independent columns sharing each matrix read, or bpcg_mixed,
+    one block Krylov space)."""
+    import numpy as np
+    from nsfem_amd import F64, fem
+    B = args.ensemble
+    scales = [1.0 - 0.4 * b / max(1, B - 1) for b in range(B)]
+    starts = [u0]
+    N = ctx.numVelocityNodes
+    for b in range(1, B):
+        u = initial_velocity(ctx, cond, 1 + b)
+        ids, _ = ctx.dirichlet("vx")
+        idy, _ = ctx.dirichlet("vy")
+        u[ids] *= scales[b]
+        u[idy + N] *= scales[b]
+        starts.append(u)
+    pressure = args.ensemble_pressure
+    cfg = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 5000, target, 1), fem.SolverConfig(pressure, 50000, target, 1),
+                           dtype=F64, timeStepPolicy=fem.TAU_CLAMP)
+    ch, r, _, last = run_path(f"ensemble{B}", ctx, cfg, args.tau, np.concatenate(starts), args.warmup, args.steps,
+                              not args.no_autotune, make=lambda c: fem.Chorin(ctx, c, instances=B, inflow_scale=scales))
+    del ch
+    r["instances"] = B
+    r["inflow_scales"] = scales
+    r["solvers"] = {"velocity": "pcg_mixed", "pressure": pressure}
+    r["instance_steps_per_s"] = r["steps_per_s"] * B
+    return r, last
+
+
 def kernel_table(ch, res, reps=30, with_iteration=True):
     """Per-kernel average launch time (HIP events on the library stream,
     replaying the launches one solver iteration makes), algorithmic bytes per
@@ -510,6 +543,11 @@ def main():
                     help="seconds of timed CPU-baseline steps (at most 10 steps, at least 2)")
     ap.add_argument("--eager", action="store_true",
                     help="launch solver iterations from the host (no graph replay; for kernel tracing tools)")
+    ap.add_argument("--ensemble", type=int, default=8,
+                    help="also run an ensemble of this many flows on the same mesh (inflow scales 1.0 down to 0.6; "
+                         "mixed velocity solves over 2B columns, pressure over B); 0 = skip")
+    ap.add_argument("--ensemble-pressure", default="pcg_mixed", choices=["pcg_mixed", "bpcg_mixed"],
+                    help="the ensemble's pressure solver")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run 3 in-place scalings of an array this size (known traffic, to scale HBM byte counters)")
     args = ap.parse_args()
@@ -603,6 +641,20 @@ def main():
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
+    ensemble = None
+    if args.ensemble > 0:
+        ensemble, ens_last = run_ensemble(args, ctx, cond, u0, target)
+        if "fp64" in results:
+            # instance 0 has the single flow's inflow and start
+            N = ctx.numVelocityNodes
+            ensemble["instance0_vs_fp64"] = {"velocity": rel(ens_last[0][:2 * N], lasts["fp64"][0]),
+                                            "pressure": rel(ens_last[1][:ctx.numPressureNodes], lasts["fp64"][1])}
+            ensemble["instance_steps_over_fp64_steps"] = ensemble["instance_steps_per_s"] / results["fp64"]["steps_per_s"]
+        if "mixed" in results:
+            ensemble["instance_steps_over_mixed_steps"] = ensemble["instance_steps_per_s"] / results["mixed"]["steps_per_s"]
+        log(f"ensemble: {ensemble['instance_steps_per_s']:.2f} instance-steps/s, "
+            f"{ensemble.get('instance_steps_over_fp64_steps', 0):.2f}x the FP64 single flow")
+
     errors = {}
     if "fp64" in results and "mixed" in results:
         errors["mixed_vs_fp64_after_steps"] = {"velocity": rel(lasts["mixed"][0], lasts["fp64"][0]),
@@ -660,6 +712,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "ensemble": ensemble,
         "setup_s": setup_s,
     }
     print(json.dumps(out))

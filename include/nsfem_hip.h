@@ -370,6 +370,20 @@ typedef struct nsfem_step_stats
 
 typedef struct nsfem_chorin nsfem_chorin;
 int nsfem_chorin_create(const nsfem_context * c, const nsfem_chorin_config * cfg, nsfem_chorin ** out);
+/* Ensemble: `instances` (1, 2, 4, 8 or 16) independent flows on the one mesh
+ * of a single-mesh context, instance b's Dirichlet velocity (the inflow
+ * profile) scaled by inflowScale[b] (NULL: all 1).  They share every matrix:
+ * each velocity solve is one solve of 2 x instances right-hand sides, the
+ * pressure solve one of `instances` (a block solver such as bpcg_mixed for
+ * 4 or more); the convection is each instance's own.  The state functions
+ * then hold the instances one after another: velocity [u_0 .. u_B-1] (2N
+ * each), pressure [p_0 .. p_B-1] (V each); the step statistics are the worst
+ * over the instances.  Parameter sweeps over one geometry (inflow speeds,
+ * perturbed starts) in place of one chorinCuda.cu run per case.  No reference
+ * counterpart. */
+int nsfem_chorin_create_ensemble(const nsfem_context * c, const nsfem_chorin_config * cfg, int instances, const double * inflowScale,
+                                 nsfem_chorin ** out);
+int nsfem_chorin_instances(const nsfem_chorin * ch, int * instances);
 int nsfem_chorin_destroy(nsfem_chorin * ch);
 /* Replaces the fixed tau = maxT / ceil(maxT / tau0) of chorinCuda.cu:221-222.
  * Under NSFEM_TAU_REJECT a tau above tau_max fails with NSFEM_EINVAL (the
@@ -393,7 +407,8 @@ int nsfem_chorin_set_step_graph_budgets(nsfem_chorin * ch, int tentative, int pr
 /* One projection step; stats may be NULL.  NSFEM_ENONFINITE on a bad
  * velocity solve ("Tentative/Final acceleration is bad", chorinCuda.cu:270-273, 349-352). */
 int nsfem_chorin_step(nsfem_chorin * ch, nsfem_step_stats * stats);
-/* Device state: velocity 2N ([ux; uy]), pressure V, in the chorin dtype. */
+/* Device state: velocity 2N ([ux; uy]), pressure V, in the chorin dtype
+ * (an ensemble: B times each, instance after instance). */
 int nsfem_chorin_state(nsfem_chorin * ch, void ** velocity, void ** pressure);
 int nsfem_chorin_download(nsfem_chorin * ch, void * velocity, void * pressure);
 /* Output path, replacing the two blocking per-step downloads
@@ -413,8 +428,8 @@ int nsfem_chorin_reset(nsfem_chorin * ch);
  * four for the colour sweeps, velocity direction / update / rho rows,
  * pressure direction / update / rho rows, the velocity and pressure
  * direction kernels' lanes, then whether each solver fuses its reductions};
- * autotune as nsfem_solver_autotune for both solvers (the state u, p is
- * kept). */
+ * autotune as nsfem_solver_autotune for each pcg / pcg_mixed solver of
+ * the step (other methods keep their shapes; the state u, p is kept). */
 int nsfem_chorin_shape(const nsfem_chorin * ch, int * shape);
 int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape);
 /* nsfem_solver_set_option for the step's solvers: which = 0 velocity,

@@ -164,6 +164,8 @@ _SIGNATURES = {
                                   ctypes.POINTER(_vp)],
     "nsfem_host_free": [_vp],
     "nsfem_chorin_create": [_vp, ctypes.POINTER(ChorinConfigC), ctypes.POINTER(_vp)],
+    "nsfem_chorin_create_ensemble": [_vp, ctypes.POINTER(ChorinConfigC), ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_vp)],
+    "nsfem_chorin_instances": [_vp, ctypes.POINTER(ctypes.c_int)],
     "nsfem_chorin_destroy": [_vp],
     "nsfem_chorin_set_time_step": [_vp, ctypes.c_double],
     "nsfem_chorin_time_step_info": [_vp, _c_double_p, _c_double_p, _c_double_p, _c_double_p],

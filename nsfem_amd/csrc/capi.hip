@@ -1104,6 +1104,12 @@ extern "C"
 
     int nsfem_chorin_create(const nsfem_context * c, const nsfem_chorin_config * cfg, nsfem_chorin ** out)
     {
+        return nsfem_chorin_create_ensemble(c, cfg, 1, nullptr, out);
+    }
+
+    int nsfem_chorin_create_ensemble(const nsfem_context * c, const nsfem_chorin_config * cfg, int instances, const double * inflowScale,
+                                     nsfem_chorin ** out)
+    {
         return guarded([&]
                        {
             need(c, "context");
@@ -1148,6 +1154,15 @@ extern "C"
                     {
                         throw std::invalid_argument("stepGraph needs pcg or pcg_mixed solvers");
                     }
+                }
+            }
+            cc.instances = instances;
+            if (inflowScale && instances > 0)
+            {
+                cc.inflowScale.assign(inflowScale, inflowScale + instances);
+                for (double v : cc.inflowScale)
+                {
+                    if (!std::isfinite(v)) throw std::invalid_argument("inflowScale values must be finite");
                 }
             }
             auto h = std::make_unique<nsfem_chorin>();
@@ -1256,9 +1271,11 @@ extern "C"
             const size_t ts = dtypeSize(ch->core->dtype());
             hipStream_t s = libraryStream();
             if (velocity)
-                NSFEM_HIP(hipMemcpyAsync(velocity, ch->core->velocity(), 2 * ts * ch->core->numVelocityNodes(), hipMemcpyDeviceToHost, s));
+                NSFEM_HIP(hipMemcpyAsync(velocity, ch->core->velocity(), 2 * ts * ch->core->numVelocityNodes() * ch->core->instances(),
+                                         hipMemcpyDeviceToHost, s));
             if (pressure)
-                NSFEM_HIP(hipMemcpyAsync(pressure, ch->core->pressure(), ts * ch->core->numPressureNodes(), hipMemcpyDeviceToHost, s));
+                NSFEM_HIP(hipMemcpyAsync(pressure, ch->core->pressure(), ts * ch->core->numPressureNodes() * ch->core->instances(),
+                                         hipMemcpyDeviceToHost, s));
             syncLibraryStream(); });
     }
 
@@ -1285,9 +1302,18 @@ extern "C"
             need(ch, "chorin");
             need(velocity, "velocity");
             const size_t ts = dtypeSize(ch->core->dtype());
-            NSFEM_HIP(hipMemcpyAsync(ch->core->velocity(), velocity, 2 * ts * ch->core->numVelocityNodes(), hipMemcpyHostToDevice,
-                                     libraryStream()));
+            NSFEM_HIP(hipMemcpyAsync(ch->core->velocity(), velocity, 2 * ts * ch->core->numVelocityNodes() * ch->core->instances(),
+                                     hipMemcpyHostToDevice, libraryStream()));
             syncLibraryStream(); });
+    }
+
+    int nsfem_chorin_instances(const nsfem_chorin * ch, int * instances)
+    {
+        return guarded([&]
+                       {
+            need(ch, "chorin");
+            need(instances, "instances");
+            *instances = ch->core->instances(); });
     }
 
     int nsfem_chorin_reset(nsfem_chorin * ch)
@@ -1335,8 +1361,11 @@ extern "C"
                        {
             need(ch, "chorin");
             if (reps < 1) throw std::invalid_argument("reps must be positive");
-            ch->core->velocitySolver().autotune(reps);
-            ch->core->pressureSolver().autotune(reps);
+            // block PCG keeps its fixed shapes
+            for (McgsSolver * sv : {&ch->core->velocitySolver(), &ch->core->pressureSolver()})
+            {
+                if (sv->method() == SolverMethod::Pcg) sv->autotune(reps);
+            }
             if (shape)
             {
                 chorinShape(*ch, shape);

@@ -103,6 +103,19 @@ namespace nsfem
         V_ = ctx.numPressureNodes;
         E_ = ctx.convection.numElements;
         nu_ = ctx.viscosity;
+        B_ = cfg_.instances;
+        if (B_ != 1 && B_ != 2 && B_ != 4 && B_ != 8 && B_ != 16)
+        {
+            throw std::invalid_argument("ensemble instances must be 1, 2, 4, 8 or 16");
+        }
+        if (B_ > 1 && !ctx.massBlocks.empty())
+        {
+            throw std::invalid_argument("an ensemble runs on a single-mesh context, not a batch");
+        }
+        if (!cfg_.inflowScale.empty() && static_cast<int>(cfg_.inflowScale.size()) != B_)
+        {
+            throw std::invalid_argument("inflowScale needs one value per instance");
+        }
         const hipStream_t s = libraryStream();
 
         std::vector<double> nuK(ctx.velocityStiffness.values);
@@ -112,6 +125,10 @@ namespace nsfem
         }
         upload(nuK_, ctx.velocityStiffness, &nuK);
         upload(A_, ctx.velocityStiffness, &nuK);
+        if (B_ > 1)
+        {
+            A_.values.resize(static_cast<size_t>(A_.nnz) * B_ * tsize_); // each instance's A (set every step)
+        }
         upload(vpd_, ctx.velocityPressureDiv);
         upload(pvd_, ctx.pressureVelocityDiv);
 
@@ -135,19 +152,26 @@ namespace nsfem
         NSFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gConvTx), &T.tx[0][0][0], sizeof(double) * 216));
         NSFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gConvTy), &T.ty[0][0][0], sizeof(double) * 216));
 
-        // Dirichlet velocity: X ids, then Y ids offset by N (chorinCuda.cu:41-54)
-        nDir_ = static_cast<int>(ctx.dirichletVx.size() + ctx.dirichletVy.size());
+        // Dirichlet velocity: X ids, then Y ids offset by N (chorinCuda.cu:41-54),
+        // for every instance (offset 2N b, values scaled by its inflow scale)
+        const int perInstance = static_cast<int>(ctx.dirichletVx.size() + ctx.dirichletVy.size());
+        nDir_ = perInstance * B_;
         std::vector<int> ids;
         std::vector<double> vals;
-        for (const auto & d : ctx.dirichletVx)
+        for (int b = 0; b < B_; b++)
         {
-            ids.push_back(d.id);
-            vals.push_back(d.value);
-        }
-        for (const auto & d : ctx.dirichletVy)
-        {
-            ids.push_back(d.id + N_);
-            vals.push_back(d.value);
+            const double scale = cfg_.inflowScale.empty() ? 1.0 : cfg_.inflowScale[b];
+            const int off = 2 * N_ * b;
+            for (const auto & d : ctx.dirichletVx)
+            {
+                ids.push_back(d.id + off);
+                vals.push_back(d.value * scale);
+            }
+            for (const auto & d : ctx.dirichletVy)
+            {
+                ids.push_back(d.id + N_ + off);
+                vals.push_back(d.value * scale);
+            }
         }
         dirIds_.upload(ids.empty() ? std::vector<int>{0} : ids);
         if (cfg_.dtype == F64)
@@ -162,9 +186,22 @@ namespace nsfem
         }
         nIntP_ = static_cast<int>(ctx.internalPressureNodes.size());
         internalP_.upload(ctx.internalPressureNodes);
+        if (B_ > 1)
+        {
+            std::vector<int> all;
+            all.reserve(static_cast<size_t>(nIntP_) * B_);
+            for (int b = 0; b < B_; b++)
+            {
+                for (int id : ctx.internalPressureNodes)
+                {
+                    all.push_back(id + b * V_);
+                }
+            }
+            pScatterIds_.upload(all);
+        }
 
-        u_.resize(static_cast<size_t>(2) * N_ * tsize_);
-        p_.resize(static_cast<size_t>(V_) * tsize_);
+        u_.resize(static_cast<size_t>(2) * N_ * B_ * tsize_);
+        p_.resize(static_cast<size_t>(V_) * B_ * tsize_);
 
         auto solverOpts = [&](const ChorinSolverConfig & sc, int nch)
         {
@@ -204,8 +241,8 @@ namespace nsfem
             for (const auto & m : ctx.massBlocks) vsys.push_back(&m);
             for (const auto & m : ctx.pressureBlocks) psys.push_back(&m);
         }
-        vSolver_ = std::make_unique<McgsSolver>(vsys, solverOpts(cfg_.velocity, 2));
-        pSolver_ = std::make_unique<McgsSolver>(psys, solverOpts(cfg_.pressure, 1));
+        vSolver_ = std::make_unique<McgsSolver>(vsys, solverOpts(cfg_.velocity, 2 * B_));
+        pSolver_ = std::make_unique<McgsSolver>(psys, solverOpts(cfg_.pressure, B_));
         vSolver_->setUseGraphs(cfg_.graphs);
         pSolver_->setUseGraphs(cfg_.graphs);
         for (auto & e : ev_)
@@ -253,8 +290,8 @@ namespace nsfem
         {
             throw std::invalid_argument("a state download is still pending (wait for it first)");
         }
-        const size_t bu = 2 * tsize_ * static_cast<size_t>(N_);
-        const size_t bp = tsize_ * static_cast<size_t>(V_);
+        const size_t bu = 2 * tsize_ * static_cast<size_t>(N_) * B_;
+        const size_t bp = tsize_ * static_cast<size_t>(V_) * B_;
         if (!copyStream_)
         {
             // set up on first use: snapshot buffers, pinned staging, a copy
@@ -289,11 +326,11 @@ namespace nsfem
         dlPending_ = false;
         if (velocity)
         {
-            std::memcpy(velocity, hostU_, 2 * tsize_ * static_cast<size_t>(N_));
+            std::memcpy(velocity, hostU_, 2 * tsize_ * static_cast<size_t>(N_) * B_);
         }
         if (pressure)
         {
-            std::memcpy(pressure, hostP_, tsize_ * static_cast<size_t>(V_));
+            std::memcpy(pressure, hostP_, tsize_ * static_cast<size_t>(V_) * B_);
         }
     }
 
@@ -305,8 +342,9 @@ namespace nsfem
         }
         const hipStream_t s = libraryStream();
         const int dt = cfg_.dtype;
-        const size_t n2 = 2 * static_cast<size_t>(N_);
-        // two independent seeded starts, one per solver channel
+        const int nch = vSolver_->numCh();
+        const size_t n2 = static_cast<size_t>(nch) * N_;
+        // independent seeded starts, one per solver channel
         std::vector<double> h(n2);
         uint64_t st = 0x9E3779B97F4A7C15ull;
         for (auto & v : h)
@@ -349,7 +387,7 @@ namespace nsfem
             for (int k = 0; k < S; k++)
             {
                 const long n = off[k + 1] - off[k];
-                for (int c = 0; c < 2; c++)
+                for (int c = 0; c < nch; c++)
                 {
                     double nrm = 0;
                     dot(dt, n, 1, slice(x.get(), k, c), n, slice(x.get(), k, c), n, &nrm, s);
@@ -360,14 +398,14 @@ namespace nsfem
                     scale(dt, n, std::sqrt(n / nrm), slice(x.get(), k, c), s);
                 }
             }
-            csrMul(K, 2, x.get(), N_, rhs, N_, 1.0, 0.0, nullptr, N_, 0, s);
+            csrMul(K, nch, x.get(), N_, rhs, N_, 1.0, 0.0, nullptr, N_, 0, s);
             vSolver_->solve(true);
-            csrMul(K, 2, y, N_, w.get(), N_, 1.0, 0.0, nullptr, N_, 0, s);
+            csrMul(K, nch, y, N_, w.get(), N_, 1.0, 0.0, nullptr, N_, 0, s);
             lam = 0;
             for (int k = 0; k < S; k++)
             {
                 const long n = off[k + 1] - off[k];
-                for (int c = 0; c < 2; c++)
+                for (int c = 0; c < nch; c++)
                 {
                     double yKy = 0, yMy = 0;
                     dot(dt, n, 1, slice(y, k, c), n, slice(w.get(), k, c), n, &yKy, s);
@@ -424,10 +462,10 @@ namespace nsfem
     void ChorinGpu::resetState()
     {
         const hipStream_t s = libraryStream();
-        fill(cfg_.dtype, 2L * N_, 0.0, u_.get(), s);
-        fill(cfg_.dtype, V_, 0.0, p_.get(), s);
+        fill(cfg_.dtype, 2L * N_ * B_, 0.0, u_.get(), s);
+        fill(cfg_.dtype, static_cast<long>(V_) * B_, 0.0, p_.get(), s);
         imposeDirichlet(s);
-        fill(pSolver_->ioDtype(), static_cast<long>(pSolver_->n()), 0.0, pSolver_->ioSol(), s);
+        fill(pSolver_->ioDtype(), static_cast<long>(pSolver_->n()) * pSolver_->numCh(), 0.0, pSolver_->ioSol(), s);
         syncLibraryStream();
     }
 
@@ -518,14 +556,18 @@ namespace nsfem
 
     namespace
     {
-        // Worst instance of a (possibly batched) solve.
-        double maxMse(const McgsSolver & s, int c)
+        // Worst instance of a (possibly batched) solve over the channels
+        // c0, c0 + stride, ... (an ensemble's channels of one component).
+        double maxMse(const McgsSolver & s, int c0, int stride = 1)
         {
-            double m = s.lastMse(0, c);
-            for (int k = 1; k < s.numSystems(); k++)
+            double m = s.lastMse(0, c0);
+            for (int c = c0; c < s.numCh(); c += stride)
             {
-                const double v = s.lastMse(k, c);
-                m = (v != v || v > m) ? v : m;
+                for (int k = 0; k < s.numSystems(); k++)
+                {
+                    const double v = s.lastMse(k, c);
+                    m = (v != v || v > m) ? v : m;
+                }
             }
             return m;
         }
@@ -552,6 +594,39 @@ namespace nsfem
         }
     } // namespace
 
+    void ChorinGpu::launchConvectionRhs(hipStream_t s)
+    {
+        // per instance: A_b = nu K + C(u_b), then rhs_b = A_b u_b (the
+        // product's lanes per row follow its mean row length)
+        const int dt = cfg_.dtype;
+        const size_t uStride = 2 * static_cast<size_t>(N_) * tsize_;
+        const size_t aStride = static_cast<size_t>(A_.nnz) * tsize_;
+        for (int b = 0; b < B_; b++)
+        {
+            const char * u = static_cast<const char *>(u_.get()) + b * uStride;
+            char * a = static_cast<char *>(A_.values.get()) + b * aStride;
+            char * rhs = static_cast<char *>(vSolver_->ioRhs()) + b * uStride;
+            assembleConvection(u, a, nullptr, s);
+            CsrView Ab = A_.view(dt);
+            Ab.values = a;
+            csrMul(Ab, 2, u, N_, rhs, N_, 1.0, 0.0, nullptr, N_, 0, s);
+        }
+    }
+
+    void ChorinGpu::launchPressureRhs(hipStream_t s)
+    {
+        // rhs_b = -(1/tau) (VPD u_b)[internal]: instance b is pressure channel b
+        csrMul(vpd_.view(cfg_.dtype), B_, u_.get(), 2L * N_, pSolver_->ioRhs(), nIntP_, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
+    }
+
+    void ChorinGpu::launchScatterGradient(hipStream_t s)
+    {
+        // p_b[internal] = solution channel b, then rhs_b = PVD p_b
+        const int dt = cfg_.dtype;
+        scatter(dt, nIntP_ * B_, B_ > 1 ? pScatterIds_.get() : internalP_.get(), pSolver_->ioSol(), p_.get(), s);
+        csrMul(pvd_.view(dt), B_, p_.get(), V_, vSolver_->ioRhs(), 2L * N_, 1.0, 0.0, nullptr, 2 * N_, 0, s);
+    }
+
     void ChorinGpu::stepRegular(ChorinStepStats * stats)
     {
         const hipStream_t s = libraryStream();
@@ -560,16 +635,14 @@ namespace nsfem
         ChorinStepStats st{};
         NSFEM_HIP(hipEventRecord(ev_[0], s));
 
-        // Convection: A = nu K + C(u)
-        assembleConvection(u_.get(), A_.values.get(), nullptr, s);
+        // Convection: A = nu K + C(u) and the tentative right-hand side A u
+        launchConvectionRhs(s);
         NSFEM_HIP(hipEventRecord(ev_[1], s));
 
-        // Tentative velocity: M a = A u;  u -= tau a (the product's lanes per
-        // row follow its mean row length, not the solver shapes)
-        csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, 0, s);
+        // Tentative velocity: M a = A u;  u -= tau a
         vSolver_->solve(true); // cold start
-        st.mseTentX = maxMse(*vSolver_, 0);
-        st.mseTentY = maxMse(*vSolver_, 1);
+        st.mseTentX = maxMse(*vSolver_, 0, 2);
+        st.mseTentY = maxMse(*vSolver_, 1, 2);
         st.relTent = maxRel(*vSolver_);
         st.itersTent = maxIters(*vSolver_);
         st.outerTent = vSolver_->lastOuterIterations();
@@ -577,25 +650,24 @@ namespace nsfem
         {
             throw NonFiniteError("Tentative acceleration is bad");
         }
-        axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+        axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
         imposeDirichlet(s);
         NSFEM_HIP(hipEventRecord(ev_[2], s));
 
         // Pressure: Kp_int p = -(1/tau) (VPD u)[internal], warm start
-        csrMul(vpd_.view(dt), 1, u_.get(), 0, pSolver_->ioRhs(), 0, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
+        launchPressureRhs(s);
         pSolver_->solve();
         st.msePressure = maxMse(*pSolver_, 0);
         st.relPressure = maxRel(*pSolver_);
         st.itersPressure = maxIters(*pSolver_);
         st.outerPressure = pSolver_->lastOuterIterations();
-        scatter(dt, nIntP_, internalP_.get(), pSolver_->ioSol(), p_.get(), s);
         NSFEM_HIP(hipEventRecord(ev_[3], s));
 
         // Projection: M a = PVD p;  u -= tau a
-        csrMul(pvd_.view(dt), 1, p_.get(), 0, vSolver_->ioRhs(), 0, 1.0, 0.0, nullptr, 2 * N_, 0, s);
+        launchScatterGradient(s);
         vSolver_->solve(true); // cold start
-        st.mseFinalX = maxMse(*vSolver_, 0);
-        st.mseFinalY = maxMse(*vSolver_, 1);
+        st.mseFinalX = maxMse(*vSolver_, 0, 2);
+        st.mseFinalY = maxMse(*vSolver_, 1, 2);
         st.relFinal = maxRel(*vSolver_);
         st.itersFinal = maxIters(*vSolver_);
         st.outerFinal = vSolver_->lastOuterIterations();
@@ -603,7 +675,7 @@ namespace nsfem
         {
             throw NonFiniteError("Final acceleration is bad");
         }
-        axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+        axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
         imposeDirichlet(s);
         NSFEM_HIP(hipEventRecord(ev_[4], s));
         NSFEM_HIP(hipEventSynchronize(ev_[4]));
@@ -686,10 +758,11 @@ namespace nsfem
         const hipStream_t s = libraryStream();
         const int dt = cfg_.dtype;
         const long twoN = 2L * N_;
-        const size_t pBytes = static_cast<size_t>(pSolver_->n()) * dtypeSize(pSolver_->ioDtype());
-        if (stepSnapU_.size() < static_cast<size_t>(twoN) * tsize_)
+        const size_t pBytes = static_cast<size_t>(pSolver_->n()) * pSolver_->numCh() * dtypeSize(pSolver_->ioDtype());
+        const size_t uBytes = static_cast<size_t>(twoN) * B_ * tsize_;
+        if (stepSnapU_.size() < uBytes)
         {
-            stepSnapU_.resize(static_cast<size_t>(twoN) * tsize_);
+            stepSnapU_.resize(uBytes);
         }
         if (stepSnapP_.size() < pBytes)
         {
@@ -702,22 +775,20 @@ namespace nsfem
         try
         {
             // snapshot for a rollback
-            NSFEM_HIP(hipMemcpyAsync(stepSnapU_.get(), u_.get(), twoN * tsize_, hipMemcpyDeviceToDevice, s));
+            NSFEM_HIP(hipMemcpyAsync(stepSnapU_.get(), u_.get(), uBytes, hipMemcpyDeviceToDevice, s));
             NSFEM_HIP(hipMemcpyAsync(stepSnapP_.get(), pSolver_->ioSol(), pBytes, hipMemcpyDeviceToDevice, s));
-            assembleConvection(u_.get(), A_.values.get(), nullptr, s);
-            csrMul(A_.view(dt), 2, u_.get(), N_, vSolver_->ioRhs(), N_, 1.0, 0.0, nullptr, N_, 0, s);
+            launchConvectionRhs(s);
             vSolver_->enqueueSolve(s, true, budget_[0]);
             vSolver_->snapshotStats(s, 0);
-            axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+            axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
             imposeDirichlet(s);
-            csrMul(vpd_.view(dt), 1, u_.get(), 0, pSolver_->ioRhs(), 0, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
+            launchPressureRhs(s);
             pSolver_->enqueueSolve(s, false, budget_[1]);
             pSolver_->snapshotStats(s, 1);
-            scatter(dt, nIntP_, internalP_.get(), pSolver_->ioSol(), p_.get(), s);
-            csrMul(pvd_.view(dt), 1, p_.get(), 0, vSolver_->ioRhs(), 0, 1.0, 0.0, nullptr, 2 * N_, 0, s);
+            launchScatterGradient(s);
             vSolver_->enqueueSolve(s, true, budget_[2]);
             vSolver_->snapshotStats(s, 2);
-            axpy(dt, twoN, -tau_, vSolver_->ioSol(), u_.get(), s);
+            axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
             imposeDirichlet(s);
         }
         catch (...)
@@ -756,8 +827,8 @@ namespace nsfem
         NSFEM_HIP(hipEventSynchronize(ev_[4]));
         ChorinStepStats st{};
         const bool okT = vSolver_->collectStats(0);
-        st.mseTentX = maxMse(*vSolver_, 0);
-        st.mseTentY = maxMse(*vSolver_, 1);
+        st.mseTentX = maxMse(*vSolver_, 0, 2);
+        st.mseTentY = maxMse(*vSolver_, 1, 2);
         st.relTent = maxRel(*vSolver_);
         st.itersTent = maxIters(*vSolver_);
         const bool badT = !std::isfinite(st.mseTentX) || !std::isfinite(st.mseTentY) || vSolver_->lastNonFinite();
@@ -766,8 +837,8 @@ namespace nsfem
         st.relPressure = maxRel(*pSolver_);
         st.itersPressure = maxIters(*pSolver_);
         const bool okF = vSolver_->collectStats(2);
-        st.mseFinalX = maxMse(*vSolver_, 0);
-        st.mseFinalY = maxMse(*vSolver_, 1);
+        st.mseFinalX = maxMse(*vSolver_, 0, 2);
+        st.mseFinalY = maxMse(*vSolver_, 1, 2);
         st.relFinal = maxRel(*vSolver_);
         st.itersFinal = maxIters(*vSolver_);
         const bool badF = !std::isfinite(st.mseFinalX) || !std::isfinite(st.mseFinalY) || vSolver_->lastNonFinite();
@@ -775,8 +846,8 @@ namespace nsfem
         {
             // back to the state before the step (the regular path's state at
             // its error, or the start of the rerun)
-            const size_t pBytes = static_cast<size_t>(pSolver_->n()) * dtypeSize(pSolver_->ioDtype());
-            NSFEM_HIP(hipMemcpyAsync(u_.get(), stepSnapU_.get(), 2 * static_cast<size_t>(N_) * tsize_, hipMemcpyDeviceToDevice, s));
+            const size_t pBytes = static_cast<size_t>(pSolver_->n()) * pSolver_->numCh() * dtypeSize(pSolver_->ioDtype());
+            NSFEM_HIP(hipMemcpyAsync(u_.get(), stepSnapU_.get(), 2 * static_cast<size_t>(N_) * B_ * tsize_, hipMemcpyDeviceToDevice, s));
             NSFEM_HIP(hipMemcpyAsync(pSolver_->ioSol(), stepSnapP_.get(), pBytes, hipMemcpyDeviceToDevice, s));
             syncLibraryStream();
             if (badT)

@@ -61,6 +61,13 @@ namespace nsfem
         // The whole step as one captured graph (pcg solvers without
         // mixedRestart); see ChorinGpu::step.
         bool stepGraph = false;
+        // Ensemble: this many independent instances (1, 2, 4, 8 or 16) on the
+        // context's one mesh, instance b's inflow (the Dirichlet velocity)
+        // scaled by inflowScale[b] (empty: 1).  They share every matrix: the
+        // velocity solves are one solve of 2B right-hand sides, the pressure
+        // solve one of B; the convection is each instance's own.
+        int instances = 1;
+        std::vector<double> inflowScale;
     };
 
     struct ChorinStepStats
@@ -122,6 +129,7 @@ namespace nsfem
         void assembleConvection(const void * u, void * aOut, void * cOut, hipStream_t s);
 
         int dtype() const { return cfg_.dtype; }
+        int instances() const { return B_; }
         int numVelocityNodes() const { return N_; }
         int numPressureNodes() const { return V_; }
         void * velocity() { return u_.get(); }
@@ -157,6 +165,8 @@ namespace nsfem
 
         ChorinConfig cfg_;
         int N_ = 0, V_ = 0, E_ = 0;
+        int B_ = 1; // ensemble instances: state [u_0 .. u_B-1] (2N each), [p_0 .. p_B-1] (V each)
+        DeviceArray<int> pScatterIds_; // internal pressure ids of every instance (instance b: + b V)
         double tau_ = 1e-3;
         double tauRequested_ = 1e-3;
         bool tauChecked_ = false;
@@ -165,6 +175,10 @@ namespace nsfem
         size_t tsize_ = 8;
 
         void stepRegular(ChorinStepStats * stats);
+        // the step's launches around the solves (every instance)
+        void launchConvectionRhs(hipStream_t s);
+        void launchPressureRhs(hipStream_t s);
+        void launchScatterGradient(hipStream_t s);
         bool stepCaptured(ChorinStepStats * stats);
         void buildStepGraph();
         void destroyStepGraph();

@@ -125,6 +125,7 @@ namespace nsfem
         int ioDtype() const { return opt_.precision == SolverPrecision::F32 ? F32 : F64; }
         int n() const { return n_; }
         int numCh() const { return opt_.numCh; }
+        SolverMethod method() const { return opt_.method; }
         int numSystems() const { return numSystems_; }
         int numColors() const { return numColors_; }
         int systemOffset(int s) const { return sysOffset_[s]; }

@@ -247,9 +247,14 @@ class ChorinConfig:
 
 
 class Chorin:
-    """GPU Chorin projection stepping over a ChorinContext."""
+    """GPU Chorin projection stepping over a ChorinContext.
 
-    def __init__(self, ctx: ChorinContext, cfg: ChorinConfig):
+    instances > 1 (2, 4, 8 or 16, single-mesh context): an ensemble of
+    independent flows on the one mesh sharing every matrix, instance b's
+    inflow scaled by inflow_scale[b]; the state arrays then hold the
+    instances one after another (velocity B x 2N, pressure B x V)."""
+
+    def __init__(self, ctx: ChorinContext, cfg: ChorinConfig, instances=1, inflow_scale=None):
         self.ctx = ctx
         self.cfg = cfg
         self._m1 = cfg.velocitySolver.method.encode()
@@ -261,7 +266,15 @@ class Chorin:
                                                  p.rowsPerBlock, p.sweepLanes, p.sweepRowsPerBlock),
                                cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy, cfg.stepGraph)
         self._h = ctypes.c_void_p()
-        call("nsfem_chorin_create", ctx.handle, ctypes.byref(c), ctypes.byref(self._h))
+        self.instances = int(instances)
+        if inflow_scale is None:
+            scale = None
+        else:
+            scale = np.ascontiguousarray(inflow_scale, np.float64)
+            if scale.size != self.instances:
+                raise ValueError("inflow_scale needs one value per instance")
+            scale = scale.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        call("nsfem_chorin_create_ensemble", ctx.handle, ctypes.byref(c), self.instances, scale, ctypes.byref(self._h))
         self.dtype = _NP[cfg.dtype]
         self.N = ctx.numVelocityNodes
         self.V = ctx.numPressureNodes
@@ -295,8 +308,8 @@ class Chorin:
         return st.as_dict()
 
     def download(self):
-        u = np.empty(2 * self.N, self.dtype)
-        p = np.empty(self.V, self.dtype)
+        u = np.empty(2 * self.N * self.instances, self.dtype)
+        p = np.empty(self.V * self.instances, self.dtype)
         call("nsfem_chorin_download", self._h, _ptr(u), _ptr(p))
         return u, p
 
@@ -306,15 +319,15 @@ class Chorin:
         call("nsfem_chorin_download_async", self._h)
 
     def download_wait(self):
-        u = np.empty(2 * self.N, self.dtype)
-        p = np.empty(self.V, self.dtype)
+        u = np.empty(2 * self.N * self.instances, self.dtype)
+        p = np.empty(self.V * self.instances, self.dtype)
         call("nsfem_chorin_download_wait", self._h, _ptr(u), _ptr(p))
         return u, p
 
     def upload_velocity(self, u):
         u = np.ascontiguousarray(u, self.dtype)
-        if u.size != 2 * self.N:
-            raise ValueError("velocity must have 2N entries")
+        if u.size != 2 * self.N * self.instances:
+            raise ValueError("velocity must have 2N entries per instance")
         call("nsfem_chorin_upload_velocity", self._h, _ptr(u))
 
     def reset(self):
