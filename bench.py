@@ -136,7 +136,8 @@ def run_ensemble(args, ctx, cond, u0, target):
         starts.append(u)
     pressure = args.ensemble_pressure
     cfg = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 5000, target, 1), fem.SolverConfig(pressure, 50000, target, 1),
-                           dtype=F64, timeStepPolicy=fem.TAU_CLAMP)
+                           dtype=F64, timeStepPolicy=fem.TAU_CLAMP,
+                           pressureStart=fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else 0)
     ch, r, _, last = run_path(f"ensemble{B}", ctx, cfg, args.tau, np.concatenate(starts), args.warmup, args.steps,
                               not args.no_autotune, make=lambda c: fem.Chorin(ctx, c, instances=B, inflow_scale=scales))
     del ch
@@ -546,6 +547,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=8,
                     help="also run an ensemble of this many flows on the same mesh (inflow scales 1.0 down to 0.6; "
                          "mixed velocity solves over 2B columns, pressure over B); 0 = skip")
+    ap.add_argument("--pressure-start", default="last", choices=["last", "extrapolate"],
+                    help="pressure solves' initial guess: the last pressure, or 2 p_n - p_n-1")
     ap.add_argument("--ensemble-pressure", default="pcg_mixed", choices=["pcg_mixed", "bpcg_mixed"],
                     help="the ensemble's pressure solver")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -610,8 +613,11 @@ def main():
 
     eager = 1 if args.eager else 0
     sg = 1 if args.step_graph else 0
-    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg)
-    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg)
+    ps = fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else fem.PRESSURE_START_LAST
+    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg,
+                             pressureStart=ps)
+    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg,
+                             pressureStart=ps)
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     if args.kernels_only:
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
@@ -703,6 +709,7 @@ def main():
                    "reynolds": 100, "peak_inflow": cond.peakVelocity, "viscosity": cond.viscosity,
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
                    "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 symmetric GS preconditioner",
+                   "pressure_start": args.pressure_start,
                    "seed": args.seed},
         "path": head,
         "fp64": r64,

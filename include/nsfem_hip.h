@@ -345,7 +345,14 @@ typedef struct nsfem_chorin_config
                          * impositions; a solve out of budget rolls the step back
                          * to the snapshot and reruns it regularly.  pcg solvers
                          * without mixedRestart.  No reference counterpart. */
+    int pressureStart;  /* the pressure solve's initial guess: NSFEM_PRESSURE_START_LAST
+                         * (0, the last pressure, as the reference's in-place
+                         * solve) or NSFEM_PRESSURE_START_EXTRAPOLATE (2 p_n - p_n-1;
+                         * p_n on the first step after a reset) */
 } nsfem_chorin_config;
+
+#define NSFEM_PRESSURE_START_LAST 0
+#define NSFEM_PRESSURE_START_EXTRAPOLATE 1
 
 /* Time-step policies.  The step is explicit (forward Euler) in diffusion and
  * convection (chorinCuda.cu:263-278) and stays bounded only while

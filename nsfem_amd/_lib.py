@@ -20,6 +20,8 @@ F32 = 0
 F64 = 1
 TAU_REJECT = 0     # time-step policies (nsfem_chorin_config.timeStepPolicy)
 TAU_CLAMP = 1
+PRESSURE_START_LAST = 0
+PRESSURE_START_EXTRAPOLATE = 1
 TAU_UNCHECKED = 2
 NSFEM_CONTEXT_GPU_ASSEMBLY = 1
 
@@ -69,7 +71,7 @@ class SolverConfig(ctypes.Structure):
 class ChorinConfigC(ctypes.Structure):
     _fields_ = [("velocitySolver", SolverConfig), ("pressureSolver", SolverConfig), ("dtype", ctypes.c_int),
                 ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int), ("timeStepPolicy", ctypes.c_int),
-                ("stepGraph", ctypes.c_int)]
+                ("stepGraph", ctypes.c_int), ("pressureStart", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):

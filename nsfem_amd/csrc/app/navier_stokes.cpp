@@ -123,6 +123,14 @@ int main(int argc, char ** argv)
             std::cerr << "hip.timeStepPolicy must be reject, clamp or unchecked\n";
             return 1;
         }
+        // the pressure solve's initial guess: the last pressure or 2 p_n - p_n-1
+        const std::string start = cfg.str("hip.pressureStart", "last");
+        if (start != "last" && start != "extrapolate")
+        {
+            std::cerr << "hip.pressureStart must be last or extrapolate\n";
+            return 1;
+        }
+        cc.pressureStart = start == "last" ? NSFEM_PRESSURE_START_LAST : NSFEM_PRESSURE_START_EXTRAPOLATE;
         nsfem_chorin * ch = nullptr;
         cu::check(nsfem_chorin_create(ctx, &cc, &ch));
 

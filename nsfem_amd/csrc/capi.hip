@@ -1145,6 +1145,11 @@ extern "C"
                 throw std::invalid_argument("stepGraph must be 0 or 1");
             }
             cc.stepGraph = cfg->stepGraph == 1;
+            if (cfg->pressureStart != NSFEM_PRESSURE_START_LAST && cfg->pressureStart != NSFEM_PRESSURE_START_EXTRAPOLATE)
+            {
+                throw std::invalid_argument("pressureStart must be NSFEM_PRESSURE_START_LAST or NSFEM_PRESSURE_START_EXTRAPOLATE");
+            }
+            cc.pressureExtrapolation = cfg->pressureStart == NSFEM_PRESSURE_START_EXTRAPOLATE;
             if (cc.stepGraph)
             {
                 for (const auto * sc : {&cc.velocity, &cc.pressure})

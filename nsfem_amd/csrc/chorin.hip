@@ -466,6 +466,15 @@ namespace nsfem
         fill(cfg_.dtype, static_cast<long>(V_) * B_, 0.0, p_.get(), s);
         imposeDirichlet(s);
         fill(pSolver_->ioDtype(), static_cast<long>(pSolver_->n()) * pSolver_->numCh(), 0.0, pSolver_->ioSol(), s);
+        if (cfg_.pressureExtrapolation)
+        {
+            const long n = static_cast<long>(pSolver_->n()) * pSolver_->numCh();
+            if (pPrev_.size() < static_cast<size_t>(n) * dtypeSize(pSolver_->ioDtype()))
+            {
+                pPrev_.resize(std::max<size_t>(8, static_cast<size_t>(n) * dtypeSize(pSolver_->ioDtype())));
+            }
+            fill(pSolver_->ioDtype(), n, 0.0, pPrev_.get(), s);
+        }
         syncLibraryStream();
     }
 
@@ -619,6 +628,14 @@ namespace nsfem
         csrMul(vpd_.view(cfg_.dtype), B_, u_.get(), 2L * N_, pSolver_->ioRhs(), nIntP_, -1.0 / tau_, 0.0, internalP_.get(), nIntP_, 0, s);
     }
 
+    void ChorinGpu::startPressure(hipStream_t s)
+    {
+        if (cfg_.pressureExtrapolation)
+        {
+            extrapolate(pSolver_->ioDtype(), static_cast<long>(pSolver_->n()) * pSolver_->numCh(), pSolver_->ioSol(), pPrev_.get(), s);
+        }
+    }
+
     void ChorinGpu::launchScatterGradient(hipStream_t s)
     {
         // p_b[internal] = solution channel b, then rhs_b = PVD p_b
@@ -656,6 +673,7 @@ namespace nsfem
 
         // Pressure: Kp_int p = -(1/tau) (VPD u)[internal], warm start
         launchPressureRhs(s);
+        startPressure(s);
         pSolver_->solve();
         st.msePressure = maxMse(*pSolver_, 0);
         st.relPressure = maxRel(*pSolver_);
@@ -783,6 +801,7 @@ namespace nsfem
             axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
             imposeDirichlet(s);
             launchPressureRhs(s);
+            startPressure(s);
             pSolver_->enqueueSolve(s, false, budget_[1]);
             pSolver_->snapshotStats(s, 1);
             launchScatterGradient(s);

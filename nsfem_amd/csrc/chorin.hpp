@@ -68,6 +68,10 @@ namespace nsfem
         // solve one of B; the convection is each instance's own.
         int instances = 1;
         std::vector<double> inflowScale;
+        // Pressure solve's initial guess: false = the last pressure (as the
+        // reference's in-place solve), true = 2 p_n - p_n-1 (linear
+        // extrapolation in time; the first step after a reset uses p_n).
+        bool pressureExtrapolation = false;
     };
 
     struct ChorinStepStats
@@ -179,6 +183,8 @@ namespace nsfem
         void launchConvectionRhs(hipStream_t s);
         void launchPressureRhs(hipStream_t s);
         void launchScatterGradient(hipStream_t s);
+        void startPressure(hipStream_t s);
+        DeviceArray<char> pPrev_; // the pressure solution of the step before (extrapolation)
         bool stepCaptured(ChorinStepStats * stats);
         void buildStepGraph();
         void destroyStepGraph();

@@ -209,6 +209,19 @@ namespace nsfem
         }
     }
 
+    // x <- 2 x - prev, prev <- x (old): a linear extrapolation in time of a
+    // solver's warm start, keeping the history for the next step
+    template <typename T>
+    __global__ void extrapolateKernel(long n, T * __restrict__ x, T * __restrict__ prev)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            const T a = x[i];
+            x[i] = 2 * a - prev[i];
+            prev[i] = a;
+        }
+    }
+
     // One wave that idles for the given number of ticks of the constant
     // 100 MHz wall clock, then exits (holds a stream while the host queues
     // work behind it, so that work runs back to back).

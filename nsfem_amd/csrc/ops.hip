@@ -209,6 +209,17 @@ namespace nsfem
                                                           static_cast<double *>(c))));
     }
 
+    void extrapolate(int dt, long n, void * x, void * prev, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        NSFEM_TYPED(dt, (extrapolateKernel<float><<<g, 256, 0, s>>>(n, static_cast<float *>(x), static_cast<float *>(prev))),
+                    (extrapolateKernel<double><<<g, 256, 0, s>>>(n, static_cast<double *>(x), static_cast<double *>(prev))));
+    }
+
     void fill(int dt, long n, double value, void * dst, hipStream_t s)
     {
         if (n <= 0)

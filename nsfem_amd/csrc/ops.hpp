@@ -38,6 +38,8 @@ namespace nsfem
     void scale(int dt, long n, double alpha, void * dst, hipStream_t s);
     void geam(int dt, long n, double alpha, const void * a, double beta, const void * b, void * c, hipStream_t s);
     void fill(int dt, long n, double value, void * dst, hipStream_t s);
+    // x <- 2 x - prev and prev <- the old x, elementwise
+    void extrapolate(int dt, long n, void * x, void * prev, hipStream_t s);
     void fillInt(long n, int value, int * dst, hipStream_t s);
     void convert(int srcDt, int dstDt, long n, const void * src, void * dst, hipStream_t s);
     void scatter(int dt, int nnz, const int * idx, const void * vals, void * dense, hipStream_t s);

@@ -15,7 +15,8 @@ import time
 import numpy as np
 
 from . import _lib
-from ._lib import F32, F64, TAU_CLAMP, TAU_REJECT, TAU_UNCHECKED, call  # noqa: F401
+from ._lib import (F32, F64, PRESSURE_START_EXTRAPOLATE, PRESSURE_START_LAST, TAU_CLAMP, TAU_REJECT,  # noqa: F401
+                   TAU_UNCHECKED, call)
 
 _NP = {F32: np.float32, F64: np.float64}
 
@@ -244,6 +245,7 @@ class ChorinConfig:
     eagerLaunches: int = 0  # 1: no graph replay of solver iterations (for tracing tools)
     timeStepPolicy: int = 0  # TAU_REJECT: an unstable step is an error; TAU_CLAMP, TAU_UNCHECKED
     stepGraph: int = 0  # 1: each step is one captured graph with device-side convergence (pcg solvers)
+    pressureStart: int = 0  # PRESSURE_START_LAST: warm start from p_n; PRESSURE_START_EXTRAPOLATE: 2 p_n - p_n-1
 
 
 class Chorin:
@@ -264,7 +266,8 @@ class Chorin:
                                                  v.rowsPerBlock, v.sweepLanes, v.sweepRowsPerBlock),
                                _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
                                                  p.rowsPerBlock, p.sweepLanes, p.sweepRowsPerBlock),
-                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy, cfg.stepGraph)
+                               cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy, cfg.stepGraph,
+                               cfg.pressureStart)
         self._h = ctypes.c_void_p()
         self.instances = int(instances)
         if inflow_scale is None:
