@@ -549,6 +549,9 @@ def main():
                          "mixed velocity solves over 2B columns, pressure over B); 0 = skip")
     ap.add_argument("--pressure-start", default="last", choices=["last", "extrapolate"],
                     help="pressure solves' initial guess: the last pressure, or 2 p_n - p_n-1")
+    ap.add_argument("--pressure-solver", default="mcgs", choices=["mcgs", "amg"],
+                    help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
+                         "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
     ap.add_argument("--ensemble-pressure", default="pcg_mixed", choices=["pcg_mixed", "bpcg_mixed"],
                     help="the ensemble's pressure solver")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -608,8 +611,9 @@ def main():
         return ch
 
     def solvers(method):
+        pressure = method if args.pressure_solver == "mcgs" else method.replace("pcg", "amg_pcg")
         return (fem.SolverConfig(method, 5000, target, 1, vl, vr, vsl, vsr),
-                fem.SolverConfig(method, 50000, target, 1, pl, pr, psl, psr))
+                fem.SolverConfig(pressure, 50000, target, 1, pl, pr, psl, psr))
 
     eager = 1 if args.eager else 0
     sg = 1 if args.step_graph else 0

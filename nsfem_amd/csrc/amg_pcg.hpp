@@ -1,0 +1,111 @@
+// CG preconditioned by a smoothed-aggregation V-cycle whose smoother on
+// every level is one multicolour Gauss-Seidel sweep (forward on the way
+// down, backward on the way up: a symmetric preconditioner), for the
+// pressure Poisson solve.
+//
+// No counterpart in the reference (its pressure solve is multicolour GS,
+// cpp/libs/CudaUtils/src/gaussSeidel.cu:121-427, or cuDSS); the stopping
+// rule is the pcg solver's, |r| <= target |b| in FP64, and the Krylov
+// vectors, products and reductions are FP64.  Precision of the V-cycle:
+// FP64, or FP32 (mixed: the FP32 V-cycle on an FP32 copy of every level).
+//
+// Launches per iteration: the direction (p, q = A p and p.q partials), the
+// update (x, r, |r|^2 partials, the FP32 residual copy), the V-cycle (per
+// smoothing level 2 x colours + 3, one dense product on the coarsest), the
+// r.z partials and a one-block finalize (rho, convergence).  Every kernel
+// returns at once after the solve has stopped, so whole iterations are
+// replayed from one captured graph and polled one iteration behind.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "host/amg.hpp"
+#include "kernels.hpp"
+
+namespace nsfem
+{
+    class AmgPcg
+    {
+    public:
+        // Finest level: the caller's colour-major system (stripped rows with
+        // rowSplit, diagonal, colour boundaries); a64 its FP64 device copy
+        // (the CG products), level0 the same matrix in the V-cycle precision
+        // (FP64 runs pass a64 again, mixed an FP32 copy).
+        AmgPcg(const CsrD & stripped, const std::vector<int> & rowSplit, const std::vector<double> & diag,
+               const std::vector<int> & colorStart, bool mixed, McgsMatrix<double> a64, McgsMatrix<float> level0F32, hipStream_t s);
+        ~AmgPcg();
+
+        struct Result
+        {
+            int iterations = 0;
+            double bb = 0, rr = 0;
+            bool nonFinite = false;
+            bool converged = false;
+        };
+        // Solves A x = b (device, colour order, FP64); x holds the initial
+        // guess unless zeroGuess.  Blocking.
+        Result solve(const double * b, double * x, bool zeroGuess, double target, int maxIters, bool useGraphs);
+
+        int numLevels() const { return static_cast<int>(levels_.size()); }
+        int levelRows(int l) const { return levels_[l].n; }
+        int levelColors(int l) const { return static_cast<int>(levels_[l].colorStart.size()) - 1; }
+        double operatorComplexity() const { return complexity_; }
+        double setupSeconds() const { return setupSeconds_; }
+
+    private:
+        struct Level
+        {
+            int n = 0;
+            std::vector<int> colorStart;
+            int lanes = 4, pLanes = 4, rLanes = 8;
+            // this level's matrix (levels >= 1; level 0 uses the caller's)
+            DeviceArray<int> rowStart, rowSplit, column;
+            DeviceArray<double> values64, diag64, invDiag64;
+            DeviceArray<float> values32, diag32, invDiag32;
+            // prolongation from / restriction to the next level
+            int pRows = 0, rRows = 0;
+            DeviceArray<int> pRowStart, pColumn, rRowStart, rColumn;
+            DeviceArray<double> pValues64, rValues64;
+            DeviceArray<float> pValues32, rValues32;
+            // vectors in the V-cycle precision (bytes)
+            DeviceArray<char> b, x, r;
+        };
+
+        template <typename T>
+        McgsMatrix<T> levelMatrix(int l) const;
+        template <typename T>
+        void enqueueVcycle(hipStream_t s, const T * b0, T * x0);
+        template <typename P>
+        void enqueueInit(hipStream_t s, const double * b, double * x, bool zeroGuess, double tol2);
+        template <typename P>
+        void enqueueIteration(hipStream_t s, double tol2);
+        void destroyGraphs();
+
+        bool mixed_ = false;
+        int n_ = 0;
+        McgsMatrix<double> a64_{};
+        McgsMatrix<float> a32_{};
+        std::vector<Level> levels_;
+        int coarseN_ = 0;
+        DeviceArray<double> coarse64_;
+        DeviceArray<float> coarse32_;
+        double complexity_ = 0, setupSeconds_ = 0;
+
+        // Krylov vectors (FP64) and the V-cycle operands (P)
+        DeviceArray<double> r_, p_, q_;
+        DeviceArray<char> rP_, z_;
+        int parts_ = 0;
+        DeviceArray<double> scal_, partPq_, partRr_, partRho_, partBb_;
+        DeviceArray<int> ints_;
+        int * hostPoll_ = nullptr; // pinned, 2 slots x 4 ints
+        hipEvent_t pollEvents_[2] = {nullptr, nullptr};
+        hipStream_t stream_ = nullptr;
+
+        double * xDevice_ = nullptr; // the solution vector the captured iteration updates
+        hipGraphExec_t iterExec_ = nullptr;
+        double graphTol2_ = -1;
+        int predicted_ = 0;
+    };
+} // namespace nsfem

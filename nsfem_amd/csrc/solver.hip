@@ -125,6 +125,15 @@ namespace nsfem
             o.method = SolverMethod::BlockPcg;
             o.precision = SolverPrecision::Mixed;
         }
+        else if (name == "amg_pcg")
+        {
+            o.method = SolverMethod::AmgPcg;
+        }
+        else if (name == "amg_pcg_mixed")
+        {
+            o.method = SolverMethod::AmgPcg;
+            o.precision = SolverPrecision::Mixed;
+        }
         else if (name == "dss")
         {
             throw std::invalid_argument("Solver 'dss' (cuDSS direct solve) is not available in this build; use 'gs', 'pcg' or 'pcg_mixed'");
@@ -203,6 +212,17 @@ namespace nsfem
             if (opt_.precision == SolverPrecision::F32)
             {
                 throw std::invalid_argument("block PCG keeps FP64 Krylov vectors: use an FP64 matrix (bpcg or bpcg_mixed)");
+            }
+        }
+        if (opt_.method == SolverMethod::AmgPcg)
+        {
+            if (systems.size() != 1 || opt_.numCh != 1)
+            {
+                throw std::invalid_argument("amg_pcg solves one system with one right-hand side");
+            }
+            if (opt_.precision == SolverPrecision::F32)
+            {
+                throw std::invalid_argument("amg_pcg keeps FP64 Krylov vectors: use amg_pcg or amg_pcg_mixed");
             }
         }
         stream_ = libraryStream();
@@ -413,6 +433,17 @@ namespace nsfem
 
         perm_.upload(perm);
         systemRows_.upload(sysRows_);
+        if (opt_.method == SolverMethod::AmgPcg)
+        {
+            CsrD s0;
+            s0.rows = s0.cols = n_;
+            s0.rowStart = rowStart;
+            s0.column = column;
+            s0.values = values;
+            const bool mixed = opt_.precision == SolverPrecision::Mixed;
+            amg_ = std::make_unique<AmgPcg>(s0, rowSplit, diag, colorRowStart_, mixed, A64_.view(n_),
+                                            mixed ? A32_.view(n_) : McgsMatrix<float>{}, stream_);
+        }
 
         const size_t nv = static_cast<size_t>(n_) * opt_.numCh;
         const size_t ioBytes = nv * (ioDtype() == F64 ? 8 : 4);
@@ -1574,6 +1605,22 @@ namespace nsfem
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
         lastOuter_ = 0;
+        if (opt_.method == SolverMethod::AmgPcg)
+        {
+            permute(F64, F64, n_, 1, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            if (!zeroInitial)
+            {
+                permute(F64, F64, n_, 1, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            }
+            const AmgPcg::Result r = amg_->solve(b64_.get(), x64_.get(), zeroInitial, opt_.target, opt_.maxIters, useGraphs_);
+            lastMse_[0] = std::sqrt(r.rr / n_);
+            lastRel_[0] = r.bb > 0 ? std::sqrt(r.rr / r.bb) : 0.0;
+            lastIters_[0] = r.iterations;
+            lastNonFinite_ = r.nonFinite;
+            permute(F64, F64, n_, 1, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            NSFEM_HIP(hipStreamSynchronize(s));
+            return;
+        }
         if (opt_.method == SolverMethod::BlockPcg)
         {
             permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);

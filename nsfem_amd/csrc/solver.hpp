@@ -8,6 +8,9 @@
 //           stop when every channel is below targetMse.
 //   "pcg" - CG preconditioned by one symmetric multicolour GS sweep, stop
 //           when |r| <= targetMse * |b| for every channel (relative).
+//   "amg_pcg" - the same stopping rule, CG preconditioned by a smoothed-
+//           aggregation V-cycle smoothed by multicolour GS (one system, one
+//           channel; "amg_pcg_mixed": the V-cycle in FP32).
 // Precision: F64, F32, or Mixed (pcg only): FP64 outer refinement on the
 // FP64 matrix around FP32 inner PCG solves; reductions are FP64 throughout.
 //
@@ -23,6 +26,7 @@
 
 #include "common.hpp"
 #include "csr.hpp"
+#include "amg_pcg.hpp"
 #include "bcg_kernels.hpp"
 #include "kernels.hpp"
 #include "ops.hpp"
@@ -33,7 +37,8 @@ namespace nsfem
     {
         GaussSeidel,
         Pcg,
-        BlockPcg // one Krylov space per tile of 4-32 right-hand sides
+        BlockPcg, // one Krylov space per tile of 4-32 right-hand sides
+        AmgPcg    // CG preconditioned by a smoothed-aggregation V-cycle with multicolour GS smoothing
     };
 
     enum class SolverPrecision
@@ -160,6 +165,8 @@ namespace nsfem
         int lastIterations(int s) const { return lastIters_[s]; }
         int lastOuterIterations() const { return lastOuter_; }
         bool lastNonFinite() const { return lastNonFinite_; }
+        // amg_pcg: the hierarchy (nullptr for the other methods)
+        const AmgPcg * amg() const { return amg_.get(); }
         // block PCG: whether the last solve dropped a direction (a rank-
         // deficient block)
         bool lastDroppedDirection() const { return lastDropped_; }
@@ -372,6 +379,7 @@ namespace nsfem
         bool lastNonFinite_ = false;
         hipStream_t stream_;
 
+        std::unique_ptr<AmgPcg> amg_;
         ChunkGraph graph64_, graph32_, graphMixed_;
         bool useGraphs_ = true;
         int predicted64_ = 0, predicted32_ = 0, predictedMixed_ = 0;

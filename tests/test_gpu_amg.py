@@ -1,0 +1,122 @@
+"""CG with the smoothed-aggregation V-cycle preconditioner (amg_pcg,
+amg_pcg_mixed) on the pressure systems.
+
+The preconditioner changes only the path to the solution: the stopping rule
+is the pcg solver's relative residual in FP64, so the solutions are held to
+the direct solve (scipy) with the same bounds as the one-level pcg, and the
+Chorin steps with an AMG pressure solve to the FP64 oracle as the pcg steps
+are (1e-10 at a 1e-13 residual; the mixed bounds at 1e-10)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import mesh_path
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(name, m, b, max_iters=500, target=1e-12, x0=None):
+    from nsfem_amd import cu
+    s = cu.createSolver(name, 1, m, max_iters, target, 1)
+    s.getRhs().upload(b)
+    if x0 is None:
+        s.getSol().memsetZero()
+    else:
+        s.getSol().upload(x0)
+    s.solve()
+    return s, s.getSol().download()
+
+
+def _channel_pressure(nodes, seed=7):
+    """Kp_int of a seeded channel mesh (Dirichlet outflow rows removed)."""
+    from nsfem_amd import fem, mesher
+    m = mesher.channel_mesh(nodes, seed=seed)
+    cond = fem.DfgConditions(viscosity=1e-3, peakVelocity=1.0, channelHeight=m["height"], profileScale=4.0)
+    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), cond)
+    return ctx.scipy("pressureStiffnessInternal")
+
+
+@pytest.mark.parametrize("name,target,bound", [("amg_pcg", 1e-12, 1e-9), ("amg_pcg_mixed", 1e-10, 1e-7)])
+def test_amg_pcg_matches_direct_solve(octx_c3, name, target, bound):
+    m = octx_c3["Kp_int"]
+    rng = np.random.default_rng(31)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x = _solve(name, m, b, target=target)
+    xd = spla.spsolve(sp.csc_matrix(m), b)
+    true_rel = np.linalg.norm(b - m @ x) / np.linalg.norm(b)
+    assert s.getLastRelRes() <= target and true_rel <= 1.01 * target
+    assert abs(true_rel - s.getLastRelRes()) <= 1e-3 * target + 1e-16
+    assert np.max(np.abs(x - xd)) <= bound * np.max(np.abs(xd))
+    assert 1 <= s.getLastIterations() < 60
+
+
+def test_amg_pcg_is_bit_repeatable_and_warm_starts(octx_c3):
+    m = octx_c3["Kp_int"]
+    rng = np.random.default_rng(32)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s1, x1 = _solve("amg_pcg", m, b, target=1e-10)
+    s2, x2 = _solve("amg_pcg", m, b, target=1e-10)
+    assert np.array_equal(x1, x2)
+    # from its own solution: converged at once
+    s3, x3 = _solve("amg_pcg", m, b, target=1e-10, x0=x1)
+    assert s3.getLastIterations() == 0 and np.array_equal(x3, x1)
+    # a zero right-hand side gives the zero solution
+    _, x0 = _solve("amg_pcg", m, np.zeros_like(b), target=1e-10)
+    assert not x0.any()
+
+
+@pytest.mark.parametrize("name", ["amg_pcg", "amg_pcg_mixed"])
+def test_amg_pcg_on_a_200k_channel(name):
+    """Iterations stay few at 200k rows where the one-level pcg needs
+    hundreds; the true residual meets the target."""
+    m = _channel_pressure(200_000)
+    rng = np.random.default_rng(33)
+    b = rng.uniform(-1, 1, m.shape[0])
+    s, x = _solve(name, m, b, target=1e-10, max_iters=300)
+    true_rel = np.linalg.norm(b - m @ x) / np.linalg.norm(b)
+    sp_, xp = _solve("pcg", m, b, target=1e-10, max_iters=20000)
+    print(f"{name}: {s.getLastIterations()} iterations (pcg {sp_.getLastIterations()}), true relres {true_rel:.2e}")
+    assert true_rel <= 1.01e-10
+    assert s.getLastIterations() < 60 < sp_.getLastIterations()
+    assert np.max(np.abs(x - xp)) <= 1e-6 * np.max(np.abs(xp))
+
+
+def test_chorin_steps_with_amg_pressure_match_oracle(oracle, octx_c3):
+    from nsfem_amd import fem
+    tau = 1e-3
+    orc = oracle.ChorinOracle(octx_c3, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    ctx = fem.ChorinContext(fem.TriangleMesh.load_gmsh(mesh_path("c3")), fem.DfgConditions(1e-3, 1.0))
+    cfg = fem.ChorinConfig(fem.SolverConfig("pcg", 2000, 1e-13, 1), fem.SolverConfig("amg_pcg", 500, 1e-13, 1), dtype=1)
+    mixed = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 2000, 1e-10, 1), fem.SolverConfig("amg_pcg_mixed", 500, 1e-10, 1),
+                             dtype=1)
+    ch, chm = fem.Chorin(ctx, cfg), fem.Chorin(ctx, mixed)
+    ch.set_time_step(tau)
+    chm.set_time_step(tau)
+    for k in range(3):
+        st = ch.step()
+        stm = chm.step()
+        uo, po = orc.step()
+        u, p = ch.download()
+        assert st["relPressure"] <= 1e-13 and st["itersPressure"] < 60
+        assert stm["relPressure"] <= 1e-10
+        assert np.max(np.abs(u - uo)) <= 1e-10 * np.max(np.abs(uo)), k
+        assert np.max(np.abs(p - po)) <= 1e-10 * np.max(np.abs(po)), k
+    um, pm = chm.download()
+    assert np.max(np.abs(um - uo)) <= 1e-8 * np.max(np.abs(uo))
+    assert np.max(np.abs(pm - po)) <= 1e-7 * np.max(np.abs(po))
+
+
+def test_amg_pcg_rejects_what_it_cannot_solve(octx_c3):
+    from nsfem_amd import cu
+    from nsfem_amd._lib import NsfemError
+    m = octx_c3["Kp_int"]
+    with pytest.raises(NsfemError, match="one right-hand side"):
+        cu.createSolver("amg_pcg", 2, m, 100, 1e-10, 1)
+    with pytest.raises(NsfemError, match="FP64 Krylov"):
+        cu.createSolver("amg_pcg", 1, sp.csr_matrix(m, dtype=np.float32), 100, 1e-10, 1)
+    # the pure-Neumann Laplacian is singular: no coarsest factorisation
+    lap = octx_c3["Kp"]
+    with pytest.raises(NsfemError, match="positive definite"):
+        cu.createSolver("amg_pcg", 1, lap, 100, 1e-10, 1)
