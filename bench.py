@@ -549,6 +549,8 @@ def main():
                          "mixed velocity solves over 2B columns, pressure over B); 0 = skip")
     ap.add_argument("--pressure-start", default="last", choices=["last", "extrapolate"],
                     help="pressure solves' initial guess: the last pressure, or 2 p_n - p_n-1")
+    ap.add_argument("--velocity-start", default="zero", choices=["zero", "last", "extrapolate"],
+                    help="velocity solves' initial guess: zero (as the reference), the last solution, or 2 a_n - a_n-1")
     ap.add_argument("--pressure-solver", default="mcgs", choices=["mcgs", "amg"],
                     help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
                          "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
@@ -618,10 +620,12 @@ def main():
     eager = 1 if args.eager else 0
     sg = 1 if args.step_graph else 0
     ps = fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else fem.PRESSURE_START_LAST
+    vs = {"zero": fem.VELOCITY_START_ZERO, "last": fem.VELOCITY_START_LAST, "extrapolate": fem.VELOCITY_START_EXTRAPOLATE}[
+        args.velocity_start]
     cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg,
-                             pressureStart=ps)
+                             pressureStart=ps, velocityStart=vs)
     cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg,
-                             pressureStart=ps)
+                             pressureStart=ps, velocityStart=vs)
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     if args.kernels_only:
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
@@ -714,6 +718,8 @@ def main():
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
                    "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 symmetric GS preconditioner",
                    "pressure_start": args.pressure_start,
+                   "velocity_start": args.velocity_start,
+                   "pressure_solver": args.pressure_solver,
                    "seed": args.seed},
         "path": head,
         "fp64": r64,

@@ -349,10 +349,19 @@ typedef struct nsfem_chorin_config
                          * (0, the last pressure, as the reference's in-place
                          * solve) or NSFEM_PRESSURE_START_EXTRAPOLATE (2 p_n - p_n-1;
                          * p_n on the first step after a reset) */
+    int velocityStart;  /* the velocity solves' initial guess, each of the two solves
+                         * from its own history: NSFEM_VELOCITY_START_ZERO (0, as
+                         * the reference's accel.memsetZero(), chorinCuda.cu:265,
+                         * 343), NSFEM_VELOCITY_START_LAST (its solution one step
+                         * before) or NSFEM_VELOCITY_START_EXTRAPOLATE (2 a_n - a_n-1);
+                         * zero after a reset */
 } nsfem_chorin_config;
 
 #define NSFEM_PRESSURE_START_LAST 0
 #define NSFEM_PRESSURE_START_EXTRAPOLATE 1
+#define NSFEM_VELOCITY_START_ZERO 0
+#define NSFEM_VELOCITY_START_LAST 1
+#define NSFEM_VELOCITY_START_EXTRAPOLATE 2
 
 /* Time-step policies.  The step is explicit (forward Euler) in diffusion and
  * convection (chorinCuda.cu:263-278) and stays bounded only while

@@ -22,6 +22,9 @@ TAU_REJECT = 0     # time-step policies (nsfem_chorin_config.timeStepPolicy)
 TAU_CLAMP = 1
 PRESSURE_START_LAST = 0
 PRESSURE_START_EXTRAPOLATE = 1
+VELOCITY_START_ZERO = 0
+VELOCITY_START_LAST = 1
+VELOCITY_START_EXTRAPOLATE = 2
 TAU_UNCHECKED = 2
 NSFEM_CONTEXT_GPU_ASSEMBLY = 1
 
@@ -71,7 +74,7 @@ class SolverConfig(ctypes.Structure):
 class ChorinConfigC(ctypes.Structure):
     _fields_ = [("velocitySolver", SolverConfig), ("pressureSolver", SolverConfig), ("dtype", ctypes.c_int),
                 ("innerTol", ctypes.c_double), ("eagerLaunches", ctypes.c_int), ("timeStepPolicy", ctypes.c_int),
-                ("stepGraph", ctypes.c_int), ("pressureStart", ctypes.c_int)]
+                ("stepGraph", ctypes.c_int), ("pressureStart", ctypes.c_int), ("velocityStart", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):

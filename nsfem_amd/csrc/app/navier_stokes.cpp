@@ -131,6 +131,25 @@ int main(int argc, char ** argv)
             return 1;
         }
         cc.pressureStart = start == "last" ? NSFEM_PRESSURE_START_LAST : NSFEM_PRESSURE_START_EXTRAPOLATE;
+        // the velocity solves' initial guess: zero (as the reference), last or extrapolate
+        const std::string vstart = cfg.str("hip.velocityStart", "zero");
+        if (vstart == "zero")
+        {
+            cc.velocityStart = NSFEM_VELOCITY_START_ZERO;
+        }
+        else if (vstart == "last")
+        {
+            cc.velocityStart = NSFEM_VELOCITY_START_LAST;
+        }
+        else if (vstart == "extrapolate")
+        {
+            cc.velocityStart = NSFEM_VELOCITY_START_EXTRAPOLATE;
+        }
+        else
+        {
+            std::cerr << "hip.velocityStart must be zero, last or extrapolate\n";
+            return 1;
+        }
         nsfem_chorin * ch = nullptr;
         cu::check(nsfem_chorin_create(ctx, &cc, &ch));
 

@@ -1150,6 +1150,12 @@ extern "C"
                 throw std::invalid_argument("pressureStart must be NSFEM_PRESSURE_START_LAST or NSFEM_PRESSURE_START_EXTRAPOLATE");
             }
             cc.pressureExtrapolation = cfg->pressureStart == NSFEM_PRESSURE_START_EXTRAPOLATE;
+            if (cfg->velocityStart < NSFEM_VELOCITY_START_ZERO || cfg->velocityStart > NSFEM_VELOCITY_START_EXTRAPOLATE)
+            {
+                throw std::invalid_argument(
+                    "velocityStart must be NSFEM_VELOCITY_START_ZERO, NSFEM_VELOCITY_START_LAST or NSFEM_VELOCITY_START_EXTRAPOLATE");
+            }
+            cc.velocityStart = cfg->velocityStart;
             if (cc.stepGraph)
             {
                 for (const auto * sc : {&cc.velocity, &cc.pressure})

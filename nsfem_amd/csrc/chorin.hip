@@ -466,6 +466,22 @@ namespace nsfem
         fill(cfg_.dtype, static_cast<long>(V_) * B_, 0.0, p_.get(), s);
         imposeDirichlet(s);
         fill(pSolver_->ioDtype(), static_cast<long>(pSolver_->n()) * pSolver_->numCh(), 0.0, pSolver_->ioSol(), s);
+        if (cfg_.velocityStart != 0)
+        {
+            const long n = static_cast<long>(vSolver_->n()) * vSolver_->numCh();
+            const size_t bytes = static_cast<size_t>(n) * dtypeSize(vSolver_->ioDtype());
+            for (auto & kind : vHist_)
+            {
+                for (auto & h : kind)
+                {
+                    if (h.size() < bytes)
+                    {
+                        h.resize(bytes);
+                    }
+                    fill(vSolver_->ioDtype(), n, 0.0, h.get(), s);
+                }
+            }
+        }
         if (cfg_.pressureExtrapolation)
         {
             const long n = static_cast<long>(pSolver_->n()) * pSolver_->numCh();
@@ -636,6 +652,32 @@ namespace nsfem
         }
     }
 
+    bool ChorinGpu::startVelocity(int kind, hipStream_t s)
+    {
+        const long n = static_cast<long>(vSolver_->n()) * vSolver_->numCh();
+        const int dt = vSolver_->ioDtype();
+        if (cfg_.velocityStart == 1)
+        {
+            NSFEM_HIP(hipMemcpyAsync(vSolver_->ioSol(), vHist_[kind][0].get(), static_cast<size_t>(n) * dtypeSize(dt),
+                                     hipMemcpyDeviceToDevice, s));
+        }
+        else if (cfg_.velocityStart == 2)
+        {
+            extrapolateTo(dt, n, vHist_[kind][0].get(), vHist_[kind][1].get(), vSolver_->ioSol(), s);
+        }
+        return cfg_.velocityStart == 0;
+    }
+
+    void ChorinGpu::keepVelocity(int kind, hipStream_t s)
+    {
+        if (cfg_.velocityStart != 0)
+        {
+            const long n = static_cast<long>(vSolver_->n()) * vSolver_->numCh();
+            NSFEM_HIP(hipMemcpyAsync(vHist_[kind][0].get(), vSolver_->ioSol(), static_cast<size_t>(n) * dtypeSize(vSolver_->ioDtype()),
+                                     hipMemcpyDeviceToDevice, s));
+        }
+    }
+
     void ChorinGpu::launchScatterGradient(hipStream_t s)
     {
         // p_b[internal] = solution channel b, then rhs_b = PVD p_b
@@ -657,7 +699,8 @@ namespace nsfem
         NSFEM_HIP(hipEventRecord(ev_[1], s));
 
         // Tentative velocity: M a = A u;  u -= tau a
-        vSolver_->solve(true); // cold start
+        vSolver_->solve(startVelocity(0, s));
+        keepVelocity(0, s);
         st.mseTentX = maxMse(*vSolver_, 0, 2);
         st.mseTentY = maxMse(*vSolver_, 1, 2);
         st.relTent = maxRel(*vSolver_);
@@ -683,7 +726,8 @@ namespace nsfem
 
         // Projection: M a = PVD p;  u -= tau a
         launchScatterGradient(s);
-        vSolver_->solve(true); // cold start
+        vSolver_->solve(startVelocity(1, s));
+        keepVelocity(1, s);
         st.mseFinalX = maxMse(*vSolver_, 0, 2);
         st.mseFinalY = maxMse(*vSolver_, 1, 2);
         st.relFinal = maxRel(*vSolver_);
@@ -796,7 +840,8 @@ namespace nsfem
             NSFEM_HIP(hipMemcpyAsync(stepSnapU_.get(), u_.get(), uBytes, hipMemcpyDeviceToDevice, s));
             NSFEM_HIP(hipMemcpyAsync(stepSnapP_.get(), pSolver_->ioSol(), pBytes, hipMemcpyDeviceToDevice, s));
             launchConvectionRhs(s);
-            vSolver_->enqueueSolve(s, true, budget_[0]);
+            vSolver_->enqueueSolve(s, startVelocity(0, s), budget_[0]);
+            keepVelocity(0, s);
             vSolver_->snapshotStats(s, 0);
             axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
             imposeDirichlet(s);
@@ -805,7 +850,8 @@ namespace nsfem
             pSolver_->enqueueSolve(s, false, budget_[1]);
             pSolver_->snapshotStats(s, 1);
             launchScatterGradient(s);
-            vSolver_->enqueueSolve(s, true, budget_[2]);
+            vSolver_->enqueueSolve(s, startVelocity(1, s), budget_[2]);
+            keepVelocity(1, s);
             vSolver_->snapshotStats(s, 2);
             axpy(dt, twoN * B_, -tau_, vSolver_->ioSol(), u_.get(), s);
             imposeDirichlet(s);

@@ -72,6 +72,10 @@ namespace nsfem
         // reference's in-place solve), true = 2 p_n - p_n-1 (linear
         // extrapolation in time; the first step after a reset uses p_n).
         bool pressureExtrapolation = false;
+        // Velocity solves' initial guess (tentative and final each from its
+        // own history): 0 = zero (the reference's accel.memsetZero()),
+        // 1 = the same solve's solution one step before, 2 = 2 a_n - a_n-1.
+        int velocityStart = 0;
     };
 
     struct ChorinStepStats
@@ -184,6 +188,11 @@ namespace nsfem
         void launchPressureRhs(hipStream_t s);
         void launchScatterGradient(hipStream_t s);
         void startPressure(hipStream_t s);
+        // velocity solve `kind` (0 tentative, 1 final): the initial guess
+        // before it (returns whether it starts from zero), the history after
+        bool startVelocity(int kind, hipStream_t s);
+        void keepVelocity(int kind, hipStream_t s);
+        DeviceArray<char> vHist_[2][2]; // [kind][0 last solution, 1 the one before]
         DeviceArray<char> pPrev_; // the pressure solution of the step before (extrapolation)
         bool stepCaptured(ChorinStepStats * stats);
         void buildStepGraph();

@@ -222,6 +222,19 @@ namespace nsfem
         }
     }
 
+    // out <- 2 cur - prev, prev <- cur: an extrapolated initial guess into a
+    // solver's vector, keeping the history
+    template <typename T>
+    __global__ void extrapolateToKernel(long n, const T * __restrict__ cur, T * __restrict__ prev, T * __restrict__ out)
+    {
+        for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * blockDim.x)
+        {
+            const T a = cur[i];
+            out[i] = 2 * a - prev[i];
+            prev[i] = a;
+        }
+    }
+
     // One wave that idles for the given number of ticks of the constant
     // 100 MHz wall clock, then exits (holds a stream while the host queues
     // work behind it, so that work runs back to back).

@@ -220,6 +220,20 @@ namespace nsfem
                     (extrapolateKernel<double><<<g, 256, 0, s>>>(n, static_cast<double *>(x), static_cast<double *>(prev))));
     }
 
+    void extrapolateTo(int dt, long n, const void * cur, void * prev, void * out, hipStream_t s)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        const int g = elementwiseGrid(n);
+        NSFEM_TYPED(dt,
+                    (extrapolateToKernel<float><<<g, 256, 0, s>>>(n, static_cast<const float *>(cur), static_cast<float *>(prev),
+                                                                  static_cast<float *>(out))),
+                    (extrapolateToKernel<double><<<g, 256, 0, s>>>(n, static_cast<const double *>(cur), static_cast<double *>(prev),
+                                                                   static_cast<double *>(out))));
+    }
+
     void fill(int dt, long n, double value, void * dst, hipStream_t s)
     {
         if (n <= 0)

@@ -40,6 +40,8 @@ namespace nsfem
     void fill(int dt, long n, double value, void * dst, hipStream_t s);
     // x <- 2 x - prev and prev <- the old x, elementwise
     void extrapolate(int dt, long n, void * x, void * prev, hipStream_t s);
+    // out <- 2 cur - prev and prev <- cur, elementwise
+    void extrapolateTo(int dt, long n, const void * cur, void * prev, void * out, hipStream_t s);
     void fillInt(long n, int value, int * dst, hipStream_t s);
     void convert(int srcDt, int dstDt, long n, const void * src, void * dst, hipStream_t s);
     void scatter(int dt, int nnz, const int * idx, const void * vals, void * dense, hipStream_t s);

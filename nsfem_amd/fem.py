@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (F32, F64, PRESSURE_START_EXTRAPOLATE, PRESSURE_START_LAST, TAU_CLAMP, TAU_REJECT,  # noqa: F401
-                   TAU_UNCHECKED, call)
+                   TAU_UNCHECKED, VELOCITY_START_EXTRAPOLATE, VELOCITY_START_LAST, VELOCITY_START_ZERO, call)
 
 _NP = {F32: np.float32, F64: np.float64}
 
@@ -246,6 +246,7 @@ class ChorinConfig:
     timeStepPolicy: int = 0  # TAU_REJECT: an unstable step is an error; TAU_CLAMP, TAU_UNCHECKED
     stepGraph: int = 0  # 1: each step is one captured graph with device-side convergence (pcg solvers)
     pressureStart: int = 0  # PRESSURE_START_LAST: warm start from p_n; PRESSURE_START_EXTRAPOLATE: 2 p_n - p_n-1
+    velocityStart: int = 0  # VELOCITY_START_ZERO (reference), VELOCITY_START_LAST, VELOCITY_START_EXTRAPOLATE
 
 
 class Chorin:
@@ -267,7 +268,7 @@ class Chorin:
                                _lib.SolverConfig(self._m2, p.maxIterations, p.targetMse, p.mseCheckInterval, p.lanes,
                                                  p.rowsPerBlock, p.sweepLanes, p.sweepRowsPerBlock),
                                cfg.dtype, cfg.innerTol, cfg.eagerLaunches, cfg.timeStepPolicy, cfg.stepGraph,
-                               cfg.pressureStart)
+                               cfg.pressureStart, cfg.velocityStart)
         self._h = ctypes.c_void_p()
         self.instances = int(instances)
         if inflow_scale is None:

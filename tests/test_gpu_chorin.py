@@ -146,6 +146,35 @@ def test_extrapolated_pressure_start_matches_oracle(oracle, octx_c3, step_graph)
     assert extra.download()[1].shape == last.download()[1].shape
 
 
+@pytest.mark.parametrize("start,step_graph", [(1, 0), (2, 0), (2, 1)])
+def test_warm_velocity_starts_match_oracle(oracle, octx_c3, start, step_graph):
+    """The velocity solves started from their own last solution (1) or its
+    linear extrapolation (2) instead of zero converge to the same steps: the
+    oracle's to 1e-10."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    orc = oracle.ChorinOracle(octx_c3, tau, vel=("pcg", 2000, 1e-13, 1), pres=("pcg", 8000, 1e-13, 1))
+    cold = fem.Chorin(_ctx(), _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1)))
+    warm = fem.Chorin(_ctx(), fem.ChorinConfig(fem.SolverConfig("pcg", 2000, 1e-13, 1), fem.SolverConfig("pcg", 8000, 1e-13, 1),
+                                               dtype=1, stepGraph=step_graph, velocityStart=start))
+    its = {"cold": [], "warm": []}
+    for ch in (cold, warm):
+        ch.set_time_step(tau)
+    for k in range(6):
+        sc = cold.step()
+        st = warm.step()
+        its["cold"].append(sc["itersTent"] + sc["itersFinal"])
+        its["warm"].append(st["itersTent"] + st["itersFinal"])
+        assert st["relTent"] <= 1e-13 and st["relFinal"] <= 1e-13
+        uo, po = orc.step()
+        u, p = warm.download()
+        assert _rel(u, uo) <= 1e-10, k
+        assert _rel(p, po) <= 1e-10, k
+    print(f"velocity iterations per step, zero start {its['cold']}, start {start}: {its['warm']}")
+    # the first step has no history: zero start either way
+    assert its["warm"][0] == its["cold"][0]
+
+
 def test_non_finite_velocity_is_reported():
     """A NaN in the state makes the tentative solve fail like the reference
     ("Tentative acceleration is bad", chorinCuda.cu:270-273)."""
