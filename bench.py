@@ -148,7 +148,7 @@ def run_ensemble(args, ctx, cond, u0, target):
     return r, last
 
 
-def kernel_table(ch, res, reps=30, with_iteration=True):
+def kernel_table(ch, res, reps=30, with_iteration=True, amg_pressure=False):
     """Per-kernel average launch time (HIP events on the library stream,
     replaying the launches one solver iteration makes), algorithmic bytes per
     launch, GB/s, and each kernel's estimated share of one step: launches per
@@ -158,6 +158,8 @@ def kernel_table(ch, res, reps=30, with_iteration=True):
     out = {}
     for solver, iters in (("velocity", vel_iters), ("pressure", it["pressure"] if it else None)):
         kinds = ("direction", "sweep", "rho", "update", "finalize") + (("iteration",) if with_iteration else ())
+        if solver == "pressure" and amg_pressure:
+            kinds = ("direction", "update", "rho", "vcycle", "iteration")
         for k in kinds:
             ms, b, per_it = ch.time_kernel(f"{solver}_{k}", reps=reps)
             out[f"{solver}_{k}"] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9 if b else None,
@@ -171,6 +173,7 @@ def kernel_table(ch, res, reps=30, with_iteration=True):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_chorin_500k_mixed.md")
+LLC_BYTES = 256e6  # MI355X Infinity Cache
 
 
 def copy_peak_gbps(mib=2048, reps=10):
@@ -518,7 +521,9 @@ def main():
     ap.add_argument("--tau", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=20241103)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--paths", default="fp64,mixed,fp64_mcgs,mixed_mcgs",
+                    help="comma list of fp64, mixed (pressure preconditioner of --pressure-solver) and fp64_mcgs, "
+                         "mixed_mcgs (the one-level multicolour GS preconditioner)")
     ap.add_argument("--shape", default="0,0,0,0",
                     help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default), "
                          "optionally followed by the same four for the colour sweeps and then by the velocity and "
@@ -551,7 +556,7 @@ def main():
                     help="pressure solves' initial guess: the last pressure, or 2 p_n - p_n-1")
     ap.add_argument("--velocity-start", default="zero", choices=["zero", "last", "extrapolate"],
                     help="velocity solves' initial guess: zero (as the reference), the last solution, or 2 a_n - a_n-1")
-    ap.add_argument("--pressure-solver", default="mcgs", choices=["mcgs", "amg"],
+    ap.add_argument("--pressure-solver", default="amg", choices=["mcgs", "amg"],
                     help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
                          "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
     ap.add_argument("--ensemble-pressure", default="pcg_mixed", choices=["pcg_mixed", "bpcg_mixed"],
@@ -612,8 +617,8 @@ def main():
                 ch.set_solver_option("lanesDirection", v, which)
         return ch
 
-    def solvers(method):
-        pressure = method if args.pressure_solver == "mcgs" else method.replace("pcg", "amg_pcg")
+    def solvers(method, one_level=False):
+        pressure = method if (one_level or args.pressure_solver == "mcgs") else method.replace("pcg", "amg_pcg")
         return (fem.SolverConfig(method, 5000, target, 1, vl, vr, vsl, vsr),
                 fem.SolverConfig(pressure, 50000, target, 1, pl, pr, psl, psr))
 
@@ -622,18 +627,26 @@ def main():
     ps = fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else fem.PRESSURE_START_LAST
     vs = {"zero": fem.VELOCITY_START_ZERO, "last": fem.VELOCITY_START_LAST, "extrapolate": fem.VELOCITY_START_EXTRAPOLATE}[
         args.velocity_start]
-    cfg64 = fem.ChorinConfig(*solvers("pcg"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg,
-                             pressureStart=ps, velocityStart=vs)
-    cfgmx = fem.ChorinConfig(*solvers("pcg_mixed"), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP, stepGraph=sg,
-                             pressureStart=ps, velocityStart=vs)
+    def config(method, one_level=False):
+        return fem.ChorinConfig(*solvers(method, one_level), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP,
+                                stepGraph=sg, pressureStart=ps, velocityStart=vs)
+
+    # fp64 / mixed: the pressure preconditioner of --pressure-solver; the
+    # *_mcgs paths: one symmetric multicolour GS sweep (the one-level pcg)
+    configs = {"fp64": config("pcg"), "mixed": config("pcg_mixed"), "fp64_mcgs": config("pcg", True),
+               "mixed_mcgs": config("pcg_mixed", True)}
+    amg_paths = {"fp64", "mixed"} if args.pressure_solver == "amg" else set()
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
+    for p in paths:
+        if p not in configs:
+            raise SystemExit(f"unknown path {p}; choose from {', '.join(configs)}")
     if args.kernels_only:
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
                "kernels": {}}
         for name in paths:
-            ch = make_chorin({"fp64": cfg64, "mixed": cfgmx}[name])
+            ch = make_chorin(configs[name])
             ch.upload_velocity(u0)
-            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps)
+            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps, amg_pressure=name in amg_paths)
             out[f"{name}_shape"] = ch.shape()
             del ch
         print(json.dumps(out))
@@ -642,13 +655,17 @@ def main():
 
     results, kernels, firsts, lasts = {}, {}, {}, {}
     for name in paths:
-        cfg = {"fp64": cfg64, "mixed": cfgmx}[name]
+        cfg = configs[name]
         # the library's guard clamps the requested tau to 1 / lambda_max(M^-1 nu K)
         ch, r, first, last = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune, make_chorin)
         ti = r["time_step"]
         log(f"{name}: tau requested {ti['tauRequested']:g}, used {ti['tauUsed']:.4g} "
             f"(lambda_max(M^-1 nu K) = {ti['lambdaNu']:.4g})")
-        kernels[name] = {} if args.no_kernels else kernel_table(ch, r)
+        r["pressure_solver"] = cfg.pressureSolver.method
+        r["velocity_solver"] = cfg.velocitySolver.method
+        # the per-kernel table for the two headline paths (the comparators are timed whole)
+        kernels[name] = {} if (args.no_kernels or name not in ("fp64", "mixed")) else \
+            kernel_table(ch, r, amg_pressure=name in amg_paths)
         results[name], firsts[name], lasts[name] = r, first, last
         del ch
 
@@ -673,6 +690,9 @@ def main():
     if "fp64" in results and "mixed" in results:
         errors["mixed_vs_fp64_after_steps"] = {"velocity": rel(lasts["mixed"][0], lasts["fp64"][0]),
                                                "pressure": rel(lasts["mixed"][1], lasts["fp64"][1])}
+    if "fp64" in results and "fp64_mcgs" in results:
+        errors["fp64_vs_fp64_mcgs_after_steps"] = {"velocity": rel(lasts["fp64"][0], lasts["fp64_mcgs"][0]),
+                                                   "pressure": rel(lasts["fp64"][1], lasts["fp64_mcgs"][1])}
     head = "mixed" if "mixed" in results else paths[0]
     tau = results[head]["time_step"]["tauUsed"]
     lam = results[head]["time_step"]["lambdaNu"] / cond.viscosity
@@ -686,19 +706,30 @@ def main():
     r64 = results.get("fp64")
 
     # Dominant kernel of the headline path by estimated time per step.
-    roofline = None
+    roofline = roofline_llc = None
     if kernmx:
-        dom_name = max((k for k in kernmx if not k.endswith("_iteration")), key=lambda k: kernmx[k]["ms_per_step"])
+        # single kernels only (an iteration or a V-cycle is a sequence of launches)
+        dom_name = max((k for k in kernmx if not k.endswith(("_iteration", "_vcycle"))), key=lambda k: kernmx[k]["ms_per_step"])
         dom = kernmx[dom_name]
         peak = 8000.0
         traffic = measured_traffic(dom_name) if head == "mixed" and args.nodes == 500_000 else None
         copy = copy_peak_gbps()
         log(f"achievable copy bandwidth: {copy:.0f} GB/s")
-        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": dom["GBps"], "peak": peak, "unit": "GB/s",
-                    "frac": dom["GBps"] / peak, "traffic": traffic, "bytes_per_launch": dom["bytes"],
-                    "avg_launch_ms": dom["avg_ms"],
-                    "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
-                    "achievable_copy": copy, "frac_of_achievable": dom["GBps"] / copy}
+        def line(name):
+            k = kernmx[name]
+            t = measured_traffic(name) if head == "mixed" and args.nodes == 500_000 else None
+            return {"bound": "hbm", "kernel": name, "achieved": k["GBps"], "peak": peak, "unit": "GB/s",
+                    "frac": k["GBps"] / peak, "traffic": t, "bytes_per_launch": k["bytes"], "avg_launch_ms": k["avg_ms"],
+                    # a launch's algorithmic bytes within the 256 MB Infinity Cache: its operands can stay
+                    # cache-resident between launches, so the HBM fraction understates what bounds it
+                    "working_set": "exceeds the 256 MB Infinity Cache" if k["bytes"] > LLC_BYTES else "Infinity-Cache resident",
+                    "traffic_source": (os.path.relpath(TRAFFIC_FILE, ROOT) + " (counter passes of the same kernel and "
+                                       "workload, committed; not measured in this run)") if t else None,
+                    "achievable_copy": copy, "frac_of_achievable": k["GBps"] / copy}
+
+        roofline = line(dom_name)
+        big = [k for k in kernmx if not k.endswith(("_iteration", "_vcycle")) and kernmx[k]["bytes"] > LLC_BYTES]
+        roofline_llc = line(max(big, key=lambda k: kernmx[k]["ms_per_step"])) if big else None
     out = {
         "metric": "Chorin steps/s @500k p-DOFs, mixed vs FP64 GPU, same residual; SpMV % HBM peak",
         "value": rmx["steps_per_s"],
@@ -716,7 +747,10 @@ def main():
                    "velocity_dofs_per_component": ctx.numVelocityNodes, "elements": ctx.numElements,
                    "reynolds": 100, "peak_inflow": cond.peakVelocity, "viscosity": cond.viscosity,
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
-                   "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 symmetric GS preconditioner",
+                   "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 preconditioners",
+                   "preconditioners": {"velocity": "one symmetric multicolour GS sweep",
+                                       "pressure": ("smoothed-aggregation V-cycle, multicolour GS smoothing"
+                                                    if args.pressure_solver == "amg" else "one symmetric multicolour GS sweep")},
                    "pressure_start": args.pressure_start,
                    "velocity_start": args.velocity_start,
                    "pressure_solver": args.pressure_solver,
@@ -727,6 +761,12 @@ def main():
         "mixed_over_fp64": (results["mixed"]["steps_per_s"] / r64["steps_per_s"]) if (r64 and "mixed" in results) else None,
         "errors": errors,
         "roofline": roofline,
+        "roofline_llc_exceeding": roofline_llc,
+        "fp64_mcgs": results.get("fp64_mcgs"),
+        "mixed_mcgs": results.get("mixed_mcgs"),
+        "mixed_mcgs_over_fp64_mcgs": (results["mixed_mcgs"]["steps_per_s"] / results["fp64_mcgs"]["steps_per_s"])
+        if ("mixed_mcgs" in results and "fp64_mcgs" in results) else None,
+        "mixed_over_fp64_mcgs": (rmx["steps_per_s"] / results["fp64_mcgs"]["steps_per_s"]) if "fp64_mcgs" in results else None,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "ensemble": ensemble,

@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <stdexcept>
 #include <type_traits>
 
@@ -348,6 +349,13 @@ namespace nsfem
             const double rr = init ? 0.0 : sumParts(st.partRr, st.parts, sm);
             if (threadIdx.x == 0)
             {
+                if (st.tol2 < 0)
+                {
+                    // kernel timing: never stops, whatever the vectors hold
+                    st.scal[1] = st.scal[0];
+                    st.scal[0] = std::isfinite(rho) && rho > 0 ? rho : 1.0;
+                    return;
+                }
                 if (!init)
                 {
                     st.scal[3] = rr;
@@ -410,6 +418,13 @@ namespace nsfem
             L.n = H.n;
             L.colorStart = H.colorStart;
             L.lanes = lanesFor(meanRow(H.stripped.rowStart));
+            L.nnz = H.stripped.nnz();
+            for (int i = 0; i < H.n; i++)
+            {
+                L.nnzLower += H.rowSplit[i] - H.stripped.rowStart[i];
+            }
+            L.nnzP = H.P.nnz();
+            L.nnzR = H.R.nnz();
             if (l > 0)
             {
                 L.rowStart.upload(H.stripped.rowStart);
@@ -668,6 +683,128 @@ namespace nsfem
         amgRhoKernel<P><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st);
         amgFinalizeKernel<<<1, kBlock, 0, s>>>(st, 0);
         NSFEM_HIP(hipGetLastError());
+    }
+
+    double AmgPcg::timeKernel(const std::string & which, int reps, double * bytesOut, double * launchesOut, double * x)
+    {
+        if (reps < 1)
+        {
+            throw std::invalid_argument("reps must be positive");
+        }
+        const hipStream_t s = stream_;
+        const double vs = mixed_ ? 4 : 8, n = n_;
+        const double nnz0 = levels_[0].nnz;
+        const double bDir = nnz0 * 12 + n * (4 + 8 + vs + 32);
+        const double bUpd = n * (8 + 8 + 16 + 16 + (mixed_ ? 4 : 0));
+        const double bRho = n * (8 + vs);
+        double bV = static_cast<double>(coarseN_) * coarseN_ * vs;
+        for (int l = 0; l + 1 < numLevels(); l++)
+        {
+            const Level & L = levels_[l];
+            // forward lower parts, residual and backward full rows; the diagonal
+            // or its inverse three times; b, x and r about eight passes; P and R
+            bV += (L.nnzLower + 2 * L.nnz) * (vs + 4) + 3 * 4.0 * (L.n + 1) + 3 * vs * L.n + 8 * vs * L.n;
+            bV += (L.nnzP + L.nnzR) * (vs + 4) + 4.0 * (L.pRows + L.rRows + 2);
+        }
+        double bytes = 0;
+        std::function<void()> launch;
+        // a stop flag that never rises: every kernel does its work; finite
+        // step scalars
+        fillInt(4, 0, ints_.get(), s);
+        fill(F64, 4, 1.0, scal_.get(), s);
+        fill(F64, parts_, 1.0, partPq_.get(), s);
+        const double tol2 = -1.0;
+        const AmgState st{scal_.get(), ints_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
+        if (x != xDevice_)
+        {
+            xDevice_ = x;
+            destroyGraphs();
+        }
+        auto typed = [&](auto tag)
+        {
+            using P = decltype(tag);
+            P * rP = mixed_ ? reinterpret_cast<P *>(rP_.get()) : reinterpret_cast<P *>(r_.get());
+            P * z = reinterpret_cast<P *>(z_.get());
+            if (which == "direction")
+            {
+                bytes = bDir;
+                launch = [&, rP, z]()
+                {
+                    byLanes(levels_[0].lanes, [&]<int LN>()
+                            { amgDirectionKernel<P, LN><<<parts_, kBlock, 0, s>>>(a64_, z, p_.get(), q_.get(), st); });
+                };
+            }
+            else if (which == "update")
+            {
+                bytes = bUpd;
+                launch = [&, rP]() { amgUpdateKernel<P><<<parts_, kBlock, 0, s>>>(n_, p_.get(), q_.get(), x, r_.get(), rP, st); };
+            }
+            else if (which == "rho")
+            {
+                bytes = bRho;
+                launch = [&, z]() { amgRhoKernel<P><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st); };
+            }
+            else if (which == "vcycle")
+            {
+                bytes = bV;
+                launch = [&, rP, z]() { enqueueVcycle<P>(s, rP, z); };
+            }
+            else if (which == "iteration")
+            {
+                bytes = bDir + bUpd + bRho + bV;
+                if (!iterExec_ || graphTol2_ != tol2)
+                {
+                    destroyGraphs();
+                    hipGraph_t g = nullptr;
+                    NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    enqueueIteration<P>(s, tol2);
+                    NSFEM_HIP(hipStreamEndCapture(s, &g));
+                    const hipError_t rc = hipGraphInstantiate(&iterExec_, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    NSFEM_HIP(rc);
+                    graphTol2_ = tol2;
+                }
+                launch = [&]() { NSFEM_HIP(hipGraphLaunch(iterExec_, s)); };
+            }
+            else
+            {
+                throw std::invalid_argument("unknown amg_pcg kernel '" + which + "' (direction, update, rho, vcycle, iteration)");
+            }
+        };
+        if (mixed_)
+        {
+            typed(float{});
+        }
+        else
+        {
+            typed(double{});
+        }
+        launch(); // warm
+        hipEvent_t e0, e1;
+        NSFEM_HIP(hipEventCreate(&e0));
+        NSFEM_HIP(hipEventCreate(&e1));
+        NSFEM_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; i++)
+        {
+            launch();
+        }
+        NSFEM_HIP(hipEventRecord(e1, s));
+        NSFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        NSFEM_HIP(hipEventDestroy(e0));
+        NSFEM_HIP(hipEventDestroy(e1));
+        NSFEM_HIP(hipGetLastError());
+        destroyGraphs(); // the timing graph holds a negative tolerance
+        if (bytesOut)
+        {
+            *bytesOut = bytes;
+        }
+        if (launchesOut)
+        {
+            *launchesOut = 1;
+        }
+        return static_cast<double>(ms) / reps;
     }
 
     AmgPcg::Result AmgPcg::solve(const double * b, double * x, bool zeroGuess, double target, int maxIters, bool useGraphs)

@@ -18,6 +18,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -48,6 +49,15 @@ namespace nsfem
         // guess unless zeroGuess.  Blocking.
         Result solve(const double * b, double * x, bool zeroGuess, double target, int maxIters, bool useGraphs);
 
+        // Average duration (ms) of one launch of `which`, HIP events around
+        // reps back-to-back replays on the solver's stream: "direction",
+        // "update", "rho", "vcycle" (the V-cycle's launches in sequence) or
+        // "iteration" (every launch of one CG iteration, replayed from its
+        // captured graph); the algorithmic bytes of one launch and the
+        // launches per iteration.  x is the caller's solution vector
+        // (overwritten, as are the internal work vectors).
+        double timeKernel(const std::string & which, int reps, double * bytes, double * launches, double * x);
+
         int numLevels() const { return static_cast<int>(levels_.size()); }
         int levelRows(int l) const { return levels_[l].n; }
         int levelColors(int l) const { return static_cast<int>(levels_[l].colorStart.size()) - 1; }
@@ -60,6 +70,7 @@ namespace nsfem
             int n = 0;
             std::vector<int> colorStart;
             int lanes = 4, pLanes = 4, rLanes = 8;
+            double nnz = 0, nnzLower = 0, nnzP = 0, nnzR = 0; // entry counts (algorithmic bytes)
             // this level's matrix (levels >= 1; level 0 uses the caller's)
             DeviceArray<int> rowStart, rowSplit, column;
             DeviceArray<double> values64, diag64, invDiag64;

@@ -1863,9 +1863,13 @@ namespace nsfem
         {
             throw std::invalid_argument("reps must be positive");
         }
+        if (amg_)
+        {
+            return amg_->timeKernel(which, reps, bytesPerLaunch, launchesPerIteration, x64_.get());
+        }
         if (opt_.method != SolverMethod::Pcg)
         {
-            throw std::invalid_argument("kernel timing is available for pcg solvers");
+            throw std::invalid_argument("kernel timing is available for pcg and amg_pcg solvers");
         }
         hipStream_t s = stream_;
         const int nch = opt_.numCh;
