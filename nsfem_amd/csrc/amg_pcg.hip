@@ -14,8 +14,9 @@ namespace nsfem
     {
         struct AmgState
         {
-            double * scal; // rho, rho of the iteration before, |b|^2, |r|^2
-            int * ints;    // stop (0 running, 1 converged, 2 non-finite or breakdown), iterations, zero right-hand side
+            double * scal; // per channel, blocks of NCH: rho, rho of the iteration before, |b|^2, |r|^2
+            int * ints;    // stop (0 running, 1 converged, 2 non-finite or breakdown), iterations, zero right-hand sides
+            int * active;  // per channel
             double * partPq;
             double * partRr;
             double * partRho;
@@ -24,14 +25,14 @@ namespace nsfem
             double tol2;
         };
 
-        // ---------------- V-cycle ----------------
+        // ---------------- V-cycle (channels interleaved per row) ----------------
 
         // One colour of a multicolour Gauss-Seidel sweep: rows [rowBegin,
         // rowEnd) (one colour, independent of each other) take
         // x_i = (b_i - sum_j a_ij x_j) / a_ii over the lower-colour part of
         // the row (lowerOnly: the first sweep from x = 0, where the higher
         // colours are still zero) or the whole row.  L lanes per row.
-        template <typename T, int L>
+        template <typename T, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgSweepKernel(McgsMatrix<T> A, int rowBegin, int rowEnd, const T * __restrict__ b,
                                                                  T * __restrict__ x, int lowerOnly, const int * __restrict__ stop)
         {
@@ -42,24 +43,45 @@ namespace nsfem
             const int t = blockIdx.x * kBlock + threadIdx.x;
             const int row = rowBegin + t / L;
             const int lane = t % L;
-            T s = 0;
+            T s[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = 0;
+            }
             if (row < rowEnd)
             {
                 const int j1 = lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1];
                 for (int j = A.rowStart[row] + lane; j < j1; j += L)
                 {
-                    s = fmaT(A.values[j], x[A.column[j]], s);
+                    const T a = A.values[j];
+                    const long col = static_cast<long>(A.column[j]) * NCH;
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        s[c] = fmaT(a, x[col + c], s[c]);
+                    }
                 }
             }
-            s = laneGroupSum<L>(s);
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = laneGroupSum<L>(s[c]);
+            }
             if (row < rowEnd && lane == 0)
             {
-                x[row] = (b[row] - s) * A.invDiag[row];
+                const T d = A.invDiag[row];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    x[i] = (b[i] - s[c]) * d;
+                }
             }
         }
 
         // r = b - A x (A = stripped + diagonal).
-        template <typename T, int L>
+        template <typename T, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgResidualKernel(McgsMatrix<T> A, const T * __restrict__ b, const T * __restrict__ x,
                                                                     T * __restrict__ r, const int * __restrict__ stop)
         {
@@ -70,23 +92,44 @@ namespace nsfem
             const int t = blockIdx.x * kBlock + threadIdx.x;
             const int row = t / L;
             const int lane = t % L;
-            T s = 0;
+            T s[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = 0;
+            }
             if (row < A.n)
             {
                 for (int j = A.rowStart[row] + lane; j < A.rowStart[row + 1]; j += L)
                 {
-                    s = fmaT(A.values[j], x[A.column[j]], s);
+                    const T a = A.values[j];
+                    const long col = static_cast<long>(A.column[j]) * NCH;
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        s[c] = fmaT(a, x[col + c], s[c]);
+                    }
                 }
             }
-            s = laneGroupSum<L>(s);
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = laneGroupSum<L>(s[c]);
+            }
             if (row < A.n && lane == 0)
             {
-                r[row] = b[row] - fmaT(A.diag[row], x[row], s);
+                const T d = A.diag[row];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    r[i] = b[i] - fmaT(d, x[i], s[c]);
+                }
             }
         }
 
         // y = M x (accumulate: y += M x) for the restriction and prolongation.
-        template <typename T, int L>
+        template <typename T, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgSpmvKernel(int rows, const int * __restrict__ rowStart, const int * __restrict__ column,
                                                                 const T * __restrict__ values, const T * __restrict__ x, T * __restrict__ y,
                                                                 int accumulate, const int * __restrict__ stop)
@@ -98,23 +141,43 @@ namespace nsfem
             const int t = blockIdx.x * kBlock + threadIdx.x;
             const int row = t / L;
             const int lane = t % L;
-            T s = 0;
+            T s[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = 0;
+            }
             if (row < rows)
             {
                 for (int j = rowStart[row] + lane; j < rowStart[row + 1]; j += L)
                 {
-                    s = fmaT(values[j], x[column[j]], s);
+                    const T a = values[j];
+                    const long col = static_cast<long>(column[j]) * NCH;
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        s[c] = fmaT(a, x[col + c], s[c]);
+                    }
                 }
             }
-            s = laneGroupSum<L>(s);
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = laneGroupSum<L>(s[c]);
+            }
             if (row < rows && lane == 0)
             {
-                y[row] = accumulate ? y[row] + s : s;
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    y[i] = accumulate ? y[i] + s[c] : s[c];
+                }
             }
         }
 
         // x = Ainv b on the coarsest level (dense, row-major): one wavefront per row.
-        template <typename T>
+        template <typename T, int NCH>
         __global__ void __launch_bounds__(kBlock) amgDenseKernel(int n, const T * __restrict__ inv, const T * __restrict__ b,
                                                                  T * __restrict__ x, const int * __restrict__ stop)
         {
@@ -124,114 +187,187 @@ namespace nsfem
             }
             const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
             const int lane = threadIdx.x & 63;
-            T s = 0;
+            T s[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = 0;
+            }
             if (row < n)
             {
                 const T * a = inv + static_cast<size_t>(row) * n;
                 for (int j = lane; j < n; j += 64)
                 {
-                    s = fmaT(a[j], b[j], s);
+                    const T v = a[j];
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        s[c] = fmaT(v, b[static_cast<long>(j) * NCH + c], s[c]);
+                    }
                 }
             }
-            s = laneGroupSum<64>(s);
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                s[c] = laneGroupSum<64>(s[c]);
+            }
             if (row < n && lane == 0)
             {
-                x[row] = s;
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    x[static_cast<long>(row) * NCH + c] = s[c];
+                }
             }
         }
 
         // ---------------- CG around it ----------------
 
-        // Sum of parts partials in a fixed order (the same in every block).
-        __device__ __forceinline__ double sumParts(const double * __restrict__ parts, int count, double * sm)
+        // Per-channel sums of count partials (parts[k * K + c]) in a fixed
+        // order, the same in every block; valid in thread 0.
+        template <int K>
+        __device__ __forceinline__ void sumParts(const double * __restrict__ parts, int count, double (&v)[K], double * sm)
         {
-            double v[1] = {0};
+#pragma unroll
+            for (int c = 0; c < K; c++)
+            {
+                v[c] = 0;
+            }
             for (int k = threadIdx.x; k < count; k += kBlock)
             {
-                v[0] += parts[k];
+#pragma unroll
+                for (int c = 0; c < K; c++)
+                {
+                    v[c] += parts[static_cast<size_t>(k) * K + c];
+                }
             }
-            blockSum<1>(v, sm);
-            return v[0]; // thread 0
+            blockSum<K>(v, sm);
+        }
+
+        template <int K>
+        __device__ __forceinline__ void storeParts(const double (&v)[K], double * parts)
+        {
+            if (threadIdx.x == 0)
+            {
+#pragma unroll
+                for (int c = 0; c < K; c++)
+                {
+                    parts[static_cast<size_t>(blockIdx.x) * K + c] = v[c];
+                }
+            }
         }
 
         // r = b - A x (x = 0 and r = b for a zero guess), the V-cycle's copy
-        // of r, partials of |b|^2 and |r|^2.
-        template <typename P, int L>
+        // of r, per-channel partials of |b|^2 and |r|^2.
+        template <typename P, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgInitKernel(McgsMatrix<double> A, const double * __restrict__ b, double * __restrict__ x,
                                                                 double * __restrict__ r, P * __restrict__ rP, int zeroGuess, AmgState st)
         {
-            __shared__ double sm[2 * kWaves];
+            __shared__ double sm[2 * NCH * kWaves];
             constexpr int G = kBlock / L;
             const int lane = threadIdx.x % L;
-            double acc[2] = {0, 0};
+            double bb[NCH], rr[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                bb[c] = rr[c] = 0;
+            }
             for (int base = blockIdx.x * G; base < A.n; base += gridDim.x * G)
             {
                 const int row = base + static_cast<int>(threadIdx.x) / L;
-                double s = 0;
+                double s[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    s[c] = 0;
+                }
                 if (row < A.n && !zeroGuess)
                 {
                     for (int j = A.rowStart[row] + lane; j < A.rowStart[row + 1]; j += L)
                     {
-                        s = fmaT(A.values[j], x[A.column[j]], s);
+                        const double a = A.values[j];
+                        const long col = static_cast<long>(A.column[j]) * NCH;
+#pragma unroll
+                        for (int c = 0; c < NCH; c++)
+                        {
+                            s[c] = fmaT(a, x[col + c], s[c]);
+                        }
                     }
                 }
-                s = laneGroupSum<L>(s);
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    s[c] = laneGroupSum<L>(s[c]);
+                }
                 if (row < A.n && lane == 0)
                 {
-                    const double bi = b[row];
-                    double ri = bi;
-                    if (zeroGuess)
+                    const double d = A.diag[row];
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
                     {
-                        x[row] = 0;
+                        const long i = static_cast<long>(row) * NCH + c;
+                        const double bi = b[i];
+                        double ri = bi;
+                        if (zeroGuess)
+                        {
+                            x[i] = 0;
+                        }
+                        else
+                        {
+                            ri = bi - fmaT(d, x[i], s[c]);
+                        }
+                        r[i] = ri;
+                        if constexpr (!std::is_same_v<P, double>)
+                        {
+                            rP[i] = static_cast<P>(ri);
+                        }
+                        bb[c] = fmaT(bi, bi, bb[c]);
+                        rr[c] = fmaT(ri, ri, rr[c]);
                     }
-                    else
-                    {
-                        ri = bi - fmaT(A.diag[row], x[row], s);
-                    }
-                    r[row] = ri;
-                    if constexpr (!std::is_same_v<P, double>)
-                    {
-                        rP[row] = static_cast<P>(ri);
-                    }
-                    acc[0] = fmaT(bi, bi, acc[0]);
-                    acc[1] = fmaT(ri, ri, acc[1]);
                 }
             }
-            blockSum<2>(acc, sm);
-            if (threadIdx.x == 0)
-            {
-                st.partBb[blockIdx.x] = acc[0];
-                st.partRr[blockIdx.x] = acc[1];
-            }
+            blockSum<NCH>(bb, sm);
+            blockSum<NCH>(rr, sm);
+            storeParts<NCH>(bb, st.partBb);
+            storeParts<NCH>(rr, st.partRr);
         }
 
+        template <int NCH>
         __global__ void __launch_bounds__(kBlock) amgInitFinalizeKernel(AmgState st)
         {
-            __shared__ double sm[kWaves];
-            const double bb = sumParts(st.partBb, st.parts, sm);
-            const double rr = sumParts(st.partRr, st.parts, sm);
+            __shared__ double sm[NCH * kWaves];
+            double bb[NCH], rr[NCH];
+            sumParts<NCH>(st.partBb, st.parts, bb, sm);
+            sumParts<NCH>(st.partRr, st.parts, rr, sm);
             if (threadIdx.x == 0)
             {
-                st.scal[0] = 0;
-                st.scal[1] = 0;
-                st.scal[2] = bb;
-                st.scal[3] = rr;
+                int any = 0, bad = 0, zero = 0;
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    st.scal[c] = 0;
+                    st.scal[NCH + c] = 0;
+                    st.scal[2 * NCH + c] = bb[c];
+                    st.scal[3 * NCH + c] = rr[c];
+                    bad |= !std::isfinite(bb[c]) || !std::isfinite(rr[c]);
+                    const int act = bb[c] > 0 && rr[c] > st.tol2 * bb[c];
+                    st.active[c] = act;
+                    any |= act;
+                    if (bb[c] == 0 && rr[c] > 0)
+                    {
+                        zero |= 1 << c; // b = 0 from a nonzero guess: the solution is 0
+                    }
+                }
                 st.ints[1] = 0;
-                st.ints[2] = bb == 0 ? 1 : 0;
-                if (!std::isfinite(bb) || !std::isfinite(rr))
-                {
-                    st.ints[0] = 2;
-                }
-                else
-                {
-                    st.ints[0] = (bb == 0 || rr <= st.tol2 * bb) ? 1 : 0;
-                }
+                st.ints[2] = zero;
+                st.ints[0] = bad ? 2 : (any ? 0 : 1);
             }
         }
 
-        // beta = rho / rho_before (0 on the first direction); p = z + beta p,
-        // q = A z + beta q (= A p, as A p_before = q), partials of p.q.
-        template <typename P, int L>
+        // beta = rho / rho_before (0 on the first direction and for stopped
+        // channels); p = z + beta p, q = A z + beta q (= A p, as
+        // A p_before = q), per-channel partials of p.q.
+        template <typename P, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgDirectionKernel(McgsMatrix<double> A, const P * __restrict__ z, double * __restrict__ p,
                                                                      double * __restrict__ q, AmgState st)
         {
@@ -239,45 +375,74 @@ namespace nsfem
             {
                 return;
             }
-            __shared__ double sm[kWaves];
-            const double rho = st.scal[0], before = st.scal[1];
-            const double beta = before != 0 ? rho / before : 0.0;
+            __shared__ double sm[NCH * kWaves];
+            double beta[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                const double before = st.scal[NCH + c];
+                beta[c] = (st.active[c] && before != 0) ? st.scal[c] / before : 0.0;
+            }
             constexpr int G = kBlock / L;
             const int lane = threadIdx.x % L;
-            double acc[1] = {0};
+            double acc[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                acc[c] = 0;
+            }
             for (int base = blockIdx.x * G; base < A.n; base += gridDim.x * G)
             {
                 const int row = base + static_cast<int>(threadIdx.x) / L;
-                double s = 0;
+                double s[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    s[c] = 0;
+                }
                 if (row < A.n)
                 {
                     for (int j = A.rowStart[row] + lane; j < A.rowStart[row + 1]; j += L)
                     {
-                        s = fmaT(A.values[j], static_cast<double>(z[A.column[j]]), s);
+                        const double a = A.values[j];
+                        const long col = static_cast<long>(A.column[j]) * NCH;
+#pragma unroll
+                        for (int c = 0; c < NCH; c++)
+                        {
+                            s[c] = fmaT(a, static_cast<double>(z[col + c]), s[c]);
+                        }
                     }
                 }
-                s = laneGroupSum<L>(s);
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    s[c] = laneGroupSum<L>(s[c]);
+                }
                 if (row < A.n && lane == 0)
                 {
-                    const double zi = static_cast<double>(z[row]);
-                    const double pi = fmaT(beta, p[row], zi);
-                    const double qi = fmaT(beta, q[row], fmaT(A.diag[row], zi, s));
-                    p[row] = pi;
-                    q[row] = qi;
-                    acc[0] = fmaT(pi, qi, acc[0]);
+                    const double d = A.diag[row];
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const long i = static_cast<long>(row) * NCH + c;
+                        const double zi = static_cast<double>(z[i]);
+                        const double pi = fmaT(beta[c], p[i], zi);
+                        const double qi = fmaT(beta[c], q[i], fmaT(d, zi, s[c]));
+                        p[i] = pi;
+                        q[i] = qi;
+                        acc[c] = fmaT(pi, qi, acc[c]);
+                    }
                 }
             }
-            blockSum<1>(acc, sm);
-            if (threadIdx.x == 0)
-            {
-                st.partPq[blockIdx.x] = acc[0];
-            }
+            blockSum<NCH>(acc, sm);
+            storeParts<NCH>(acc, st.partPq);
         }
 
-        // alpha = rho / p.q (every block sums the partials itself, in the same
+        // alpha = rho / p.q for the channels still iterating, 0 for the
+        // stopped ones (every block sums the partials itself, in the same
         // order); x += alpha p, r -= alpha q, the V-cycle's copy of r,
-        // partials of |r|^2.
-        template <typename P>
+        // per-channel partials of |r|^2.
+        template <typename P, int NCH>
         __global__ void __launch_bounds__(kBlock) amgUpdateKernel(int n, const double * __restrict__ p, const double * __restrict__ q,
                                                                   double * __restrict__ x, double * __restrict__ r, P * __restrict__ rP,
                                                                   AmgState st)
@@ -286,98 +451,152 @@ namespace nsfem
             {
                 return;
             }
-            __shared__ double sm[kWaves];
-            __shared__ double shAlpha;
-            const double pq = sumParts(st.partPq, st.parts, sm);
+            __shared__ double sm[NCH * kWaves];
+            __shared__ double shAlpha[NCH];
+            double pq[NCH];
+            sumParts<NCH>(st.partPq, st.parts, pq, sm);
             if (threadIdx.x == 0)
             {
-                shAlpha = st.scal[0] / pq;
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    shAlpha[c] = st.active[c] ? st.scal[c] / pq[c] : 0.0;
+                }
             }
             __syncthreads();
-            const double alpha = shAlpha;
-            double acc[1] = {0};
-            for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+            double alpha[NCH], acc[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
             {
-                x[i] = fmaT(alpha, p[i], x[i]);
-                const double ri = fmaT(-alpha, q[i], r[i]);
-                r[i] = ri;
-                if constexpr (!std::is_same_v<P, double>)
+                alpha[c] = shAlpha[c];
+                acc[c] = 0;
+            }
+            for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock)
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
                 {
-                    rP[i] = static_cast<P>(ri);
+                    const long i = static_cast<long>(row) * NCH + c;
+                    x[i] = fmaT(alpha[c], p[i], x[i]);
+                    const double ri = fmaT(-alpha[c], q[i], r[i]);
+                    r[i] = ri;
+                    if constexpr (!std::is_same_v<P, double>)
+                    {
+                        rP[i] = static_cast<P>(ri);
+                    }
+                    acc[c] = fmaT(ri, ri, acc[c]);
                 }
-                acc[0] = fmaT(ri, ri, acc[0]);
             }
-            blockSum<1>(acc, sm);
-            if (threadIdx.x == 0)
-            {
-                st.partRr[blockIdx.x] = acc[0];
-            }
+            blockSum<NCH>(acc, sm);
+            storeParts<NCH>(acc, st.partRr);
         }
 
-        // partials of r.z
-        template <typename P>
+        // per-channel partials of r.z
+        template <typename P, int NCH>
         __global__ void __launch_bounds__(kBlock) amgRhoKernel(int n, const double * __restrict__ r, const P * __restrict__ z, AmgState st)
         {
             if (st.ints[0])
             {
                 return;
             }
-            __shared__ double sm[kWaves];
-            double acc[1] = {0};
-            for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+            __shared__ double sm[NCH * kWaves];
+            double acc[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
             {
-                acc[0] = fmaT(r[i], static_cast<double>(z[i]), acc[0]);
+                acc[c] = 0;
             }
-            blockSum<1>(acc, sm);
-            if (threadIdx.x == 0)
+            for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock)
             {
-                st.partRho[blockIdx.x] = acc[0];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = static_cast<long>(row) * NCH + c;
+                    acc[c] = fmaT(r[i], static_cast<double>(z[i]), acc[c]);
+                }
             }
+            blockSum<NCH>(acc, sm);
+            storeParts<NCH>(acc, st.partRho);
         }
 
-        // rho = r.z; after an update (init = 0) also |r|^2, the iteration
-        // count and the stopping test.  A preconditioned residual with
-        // r.z <= 0 (a breakdown) stops the solve as non-finite.
+        // rho = r.z per channel; after an update (init = 0) also |r|^2, the
+        // iteration count and each channel's stopping test.  A channel still
+        // iterating with r.z <= 0 (a breakdown) or a non-finite value stops
+        // the solve as non-finite; the solve stops when no channel iterates.
+        template <int NCH>
         __global__ void __launch_bounds__(kBlock) amgFinalizeKernel(AmgState st, int init)
         {
             if (st.ints[0])
             {
                 return;
             }
-            __shared__ double sm[kWaves];
-            const double rho = sumParts(st.partRho, st.parts, sm);
-            const double rr = init ? 0.0 : sumParts(st.partRr, st.parts, sm);
+            __shared__ double sm[NCH * kWaves];
+            double rho[NCH], rr[NCH];
+            sumParts<NCH>(st.partRho, st.parts, rho, sm);
+            if (!init)
+            {
+                sumParts<NCH>(st.partRr, st.parts, rr, sm);
+            }
             if (threadIdx.x == 0)
             {
                 if (st.tol2 < 0)
                 {
                     // kernel timing: never stops, whatever the vectors hold
-                    st.scal[1] = st.scal[0];
-                    st.scal[0] = std::isfinite(rho) && rho > 0 ? rho : 1.0;
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        st.scal[NCH + c] = st.scal[c];
+                        st.scal[c] = std::isfinite(rho[c]) && rho[c] > 0 ? rho[c] : 1.0;
+                    }
                     return;
+                }
+                int any = 0, bad = 0;
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    int act = st.active[c];
+                    if (!init)
+                    {
+                        st.scal[3 * NCH + c] = rr[c];
+                        if (act)
+                        {
+                            if (!std::isfinite(rr[c]))
+                            {
+                                bad = 1;
+                            }
+                            else if (rr[c] <= st.tol2 * st.scal[2 * NCH + c])
+                            {
+                                act = 0;
+                            }
+                        }
+                    }
+                    if (act && (!std::isfinite(rho[c]) || !(rho[c] > 0)))
+                    {
+                        bad = 1;
+                    }
+                    st.active[c] = act;
+                    any |= act;
+                    st.scal[NCH + c] = init ? 0.0 : st.scal[c];
+                    st.scal[c] = rho[c];
                 }
                 if (!init)
                 {
-                    st.scal[3] = rr;
                     st.ints[1] += 1;
-                    if (!std::isfinite(rr))
-                    {
-                        st.ints[0] = 2;
-                        return;
-                    }
-                    if (rr <= st.tol2 * st.scal[2])
-                    {
-                        st.ints[0] = 1;
-                        return;
-                    }
                 }
-                if (!std::isfinite(rho) || !(rho > 0))
+                st.ints[0] = bad ? 2 : (any ? 0 : 1);
+            }
+        }
+
+        // Channels of x (interleaved) whose bit is set in mask become 0.
+        __global__ void amgZeroChannelsKernel(int n, int nch, int mask, double * __restrict__ x)
+        {
+            const long total = static_cast<long>(n) * nch;
+            for (long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; t < total; t += static_cast<long>(gridDim.x) * blockDim.x)
+            {
+                if ((mask >> static_cast<int>(t % nch)) & 1)
                 {
-                    st.ints[0] = 2;
-                    return;
+                    x[t] = 0;
                 }
-                st.scal[1] = init ? 0.0 : st.scal[0];
-                st.scal[0] = rho;
             }
         }
 
@@ -403,14 +622,20 @@ namespace nsfem
     } // namespace
 
     AmgPcg::AmgPcg(const CsrD & stripped, const std::vector<int> & rowSplit, const std::vector<double> & diag,
-                   const std::vector<int> & colorStart, bool mixed, McgsMatrix<double> a64, McgsMatrix<float> level0F32, hipStream_t s)
-        : mixed_(mixed), n_(stripped.rows), a64_(a64), a32_(level0F32), stream_(s)
+                   const std::vector<int> & colorStart, bool mixed, int numCh, McgsMatrix<double> a64, McgsMatrix<float> level0F32,
+                   hipStream_t s)
+        : mixed_(mixed), n_(stripped.rows), nch_(numCh), a64_(a64), a32_(level0F32), stream_(s)
     {
+        if (nch_ != 1 && nch_ != 2 && nch_ != 4 && nch_ != 8 && nch_ != 16)
+        {
+            throw std::invalid_argument("amg_pcg solves 1, 2, 4, 8 or 16 right-hand sides");
+        }
         const auto t0 = std::chrono::steady_clock::now();
         const AmgHierarchy h = buildAmgHierarchy(stripped, rowSplit, diag, colorStart);
         complexity_ = h.operatorComplexity;
         levels_.resize(h.levels.size());
         const size_t ts = mixed_ ? 4 : 8;
+        const size_t vb = ts * nch_; // bytes per row of a V-cycle vector
         for (size_t l = 0; l < h.levels.size(); l++)
         {
             const AmgLevel & H = h.levels[l];
@@ -448,12 +673,12 @@ namespace nsfem
                     L.diag64.upload(H.diag);
                     L.invDiag64.upload(H.invDiag);
                 }
-                L.b.resize(H.n * ts);
-                L.x.resize(H.n * ts);
+                L.b.resize(H.n * vb);
+                L.x.resize(H.n * vb);
             }
             if (l + 1 < h.levels.size())
             {
-                L.r.resize(H.n * ts);
+                L.r.resize(H.n * vb);
                 L.pRows = H.P.rows;
                 L.rRows = H.R.rows;
                 L.pLanes = lanesFor(meanRow(H.P.rowStart));
@@ -483,23 +708,26 @@ namespace nsfem
         {
             coarse64_.upload(h.coarseInverse);
         }
-        r_.resize(n_);
-        p_.resize(n_);
-        q_.resize(n_);
-        z_.resize(n_ * ts);
+        const size_t nv = static_cast<size_t>(n_) * nch_;
+        r_.resize(nv);
+        p_.resize(nv);
+        q_.resize(nv);
+        z_.resize(nv * ts);
         if (mixed_)
         {
-            rP_.resize(n_ * ts);
+            rP_.resize(nv * ts);
         }
         parts_ = std::max(1, std::min(1024, (n_ + kBlock - 1) / kBlock));
-        scal_.resize(4);
+        scal_.resize(4 * nch_);
         ints_.resize(4);
+        active_.resize(nch_);
         for (auto * a : {&partPq_, &partRr_, &partRho_, &partBb_})
         {
-            a->resize(parts_);
+            a->resize(static_cast<size_t>(parts_) * nch_);
         }
         ints_.zero();
         scal_.zero();
+        active_.zero();
         NSFEM_HIP(hipHostMalloc(reinterpret_cast<void **>(&hostPoll_), 8 * sizeof(int), hipHostMallocDefault));
         for (hipEvent_t & e : pollEvents_)
         {
@@ -533,6 +761,31 @@ namespace nsfem
         }
     }
 
+    template <typename F>
+    void AmgPcg::typed(F && f)
+    {
+        auto byCh = [&]<typename P>()
+        {
+            switch (nch_)
+            {
+            case 1: f.template operator()<P, 1>(); break;
+            case 2: f.template operator()<P, 2>(); break;
+            case 4: f.template operator()<P, 4>(); break;
+            case 8: f.template operator()<P, 8>(); break;
+            case 16: f.template operator()<P, 16>(); break;
+            default: throw std::invalid_argument("amg_pcg: unsupported channel count");
+            }
+        };
+        if (mixed_)
+        {
+            byCh.template operator()<float>();
+        }
+        else
+        {
+            byCh.template operator()<double>();
+        }
+    }
+
     template <typename T>
     McgsMatrix<T> AmgPcg::levelMatrix(int l) const
     {
@@ -560,7 +813,7 @@ namespace nsfem
         }
     }
 
-    template <typename T>
+    template <typename T, int NCH>
     void AmgPcg::enqueueVcycle(hipStream_t s, const T * b0, T * x0)
     {
         const int * stop = ints_.get();
@@ -589,6 +842,7 @@ namespace nsfem
                 return L.rValues32.get();
             }
         };
+        auto grid = [](long rows, int lanes) { return static_cast<int>((rows * lanes + kBlock - 1) / kBlock); };
         auto sweep = [&](int l, int c, int lowerOnly)
         {
             const Level & L = levels_[l];
@@ -599,10 +853,7 @@ namespace nsfem
             }
             const McgsMatrix<T> A = levelMatrix<T>(l);
             byLanes(L.lanes, [&]<int LN>()
-                    {
-                        const int grid = static_cast<int>((static_cast<long>(r1 - r0) * LN + kBlock - 1) / kBlock);
-                        amgSweepKernel<T, LN><<<grid, kBlock, 0, s>>>(A, r0, r1, bOf(l), xOf(l), lowerOnly, stop);
-                    });
+                    { amgSweepKernel<T, LN, NCH><<<grid(r1 - r0, LN), kBlock, 0, s>>>(A, r0, r1, bOf(l), xOf(l), lowerOnly, stop); });
         };
         for (int l = 0; l < last; l++)
         {
@@ -613,16 +864,12 @@ namespace nsfem
             }
             T * r = reinterpret_cast<T *>(L.r.get());
             const McgsMatrix<T> A = levelMatrix<T>(l);
-            byLanes(L.lanes, [&]<int LN>()
-                    {
-                        const int grid = static_cast<int>((static_cast<long>(L.n) * LN + kBlock - 1) / kBlock);
-                        amgResidualKernel<T, LN><<<grid, kBlock, 0, s>>>(A, bOf(l), xOf(l), r, stop);
-                    });
+            byLanes(L.lanes, [&]<int LN>() { amgResidualKernel<T, LN, NCH><<<grid(L.n, LN), kBlock, 0, s>>>(A, bOf(l), xOf(l), r, stop); });
             byLanes(L.rLanes, [&]<int LN>()
                     {
-                        const int grid = static_cast<int>((static_cast<long>(L.rRows) * LN + kBlock - 1) / kBlock);
-                        amgSpmvKernel<T, LN><<<grid, kBlock, 0, s>>>(L.rRows, L.rRowStart.get(), L.rColumn.get(), rVals(L), r,
-                                                                     reinterpret_cast<T *>(levels_[l + 1].b.get()), 0, stop);
+                        amgSpmvKernel<T, LN, NCH><<<grid(L.rRows, LN), kBlock, 0, s>>>(L.rRows, L.rRowStart.get(), L.rColumn.get(), rVals(L),
+                                                                                      r, reinterpret_cast<T *>(levels_[l + 1].b.get()), 0,
+                                                                                      stop);
                     });
         }
         {
@@ -635,17 +882,15 @@ namespace nsfem
             {
                 inv = coarse32_.get();
             }
-            const int grid = (coarseN_ * 64 + kBlock - 1) / kBlock;
-            amgDenseKernel<T><<<grid, kBlock, 0, s>>>(coarseN_, inv, bOf(last), xOf(last), stop);
+            amgDenseKernel<T, NCH><<<grid(coarseN_, 64), kBlock, 0, s>>>(coarseN_, inv, bOf(last), xOf(last), stop);
         }
         for (int l = last - 1; l >= 0; l--)
         {
             const Level & L = levels_[l];
             byLanes(L.pLanes, [&]<int LN>()
                     {
-                        const int grid = static_cast<int>((static_cast<long>(L.pRows) * LN + kBlock - 1) / kBlock);
-                        amgSpmvKernel<T, LN><<<grid, kBlock, 0, s>>>(L.pRows, L.pRowStart.get(), L.pColumn.get(), pVals(L), xOf(l + 1),
-                                                                     xOf(l), 1, stop);
+                        amgSpmvKernel<T, LN, NCH><<<grid(L.pRows, LN), kBlock, 0, s>>>(L.pRows, L.pRowStart.get(), L.pColumn.get(), pVals(L),
+                                                                                      xOf(l + 1), xOf(l), 1, stop);
                     });
             for (int c = static_cast<int>(L.colorStart.size()) - 2; c >= 0; c--)
             {
@@ -655,33 +900,33 @@ namespace nsfem
         NSFEM_HIP(hipGetLastError());
     }
 
-    template <typename P>
+    template <typename P, int NCH>
     void AmgPcg::enqueueInit(hipStream_t s, const double * b, double * x, bool zeroGuess, double tol2)
     {
-        const AmgState st{scal_.get(), ints_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
+        const AmgState st{scal_.get(), ints_.get(), active_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
         P * rP = mixed_ ? reinterpret_cast<P *>(rP_.get()) : reinterpret_cast<P *>(r_.get());
         P * z = reinterpret_cast<P *>(z_.get());
         byLanes(levels_[0].lanes, [&]<int LN>()
-                { amgInitKernel<P, LN><<<parts_, kBlock, 0, s>>>(a64_, b, x, r_.get(), rP, zeroGuess ? 1 : 0, st); });
-        amgInitFinalizeKernel<<<1, kBlock, 0, s>>>(st);
-        enqueueVcycle<P>(s, rP, z);
-        amgRhoKernel<P><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st);
-        amgFinalizeKernel<<<1, kBlock, 0, s>>>(st, 1);
+                { amgInitKernel<P, LN, NCH><<<parts_, kBlock, 0, s>>>(a64_, b, x, r_.get(), rP, zeroGuess ? 1 : 0, st); });
+        amgInitFinalizeKernel<NCH><<<1, kBlock, 0, s>>>(st);
+        enqueueVcycle<P, NCH>(s, rP, z);
+        amgRhoKernel<P, NCH><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st);
+        amgFinalizeKernel<NCH><<<1, kBlock, 0, s>>>(st, 1);
         NSFEM_HIP(hipGetLastError());
     }
 
-    template <typename P>
+    template <typename P, int NCH>
     void AmgPcg::enqueueIteration(hipStream_t s, double tol2)
     {
-        const AmgState st{scal_.get(), ints_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
+        const AmgState st{scal_.get(), ints_.get(), active_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
         P * rP = mixed_ ? reinterpret_cast<P *>(rP_.get()) : reinterpret_cast<P *>(r_.get());
         P * z = reinterpret_cast<P *>(z_.get());
         byLanes(levels_[0].lanes, [&]<int LN>()
-                { amgDirectionKernel<P, LN><<<parts_, kBlock, 0, s>>>(a64_, z, p_.get(), q_.get(), st); });
-        amgUpdateKernel<P><<<parts_, kBlock, 0, s>>>(n_, p_.get(), q_.get(), xDevice_, r_.get(), rP, st);
-        enqueueVcycle<P>(s, rP, z);
-        amgRhoKernel<P><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st);
-        amgFinalizeKernel<<<1, kBlock, 0, s>>>(st, 0);
+                { amgDirectionKernel<P, LN, NCH><<<parts_, kBlock, 0, s>>>(a64_, z, p_.get(), q_.get(), st); });
+        amgUpdateKernel<P, NCH><<<parts_, kBlock, 0, s>>>(n_, p_.get(), q_.get(), xDevice_, r_.get(), rP, st);
+        enqueueVcycle<P, NCH>(s, rP, z);
+        amgRhoKernel<P, NCH><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st);
+        amgFinalizeKernel<NCH><<<1, kBlock, 0, s>>>(st, 0);
         NSFEM_HIP(hipGetLastError());
     }
 
@@ -692,93 +937,86 @@ namespace nsfem
             throw std::invalid_argument("reps must be positive");
         }
         const hipStream_t s = stream_;
-        const double vs = mixed_ ? 4 : 8, n = n_;
+        const double vs = mixed_ ? 4 : 8, n = n_, k = nch_;
         const double nnz0 = levels_[0].nnz;
-        const double bDir = nnz0 * 12 + n * (4 + 8 + vs + 32);
-        const double bUpd = n * (8 + 8 + 16 + 16 + (mixed_ ? 4 : 0));
-        const double bRho = n * (8 + vs);
-        double bV = static_cast<double>(coarseN_) * coarseN_ * vs;
+        // matrix once, diagonal, z gathered once, p and q read and written
+        const double bDir = nnz0 * 12 + n * (4 + 8) + k * n * (vs + 32);
+        const double bUpd = k * n * (8 + 8 + 16 + 16 + (mixed_ ? 4 : 0));
+        const double bRho = k * n * (8 + vs);
+        double bV = static_cast<double>(coarseN_) * coarseN_ * vs + k * coarseN_ * 2 * vs;
         for (int l = 0; l + 1 < numLevels(); l++)
         {
             const Level & L = levels_[l];
             // forward lower parts, residual and backward full rows; the diagonal
             // or its inverse three times; b, x and r about eight passes; P and R
-            bV += (L.nnzLower + 2 * L.nnz) * (vs + 4) + 3 * 4.0 * (L.n + 1) + 3 * vs * L.n + 8 * vs * L.n;
+            bV += (L.nnzLower + 2 * L.nnz) * (vs + 4) + 3 * 4.0 * (L.n + 1) + 3 * vs * L.n + 8 * k * vs * L.n;
             bV += (L.nnzP + L.nnzR) * (vs + 4) + 4.0 * (L.pRows + L.rRows + 2);
         }
         double bytes = 0;
         std::function<void()> launch;
         // a stop flag that never rises: every kernel does its work; finite
-        // step scalars
-        fillInt(4, 0, ints_.get(), s);
-        fill(F64, 4, 1.0, scal_.get(), s);
-        fill(F64, parts_, 1.0, partPq_.get(), s);
+        // step scalars, every channel active
         const double tol2 = -1.0;
-        const AmgState st{scal_.get(), ints_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
+        fillInt(4, 0, ints_.get(), s);
+        fillInt(nch_, 1, active_.get(), s);
+        fill(F64, 4L * nch_, 1.0, scal_.get(), s);
+        fill(F64, static_cast<long>(parts_) * nch_, 1.0, partPq_.get(), s);
+        const AmgState st{scal_.get(), ints_.get(), active_.get(), partPq_.get(), partRr_.get(), partRho_.get(), partBb_.get(), parts_, tol2};
         if (x != xDevice_)
         {
             xDevice_ = x;
             destroyGraphs();
         }
-        auto typed = [&](auto tag)
-        {
-            using P = decltype(tag);
-            P * rP = mixed_ ? reinterpret_cast<P *>(rP_.get()) : reinterpret_cast<P *>(r_.get());
-            P * z = reinterpret_cast<P *>(z_.get());
-            if (which == "direction")
-            {
-                bytes = bDir;
-                launch = [&, rP, z]()
-                {
-                    byLanes(levels_[0].lanes, [&]<int LN>()
-                            { amgDirectionKernel<P, LN><<<parts_, kBlock, 0, s>>>(a64_, z, p_.get(), q_.get(), st); });
-                };
-            }
-            else if (which == "update")
-            {
-                bytes = bUpd;
-                launch = [&, rP]() { amgUpdateKernel<P><<<parts_, kBlock, 0, s>>>(n_, p_.get(), q_.get(), x, r_.get(), rP, st); };
-            }
-            else if (which == "rho")
-            {
-                bytes = bRho;
-                launch = [&, z]() { amgRhoKernel<P><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st); };
-            }
-            else if (which == "vcycle")
-            {
-                bytes = bV;
-                launch = [&, rP, z]() { enqueueVcycle<P>(s, rP, z); };
-            }
-            else if (which == "iteration")
-            {
-                bytes = bDir + bUpd + bRho + bV;
-                if (!iterExec_ || graphTol2_ != tol2)
-                {
-                    destroyGraphs();
-                    hipGraph_t g = nullptr;
-                    NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    enqueueIteration<P>(s, tol2);
-                    NSFEM_HIP(hipStreamEndCapture(s, &g));
-                    const hipError_t rc = hipGraphInstantiate(&iterExec_, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    NSFEM_HIP(rc);
-                    graphTol2_ = tol2;
-                }
-                launch = [&]() { NSFEM_HIP(hipGraphLaunch(iterExec_, s)); };
-            }
-            else
-            {
-                throw std::invalid_argument("unknown amg_pcg kernel '" + which + "' (direction, update, rho, vcycle, iteration)");
-            }
-        };
-        if (mixed_)
-        {
-            typed(float{});
-        }
-        else
-        {
-            typed(double{});
-        }
+        typed([&]<typename P, int NCH>()
+              {
+                  P * rP = mixed_ ? reinterpret_cast<P *>(rP_.get()) : reinterpret_cast<P *>(r_.get());
+                  P * z = reinterpret_cast<P *>(z_.get());
+                  if (which == "direction")
+                  {
+                      bytes = bDir;
+                      launch = [=, this]()
+                      {
+                          byLanes(levels_[0].lanes, [&]<int LN>()
+                                  { amgDirectionKernel<P, LN, NCH><<<parts_, kBlock, 0, s>>>(a64_, z, p_.get(), q_.get(), st); });
+                      };
+                  }
+                  else if (which == "update")
+                  {
+                      bytes = bUpd;
+                      launch = [=, this]() { amgUpdateKernel<P, NCH><<<parts_, kBlock, 0, s>>>(n_, p_.get(), q_.get(), x, r_.get(), rP, st); };
+                  }
+                  else if (which == "rho")
+                  {
+                      bytes = bRho;
+                      launch = [=, this]() { amgRhoKernel<P, NCH><<<parts_, kBlock, 0, s>>>(n_, r_.get(), z, st); };
+                  }
+                  else if (which == "vcycle")
+                  {
+                      bytes = bV;
+                      launch = [=, this]() { enqueueVcycle<P, NCH>(s, rP, z); };
+                  }
+                  else if (which == "iteration")
+                  {
+                      bytes = bDir + bUpd + bRho + bV;
+                      if (!iterExec_ || graphTol2_ != tol2)
+                      {
+                          destroyGraphs();
+                          hipGraph_t g = nullptr;
+                          NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                          enqueueIteration<P, NCH>(s, tol2);
+                          NSFEM_HIP(hipStreamEndCapture(s, &g));
+                          const hipError_t rc = hipGraphInstantiate(&iterExec_, g, nullptr, nullptr, 0);
+                          (void)hipGraphDestroy(g);
+                          NSFEM_HIP(rc);
+                          graphTol2_ = tol2;
+                      }
+                      launch = [this, s]() { NSFEM_HIP(hipGraphLaunch(iterExec_, s)); };
+                  }
+                  else
+                  {
+                      throw std::invalid_argument("unknown amg_pcg kernel '" + which + "' (direction, update, rho, vcycle, iteration)");
+                  }
+              });
         launch(); // warm
         hipEvent_t e0, e1;
         NSFEM_HIP(hipEventCreate(&e0));
@@ -816,26 +1054,12 @@ namespace nsfem
             xDevice_ = x; // a kernel argument of the captured iteration
             destroyGraphs();
         }
-        if (mixed_)
-        {
-            enqueueInit<float>(s, b, x, zeroGuess, tol2);
-        }
-        else
-        {
-            enqueueInit<double>(s, b, x, zeroGuess, tol2);
-        }
+        typed([&]<typename P, int NCH>() { enqueueInit<P, NCH>(s, b, x, zeroGuess, tol2); });
         auto iterate = [&]()
         {
             if (!useGraphs)
             {
-                if (mixed_)
-                {
-                    enqueueIteration<float>(s, tol2);
-                }
-                else
-                {
-                    enqueueIteration<double>(s, tol2);
-                }
+                typed([&]<typename P, int NCH>() { enqueueIteration<P, NCH>(s, tol2); });
                 return;
             }
             if (!iterExec_ || graphTol2_ != tol2)
@@ -845,14 +1069,7 @@ namespace nsfem
                 NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 try
                 {
-                    if (mixed_)
-                    {
-                        enqueueIteration<float>(s, tol2);
-                    }
-                    else
-                    {
-                        enqueueIteration<double>(s, tol2);
-                    }
+                    typed([&]<typename P, int NCH>() { enqueueIteration<P, NCH>(s, tol2); });
                 }
                 catch (...)
                 {
@@ -901,22 +1118,30 @@ namespace nsfem
             }
             slot = prev;
         }
-        double sc[4];
+        std::vector<double> sc(4 * nch_);
         int in[4];
-        NSFEM_HIP(hipMemcpyAsync(sc, scal_.get(), sizeof sc, hipMemcpyDeviceToHost, s));
+        NSFEM_HIP(hipMemcpyAsync(sc.data(), scal_.get(), sc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         NSFEM_HIP(hipMemcpyAsync(in, ints_.get(), sizeof in, hipMemcpyDeviceToHost, s));
         NSFEM_HIP(hipStreamSynchronize(s));
         Result res;
         res.iterations = in[1];
-        res.bb = sc[2];
-        res.rr = sc[3];
+        res.bb.assign(sc.begin() + 2 * nch_, sc.begin() + 3 * nch_);
+        res.rr.assign(sc.begin() + 3 * nch_, sc.begin() + 4 * nch_);
         res.nonFinite = in[0] == 2;
         res.converged = in[0] == 1;
         if (in[2])
         {
-            fill(F64, n_, 0.0, x, s); // b = 0: x = 0
+            // b = 0 from a nonzero guess: x = 0 in those channels
+            amgZeroChannelsKernel<<<std::max(1, std::min(4096, (n_ * nch_ + kBlock - 1) / kBlock)), kBlock, 0, s>>>(n_, nch_, in[2], x);
+            NSFEM_HIP(hipGetLastError());
             NSFEM_HIP(hipStreamSynchronize(s));
-            res.rr = 0;
+            for (int c = 0; c < nch_; c++)
+            {
+                if ((in[2] >> c) & 1)
+                {
+                    res.rr[c] = 0;
+                }
+            }
         }
         predicted_ = res.iterations;
         return res;

@@ -8,6 +8,9 @@
 // rule is the pcg solver's, |r| <= target |b| in FP64, and the Krylov
 // vectors, products and reductions are FP64.  Precision of the V-cycle:
 // FP64, or FP32 (mixed: the FP32 V-cycle on an FP32 copy of every level).
+// 1, 2, 4, 8 or 16 right-hand sides (channels, interleaved per row: entry
+// (i, c) at i * channels + c) share every matrix read; each channel has its
+// own CG scalars and stops on its own, the solve when all have stopped.
 //
 // Launches per iteration: the direction (p, q = A p and p.q partials), the
 // update (x, r, |r|^2 partials, the FP32 residual copy), the V-cycle (per
@@ -35,18 +38,19 @@ namespace nsfem
         // (the CG products), level0 the same matrix in the V-cycle precision
         // (FP64 runs pass a64 again, mixed an FP32 copy).
         AmgPcg(const CsrD & stripped, const std::vector<int> & rowSplit, const std::vector<double> & diag,
-               const std::vector<int> & colorStart, bool mixed, McgsMatrix<double> a64, McgsMatrix<float> level0F32, hipStream_t s);
+               const std::vector<int> & colorStart, bool mixed, int numCh, McgsMatrix<double> a64, McgsMatrix<float> level0F32,
+               hipStream_t s);
         ~AmgPcg();
 
         struct Result
         {
-            int iterations = 0;
-            double bb = 0, rr = 0;
+            int iterations = 0;            // of the channel that iterated longest
+            std::vector<double> bb, rr;    // per channel |b|^2 and the final |r|^2
             bool nonFinite = false;
-            bool converged = false;
+            bool converged = false;        // every channel
         };
-        // Solves A x = b (device, colour order, FP64); x holds the initial
-        // guess unless zeroGuess.  Blocking.
+        // Solves A x = b (device, colour order, FP64, channels interleaved);
+        // x holds the initial guess unless zeroGuess.  Blocking.
         Result solve(const double * b, double * x, bool zeroGuess, double target, int maxIters, bool useGraphs);
 
         // Average duration (ms) of one launch of `which`, HIP events around
@@ -86,16 +90,20 @@ namespace nsfem
 
         template <typename T>
         McgsMatrix<T> levelMatrix(int l) const;
-        template <typename T>
+        template <typename T, int NCH>
         void enqueueVcycle(hipStream_t s, const T * b0, T * x0);
-        template <typename P>
+        template <typename P, int NCH>
         void enqueueInit(hipStream_t s, const double * b, double * x, bool zeroGuess, double tol2);
-        template <typename P>
+        template <typename P, int NCH>
         void enqueueIteration(hipStream_t s, double tol2);
+        // calls f.template operator()<P, NCH>() for this solver's precision and channels
+        template <typename F>
+        void typed(F && f);
         void destroyGraphs();
 
         bool mixed_ = false;
         int n_ = 0;
+        int nch_ = 1;
         McgsMatrix<double> a64_{};
         McgsMatrix<float> a32_{};
         std::vector<Level> levels_;
@@ -109,7 +117,8 @@ namespace nsfem
         DeviceArray<char> rP_, z_;
         int parts_ = 0;
         DeviceArray<double> scal_, partPq_, partRr_, partRho_, partBb_;
-        DeviceArray<int> ints_;
+        DeviceArray<int> ints_;   // stop, iterations, channels with a zero right-hand side (bit mask), unused
+        DeviceArray<int> active_; // per channel: 1 while it iterates
         int * hostPoll_ = nullptr; // pinned, 2 slots x 4 ints
         hipEvent_t pollEvents_[2] = {nullptr, nullptr};
         hipStream_t stream_ = nullptr;

@@ -216,9 +216,9 @@ namespace nsfem
         }
         if (opt_.method == SolverMethod::AmgPcg)
         {
-            if (systems.size() != 1 || opt_.numCh != 1)
+            if (systems.size() != 1 || opt_.numCh > 16)
             {
-                throw std::invalid_argument("amg_pcg solves one system with one right-hand side");
+                throw std::invalid_argument("amg_pcg solves one system with 1, 2, 4, 8 or 16 right-hand sides");
             }
             if (opt_.precision == SolverPrecision::F32)
             {
@@ -441,7 +441,7 @@ namespace nsfem
             s0.column = column;
             s0.values = values;
             const bool mixed = opt_.precision == SolverPrecision::Mixed;
-            amg_ = std::make_unique<AmgPcg>(s0, rowSplit, diag, colorRowStart_, mixed, A64_.view(n_),
+            amg_ = std::make_unique<AmgPcg>(s0, rowSplit, diag, colorRowStart_, mixed, opt_.numCh, A64_.view(n_),
                                             mixed ? A32_.view(n_) : McgsMatrix<float>{}, stream_);
         }
 
@@ -1607,17 +1607,20 @@ namespace nsfem
         lastOuter_ = 0;
         if (opt_.method == SolverMethod::AmgPcg)
         {
-            permute(F64, F64, n_, 1, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
             if (!zeroInitial)
             {
-                permute(F64, F64, n_, 1, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
             }
             const AmgPcg::Result r = amg_->solve(b64_.get(), x64_.get(), zeroInitial, opt_.target, opt_.maxIters, useGraphs_);
-            lastMse_[0] = std::sqrt(r.rr / n_);
-            lastRel_[0] = r.bb > 0 ? std::sqrt(r.rr / r.bb) : 0.0;
+            for (int c = 0; c < nch; c++)
+            {
+                lastMse_[c] = std::sqrt(r.rr[c] / n_);
+                lastRel_[c] = r.bb[c] > 0 ? std::sqrt(r.rr[c] / r.bb[c]) : 0.0;
+            }
             lastIters_[0] = r.iterations;
             lastNonFinite_ = r.nonFinite;
-            permute(F64, F64, n_, 1, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
             NSFEM_HIP(hipStreamSynchronize(s));
             return;
         }

@@ -17,9 +17,9 @@ from conftest import mesh_path
 pytestmark = pytest.mark.gpu
 
 
-def _solve(name, m, b, max_iters=500, target=1e-12, x0=None):
+def _solve(name, m, b, max_iters=500, target=1e-12, x0=None, nch=1):
     from nsfem_amd import cu
-    s = cu.createSolver(name, 1, m, max_iters, target, 1)
+    s = cu.createSolver(name, nch, m, max_iters, target, 1)
     s.getRhs().upload(b)
     if x0 is None:
         s.getSol().memsetZero()
@@ -50,6 +50,34 @@ def test_amg_pcg_matches_direct_solve(octx_c3, name, target, bound):
     assert abs(true_rel - s.getLastRelRes()) <= 1e-3 * target + 1e-16
     assert np.max(np.abs(x - xd)) <= bound * np.max(np.abs(xd))
     assert 1 <= s.getLastIterations() < 60
+
+
+@pytest.mark.parametrize("name,nch", [("amg_pcg", 4), ("amg_pcg_mixed", 8), ("amg_pcg", 16)])
+def test_amg_pcg_channels_match_single_solves(octx_c3, name, nch):
+    """Several right-hand sides share the V-cycle's matrix reads; each
+    column has its own CG scalars and stopping test: columns scaled 1e-6
+    apart and a zero column all meet the relative target, and each column
+    equals the direct solve."""
+    m = octx_c3["Kp_int"]
+    n = m.shape[0]
+    rng = np.random.default_rng(34)
+    cols = [rng.uniform(-1, 1, n) * (1e-6 if c % 3 == 1 else 1.0) for c in range(nch)]
+    cols[-1] = np.zeros(n)
+    b = np.concatenate(cols)
+    target = 1e-10
+    # a nonzero guess for the zero column too (its solution is zero)
+    s, x = _solve(name, m, b, target=target, nch=nch, x0=rng.uniform(-1, 1, n * nch) * 1e-3)
+    lu = spla.splu(sp.csc_matrix(m))
+    for c in range(nch):
+        bc, xc = b[c * n:(c + 1) * n], x[c * n:(c + 1) * n]
+        if not bc.any():
+            assert not xc.any()
+            continue
+        rel = np.linalg.norm(bc - m @ xc) / np.linalg.norm(bc)
+        assert rel <= 1.01 * target, c
+        xd = lu.solve(bc)
+        assert np.max(np.abs(xc - xd)) <= 1e-7 * np.max(np.abs(xd)), c
+    assert s.getLastIterations() < 60
 
 
 def test_amg_pcg_is_bit_repeatable_and_warm_starts(octx_c3):
@@ -112,8 +140,8 @@ def test_amg_pcg_rejects_what_it_cannot_solve(octx_c3):
     from nsfem_amd import cu
     from nsfem_amd._lib import NsfemError
     m = octx_c3["Kp_int"]
-    with pytest.raises(NsfemError, match="one right-hand side"):
-        cu.createSolver("amg_pcg", 2, m, 100, 1e-10, 1)
+    with pytest.raises(NsfemError, match="1, 2, 4, 8 or 16 right-hand sides"):
+        cu.createSolver("amg_pcg", 32, m, 100, 1e-10, 1)
     with pytest.raises(NsfemError, match="FP64 Krylov"):
         cu.createSolver("amg_pcg", 1, sp.csr_matrix(m, dtype=np.float32), 100, 1e-10, 1)
     # the pure-Neumann Laplacian is singular: no coarsest factorisation

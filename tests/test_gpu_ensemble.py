@@ -80,6 +80,27 @@ def test_mixed_ensemble_with_block_pressure_matches_oracle(oracle):
         assert eu < 1e-8 and ep < 1e-7, (b, eu, ep)
 
 
+@pytest.mark.parametrize("pressure,B,target,bounds", [("amg_pcg", 4, 1e-13, (1e-10, 1e-10)),
+                                                     ("amg_pcg_mixed", 8, 1e-10, (1e-8, 1e-7))])
+def test_ensemble_with_amg_pressure_matches_oracle(oracle, pressure, B, target, bounds):
+    """The pressure solve of an ensemble as one AMG-preconditioned solve of B
+    right-hand sides."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    vel = "pcg" if target < 1e-12 else "pcg_mixed"
+    ch = fem.Chorin(_ctx(), _cfg((vel, 2000, target, 1), (pressure, 500, target, 1)), instances=B, inflow_scale=SCALES[:B])
+    ch.set_time_step(tau)
+    orcs = _oracles(oracle, SCALES[:B], tau)
+    for _ in range(3):
+        st = ch.step()
+        assert st["relPressure"] <= target and st["itersPressure"] < 60
+        outs = [orc.step() for orc in orcs]
+    u, p = _split(ch, *ch.download())
+    for b, (uo, po) in enumerate(outs):
+        assert _rel(u[b], uo) <= bounds[0], b
+        assert _rel(p[b], po) <= bounds[1], b
+
+
 def test_ensemble_of_one_equals_the_single_flow():
     from nsfem_amd import fem
     cfg = _cfg(("pcg", 2000, 1e-13, 1), ("pcg", 8000, 1e-13, 1))
