@@ -114,7 +114,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     return ch, res, (u1, p1), (u, p)
 
 
-def run_ensemble(args, ctx, cond, u0, target):
+def run_ensemble(args, ctx, cond, u0, target, precision="mixed"):
     """B flows on the one mesh (instance b: inflow scale 1 - 0.4 b / (B - 1),
     start u0 for b = 0, other seeds otherwise) stepped together: every
     matrix is shared, the velocity solves are one pcg_mixed solve over 2B
@@ -134,16 +134,19 @@ def run_ensemble(args, ctx, cond, u0, target):
         u[ids] *= scales[b]
         u[idy + N] *= scales[b]
         starts.append(u)
-    pressure = args.ensemble_pressure
-    cfg = fem.ChorinConfig(fem.SolverConfig("pcg_mixed", 5000, target, 1), fem.SolverConfig(pressure, 50000, target, 1),
+    pressure = args.ensemble_pressure or ("amg_pcg_mixed" if args.pressure_solver == "amg" else "pcg_mixed")
+    velocity = "pcg_mixed"
+    if precision == "fp64":
+        pressure, velocity = pressure.replace("_mixed", ""), "pcg"
+    cfg = fem.ChorinConfig(fem.SolverConfig(velocity, 5000, target, 1), fem.SolverConfig(pressure, 50000, target, 1),
                            dtype=F64, timeStepPolicy=fem.TAU_CLAMP,
                            pressureStart=fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else 0)
-    ch, r, _, last = run_path(f"ensemble{B}", ctx, cfg, args.tau, np.concatenate(starts), args.warmup, args.steps,
+    ch, r, _, last = run_path(f"ensemble{B}_{precision}", ctx, cfg, args.tau, np.concatenate(starts), args.warmup, args.steps,
                               not args.no_autotune, make=lambda c: fem.Chorin(ctx, c, instances=B, inflow_scale=scales))
     del ch
     r["instances"] = B
     r["inflow_scales"] = scales
-    r["solvers"] = {"velocity": "pcg_mixed", "pressure": pressure}
+    r["solvers"] = {"velocity": velocity, "pressure": pressure}
     r["instance_steps_per_s"] = r["steps_per_s"] * B
     return r, last
 
@@ -559,8 +562,8 @@ def main():
     ap.add_argument("--pressure-solver", default="amg", choices=["mcgs", "amg"],
                     help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
                          "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
-    ap.add_argument("--ensemble-pressure", default="pcg_mixed", choices=["pcg_mixed", "bpcg_mixed"],
-                    help="the ensemble's pressure solver")
+    ap.add_argument("--ensemble-pressure", default=None, choices=["pcg_mixed", "bpcg_mixed", "amg_pcg_mixed"],
+                    help="the ensemble's pressure solver (default: amg_pcg_mixed, or pcg_mixed with --pressure-solver mcgs)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run 3 in-place scalings of an array this size (known traffic, to scale HBM byte counters)")
     args = ap.parse_args()
@@ -675,6 +678,10 @@ def main():
     ensemble = None
     if args.ensemble > 0:
         ensemble, ens_last = run_ensemble(args, ctx, cond, u0, target)
+        # the same ensemble in FP64 throughout: what batching alone gives
+        ens64, _ = run_ensemble(args, ctx, cond, u0, target, precision="fp64")
+        ensemble["fp64_ensemble"] = ens64
+        ensemble["mixed_over_fp64_ensemble"] = ensemble["steps_per_s"] / ens64["steps_per_s"]
         if "fp64" in results:
             # instance 0 has the single flow's inflow and start
             N = ctx.numVelocityNodes
