@@ -68,6 +68,7 @@ namespace nsfem
         int frameStep = 1;                        // keep every frameStep-th state (1: all, as the reference)
         double channelHeight = 0.41;              // inflow profile scale * peak * y (H - y) / H^2
         double profileScale = 1.0;                // (dfgCondtions.h:9-12 has H = 0.41, scale 1)
+        int autotuneReps = 10;                    // nsfem_chorin_autotune before the first step (0: default shapes)
     };
 
     namespace detail
@@ -187,6 +188,10 @@ namespace nsfem
             nsfem_chorin * h = nullptr;
             check(nsfem_chorin_create(ctx.get(), &cc, &h));
             ch.reset(h);
+        }
+        if (opt.autotuneReps > 0)
+        {
+            check(nsfem_chorin_autotune(ch.get(), opt.autotuneReps, nullptr));
         }
 
         double tau0 = timeStep0;

@@ -123,6 +123,19 @@ int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * d
  *                      |b_c - A x_c| <= targetMse |b_c|.  FP64 values.
  *   name "bpcg_mixed": bpcg with the preconditioner in FP32.
  *                      No reference counterpart for either.
+ *   name "amg_pcg"   : FP64 CG preconditioned by a smoothed-aggregation
+ *                      V-cycle (host set-up: strength 0.08 halved per level,
+ *                      greedy aggregation, Jacobi-smoothed prolongation,
+ *                      Galerkin coarse operators down to <= 512 rows, inverted
+ *                      densely) with one multicolour GS sweep per level and
+ *                      direction (forward down, backward up); one system,
+ *                      numCh = 1, 2, 4, 8 or 16 right-hand sides (each with its
+ *                      own scalars and stopping test); the stopping rule of
+ *                      "pcg".  The matrix must be nonsingular SPD (a pure-
+ *                      Neumann Laplacian fails at set-up with NSFEM_EINVAL).
+ *   name "amg_pcg_mixed": amg_pcg with the V-cycle in FP32.  No reference
+ *                      counterpart (the reference's pressure solve is "gs" or
+ *                      the cuDSS "dss").
  *   name "dss"       : rejected with NSFEM_EINVAL (no direct solver here).
  * The compute precision is valueDtype ("gs"/"pcg").  Colouring is the
  * reference's smallest-last greedy colouring (graphs.cpp:64-176).
@@ -390,8 +403,8 @@ int nsfem_chorin_create(const nsfem_context * c, const nsfem_chorin_config * cfg
  * of a single-mesh context, instance b's Dirichlet velocity (the inflow
  * profile) scaled by inflowScale[b] (NULL: all 1).  They share every matrix:
  * each velocity solve is one solve of 2 x instances right-hand sides, the
- * pressure solve one of `instances` (a block solver such as bpcg_mixed for
- * 4 or more); the convection is each instance's own.  The state functions
+ * pressure solve one of `instances` (amg_pcg_mixed, or a block solver such
+ * as bpcg_mixed for 4 or more); the convection is each instance's own.  The state functions
  * then hold the instances one after another: velocity [u_0 .. u_B-1] (2N
  * each), pressure [p_0 .. p_B-1] (V each); the step statistics are the worst
  * over the instances.  Parameter sweeps over one geometry (inflow speeds,

@@ -152,6 +152,12 @@ int main(int argc, char ** argv)
         }
         nsfem_chorin * ch = nullptr;
         cu::check(nsfem_chorin_create(ctx, &cc, &ch));
+        // launch shapes tuned once before the first step (0: the defaults)
+        const int tuneReps = static_cast<int>(cfg.num("hip.autotune", 10));
+        if (tuneReps > 0)
+        {
+            cu::check(nsfem_chorin_autotune(ch, tuneReps, nullptr));
+        }
 
         // tau = maxT / ceil(maxT / tau0) (chorinCuda.cu:221-222); under the
         // clamp policy tau0 is first lowered to the stability limit, so the

@@ -775,18 +775,31 @@ namespace nsfem
         int maxBlocks = numBlocks_;
         if (fused() && flat())
         {
-            auto flatMap = [&](int rows, int & nb)
+            // Unset rows per workgroup: every workgroup of the next kernel
+            // re-sums all partials of this one, so the default keeps the
+            // direction and update kernels at about 1,024 workgroups and the
+            // rho kernel at about 4,096 (in whole multiples of the main rows)
+            auto defaultRows = [&](int maxBlocksWanted)
+            {
+                int rows = rowsPerBlock_;
+                while (static_cast<long>(rows) * maxBlocksWanted < n_)
+                {
+                    rows *= 2;
+                }
+                return rows;
+            };
+            auto flatMap = [&](int rows, int & nb, int maxBlocksWanted)
             {
                 BlockMap m = bm_;
-                m.rowsPerBlock = rows > 0 ? rows : rowsPerBlock_;
+                m.rowsPerBlock = rows > 0 ? rows : defaultRows(maxBlocksWanted);
                 nb = (n_ + m.rowsPerBlock - 1) / m.rowsPerBlock;
                 m.numBlocks = nb;
                 maxBlocks = std::max(maxBlocks, nb);
                 return m;
             };
-            bmDir_ = flatMap(rowsDir_, nbDir_);
-            bmUpd_ = flatMap(rowsUpd_, nbUpd_);
-            bmRho_ = flatMap(rowsRho_, nbRho_);
+            bmDir_ = flatMap(rowsDir_, nbDir_, 1024);
+            bmUpd_ = flatMap(rowsUpd_, nbUpd_, 1024);
+            bmRho_ = flatMap(rowsRho_, nbRho_, 4096);
             if (regionDir_ && n_ > 0)
             {
                 // stretch k of every colour, then stretch k + 1: the XCD

@@ -343,7 +343,8 @@ def _csr_arrays(m):
 
 
 def createSolver(name, numCh, m, maxIters, targetMse, mseCheckInterval, dtype=None):
-    """cu::createSolver: name "gs" | "pcg" | "pcg_mixed" | "bpcg" | "bpcg_mixed" ("dss" is rejected).
+    """cu::createSolver: name "gs" | "pcg" | "pcg_mixed" | "bpcg" | "bpcg_mixed" | "amg_pcg" | "amg_pcg_mixed"
+    ("dss" is rejected).
     m: scipy sparse matrix or (rows, rowStart, column, values).  The value
     dtype (float32 / float64) sets the compute precision."""
     rows, rs, col, vals = _csr_arrays(m)

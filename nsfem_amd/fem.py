@@ -399,14 +399,17 @@ class TimeStepSolution:
 
 
 def solve_ns_chorin(mesh: TriangleMesh, cond: DfgConditions, tau0: float, max_t: float, cfg: ChorinConfig,
-                    frame_step: int = 1, verbose: bool = False):
+                    frame_step: int = 1, verbose: bool = False, autotune: int = 10):
     """solveNsChorinCuda: numTimeSteps = ceil(maxT / tau0), tau = maxT / numTimeSteps,
     numTimeSteps + 1 steps from u = Dirichlet values (under TAU_CLAMP tau0 is
     first lowered to the stability limit); every frame_step-th state
     is kept (the frames the reference renders, navierStokes.cpp:109), each
-    copied to the host while the following steps run."""
+    copied to the host while the following steps run.  autotune > 0: the
+    solvers' launch shapes are tuned first (that many replays per candidate)."""
     ctx = ChorinContext(mesh, cond)
     ch = Chorin(ctx, cfg)
+    if autotune > 0:
+        ch.autotune(autotune)
     if cfg.timeStepPolicy == TAU_CLAMP:
         # lowered to the stability limit first, so the run still ends at max_t
         tau0 = min(tau0, ch.time_step_info()["tauMax"])
