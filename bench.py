@@ -285,13 +285,13 @@ def build_batch(count, seed):
     return ctx, info
 
 
-def run_batch(args):
-    """Batched uneven instances in one launch sequence: steps/s of the whole
-    batch (every instance advances one step), FP64 and mixed."""
+def batch_leg(args, count, step_graph, with_kernels):
+    """count uneven instances (C4) in one block-diagonal batch, FP64 and
+    mixed: steps/s of the whole batch (every instance advances one step)."""
     import numpy as np
     from nsfem_amd import F64, fem
     t_setup = time.perf_counter()
-    ctx, info = build_batch(args.batch, args.seed)
+    ctx, info = build_batch(count, args.seed)
     rng = np.random.default_rng(1)
     N = ctx.numVelocityNodes
     u0 = rng.uniform(-1, 1, 2 * N) * 1e-3
@@ -303,14 +303,33 @@ def run_batch(args):
     res = {}
     for name, method in (("fp64", "pcg"), ("mixed", "pcg_mixed")):
         cfg = fem.ChorinConfig(fem.SolverConfig(method, 5000, target, 1), fem.SolverConfig(method, 50000, target, 1), dtype=F64,
-                               timeStepPolicy=fem.TAU_CLAMP, stepGraph=1 if args.step_graph else 0)
+                               timeStepPolicy=fem.TAU_CLAMP, stepGraph=1 if step_graph else 0)
         # the guard clamps tau to the batch's limit (the tightest instance's)
-        ch, r, _, _ = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune)
-        r["kernels"] = {} if args.no_kernels else kernel_table(ch, r, args.kernel_reps)
+        ch, r, _, _ = run_path(f"batch{count}_{name}", ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune)
+        r["kernels"] = kernel_table(ch, r, args.kernel_reps) if with_kernels else {}
         del ch
-        r["instance_steps_per_s"] = r["steps_per_s"] * args.batch
+        r["instance_steps_per_s"] = r["steps_per_s"] * count
         r["pressure_dof_steps_per_s"] = r["steps_per_s"] * ctx.numPressureNodes
         res[name] = r
+    return {"workload": f"chorin_step_batch_{count}_uneven", "instances": count, "captured_steps": bool(step_graph),
+            "pressure_dofs_total": ctx.numPressureNodes, "velocity_dofs_per_component_total": N,
+            "tau_requested": args.tau, "tau_used": res["mixed"]["time_step"]["tauUsed"],
+            "lambda_max_nu": res["mixed"]["time_step"]["lambdaNu"], "solver_rtol": target, "seed": args.seed,
+            "solvers": "pcg / pcg_mixed (per-instance convergence in one block-diagonal solve)",
+            "instances_detail": info, "fp64": res["fp64"], "mixed": res["mixed"],
+            "mixed_over_fp64": res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], "setup_s": setup_s,
+            "_ctx": ctx}
+
+
+def run_batch(args):
+    """Batched uneven instances in one launch sequence (C4) as the bench line."""
+    b = batch_leg(args, args.batch, args.step_graph, not args.no_kernels)
+    ctx = b.pop("_ctx")
+    res = {"fp64": b["fp64"], "mixed": b["mixed"]}
+    N = ctx.numVelocityNodes
+    target = b["solver_rtol"]
+    info = b["instances_detail"]
+    setup_s = b["setup_s"]
     out = {
         "metric": "batched Chorin instance-steps/s, mixed vs FP64 GPU, same residual",
         "value": res["mixed"]["instance_steps_per_s"], "unit": "instance-steps/s", "n_gpus": 1,
@@ -562,6 +581,8 @@ def main():
     ap.add_argument("--pressure-solver", default="amg", choices=["mcgs", "amg"],
                     help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
                          "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
+    ap.add_argument("--batch-leg", type=int, default=48,
+                    help="also run this many uneven instances (10k-250k pressure DOFs) as one captured batch; 0 = skip")
     ap.add_argument("--ensemble-pressure", default=None, choices=["pcg_mixed", "bpcg_mixed", "amg_pcg_mixed"],
                     help="the ensemble's pressure solver (default: amg_pcg_mixed, or pcg_mixed with --pressure-solver mcgs)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -675,6 +696,13 @@ def main():
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
+    batch = None
+    if args.batch_leg > 0:
+        batch = batch_leg(args, args.batch_leg, True, False)
+        del batch["_ctx"]
+        log(f"batch of {args.batch_leg}: FP64 {batch['fp64']['instance_steps_per_s']:.1f}, mixed "
+            f"{batch['mixed']['instance_steps_per_s']:.1f} instance-steps/s (captured steps)")
+
     ensemble = None
     if args.ensemble > 0:
         ensemble, ens_last = run_ensemble(args, ctx, cond, u0, target)
@@ -777,6 +805,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "ensemble": ensemble,
+        "batch": batch,
         "setup_s": setup_s,
     }
     print(json.dumps(out))

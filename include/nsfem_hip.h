@@ -357,7 +357,8 @@ typedef struct nsfem_chorin_config
                          * convergence is decided on the device), the updates and
                          * impositions; a solve out of budget rolls the step back
                          * to the snapshot and reruns it regularly.  pcg solvers
-                         * without mixedRestart.  No reference counterpart. */
+                         * without mixedRestart, or amg_pcg / amg_pcg_mixed for the
+                         * pressure.  No reference counterpart. */
     int pressureStart;  /* the pressure solve's initial guess: NSFEM_PRESSURE_START_LAST
                          * (0, the last pressure, as the reference's in-place
                          * solve) or NSFEM_PRESSURE_START_EXTRAPOLATE (2 p_n - p_n-1;

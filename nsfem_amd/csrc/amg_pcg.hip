@@ -1018,6 +1018,11 @@ namespace nsfem
                   }
               });
         launch(); // warm
+        NSFEM_HIP(hipStreamSynchronize(s));
+        // the stream idles on a 20 ms delay kernel while the host queues the
+        // repetitions, so they run back to back and the events bracket GPU
+        // execution only
+        delayKernel<<<1, 64, 0, s>>>(2000000);
         hipEvent_t e0, e1;
         NSFEM_HIP(hipEventCreate(&e0));
         NSFEM_HIP(hipEventCreate(&e1));
@@ -1043,6 +1048,21 @@ namespace nsfem
             *launchesOut = 1;
         }
         return static_cast<double>(ms) / reps;
+    }
+
+    void AmgPcg::enqueueSolve(hipStream_t s, const double * b, double * x, bool zeroGuess, double target, int budget)
+    {
+        const double tol2 = target * target;
+        xDevice_ = x;
+        destroyGraphs(); // the chunk graph may hold another solution vector
+        typed([&]<typename P, int NCH>()
+              {
+                  enqueueInit<P, NCH>(s, b, x, zeroGuess, tol2);
+                  for (int i = 0; i < budget; i++)
+                  {
+                      enqueueIteration<P, NCH>(s, tol2);
+                  }
+              });
     }
 
     AmgPcg::Result AmgPcg::solve(const double * b, double * x, bool zeroGuess, double target, int maxIters, bool useGraphs)

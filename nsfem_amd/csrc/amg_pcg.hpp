@@ -52,6 +52,16 @@ namespace nsfem
         // Solves A x = b (device, colour order, FP64, channels interleaved);
         // x holds the initial guess unless zeroGuess.  Blocking.
         Result solve(const double * b, double * x, bool zeroGuess, double target, int maxIters, bool useGraphs);
+        // The whole solve on the stream with no host synchronisation (for
+        // graph capture): the prologue and exactly `budget` iterations, whose
+        // kernels do nothing once every channel has stopped.  The device
+        // scalars then hold the statistics (deviceScalars: per channel blocks
+        // of rho, rho before, |b|^2, |r|^2; deviceInts: stop, iterations,
+        // zero right-hand sides).
+        void enqueueSolve(hipStream_t s, const double * b, double * x, bool zeroGuess, double target, int budget);
+        const double * deviceScalars() const { return scal_.get(); }
+        const int * deviceInts() const { return ints_.get(); }
+        int numCh() const { return nch_; }
 
         // Average duration (ms) of one launch of `which`, HIP events around
         // reps back-to-back replays on the solver's stream: "direction",

@@ -1161,9 +1161,9 @@ extern "C"
                 for (const auto * sc : {&cc.velocity, &cc.pressure})
                 {
                     const SolverOptions o = parseSolverName(sc->method);
-                    if (o.method != SolverMethod::Pcg)
+                    if (o.method != SolverMethod::Pcg && o.method != SolverMethod::AmgPcg)
                     {
-                        throw std::invalid_argument("stepGraph needs pcg or pcg_mixed solvers");
+                        throw std::invalid_argument("stepGraph needs pcg, pcg_mixed, amg_pcg or amg_pcg_mixed solvers");
                     }
                 }
             }

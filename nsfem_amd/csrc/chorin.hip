@@ -554,6 +554,8 @@ namespace nsfem
             throw std::invalid_argument("unknown kernel '" + which + "'");
         }
         launch();
+        NSFEM_HIP(hipStreamSynchronize(s));
+        delayKernel<<<1, 64, 0, s>>>(2000000); // the repetitions queue behind it and run back to back
         hipEvent_t e0, e1;
         NSFEM_HIP(hipEventCreate(&e0));
         NSFEM_HIP(hipEventCreate(&e1));
@@ -791,19 +793,20 @@ namespace nsfem
     void ChorinGpu::setBudgets(int tent, int pres, int fin)
     {
         // A measured count c gets c + m iterations, m = max(3, c / 4) for the
-        // short velocity solves and max(16, c / 24) for the pressure solve.
+        // short velocity solves and max(3, c / 24) for the pressure solve
+        // (one-level preconditioning: ~2,000 iterations, a V-cycle: ~20).
         // The graph is rebuilt when less than half a margin is left or more
         // than two margins are idle (hysteresis: counts drift by a few
         // iterations from step to step).
         const int counts[3] = {tent, pres, fin};
-        const int minMargin[3] = {3, 16, 3};
+        const int minMargin[3] = {3, 3, 3};
         const int frac[3] = {4, 24, 4};
         bool rebuild = false;
         for (int k = 0; k < 3; k++)
         {
             const int c = counts[k];
             const int m = std::max(minMargin[k], c / frac[k]);
-            if (c + m / 2 > budget_[k] || budget_[k] > c + 2 * m + 8)
+            if (c + m / 2 > budget_[k] || budget_[k] > c + 2 * m + 2)
             {
                 budget_[k] = c + m;
                 rebuild = true;

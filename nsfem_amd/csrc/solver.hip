@@ -1769,9 +1769,25 @@ namespace nsfem
 
     void McgsSolver::enqueueSolve(hipStream_t s, bool zeroInitial, int budget)
     {
+        if (amg_)
+        {
+            if (budget < 1)
+            {
+                throw std::invalid_argument("captured solve: the iteration budget must be positive");
+            }
+            const int nch = opt_.numCh;
+            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            if (!zeroInitial)
+            {
+                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+            }
+            amg_->enqueueSolve(s, b64_.get(), x64_.get(), zeroInitial, opt_.target, std::min(budget, opt_.maxIters));
+            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            return;
+        }
         if (opt_.method != SolverMethod::Pcg || (opt_.precision == SolverPrecision::Mixed && opt_.restart))
         {
-            throw std::invalid_argument("captured solves need a pcg solver (pcg or pcg_mixed without mixedRestart)");
+            throw std::invalid_argument("captured solves need a pcg solver (pcg or pcg_mixed without mixedRestart, amg_pcg, amg_pcg_mixed)");
         }
         if (budget < 1)
         {
@@ -1829,6 +1845,13 @@ namespace nsfem
         const size_t intBytes = (3 * static_cast<size_t>(K) + 4) * sizeof(int);
         const size_t slotBytes = scalBytes + intBytes;
         char * base = statSlots_.get() + slot * slotBytes;
+        if (amg_)
+        {
+            // its per-channel scalars (4 K doubles) and 4 ints fit the slot
+            NSFEM_HIP(hipMemcpyAsync(base, amg_->deviceScalars(), 4 * static_cast<size_t>(K) * sizeof(double), hipMemcpyDeviceToDevice, s));
+            NSFEM_HIP(hipMemcpyAsync(base + scalBytes, amg_->deviceInts(), 4 * sizeof(int), hipMemcpyDeviceToDevice, s));
+            return;
+        }
         NSFEM_HIP(hipMemcpyAsync(base, inner_.scal.get(), scalBytes, hipMemcpyDeviceToDevice, s));
         NSFEM_HIP(hipMemcpyAsync(base + scalBytes, inner_.ints.get(), intBytes, hipMemcpyDeviceToDevice, s));
     }
@@ -1844,6 +1867,25 @@ namespace nsfem
             throw std::invalid_argument("no statistics in that slot");
         }
         const char * base = statSlots_.get() + slot * slotBytes;
+        if (amg_)
+        {
+            NSFEM_HIP(hipMemcpyAsync(hostScal_, base, 4 * K * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            NSFEM_HIP(hipMemcpyAsync(hostInts_, base + scalBytes, 4 * sizeof(int), hipMemcpyDeviceToHost, stream_));
+            NSFEM_HIP(hipStreamSynchronize(stream_));
+            const int stop = hostInts_[0], iters = hostInts_[1], zero = hostInts_[2];
+            for (int c = 0; c < K; c++)
+            {
+                const double bb = hostScal_[2 * K + c], rr = hostScal_[3 * K + c];
+                lastMse_[c] = std::sqrt(rr / n_);
+                lastRel_[c] = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            }
+            lastIters_[0] = iters;
+            lastNonFinite_ = stop == 2;
+            lastOuter_ = 0;
+            // a zero right-hand side from a nonzero guess is zeroed by the
+            // regular solve only: rerun
+            return zero == 0 && (stop != 0 || iters >= opt_.maxIters);
+        }
         NSFEM_HIP(hipMemcpyAsync(hostScal_, base, 7 * K * sizeof(double), hipMemcpyDeviceToHost, stream_));
         NSFEM_HIP(hipMemcpyAsync(hostInts_, base + scalBytes, (2 * K + 4) * sizeof(int), hipMemcpyDeviceToHost, stream_));
         NSFEM_HIP(hipStreamSynchronize(stream_));

@@ -136,6 +136,51 @@ def test_chorin_steps_with_amg_pressure_match_oracle(oracle, octx_c3):
     assert np.max(np.abs(pm - po)) <= 1e-7 * np.max(np.abs(po))
 
 
+@pytest.mark.parametrize("B", [1, 4])
+def test_captured_steps_with_amg_pressure(oracle, B):
+    """The whole step as one captured graph with the AMG-preconditioned
+    pressure solve (its iterations captured with a budget, idle once
+    converged): equal to regular steps and to the oracle, and a forced small
+    budget rolls the step back and reruns it."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    scales = [1.0, 0.5, 1.5, 0.8][:B]
+    def make(sg):
+        cfg = fem.ChorinConfig(fem.SolverConfig("pcg", 2000, 1e-13, 1), fem.SolverConfig("amg_pcg", 500, 1e-13, 1), dtype=1,
+                               stepGraph=sg)
+        ch = fem.Chorin(fem.ChorinContext(fem.TriangleMesh.load_gmsh(mesh_path("c3")), fem.DfgConditions(1e-3, 1.0)), cfg,
+                        instances=B, inflow_scale=scales)
+        ch.set_time_step(tau)
+        return ch
+    regular, captured = make(0), make(1)
+    kinds = []
+    for _ in range(4):
+        regular.step()
+        st = captured.step()
+        kinds.append(st["capturedStep"])
+        assert st["relPressure"] <= 1e-13
+    assert kinds[0] == 0 and sum(kinds[1:]) >= 2
+    ur, pr = regular.download()
+    uc, pc = captured.download()
+    assert np.max(np.abs(uc - ur)) <= 1e-11 * np.max(np.abs(ur))
+    assert np.max(np.abs(pc - pr)) <= 1e-10 * np.max(np.abs(pr))
+    mesh = oracle.parse_gmsh(mesh_path("c3"))
+    for b in range(B):
+        orc = oracle.ChorinOracle(oracle.build_context(mesh, 1e-3, scales[b]), tau, vel=("pcg", 2000, 1e-13, 1),
+                                  pres=("pcg", 8000, 1e-13, 1))
+        for _ in range(4):
+            uo, po = orc.step()
+        n2, v = 2 * captured.N, captured.V
+        assert np.max(np.abs(uc[b * n2:(b + 1) * n2] - uo)) <= 1e-10 * np.max(np.abs(uo))
+        assert np.max(np.abs(pc[b * v:(b + 1) * v] - po)) <= 1e-10 * np.max(np.abs(po))
+    # a pressure budget of 2 iterations cannot converge: rolled back and rerun
+    captured.set_step_graph_budgets(40, 2, 40)
+    before = captured.step_graph_info()["rollbacks"]
+    st = captured.step()
+    assert st["capturedStep"] == 0 and captured.step_graph_info()["rollbacks"] == before + 1
+    assert st["relPressure"] <= 1e-13
+
+
 def test_amg_pcg_rejects_what_it_cannot_solve(octx_c3):
     from nsfem_amd import cu
     from nsfem_amd._lib import NsfemError
