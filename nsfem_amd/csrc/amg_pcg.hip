@@ -954,6 +954,8 @@ namespace nsfem
         }
         double bytes = 0;
         std::function<void()> launch;
+        hipGraphExec_t vcycleExec = nullptr;
+        bool composite = false; // a graph of many launches per repetition
         // a stop flag that never rises: every kernel does its work; finite
         // step scalars, every channel active
         const double tol2 = -1.0;
@@ -992,8 +994,18 @@ namespace nsfem
                   }
                   else if (which == "vcycle")
                   {
+                      // one V-cycle replayed from its own captured graph (its 50-odd
+                      // short launches queued from the host would outlast them)
                       bytes = bV;
-                      launch = [=, this]() { enqueueVcycle<P, NCH>(s, rP, z); };
+                      hipGraph_t g = nullptr;
+                      NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                      enqueueVcycle<P, NCH>(s, rP, z);
+                      NSFEM_HIP(hipStreamEndCapture(s, &g));
+                      const hipError_t rc = hipGraphInstantiate(&vcycleExec, g, nullptr, nullptr, 0);
+                      (void)hipGraphDestroy(g);
+                      NSFEM_HIP(rc);
+                      composite = true;
+                      launch = [&vcycleExec, s]() { NSFEM_HIP(hipGraphLaunch(vcycleExec, s)); };
                   }
                   else if (which == "iteration")
                   {
@@ -1010,6 +1022,7 @@ namespace nsfem
                           NSFEM_HIP(rc);
                           graphTol2_ = tol2;
                       }
+                      composite = true;
                       launch = [this, s]() { NSFEM_HIP(hipGraphLaunch(iterExec_, s)); };
                   }
                   else
@@ -1019,10 +1032,15 @@ namespace nsfem
               });
         launch(); // warm
         NSFEM_HIP(hipStreamSynchronize(s));
-        // the stream idles on a 20 ms delay kernel while the host queues the
-        // repetitions, so they run back to back and the events bracket GPU
-        // execution only
-        delayKernel<<<1, 64, 0, s>>>(2000000);
+        if (!composite)
+        {
+            // single kernels: the stream idles on a 20 ms delay kernel while the
+            // host queues the repetitions, so they run back to back and the
+            // events bracket GPU execution only.  Graph replays (a V-cycle, an
+            // iteration) outlast their host launch and run without it, as in a
+            // solve (after 20 ms of one idle wave the clocks start lower).
+            delayKernel<<<1, 64, 0, s>>>(2000000);
+        }
         hipEvent_t e0, e1;
         NSFEM_HIP(hipEventCreate(&e0));
         NSFEM_HIP(hipEventCreate(&e1));
@@ -1038,6 +1056,10 @@ namespace nsfem
         NSFEM_HIP(hipEventDestroy(e0));
         NSFEM_HIP(hipEventDestroy(e1));
         NSFEM_HIP(hipGetLastError());
+        if (vcycleExec)
+        {
+            (void)hipGraphExecDestroy(vcycleExec);
+        }
         destroyGraphs(); // the timing graph holds a negative tolerance
         if (bytesOut)
         {
