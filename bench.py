@@ -366,12 +366,14 @@ def run_poisson(args):
     b = rng.uniform(-1, 1, n)
     b -= b.mean()
     out = {}
-    for name, method, restart in (("fp64", "pcg", 0), ("mixed", "pcg_mixed", 0), ("mixed_restart", "pcg_mixed", 1)):
+    for name, method, restart in (("fp64", "pcg", 0), ("mixed", "pcg_mixed", 0), ("mixed_restart", "pcg_mixed", 1),
+                                  ("fp64_amg", "amg_pcg", 0), ("mixed_amg", "amg_pcg_mixed", 0)):
         s = cu.createSolver(method, 1, K, 50000, 1e-10, 1)
         s.setOption("projectMean", 1)
         if restart:
             s.setOption("mixedRestart", 1)
-        shape = s.autotune(10) if not args.no_autotune else s.shape()
+        amg = method.startswith("amg")
+        shape = s.autotune(10) if not (args.no_autotune or amg) else s.shape()
         shape = tuple(shape) + s.sweepShape()
         times, iters = [], []
         for rep in range(args.warmup + args.steps):
@@ -400,6 +402,10 @@ def run_poisson(args):
         "config": {"workload": f"poisson_neumann_{n}", "rows": n, "nnz": int(K.nnz), "seed": args.seed},
         "fp64": out["fp64"], "mixed": out["mixed"], "mixed_restart": out["mixed_restart"],
         "mixed_over_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], "setup_s": setup_s,
+        "preconditioners": {"fp64, mixed, mixed_restart": "one symmetric multicolour GS sweep",
+                            "fp64_amg, mixed_amg": "smoothed-aggregation V-cycle, multicolour GS smoothing"},
+        "fp64_amg": out["fp64_amg"], "mixed_amg": out["mixed_amg"],
+        "mixed_amg_over_fp64": out["mixed_amg"]["solves_per_s"] / out["fp64"]["solves_per_s"],
     }
     print(json.dumps(res))
     return 0
@@ -515,6 +521,20 @@ def run_multi_rhs(args):
     ind_rel, ind_mean = check()
     log(f"  independent columns (pcg_mixed tiles of 8): {dt_ind:.2f} s, iterations {its_ind}")
     del s
+
+    # the AMG-preconditioned CG over tiles of 16 columns (each column its own
+    # CG scalars, one V-cycle for the tile)
+    s = cu.createSolver("amg_pcg_mixed", 16, K, 2000, tol, 1)
+    s.setOption("projectMean", 1)
+    amg_times = []
+    for rep in range(1 + args.steps):
+        dt_amg, its_amg = solve_all(s, 16)
+        if rep >= 1:
+            amg_times.append(dt_amg)
+    amg_rel, amg_mean = check()
+    dt_amg = float(np.mean(amg_times))
+    log(f"  AMG-preconditioned columns (amg_pcg_mixed tiles of 16): {dt_amg:.2f} s, iterations {its_amg}")
+    del s
     res = {
         "metric": "multi-RHS PCG-MCGS: right-hand sides solved per s to 1e-8 (32 columns, block PCG, autotuned RHS tile)",
         "value": nrhs / t_block, "unit": "columns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -527,6 +547,9 @@ def run_multi_rhs(args):
         "independent": {"seconds": dt_ind, "tile": 8, "iterations_per_tile": its_ind, "max_true_relres": ind_rel,
                         "max_rel_mean": ind_mean},
         "block_over_independent": dt_ind / t_block, "setup_s": setup_s,
+        "amg": {"seconds": dt_amg, "tile": 16, "iterations_per_tile": its_amg, "max_true_relres": amg_rel,
+                "max_rel_mean": amg_mean, "method": "amg_pcg_mixed (smoothed-aggregation V-cycle, multicolour GS smoothing)"},
+        "amg_over_block": t_block / dt_amg,
     }
     log(f"multi-rhs: block tile {best}, {t_block:.2f} s per {nrhs}-column solve (independent {dt_ind:.2f} s), "
         f"max true relres {block_rel:.2e}")

@@ -131,8 +131,11 @@ int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * d
  *                      direction (forward down, backward up); one system,
  *                      numCh = 1, 2, 4, 8 or 16 right-hand sides (each with its
  *                      own scalars and stopping test); the stopping rule of
- *                      "pcg".  The matrix must be nonsingular SPD (a pure-
- *                      Neumann Laplacian fails at set-up with NSFEM_EINVAL).
+ *                      "pcg".  The matrix must be SPD, or positive
+ *                      semidefinite with zero row sums (a pure-Neumann
+ *                      Laplacian: the coarsest level is completed by a rank-one
+ *                      term, right-hand sides must have zero mean; option
+ *                      "projectMean" returns the zero-mean solution).
  *   name "amg_pcg_mixed": amg_pcg with the V-cycle in FP32.  No reference
  *                      counterpart (the reference's pressure solve is "gs" or
  *                      the cuDSS "dss").

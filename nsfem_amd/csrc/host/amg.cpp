@@ -266,16 +266,31 @@ namespace nsfem
             return agg;
         }
 
-        // Dense inverse of an SPD matrix (Cholesky, then L^-T L^-1).
-        std::vector<double> denseInverse(const CsrD & a)
+        // Dense inverse of an SPD matrix (Cholesky, then L^-T L^-1); singular:
+        // of A + s 1 1^T, s = mean diagonal / n (A positive semidefinite with
+        // the constants as its null space).
+        std::vector<double> denseInverse(const CsrD & a, bool singular)
         {
             const int n = a.rows;
             std::vector<double> m(static_cast<size_t>(n) * n, 0.0);
+            double diagSum = 0;
             for (int i = 0; i < n; i++)
             {
                 for (int j = a.rowStart[i]; j < a.rowStart[i + 1]; j++)
                 {
                     m[static_cast<size_t>(i) * n + a.column[j]] = a.values[j];
+                    if (a.column[j] == i)
+                    {
+                        diagSum += a.values[j];
+                    }
+                }
+            }
+            if (singular)
+            {
+                const double shift = diagSum / n / n;
+                for (auto & v : m)
+                {
+                    v += shift;
                 }
             }
             // lower Cholesky factor in place
@@ -292,7 +307,8 @@ namespace nsfem
                 // pure-Neumann Laplacian)
                 if (!(d > 1e-10 * std::abs(ajj)))
                 {
-                    throw std::invalid_argument("amg: the coarsest operator is not positive definite (a singular system?)");
+                    throw std::invalid_argument(
+                        "amg: the coarsest operator is not positive definite (a singular system without zero row sums?)");
                 }
                 d = std::sqrt(d);
                 m[static_cast<size_t>(j) * n + j] = d;
@@ -354,6 +370,21 @@ namespace nsfem
         }
         AmgHierarchy h;
         {
+            // zero row sums (to rounding): a pure-Neumann operator
+            double worst = 0, dmax = 0;
+            for (int i = 0; i < n0; i++)
+            {
+                double sum = diag[i];
+                for (int j = stripped.rowStart[i]; j < stripped.rowStart[i + 1]; j++)
+                {
+                    sum += stripped.values[j];
+                }
+                worst = std::max(worst, std::abs(sum));
+                dmax = std::max(dmax, std::abs(diag[i]));
+            }
+            h.singular = worst <= 1e-10 * dmax;
+        }
+        {
             AmgLevel L0;
             L0.n = n0;
             L0.stripped = stripped;
@@ -375,14 +406,14 @@ namespace nsfem
             const CsrD A = assemble(L);
             if (L.n <= opt.maxCoarse || static_cast<int>(h.levels.size()) >= opt.maxLevels)
             {
-                h.coarseInverse = denseInverse(A);
+                h.coarseInverse = denseInverse(A, h.singular);
                 break;
             }
             int nc = 0;
             const std::vector<int> agg = aggregate(A, L.diag, theta, nc);
             if (nc >= L.n || nc < 1)
             {
-                h.coarseInverse = denseInverse(A); // no coarsening left
+                h.coarseInverse = denseInverse(A, h.singular); // no coarsening left
                 break;
             }
             // smoothed prolongation P = (I - omega D^-1 A) P_tent

@@ -53,13 +53,20 @@ namespace nsfem
         std::vector<AmgLevel> levels;
         std::vector<double> coarseInverse; // coarsest n x n, row-major
         double operatorComplexity = 0;     // sum of nnz over the levels / nnz of the finest
+        // Zero row sums on the finest level (a pure-Neumann Laplacian): the
+        // constants span the null space on every level (P maps constants to
+        // constants), and the coarsest "inverse" is that of A_c + s 1 1^T
+        // (s = mean diagonal / n_c), which solves A_c x = b with 1^T x = 0 for
+        // every b with 1^T b = 0.
+        bool singular = false;
     };
 
     // Level 0 is given in the caller's colour-major order (stripped
     // off-diagonal rows with rowSplit, the diagonal, colour boundaries) and
     // kept as it is.  Throws std::invalid_argument when a level is not
-    // positive definite (the coarsest factorisation meets a pivot <= 0), e.g.
-    // for the singular pure-Neumann Laplacian.
+    // positive (semi)definite: the coarsest factorisation meets a pivot that
+    // has lost all but rounding noise of its diagonal (for a singular matrix
+    // without zero row sums).
     AmgHierarchy buildAmgHierarchy(const CsrD & stripped, const std::vector<int> & rowSplit, const std::vector<double> & diag,
                                    const std::vector<int> & colorStart, const AmgOptions & opt = AmgOptions{});
 } // namespace nsfem

@@ -1633,6 +1633,10 @@ namespace nsfem
             }
             lastIters_[0] = r.iterations;
             lastNonFinite_ = r.nonFinite;
+            if (opt_.projectMean)
+            {
+                projectMean<double>(s, x64_.get());
+            }
             permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
             NSFEM_HIP(hipStreamSynchronize(s));
             return;

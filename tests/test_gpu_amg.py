@@ -189,7 +189,34 @@ def test_amg_pcg_rejects_what_it_cannot_solve(octx_c3):
         cu.createSolver("amg_pcg", 32, m, 100, 1e-10, 1)
     with pytest.raises(NsfemError, match="FP64 Krylov"):
         cu.createSolver("amg_pcg", 1, sp.csr_matrix(m, dtype=np.float32), 100, 1e-10, 1)
-    # the pure-Neumann Laplacian is singular: no coarsest factorisation
-    lap = octx_c3["Kp"]
-    with pytest.raises(NsfemError, match="positive definite"):
-        cu.createSolver("amg_pcg", 1, lap, 100, 1e-10, 1)
+
+
+
+@pytest.mark.parametrize("name", ["amg_pcg", "amg_pcg_mixed"])
+def test_amg_pcg_on_the_pure_neumann_laplacian(oracle, name):
+    """The singular pure-Neumann Laplacian (C2's operator) with zero-mean
+    right-hand sides: the coarsest level completed by a rank-one term, the
+    zero-mean solution against least squares, few iterations at 60k rows."""
+    from nsfem_amd import cu, fem, mesher
+    m = mesher.channel_mesh(60_000, seed=9)
+    ctx = fem.ChorinContext(mesher.to_triangle_mesh(m), fem.DfgConditions(0.01, 1.0, m["height"], 4.0))
+    K = ctx.scipy("pressureStiffness")
+    n = K.shape[0]
+    rng = np.random.default_rng(36)
+    b = rng.uniform(-1, 1, n)
+    b -= b.mean()
+    s = cu.createSolver(name, 1, K, 300, 1e-10, 1)
+    s.setOption("projectMean", 1)
+    s.getRhs().upload(b)
+    s.getSol().memsetZero()
+    s.solve()
+    x = s.getSol().download()
+    assert abs(x.mean()) <= 1e-12 * np.max(np.abs(x))
+    assert np.linalg.norm(b - K @ x) / np.linalg.norm(b) <= 1.01e-10
+    assert s.getLastIterations() < 60
+    # the zero-mean solution: node 0 grounded (K[1:, 1:] is nonsingular; b has
+    # zero mean, so the grounded solution solves K x = b), then the mean removed
+    K = K.tocsc()
+    xd = np.concatenate([[0.0], spla.spsolve(K[1:, 1:], b[1:])])
+    xd -= xd.mean()
+    assert np.max(np.abs(x - xd)) <= 1e-6 * np.max(np.abs(xd))
