@@ -793,20 +793,23 @@ namespace nsfem
     void ChorinGpu::setBudgets(int tent, int pres, int fin)
     {
         // A measured count c gets c + m iterations, m = max(3, c / 4) for the
-        // short velocity solves and max(3, c / 24) for the pressure solve
-        // (one-level preconditioning: ~2,000 iterations, a V-cycle: ~20).
-        // The graph is rebuilt when less than half a margin is left or more
-        // than two margins are idle (hysteresis: counts drift by a few
-        // iterations from step to step).
+        // short velocity solves and max(16, c / 24) for a one-level pressure
+        // solve (~2,000 iterations at 500k; per-instance counts of a batch
+        // drift by several), max(3, c / 24) for the V-cycle's ~20.  The graph
+        // is rebuilt when less than half a margin is left or more than two
+        // margins (and a slack of 8 iterations, 2 for the V-cycle, whose idle
+        // iterations are ~50 launches each) are idle.
         const int counts[3] = {tent, pres, fin};
-        const int minMargin[3] = {3, 3, 3};
+        const bool amgPressure = pSolver_->method() == SolverMethod::AmgPcg;
+        const int minMargin[3] = {3, amgPressure ? 3 : 16, 3};
+        const int slack[3] = {8, amgPressure ? 2 : 8, 8};
         const int frac[3] = {4, 24, 4};
         bool rebuild = false;
         for (int k = 0; k < 3; k++)
         {
             const int c = counts[k];
             const int m = std::max(minMargin[k], c / frac[k]);
-            if (c + m / 2 > budget_[k] || budget_[k] > c + 2 * m + 2)
+            if (c + m / 2 > budget_[k] || budget_[k] > c + 2 * m + slack[k])
             {
                 budget_[k] = c + m;
                 rebuild = true;
