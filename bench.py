@@ -114,7 +114,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     return ch, res, (u1, p1), (u, p)
 
 
-def run_ensemble(args, ctx, cond, u0, target, precision="mixed"):
+def run_ensemble(args, ctx, cond, u0, target, precision="mixed", tau=None):
     """B flows on the one mesh (instance b: inflow scale 1 - 0.4 b / (B - 1),
     start u0 for b = 0, other seeds otherwise) stepped together: every
     matrix is shared, the velocity solves are one pcg_mixed solve over 2B
@@ -138,10 +138,15 @@ def run_ensemble(args, ctx, cond, u0, target, precision="mixed"):
     velocity = "pcg_mixed"
     if precision == "fp64":
         pressure, velocity = pressure.replace("_mixed", ""), "pcg"
+    # tau: the single flow's step on the same mesh (already through the guard;
+    # the ensemble's own estimate of lambda_max differs by the power
+    # iteration's tolerance), so instance 0 can be compared with the single flow
     cfg = fem.ChorinConfig(fem.SolverConfig(velocity, 5000, target, 1), fem.SolverConfig(pressure, 50000, target, 1),
-                           dtype=F64, timeStepPolicy=fem.TAU_CLAMP,
-                           pressureStart=fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else 0)
-    ch, r, _, last = run_path(f"ensemble{B}_{precision}", ctx, cfg, args.tau, np.concatenate(starts), args.warmup, args.steps,
+                           dtype=F64, timeStepPolicy=fem.TAU_CLAMP if tau is None else fem.TAU_UNCHECKED,
+                           pressureStart=fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else 0,
+                           velocityStart=velocity_start_code(args.velocity_start))
+    ch, r, _, last = run_path(f"ensemble{B}_{precision}", ctx, cfg, args.tau if tau is None else tau, np.concatenate(starts),
+                              args.warmup, args.steps,
                               not args.no_autotune, make=lambda c: fem.Chorin(ctx, c, instances=B, inflow_scale=scales))
     del ch
     r["instances"] = B
@@ -177,6 +182,13 @@ def kernel_table(ch, res, reps=30, with_iteration=True, amg_pressure=False):
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_chorin_500k_mixed.md")
 LLC_BYTES = 256e6  # MI355X Infinity Cache
+
+
+def velocity_start_code(name):
+    """The ChorinConfig.velocityStart value of a --velocity-start choice."""
+    from nsfem_amd import fem
+    return {"zero": fem.VELOCITY_START_ZERO, "last": fem.VELOCITY_START_LAST,
+            "extrapolate": fem.VELOCITY_START_EXTRAPOLATE}[name]
 
 
 def copy_peak_gbps(mib=2048, reps=10):
@@ -597,10 +609,12 @@ def main():
     ap.add_argument("--ensemble", type=int, default=8,
                     help="also run an ensemble of this many flows on the same mesh (inflow scales 1.0 down to 0.6; "
                          "mixed velocity solves over 2B columns, pressure over B); 0 = skip")
-    ap.add_argument("--pressure-start", default="last", choices=["last", "extrapolate"],
-                    help="pressure solves' initial guess: the last pressure, or 2 p_n - p_n-1")
-    ap.add_argument("--velocity-start", default="zero", choices=["zero", "last", "extrapolate"],
-                    help="velocity solves' initial guess: zero (as the reference), the last solution, or 2 a_n - a_n-1")
+    ap.add_argument("--pressure-start", default="extrapolate", choices=["last", "extrapolate"],
+                    help="pressure solves' initial guess: the last pressure (as the reference), or 2 p_n - p_n-1 "
+                         "(default; the same solution to the solver tolerance in fewer iterations)")
+    ap.add_argument("--velocity-start", default="extrapolate", choices=["zero", "last", "extrapolate"],
+                    help="velocity solves' initial guess: zero (as the reference), the last solution, or 2 a_n - a_n-1 "
+                         "(default)")
     ap.add_argument("--pressure-solver", default="amg", choices=["mcgs", "amg"],
                     help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
                          "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
@@ -672,8 +686,7 @@ def main():
     eager = 1 if args.eager else 0
     sg = 1 if args.step_graph else 0
     ps = fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else fem.PRESSURE_START_LAST
-    vs = {"zero": fem.VELOCITY_START_ZERO, "last": fem.VELOCITY_START_LAST, "extrapolate": fem.VELOCITY_START_EXTRAPOLATE}[
-        args.velocity_start]
+    vs = velocity_start_code(args.velocity_start)
     def config(method, one_level=False):
         return fem.ChorinConfig(*solvers(method, one_level), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP,
                                 stepGraph=sg, pressureStart=ps, velocityStart=vs)
@@ -728,9 +741,10 @@ def main():
 
     ensemble = None
     if args.ensemble > 0:
-        ensemble, ens_last = run_ensemble(args, ctx, cond, u0, target)
+        tau_single = results[paths[0]]["time_step"]["tauUsed"] if paths else None
+        ensemble, ens_last = run_ensemble(args, ctx, cond, u0, target, tau=tau_single)
         # the same ensemble in FP64 throughout: what batching alone gives
-        ens64, _ = run_ensemble(args, ctx, cond, u0, target, precision="fp64")
+        ens64, _ = run_ensemble(args, ctx, cond, u0, target, precision="fp64", tau=tau_single)
         ensemble["fp64_ensemble"] = ens64
         ensemble["mixed_over_fp64_ensemble"] = ensemble["steps_per_s"] / ens64["steps_per_s"]
         if "fp64" in results:

@@ -36,142 +36,119 @@ namespace nsfem
         __global__ void __launch_bounds__(kBlock) amgSweepKernel(McgsMatrix<T> A, int rowBegin, int rowEnd, const T * __restrict__ b,
                                                                  T * __restrict__ x, int lowerOnly, const int * __restrict__ stop)
         {
-            if (*stop)
-            {
-                return;
-            }
+            // The stop flag, the row bounds and the row's own operands load
+            // together, with no condition around a load (one would end in a
+            // wait for every load in flight): lanes past the colour read its
+            // first row and take no entries.
             const int t = blockIdx.x * kBlock + threadIdx.x;
             const int row = rowBegin + t / L;
             const int lane = t % L;
+            const bool valid = row < rowEnd;
+            const int rs = valid ? row : rowBegin;
+            const int stopped = *stop;
+            const int * end = lowerOnly ? A.rowSplit : A.rowStart + 1;
+            const int j0 = A.rowStart[rs];
+            const int j1 = end[rs];
+            const T d = A.invDiag[rs];
+            T bi[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                bi[c] = b[static_cast<long>(rs) * NCH + c];
+            }
+            // (j1 < j0 never holds; testing it keeps the loads ahead of the
+            // branch, where the compiler would otherwise sink them)
+            if (stopped | (j1 < j0))
+            {
+                return;
+            }
             T s[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
+            rowDot<T, L, NCH>(A, j0, valid ? j1 : j0, x, s);
+            if (valid && lane == 0)
             {
-                s[c] = 0;
-            }
-            if (row < rowEnd)
-            {
-                const int j1 = lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1];
-                for (int j = A.rowStart[row] + lane; j < j1; j += L)
-                {
-                    const T a = A.values[j];
-                    const long col = static_cast<long>(A.column[j]) * NCH;
-#pragma unroll
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        s[c] = fmaT(a, x[col + c], s[c]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
-            {
-                s[c] = laneGroupSum<L>(s[c]);
-            }
-            if (row < rowEnd && lane == 0)
-            {
-                const T d = A.invDiag[row];
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = static_cast<long>(row) * NCH + c;
-                    x[i] = (b[i] - s[c]) * d;
+                    x[static_cast<long>(row) * NCH + c] = (bi[c] - s[c]) * d;
                 }
             }
         }
 
-        // r = b - A x (A = stripped + diagonal).
+        // r = b - A x (A = stripped + diagonal).  Loads as in amgSweepKernel.
         template <typename T, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgResidualKernel(McgsMatrix<T> A, const T * __restrict__ b, const T * __restrict__ x,
                                                                     T * __restrict__ r, const int * __restrict__ stop)
         {
-            if (*stop)
-            {
-                return;
-            }
             const int t = blockIdx.x * kBlock + threadIdx.x;
             const int row = t / L;
             const int lane = t % L;
+            const bool valid = row < A.n;
+            const int rs = valid ? row : 0;
+            const int stopped = *stop;
+            const int j0 = A.rowStart[rs];
+            const int j1 = A.rowStart[rs + 1];
+            const T d = A.diag[rs];
+            T bi[NCH], xi[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                const long i = static_cast<long>(rs) * NCH + c;
+                bi[c] = b[i];
+                xi[c] = x[i];
+            }
+            // (j1 < j0 never holds; testing it keeps the loads ahead of the
+            // branch, where the compiler would otherwise sink them)
+            if (stopped | (j1 < j0))
+            {
+                return;
+            }
             T s[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
+            rowDot<T, L, NCH>(A, j0, valid ? j1 : j0, x, s);
+            if (valid && lane == 0)
             {
-                s[c] = 0;
-            }
-            if (row < A.n)
-            {
-                for (int j = A.rowStart[row] + lane; j < A.rowStart[row + 1]; j += L)
-                {
-                    const T a = A.values[j];
-                    const long col = static_cast<long>(A.column[j]) * NCH;
-#pragma unroll
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        s[c] = fmaT(a, x[col + c], s[c]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
-            {
-                s[c] = laneGroupSum<L>(s[c]);
-            }
-            if (row < A.n && lane == 0)
-            {
-                const T d = A.diag[row];
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = static_cast<long>(row) * NCH + c;
-                    r[i] = b[i] - fmaT(d, x[i], s[c]);
+                    r[static_cast<long>(row) * NCH + c] = bi[c] - fmaT(d, xi[c], s[c]);
                 }
             }
         }
 
         // y = M x (accumulate: y += M x) for the restriction and prolongation.
+        // Loads as in amgSweepKernel.
         template <typename T, int L, int NCH>
         __global__ void __launch_bounds__(kBlock) amgSpmvKernel(int rows, const int * __restrict__ rowStart, const int * __restrict__ column,
                                                                 const T * __restrict__ values, const T * __restrict__ x, T * __restrict__ y,
                                                                 int accumulate, const int * __restrict__ stop)
         {
-            if (*stop)
-            {
-                return;
-            }
             const int t = blockIdx.x * kBlock + threadIdx.x;
             const int row = t / L;
             const int lane = t % L;
+            const bool valid = row < rows;
+            const int rs = valid ? row : 0;
+            const int stopped = *stop;
+            const int j0 = rowStart[rs];
+            const int j1 = rowStart[rs + 1];
+            // the old y only where it is added to (element 0 otherwise)
+            T yi[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                yi[c] = y[accumulate ? static_cast<long>(rs) * NCH + c : 0];
+            }
+            if (stopped)
+            {
+                return;
+            }
+            const McgsMatrix<T> M{rows, rowStart, nullptr, column, values, nullptr, nullptr};
             T s[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
-            {
-                s[c] = 0;
-            }
-            if (row < rows)
-            {
-                for (int j = rowStart[row] + lane; j < rowStart[row + 1]; j += L)
-                {
-                    const T a = values[j];
-                    const long col = static_cast<long>(column[j]) * NCH;
-#pragma unroll
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        s[c] = fmaT(a, x[col + c], s[c]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NCH; c++)
-            {
-                s[c] = laneGroupSum<L>(s[c]);
-            }
-            if (row < rows && lane == 0)
+            rowDot<T, L, NCH>(M, j0, valid ? j1 : j0, x, s);
+            if (valid && lane == 0)
             {
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    const long i = static_cast<long>(row) * NCH + c;
-                    y[i] = accumulate ? y[i] + s[c] : s[c];
+                    y[static_cast<long>(row) * NCH + c] = accumulate ? yi[c] + s[c] : s[c];
                 }
             }
         }
@@ -393,32 +370,18 @@ namespace nsfem
             }
             for (int base = blockIdx.x * G; base < A.n; base += gridDim.x * G)
             {
+                // the row's own operands load on its first lane after the
+                // row sum: measured faster here than loading them on every
+                // lane beside the gathers (short P1 rows, several slabs per
+                // block keep enough loads in flight)
                 const int row = base + static_cast<int>(threadIdx.x) / L;
+                const bool valid = row < A.n;
+                const int rs = valid ? row : base;
+                const int j0 = A.rowStart[rs];
+                const int j1 = A.rowStart[rs + 1];
                 double s[NCH];
-#pragma unroll
-                for (int c = 0; c < NCH; c++)
-                {
-                    s[c] = 0;
-                }
-                if (row < A.n)
-                {
-                    for (int j = A.rowStart[row] + lane; j < A.rowStart[row + 1]; j += L)
-                    {
-                        const double a = A.values[j];
-                        const long col = static_cast<long>(A.column[j]) * NCH;
-#pragma unroll
-                        for (int c = 0; c < NCH; c++)
-                        {
-                            s[c] = fmaT(a, static_cast<double>(z[col + c]), s[c]);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < NCH; c++)
-                {
-                    s[c] = laneGroupSum<L>(s[c]);
-                }
-                if (row < A.n && lane == 0)
+                rowDot<double, L, NCH>(A, j0, valid ? j1 : j0, z, s);
+                if (valid && lane == 0)
                 {
                     const double d = A.diag[row];
 #pragma unroll

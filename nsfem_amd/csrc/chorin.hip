@@ -64,16 +64,27 @@ namespace nsfem
         {
             return;
         }
+        // two contributions per step, loaded unconditionally (a missing
+        // second one reloads the first and adds -0.0, which leaves every sum
+        // unchanged), so both gathers go out together; the sum keeps the
+        // list order.  A use under a condition would let the compiler move
+        // the loads behind a branch that waits for the first pair.
+        const int t1 = cs[j + 1];
+        const T nk = nuK[j];
         double s = 0;
-        for (int t = cs[j]; t < cs[j + 1]; t++)
+        for (int t = cs[j]; t < t1; t += 2)
         {
-            s += static_cast<double>(elemC[ci[t]]);
+            const bool h = t + 1 < t1;
+            const T e0 = elemC[ci[t]];
+            const T e1 = elemC[ci[h ? t + 1 : t]];
+            s += static_cast<double>(e0);
+            s += h ? static_cast<double>(e1) : -0.0;
         }
         if (C)
         {
             C[j] = static_cast<T>(s);
         }
-        A[j] = static_cast<T>(static_cast<double>(nuK[j]) + s);
+        A[j] = static_cast<T>(static_cast<double>(nk) + s);
     }
 
     void ChorinGpu::upload(DevCsr & d, const CsrD & m, const std::vector<double> * scaled)

@@ -148,14 +148,49 @@ namespace nsfem
         {
             const int src = rowIds ? rowIds[row] : row;
             const int j1 = rowStart[src + 1];
-            for (int j = rowStart[src] + lane; j < j1; j += LANES)
+            // U strided entries per step, loaded unconditionally (an entry
+            // past the row's end reloads entry j and its term is dropped), so
+            // their loads, then their gathers, go out together; each lane
+            // adds its terms in the same order as one by one.  Lane groups
+            // of 8 or more take one entry per step: their lanes see about one
+            // entry of a row each, and the reloads measured slower (P2 rows,
+            // 8 lanes: 155 -> 171 us at 2M rows).
+            constexpr int U = LANES <= 2 ? 4 : (LANES <= 4 ? 2 : 1);
+            for (int j = rowStart[src] + lane; j < j1; j += U * LANES)
             {
-                const T v = values[j];
-                const int col = column[j];
+                T v[U];
+                int col[U];
+                bool has[U];
 #pragma unroll
-                for (int c = 0; c < NCH; c++)
+                for (int u = 0; u < U; u++)
                 {
-                    acc[c] = fmaT(v, x[col + c * ldx], acc[c]);
+                    const int ju = j + u * LANES;
+                    has[u] = ju < j1;
+                    const int js = has[u] ? ju : j;
+                    v[u] = values[js];
+                    col[u] = column[js];
+                }
+                T xv[U][NCH];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        xv[u][c] = x[col[u] + c * ldx];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                {
+                    if (has[u])
+                    {
+#pragma unroll
+                        for (int c = 0; c < NCH; c++)
+                        {
+                            acc[c] = fmaT(v[u], xv[u][c], acc[c]);
+                        }
+                    }
                 }
             }
         }
@@ -624,6 +659,10 @@ namespace nsfem
         // U strided entries per step: their loads, then their gathers, go out
         // together (the terms are added in the same order as one by one).
         // Four for short rows on 1-2 lanes, two where a lane sees few entries.
+        // The loads are unconditional: an entry past the row's end reloads
+        // the step's first entry j (in the row by the loop test) and only its
+        // term is dropped.  Conditional loads compile to branches that each
+        // end in a wait for every load in flight, which serialises the step.
         constexpr int U = LANES <= 2 ? 4 : 2;
         for (int j = j0 + lane; j < j1; j += U * LANES)
         {
@@ -635,8 +674,9 @@ namespace nsfem
             {
                 const int ju = j + u * LANES;
                 has[u] = ju < j1;
-                a[u] = has[u] ? A.values[ju] : T(0);
-                col[u] = has[u] ? A.column[ju] : 0;
+                const int js = has[u] ? ju : j;
+                a[u] = A.values[js];
+                col[u] = A.column[js];
             }
             T xv[U][NCH];
 #pragma unroll
@@ -645,7 +685,7 @@ namespace nsfem
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    xv[u][c] = has[u] ? gather(col[u], c) : T(0);
+                    xv[u][c] = gather(col[u], c);
                 }
             }
 #pragma unroll
@@ -713,22 +753,17 @@ namespace nsfem
                 j0 = valid ? A.rowStart[row] : 0;
                 j1 = valid ? (lowerOnly ? A.rowSplit[row] : A.rowStart[row + 1]) : 0;
             }
-            // the row's own operands load alongside the gathers
+            // the row's own operands load alongside the gathers, on every
+            // lane (lanes past the block's rows read the slab's first row):
+            // loads under a condition end in a wait for every load in flight
             const bool lead = valid && groupLeader<LANES>();
-            T inv = T(0), bi[NCH];
+            const int rs = valid ? row : base;
+            const T inv = A.invDiag[rs];
+            T bi[NCH];
 #pragma unroll
             for (int c = 0; c < NCH; c++)
             {
-                bi[c] = T(0);
-            }
-            if (lead)
-            {
-                inv = A.invDiag[row];
-#pragma unroll
-                for (int c = 0; c < NCH; c++)
-                {
-                    bi[c] = b[static_cast<long>(row) * NCH + c];
-                }
+                bi[c] = b[static_cast<long>(rs) * NCH + c];
             }
             T acc[NCH];
             rowDot<T, LANES, NCH>(A, j0, j1, x, acc);
@@ -813,12 +848,28 @@ namespace nsfem
         {
             const int row = base + static_cast<int>(threadIdx.x) / LANES;
             const bool valid = row < range.y;
+            // The row's own operands load before its entries, on every lane
+            // and with no condition around a load (one would end in a wait
+            // for every load in flight): lanes past the block's rows read the
+            // slab's first row; x is read but not used for a zero guess, the
+            // colour-0 scale at element 0 for the other colours.
+            const int rs = valid ? row : base;
+            const T dg = A.diag[rs];
+            const P dInv = invDiagP[rs < color0End ? rs : 0];
+            T bv[NCH], xv[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                const long i = static_cast<long>(rs) * NCH + c;
+                bv[c] = b[i];
+                xv[c] = x[i];
+            }
             T acc[NCH];
             if (!zeroGuess)
             {
-                const int j0 = valid ? A.rowStart[row] : 0;
-                const int j1 = valid ? A.rowStart[row + 1] : 0;
-                rowDot<T, LANES, NCH>(A, j0, j1, x, acc);
+                const int j0 = A.rowStart[rs];
+                const int j1 = A.rowStart[rs + 1];
+                rowDot<T, LANES, NCH>(A, j0, valid ? j1 : j0, x, acc);
             }
             if (valid && groupLeader<LANES>())
             {
@@ -826,7 +877,7 @@ namespace nsfem
                 for (int c = 0; c < NCH; c++)
                 {
                     const long i = static_cast<long>(row) * NCH + c;
-                    const T bi = b[i];
+                    const T bi = bv[c];
                     T ri;
                     if (zeroGuess)
                     {
@@ -835,7 +886,7 @@ namespace nsfem
                     }
                     else
                     {
-                        ri = bi - fmaT(A.diag[row], x[i], acc[c]);
+                        ri = bi - fmaT(dg, xv[c], acc[c]);
                     }
                     r[i] = ri;
                     const P rp = static_cast<P>(ri);
@@ -845,7 +896,7 @@ namespace nsfem
                     }
                     if (row < color0End)
                     {
-                        z[i] = rp * invDiagP[row];
+                        z[i] = rp * dInv;
                     }
                     part[c] = fma(static_cast<double>(bi), static_cast<double>(bi), part[c]);
                     part[NCH + c] = fma(static_cast<double>(ri), static_cast<double>(ri), part[NCH + c]);
@@ -914,43 +965,39 @@ namespace nsfem
                     j0 = c0 ? M.rowStart[row] : 0;
                     j1 = c0 ? M.rowStart[row + 1] : 0;
                 }
-                // the row's own operands load alongside the gathers
+                // The row's own operands load alongside the gathers, on every
+                // lane and with no condition around a load (one would end in
+                // a wait for every load in flight): lanes past the block's
+                // rows read the slab's first row, a row's own value is read
+                // through a selected pointer, and a value a row does not use
+                // is read at element 0 (one shared line for all such lanes).
                 const bool lead = valid && groupLeader<LANES>();
-                P inv = P(0), own[NCH];
+                const int rs = valid ? row : base;
+                const P inv = M.invDiag[c0 ? rs : 0];
+                // colour 0 starts from its residual (with P == T, rP is r),
+                // the other colours keep the preconditioned value
+                const P * ownSrc = c0 ? rP : z;
+                P own[NCH];
+                T rFix[NCH];
                 double rr[NCH];
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    own[c] = P(0);
-                    rr[c] = 0;
-                }
-                if (lead)
-                {
-                    if (c0)
+                    const long i = static_cast<long>(rs) * NCH + c;
+                    const bool fix = c0 && fixC[c];
+                    const T ri = r[i];
+                    const T qi = q[fix ? i : 0];
+                    const P oi = ownSrc[i];
+                    rFix[c] = fix ? fmaT(-alphaC[c], qi, ri) : ri;
+                    if constexpr (std::is_same_v<T, P>)
                     {
-                        inv = M.invDiag[row];
+                        own[c] = c0 ? rFix[c] : oi;
                     }
-#pragma unroll
-                    for (int c = 0; c < NCH; c++)
+                    else
                     {
-                        const long i = static_cast<long>(row) * NCH + c;
-                        T ri = r[i];
-                        if (c0 && fixC[c])
-                        {
-                            ri = fmaT(-alphaC[c], q[i], ri);
-                            r[i] = ri;
-                        }
-                        // with P == T, rP is r: its colour-0 value is ri
-                        if constexpr (std::is_same_v<T, P>)
-                        {
-                            own[c] = c0 ? ri : z[i];
-                        }
-                        else
-                        {
-                            own[c] = c0 ? rP[i] : z[i];
-                        }
-                        rr[c] = static_cast<double>(ri);
+                        own[c] = oi;
                     }
+                    rr[c] = static_cast<double>(rFix[c]);
                 }
                 P acc[NCH];
                 if (anyC0)
@@ -965,6 +1012,10 @@ namespace nsfem
                     {
                         const long i = static_cast<long>(row) * NCH + c;
                         P zi;
+                        if (c0 && fixC[c])
+                        {
+                            r[i] = rFix[c];
+                        }
                         if (c0)
                         {
                             zi = (own[c] - acc[c]) * inv;
@@ -1040,28 +1091,24 @@ namespace nsfem
                     j0 = valid ? A.rowStart[row] : 0;
                     j1 = valid ? A.rowStart[row + 1] : 0;
                 }
-                // the row's own operands load alongside the gathers
+                // The row's own operands load alongside the gathers, on every
+                // lane and unconverted (lanes past the block's rows read the
+                // slab's first row; p and q are read but not used on the
+                // first iteration): loads under a condition, or converted
+                // where they load, make the compiler wait for every load in
+                // flight before the row's entries are even requested.
                 const bool lead = valid && groupLeader<LANES>();
-                T dg = T(0), zr[NCH], pr[NCH], qr[NCH];
+                const int rs = valid ? row : base;
+                const T dg = A.diag[rs];
+                P zr[NCH];
+                T pr[NCH], qr[NCH];
 #pragma unroll
                 for (int c = 0; c < NCH; c++)
                 {
-                    zr[c] = pr[c] = qr[c] = T(0);
-                }
-                if (lead)
-                {
-                    dg = A.diag[row];
-#pragma unroll
-                    for (int c = 0; c < NCH; c++)
-                    {
-                        const long i = static_cast<long>(row) * NCH + c;
-                        zr[c] = static_cast<T>(z[i]);
-                        if (!first)
-                        {
-                            pr[c] = p[i];
-                            qr[c] = q[i];
-                        }
-                    }
+                    const long i = static_cast<long>(rs) * NCH + c;
+                    zr[c] = z[i];
+                    pr[c] = p[i];
+                    qr[c] = q[i];
                 }
                 T acc[NCH];
                 rowDot<T, LANES, NCH>(A, j0, j1, z, acc);
@@ -1085,7 +1132,7 @@ namespace nsfem
                     for (int c = 0; c < NCH; c++)
                     {
                         const long i = static_cast<long>(row) * NCH + c;
-                        const T zi = zr[c];
+                        const T zi = static_cast<T>(zr[c]);
                         const T az = fmaT(dg, zi, acc[c]);
                         T pi, qi;
                         if (first)
