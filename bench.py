@@ -72,9 +72,9 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     log(f"  {name}: autotuned in {tune_s:.1f} s; first step")
     first = ch.step()
     u1, p1 = ch.download()
-    for _ in range(warmup):
-        ch.step()
-    log(f"  {name}: warm-up done")
+    warm = [ch.step() for _ in range(warmup)]
+    log(f"  {name}: warm-up done; pressure iterations of the first and warm-up steps: " +
+        " ".join(str(int(s["itersPressure"])) for s in [first] + warm))
     _lib.call("nsfem_synchronize")
     stats, walls = [], []
     t0 = time.perf_counter()
@@ -86,6 +86,11 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
             log(f"  {name}: step {k + 1}/{steps}, {walls[-1]:.1f} ms")
     _lib.call("nsfem_synchronize")
     dt = time.perf_counter() - t0
+    med = float(np.median(walls))
+    for k, (w, s) in enumerate(zip(walls, stats)):
+        if w > 4 * med:
+            log(f"  {name}: step {k + 1} took {w:.1f} ms (median {med:.1f}): " +
+                ", ".join(f"{key} {s[key]:.4g}" for key in sorted(s)))
     u, p = ch.download()
     if not (np.isfinite(u).all() and np.isfinite(p).all()):
         raise RuntimeError(f"{name}: non-finite state")
@@ -112,6 +117,13 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     log(f"{name}: {res['steps_per_s']:.2f} steps/s ({res['ms_per_step']:.2f} ms/step), iters {res['iters']}, "
         f"outer {res['outer']}, max relres {res['max_relres']:.2e}, phase ms {res['ms']}")
     return ch, res, (u1, p1), (u, p)
+
+
+def ensemble_chorin(args, ctx, cfg, B, scales):
+    from nsfem_amd import fem
+    ch = fem.Chorin(ctx, cfg, instances=B, inflow_scale=scales)
+    apply_amg_smoother(args, ch, cfg)
+    return ch
 
 
 def run_ensemble(args, ctx, cond, u0, target, precision="mixed", tau=None):
@@ -147,7 +159,7 @@ def run_ensemble(args, ctx, cond, u0, target, precision="mixed", tau=None):
                            velocityStart=velocity_start_code(args.velocity_start))
     ch, r, _, last = run_path(f"ensemble{B}_{precision}", ctx, cfg, args.tau if tau is None else tau, np.concatenate(starts),
                               args.warmup, args.steps,
-                              not args.no_autotune, make=lambda c: fem.Chorin(ctx, c, instances=B, inflow_scale=scales))
+                              not args.no_autotune, make=lambda c: ensemble_chorin(args, ctx, c, B, scales))
     del ch
     r["instances"] = B
     r["inflow_scales"] = scales
@@ -182,6 +194,15 @@ def kernel_table(ch, res, reps=30, with_iteration=True, amg_pressure=False):
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_chorin_500k_mixed.md")
 LLC_BYTES = 256e6  # MI355X Infinity Cache
+
+
+def apply_amg_smoother(args, ch, cfg):
+    """--amg-chebyshev FROM[,DEGREE[,FRACTION]] on an AMG pressure solver."""
+    if not args.amg_chebyshev or not cfg.pressureSolver.method.startswith("amg"):
+        return
+    vals = [float(v) for v in args.amg_chebyshev.split(",")]
+    for key, v in zip(("amgChebyshevFrom", "amgChebyshevDegree", "amgChebyshevFraction"), vals):
+        ch.set_solver_option(key, v, "pressure")
 
 
 def velocity_start_code(name):
@@ -415,7 +436,7 @@ def run_poisson(args):
         "fp64": out["fp64"], "mixed": out["mixed"], "mixed_restart": out["mixed_restart"],
         "mixed_over_fp64": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], "setup_s": setup_s,
         "preconditioners": {"fp64, mixed, mixed_restart": "one symmetric multicolour GS sweep",
-                            "fp64_amg, mixed_amg": "smoothed-aggregation V-cycle, multicolour GS smoothing"},
+                            "fp64_amg, mixed_amg": "smoothed-aggregation V-cycle, multicolour GS smoothing on the finest level, Chebyshev on the coarser"},
         "fp64_amg": out["fp64_amg"], "mixed_amg": out["mixed_amg"],
         "mixed_amg_over_fp64": out["mixed_amg"]["solves_per_s"] / out["fp64"]["solves_per_s"],
     }
@@ -560,7 +581,7 @@ def run_multi_rhs(args):
                         "max_rel_mean": ind_mean},
         "block_over_independent": dt_ind / t_block, "setup_s": setup_s,
         "amg": {"seconds": dt_amg, "tile": 16, "iterations_per_tile": its_amg, "max_true_relres": amg_rel,
-                "max_rel_mean": amg_mean, "method": "amg_pcg_mixed (smoothed-aggregation V-cycle, multicolour GS smoothing)"},
+                "max_rel_mean": amg_mean, "method": "amg_pcg_mixed (smoothed-aggregation V-cycle, multicolour GS smoothing on the finest level, Chebyshev on the coarser)"},
         "amg_over_block": t_block / dt_amg,
     }
     log(f"multi-rhs: block tile {best}, {t_block:.2f} s per {nrhs}-column solve (independent {dt_ind:.2f} s), "
@@ -615,9 +636,12 @@ def main():
     ap.add_argument("--velocity-start", default="extrapolate", choices=["zero", "last", "extrapolate"],
                     help="velocity solves' initial guess: zero (as the reference), the last solution, or 2 a_n - a_n-1 "
                          "(default)")
+    ap.add_argument("--amg-chebyshev", default="",
+                    help="AMG pressure solves: FROM[,DEGREE[,FRACTION]] = the options amgChebyshevFrom, "
+                         "amgChebyshevDegree, amgChebyshevFraction (default: the library's)")
     ap.add_argument("--pressure-solver", default="amg", choices=["mcgs", "amg"],
                     help="pressure preconditioner: one symmetric multicolour GS sweep (pcg, pcg_mixed) or the "
-                         "smoothed-aggregation V-cycle with multicolour GS smoothing (amg_pcg, amg_pcg_mixed)")
+                         "smoothed-aggregation V-cycle (amg_pcg, amg_pcg_mixed)")
     ap.add_argument("--batch-leg", type=int, default=48,
                     help="also run this many uneven instances (10k-250k pressure DOFs) as one captured batch; 0 = skip")
     ap.add_argument("--ensemble-pressure", default=None, choices=["pcg_mixed", "bpcg_mixed", "amg_pcg_mixed"],
@@ -676,6 +700,7 @@ def main():
         if dir_lanes is not None:
             for which, v in zip(("velocity", "pressure"), dir_lanes):
                 ch.set_solver_option("lanesDirection", v, which)
+        apply_amg_smoother(args, ch, cfg)
         return ch
 
     def solvers(method, one_level=False):
@@ -821,11 +846,12 @@ def main():
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
                    "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 preconditioners",
                    "preconditioners": {"velocity": "one symmetric multicolour GS sweep",
-                                       "pressure": ("smoothed-aggregation V-cycle, multicolour GS smoothing"
+                                       "pressure": ("smoothed-aggregation V-cycle, multicolour GS smoothing on the finest level, Chebyshev on the coarser"
                                                     if args.pressure_solver == "amg" else "one symmetric multicolour GS sweep")},
                    "pressure_start": args.pressure_start,
                    "velocity_start": args.velocity_start,
                    "pressure_solver": args.pressure_solver,
+                   "amg_chebyshev_from_degree_fraction": args.amg_chebyshev or "1,2,0.2 (library default)",
                    "seed": args.seed},
         "path": head,
         "fp64": r64,

@@ -127,8 +127,10 @@ int nsfem_convert(int srcDtype, int dstDtype, long n, const void * src, void * d
  *                      V-cycle (host set-up: strength 0.08 halved per level,
  *                      greedy aggregation, Jacobi-smoothed prolongation,
  *                      Galerkin coarse operators down to <= 512 rows, inverted
- *                      densely) with one multicolour GS sweep per level and
- *                      direction (forward down, backward up); one system,
+ *                      densely) smoothed by one multicolour GS sweep per
+ *                      direction on the finest level (forward down, backward
+ *                      up) and two Chebyshev steps on D^-1 A on the coarser
+ *                      ones (options "amgChebyshev*" below); one system,
  *                      numCh = 1, 2, 4, 8 or 16 right-hand sides (each with its
  *                      own scalars and stopping test); the stopping rule of
  *                      "pcg".  The matrix must be SPD, or positive
@@ -206,7 +208,18 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  values from r and q; bit-identical results) while the
  *                  update workgroups hold at most two sweep slabs of rows
  *                  (rows per update workgroup x sweep lanes <= 512); 0: two
- *                  launches. */
+ *                  launches.
+ *   "amgChebyshevFrom", "amgChebyshevDegree", "amgChebyshevFraction"
+ *                  amg_pcg: the V-cycle levels from this one on (but the
+ *                  dense coarsest) are smoothed by this many steps of the
+ *                  Chebyshev iteration on D^-1 A over [fraction, 1] x the
+ *                  level's lambda_max bound (1.1 x its power-iteration
+ *                  estimate, at most the Gershgorin bound), the same
+ *                  polynomial on the way down and up, instead of one
+ *                  multicolour GS sweep (degree 2 to 8; 2 deg + 2 launches
+ *                  per level instead of 2 colours + 3).  Defaults 1, 2, 0.2
+ *                  (GS on the finest level only); a first level past the
+ *                  last smoothing level keeps GS everywhere. */
 int nsfem_solver_set_option(nsfem_solver * s, const char * key, double value);
 /* Kernel timing (HIP events on the library stream; reps launches queued
  * behind a 20 ms delay kernel so they run back to back and the events

@@ -1,7 +1,8 @@
-// CG preconditioned by a smoothed-aggregation V-cycle whose smoother on
-// every level is one multicolour Gauss-Seidel sweep (forward on the way
-// down, backward on the way up: a symmetric preconditioner), for the
-// pressure Poisson solve.
+// CG preconditioned by a smoothed-aggregation V-cycle, for the pressure
+// Poisson solve.  The smoother of a level is one multicolour Gauss-Seidel
+// sweep (forward on the way down, backward on the way up) or the Chebyshev
+// iteration on D^-1 A of a chosen degree (the same polynomial on both
+// ways); either keeps the preconditioner symmetric.
 //
 // No counterpart in the reference (its pressure solve is multicolour GS,
 // cpp/libs/CudaUtils/src/gaussSeidel.cu:121-427, or cuDSS); the stopping
@@ -14,8 +15,10 @@
 //
 // Launches per iteration: the direction (p, q = A p and p.q partials), the
 // update (x, r, |r|^2 partials, the FP32 residual copy), the V-cycle (per
-// smoothing level 2 x colours + 3, one dense product on the coarsest), the
-// r.z partials and a one-block finalize (rho, convergence).  Every kernel
+// Gauss-Seidel level 2 x colours + 3, per Chebyshev level of degree k
+// 2 k + 2 (+1 on the finest, whose first iterate is not written by a
+// restriction), one dense product on the coarsest), the r.z partials and a
+// one-block finalize (rho, convergence).  Every kernel
 // returns at once after the solve has stopped, so whole iterations are
 // replayed from one captured graph and polled one iteration behind.
 #pragma once
@@ -72,6 +75,18 @@ namespace nsfem
         // (overwritten, as are the internal work vectors).
         double timeKernel(const std::string & which, int reps, double * bytes, double * launches, double * x);
 
+        // Smoothers: levels from `chebyshevFrom` on (but the coarsest, which
+        // is solved densely) take `degree` steps of the Chebyshev iteration
+        // on D^-1 A over [fraction, 1] x lambda_max (the level's bound:
+        // 1.1 x its power-iteration estimate, at most the Gershgorin bound;
+        // degree 2 to 8);
+        // the levels before it one multicolour Gauss-Seidel sweep.  A first
+        // level beyond the last smoothing level keeps Gauss-Seidel everywhere.
+        void setSmoother(int chebyshevFrom, int degree, double fraction);
+        int chebyshevFrom() const { return chebFrom_; }
+        int chebyshevDegree() const { return chebDegree_; }
+        double chebyshevFraction() const { return chebFraction_; }
+
         int numLevels() const { return static_cast<int>(levels_.size()); }
         int levelRows(int l) const { return levels_[l].n; }
         int levelColors(int l) const { return static_cast<int>(levels_[l].colorStart.size()) - 1; }
@@ -96,6 +111,14 @@ namespace nsfem
             DeviceArray<float> pValues32, rValues32;
             // vectors in the V-cycle precision (bytes)
             DeviceArray<char> b, x, r;
+            // Chebyshev smoothing (chebDegree 0: Gauss-Seidel): step k forms
+            // d = chebC1[k] d + chebC2[k] D^-1 (b - A x), x += d; xa is the
+            // alternate iterate (steps read one vector and write the other),
+            // d the direction
+            double lambdaMax = 0;
+            int chebDegree = 0;
+            std::vector<double> chebC1, chebC2;
+            DeviceArray<char> xa, d;
         };
 
         template <typename T>
@@ -121,6 +144,8 @@ namespace nsfem
         DeviceArray<double> coarse64_;
         DeviceArray<float> coarse32_;
         double complexity_ = 0, setupSeconds_ = 0;
+        int chebFrom_ = 1, chebDegree_ = 2;
+        double chebFraction_ = 0.2;
 
         // Krylov vectors (FP64) and the V-cycle operands (P)
         DeviceArray<double> r_, p_, q_;

@@ -111,6 +111,28 @@ namespace
                 throw std::invalid_argument("lanesDirection must be 0, 1, 2, 4, 8 or 16");
             solver.setDirectionLanes(static_cast<int>(value));
         }
+        else if (k == "amgChebyshevFrom" || k == "amgChebyshevDegree" || k == "amgChebyshevFraction")
+        {
+            AmgPcg * amg = solver.amg();
+            if (!amg)
+            {
+                throw std::invalid_argument(k + " applies to the amg_pcg solvers only");
+            }
+            if (k == "amgChebyshevFrom")
+            {
+                if (!(value >= 0 && value <= 1 << 20)) throw std::invalid_argument("amgChebyshevFrom must be in [0, 2^20]");
+                amg->setSmoother(static_cast<int>(value), amg->chebyshevDegree(), amg->chebyshevFraction());
+            }
+            else if (k == "amgChebyshevDegree")
+            {
+                if (value != static_cast<int>(value)) throw std::invalid_argument("amgChebyshevDegree must be an integer");
+                amg->setSmoother(amg->chebyshevFrom(), static_cast<int>(value), amg->chebyshevFraction());
+            }
+            else
+            {
+                amg->setSmoother(amg->chebyshevFrom(), amg->chebyshevDegree(), value);
+            }
+        }
         else if (k == "rowsDirection" || k == "rowsUpdate" || k == "rowsRho")
         {
             if (!(value >= 0 && value <= 1 << 20)) throw std::invalid_argument(k + " must be in [0, 2^20]");
@@ -1363,7 +1385,8 @@ extern "C"
             need(key, "key");
             if (which < 0 || which > 2) throw std::invalid_argument("which must be 0 (velocity), 1 (pressure) or 2 (both)");
             if (which != 1) applySolverOption(ch->core->velocitySolver(), key, value);
-            if (which != 0) applySolverOption(ch->core->pressureSolver(), key, value); });
+            if (which != 0) applySolverOption(ch->core->pressureSolver(), key, value);
+            ch->core->invalidateStepGraph(); });
     }
 
     int nsfem_chorin_autotune(nsfem_chorin * ch, int reps, int * shape)

@@ -124,6 +124,9 @@ namespace nsfem
         // tau changes).
         void step(ChorinStepStats * stats);
         void setStepGraph(bool on);
+        // Drops the captured step, which is captured again at the next step
+        // (after a solver option changed what its launches are).
+        void invalidateStepGraph() { destroyStepGraph(); }
         bool stepGraph() const { return cfg_.stepGraph; }
         int stepGraphBuilds() const { return graphBuilds_; }
         int stepGraphRollbacks() const { return graphRollbacks_; }

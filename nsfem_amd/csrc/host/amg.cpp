@@ -417,7 +417,25 @@ namespace nsfem
                 break;
             }
             // smoothed prolongation P = (I - omega D^-1 A) P_tent
-            const double omega = opt.omegaScale / lambdaMaxDinvA(A, L.diag);
+            const double lambda = lambdaMaxDinvA(A, L.diag);
+            const double omega = opt.omegaScale / lambda;
+            {
+                // upper bound of the spectrum of D^-1 A for a polynomial
+                // smoother on this level: the power iteration's estimate
+                // (from below) raised by 10 %, but never above the
+                // Gershgorin bound max_i sum_j |a_ij| / a_ii
+                double gersh = 0;
+                for (int i = 0; i < A.rows; i++)
+                {
+                    double row = 0;
+                    for (int j = A.rowStart[i]; j < A.rowStart[i + 1]; j++)
+                    {
+                        row += std::abs(A.values[j]);
+                    }
+                    gersh = std::max(gersh, row / L.diag[i]);
+                }
+                L.lambdaMax = std::min(1.1 * lambda, gersh);
+            }
             CsrD P;
             P.rows = L.n;
             P.cols = nc;

@@ -42,6 +42,10 @@ namespace nsfem
         std::vector<int> rowSplit;
         std::vector<double> diag, invDiag;
         std::vector<int> colorStart; // numColors + 1
+        // upper bound of the spectrum of D^-1 A for a polynomial smoother:
+        // min(1.1 x the power-iteration estimate, Gershgorin bound); set on
+        // every level but the coarsest
+        double lambdaMax = 0;
         // to / from the next coarser level (empty on the coarsest)
         CsrD P; // n x nNext
         CsrD R; // nNext x n, P^T

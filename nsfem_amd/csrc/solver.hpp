@@ -167,6 +167,7 @@ namespace nsfem
         bool lastNonFinite() const { return lastNonFinite_; }
         // amg_pcg: the hierarchy (nullptr for the other methods)
         const AmgPcg * amg() const { return amg_.get(); }
+        AmgPcg * amg() { return amg_.get(); }
         // block PCG: whether the last solve dropped a direction (a rank-
         // deficient block)
         bool lastDroppedDirection() const { return lastDropped_; }
