@@ -66,7 +66,7 @@ def run_path(name, ctx, cfg, tau, u0, warmup, steps, autotune, make=None):
     ch.set_time_step(tau)
     tau_info = ch.time_step_info()
     t0 = time.perf_counter()
-    shape = ch.autotune(10) if autotune else ch.shape()
+    shape = ch.autotune(30) if autotune else ch.shape()
     tune_s = time.perf_counter() - t0 if autotune else 0.0
     ch.upload_velocity(u0)
     log(f"  {name}: autotuned in {tune_s:.1f} s; first step")
@@ -168,7 +168,7 @@ def run_ensemble(args, ctx, cond, u0, target, precision="mixed", tau=None):
     return r, last
 
 
-def kernel_table(ch, res, reps=30, with_iteration=True, amg_pressure=False):
+def kernel_table(ch, res, reps=30, with_iteration=True, amg_pressure=False, velocity_fsai=False):
     """Per-kernel average launch time (HIP events on the library stream,
     replaying the launches one solver iteration makes), algorithmic bytes per
     launch, GB/s, and each kernel's estimated share of one step: launches per
@@ -180,6 +180,8 @@ def kernel_table(ch, res, reps=30, with_iteration=True, amg_pressure=False):
         kinds = ("direction", "sweep", "rho", "update", "finalize") + (("iteration",) if with_iteration else ())
         if solver == "pressure" and amg_pressure:
             kinds = ("direction", "update", "rho", "vcycle", "iteration")
+        if solver == "velocity" and velocity_fsai:
+            kinds = ("direction", "update", "fsai") + (("iteration",) if with_iteration else ())
         for k in kinds:
             ms, b, per_it = ch.time_kernel(f"{solver}_{k}", reps=reps)
             out[f"{solver}_{k}"] = {"avg_ms": ms, "bytes": b, "GBps": b / (ms * 1e-3) / 1e9 if b else None,
@@ -636,6 +638,9 @@ def main():
     ap.add_argument("--velocity-start", default="extrapolate", choices=["zero", "last", "extrapolate"],
                     help="velocity solves' initial guess: zero (as the reference), the last solution, or 2 a_n - a_n-1 "
                          "(default)")
+    ap.add_argument("--velocity-precond", default="fsai", choices=["sgs", "fsai"],
+                    help="velocity solves' preconditioner (single flows): one symmetric multicolour GS sweep (as the "
+                         "reference) or the factorized sparse approximate inverse (default)")
     ap.add_argument("--amg-chebyshev", default="",
                     help="AMG pressure solves: FROM[,DEGREE[,FRACTION]] = the options amgChebyshevFrom, "
                          "amgChebyshevDegree, amgChebyshevFraction (default: the library's)")
@@ -701,6 +706,8 @@ def main():
             for which, v in zip(("velocity", "pressure"), dir_lanes):
                 ch.set_solver_option("lanesDirection", v, which)
         apply_amg_smoother(args, ch, cfg)
+        if args.velocity_precond == "fsai":
+            ch.set_solver_option("preconditioner", 1, "velocity")
         return ch
 
     def solvers(method, one_level=False):
@@ -731,7 +738,8 @@ def main():
         for name in paths:
             ch = make_chorin(configs[name])
             ch.upload_velocity(u0)
-            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps, amg_pressure=name in amg_paths)
+            out["kernels"][name] = kernel_table(ch, None, args.kernel_reps, amg_pressure=name in amg_paths,
+                                    velocity_fsai=args.velocity_precond == "fsai")
             out[f"{name}_shape"] = ch.shape()
             del ch
         print(json.dumps(out))
@@ -750,7 +758,7 @@ def main():
         r["velocity_solver"] = cfg.velocitySolver.method
         # the per-kernel table for the two headline paths (the comparators are timed whole)
         kernels[name] = {} if (args.no_kernels or name not in ("fp64", "mixed")) else \
-            kernel_table(ch, r, amg_pressure=name in amg_paths)
+            kernel_table(ch, r, amg_pressure=name in amg_paths, velocity_fsai=args.velocity_precond == "fsai")
         results[name], firsts[name], lasts[name] = r, first, last
         del ch
 
@@ -851,6 +859,7 @@ def main():
                    "pressure_start": args.pressure_start,
                    "velocity_start": args.velocity_start,
                    "pressure_solver": args.pressure_solver,
+                   "velocity_preconditioner": args.velocity_precond,
                    "amg_chebyshev_from_degree_fraction": args.amg_chebyshev or "1,2,0.2 (library default)",
                    "seed": args.seed},
         "path": head,

@@ -209,6 +209,16 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  update workgroups hold at most two sweep slabs of rows
  *                  (rows per update workgroup x sweep lanes <= 512); 0: two
  *                  launches.
+ *   "preconditioner" pcg / pcg_mixed on one system with 1-2 right-hand
+ *                  sides (without mixedRestart): 0 (default) one symmetric
+ *                  multicolour GS sweep, the reference's; 1 the factorized
+ *                  sparse approximate inverse z = G^T G r (Kolotilina and
+ *                  Yeremin), G lower triangular on each row's lower-colour
+ *                  neighbours and itself, G_i* from M[P_i, P_i] g = e_i
+ *                  scaled to (G M G^T)_ii = 1, built on the host when first
+ *                  chosen: two parallel launches per application instead of
+ *                  2 colours - 2 dependent sweeps (the P2 mass matrix: 12
+ *                  iterations where GS takes 13).
  *   "amgChebyshevFrom", "amgChebyshevDegree", "amgChebyshevFraction"
  *                  amg_pcg: the V-cycle levels from this one on (but the
  *                  dense coarsest) are smoothed by this many steps of the

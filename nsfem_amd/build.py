@@ -17,7 +17,7 @@ HIP_SOURCES = ["ops.hip", "solver.hip", "amg_pcg.hip", "chorin.hip", "capi.hip",
 # The GPU assembly evaluates the element integrals the host assembly shares
 # with it; without FP contraction both run the same IEEE operations.
 HIP_EXTRA_FLAGS = {"assembly.hip": ["-ffp-contract=off"]}
-CPP_SOURCES = ["host/mesh.cpp", "host/csr.cpp", "host/fem.cpp", "host/context.cpp", "host/amg.cpp"]
+CPP_SOURCES = ["host/mesh.cpp", "host/csr.cpp", "host/fem.cpp", "host/context.cpp", "host/amg.cpp", "host/fsai.cpp"]
 APPS = ["navier_stokes", "solver_test"]
 HEADERS = ["common.hpp", "kernels.hpp", "ops.hpp", "solver.hpp", "chorin.hpp", "host/mesh.hpp", "host/csr.hpp",
            "host/fem.hpp", "host/context.hpp", "host/element_math.hpp", "assembly.hpp"]

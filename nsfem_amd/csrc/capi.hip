@@ -111,6 +111,11 @@ namespace
                 throw std::invalid_argument("lanesDirection must be 0, 1, 2, 4, 8 or 16");
             solver.setDirectionLanes(static_cast<int>(value));
         }
+        else if (k == "preconditioner")
+        {
+            if (value != 0 && value != 1) throw std::invalid_argument("preconditioner must be 0 or 1");
+            solver.setPreconditioner(static_cast<int>(value));
+        }
         else if (k == "amgChebyshevFrom" || k == "amgChebyshevDegree" || k == "amgChebyshevFraction")
         {
             AmgPcg * amg = solver.amg();

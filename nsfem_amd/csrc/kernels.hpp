@@ -1714,6 +1714,123 @@ namespace nsfem
         }
     }
 
+    // Factorized sparse approximate inverse preconditioner z = G^T G r (the
+    // factor built on the host: G's values share the matrix's column
+    // structure, G(i, j) for the lower-colour entries of row i, G(j, i) for
+    // its higher-colour entries, diag the diagonal of G; rowSplit splits the
+    // two).  Two launches replace the multicolour sweeps: t = G r over the
+    // lower parts, then z = G^T t over the higher parts with the partial
+    // sums of rho = r.z, stored as the rho kernel stores them.  Same block
+    // map as the rho kernel.  1-2 channels (lanes split a row).
+    template <typename P, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) fsaiLowerKernel(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ r,
+                                                              P * __restrict__ t)
+    {
+        const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
+        const int done = st.flags[0];
+        int row = range.x + static_cast<int>(threadIdx.x) / LANES;
+        bool valid = row < range.y;
+        int j0 = valid ? G.rowStart[row] : 0;
+        int j1 = valid ? G.rowSplit[row] : 0;
+        const bool act = systemActive<NCH>(st, range.z);
+        if (done | !act | (j1 < j0))
+        {
+            return;
+        }
+        for (int base = range.x; base < range.y; base += kBlock / LANES)
+        {
+            if (base != range.x)
+            {
+                row = base + static_cast<int>(threadIdx.x) / LANES;
+                valid = row < range.y;
+                j0 = valid ? G.rowStart[row] : 0;
+                j1 = valid ? G.rowSplit[row] : 0;
+            }
+            // the row's own operands on every lane, unconditionally (lanes
+            // past the block's rows read the slab's first row)
+            const int rs = valid ? row : base;
+            const P gd = G.diag[rs];
+            P rv[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                rv[c] = r[static_cast<long>(rs) * NCH + c];
+            }
+            P acc[NCH];
+            rowDot<P, LANES, NCH>(G, j0, j1, r, acc);
+            if (valid && groupLeader<LANES>())
+            {
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    t[static_cast<long>(row) * NCH + c] = fmaT(gd, rv[c], acc[c]);
+                }
+            }
+        }
+    }
+
+    template <typename P, typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) fsaiUpperKernel(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ t,
+                                                              P * __restrict__ z, const T * __restrict__ r)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
+        const int done = st.flags[0];
+        int row = range.x + static_cast<int>(threadIdx.x) / LANES;
+        bool valid = row < range.y;
+        int j0 = valid ? G.rowSplit[row] : 0;
+        int j1 = valid ? G.rowStart[row + 1] : 0;
+        const bool act = systemActive<NCH>(st, range.z);
+        if (done || j1 < j0)
+        {
+            return;
+        }
+        double part[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            part[c] = 0;
+        }
+        if (act)
+        {
+            for (int base = range.x; base < range.y; base += kBlock / LANES)
+            {
+                if (base != range.x)
+                {
+                    row = base + static_cast<int>(threadIdx.x) / LANES;
+                    valid = row < range.y;
+                    j0 = valid ? G.rowSplit[row] : 0;
+                    j1 = valid ? G.rowStart[row + 1] : 0;
+                }
+                const int rs = valid ? row : base;
+                const P gd = G.diag[rs];
+                P tv[NCH];
+                T rv[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; c++)
+                {
+                    const long i = static_cast<long>(rs) * NCH + c;
+                    tv[c] = t[i];
+                    rv[c] = r[i];
+                }
+                P acc[NCH];
+                rowDot<P, LANES, NCH>(G, j0, j1, t, acc);
+                if (valid && groupLeader<LANES>())
+                {
+#pragma unroll
+                    for (int c = 0; c < NCH; c++)
+                    {
+                        const P zi = fmaT(gd, tv[c], acc[c]);
+                        z[static_cast<long>(row) * NCH + c] = zi;
+                        part[c] = fma(static_cast<double>(rv[c]), static_cast<double>(zi), part[c]);
+                    }
+                }
+            }
+        }
+        blockSum<NCH>(part, sm);
+        storePartial<NCH>(part, st.partials, range.w);
+    }
+
     template <typename T, typename P, int LANES, int NCH>
     __global__ void __launch_bounds__(kBlock) pcgDirectionKernel(McgsMatrix<T> A, BlockMap bm, SolveState st, const P * __restrict__ z, T * __restrict__ p, T * __restrict__ q,
                                                                  int fused, int rhoParts, int updParts)

@@ -1,5 +1,7 @@
 #include "solver.hpp"
 
+#include "host/fsai.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <functional>
@@ -1106,19 +1108,22 @@ namespace nsfem
     template <>
     PcgOperands<double, double> McgsSolver::operands<double, double>()
     {
-        return {A64_.view(n_), A64_.view(n_), b64_.get(), x64_.get(), r64_.get(), r64_.get(), z64_.get(), p64_.get(), q64_.get()};
+        return {A64_.view(n_), A64_.view(n_), b64_.get(), x64_.get(), r64_.get(), r64_.get(), z64_.get(), p64_.get(), q64_.get(),
+                fsaiView<double>(), reinterpret_cast<double *>(fsaiT_.get())};
     }
 
     template <>
     PcgOperands<float, float> McgsSolver::operands<float, float>()
     {
-        return {A32_.view(n_), A32_.view(n_), b32_.get(), x32_.get(), r32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get()};
+        return {A32_.view(n_), A32_.view(n_), b32_.get(), x32_.get(), r32_.get(), r32_.get(), z32_.get(), p32_.get(), q32_.get(),
+                fsaiView<float>(), reinterpret_cast<float *>(fsaiT_.get())};
     }
 
     template <>
     PcgOperands<double, float> McgsSolver::operands<double, float>()
     {
-        return {A64_.view(n_), A32_.view(n_), b64_.get(), x64_.get(), r64_.get(), r32_.get(), z32_.get(), p64_.get(), q64_.get()};
+        return {A64_.view(n_), A32_.view(n_), b64_.get(), x64_.get(), r64_.get(), r32_.get(), z32_.get(), p64_.get(), q64_.get(),
+                fsaiView<float>(), reinterpret_cast<float *>(fsaiT_.get())};
     }
 
     template <typename T, typename P>
@@ -1237,7 +1242,7 @@ namespace nsfem
         const int fz = fused() ? 1 : 0;
         dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                    {
-                       pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, nbRho_, nbUpd_);
+                       pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, rhoParts(), nbUpd_);
                        if (!fz)
                        {
                            pcgDirectionFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals);
@@ -1252,12 +1257,126 @@ namespace nsfem
             dispatchC(nch, [&]<int C>()
                       {
                           pcgUpdateKernel<T, P, C><<<nbUpd_, kBlock, 0, s>>>(n_, bmUpd_, st, v.p, v.q, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                            color0End_, fz, nbDir_);
+                                                                            zColour0End(), fz, nbDir_);
                       });
         }
         // the update's convergence test is finished together with rho
-        sgsSweeps<P>(s, v.M, st, v.rP, v.z, mergedUpdate() ? 2 : 1);
-        launchRho<T, P>(s, v, st, totals, true);
+        launchPrecond<T, P>(s, v, st, totals, true);
+    }
+
+    template <typename T, typename P>
+    void McgsSolver::launchPrecond(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate)
+    {
+        if (precond_ == 0)
+        {
+            sgsSweeps<P>(s, v.M, st, v.rP, v.z, withUpdate && mergedUpdate() ? 2 : 1);
+            launchRho<T, P>(s, v, st, totals, withUpdate);
+            return;
+        }
+        // on the direction kernel's map (region-major: the workgroups of one
+        // XCD take the same stretch of every colour and gather from about an
+        // eighth of the vector); the next direction kernel reads nbDir_ rho
+        // partials
+        const bool fin = !fused();
+        dispatchLC(directionLanes(), opt_.numCh, [&]<int L, int C>()
+                   {
+                       if constexpr (!kWide<C>)
+                       {
+                           fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
+                           fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
+                           if (fin)
+                           {
+                               pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
+                           }
+                       }
+                   });
+    }
+
+    template <typename P>
+    McgsMatrix<P> McgsSolver::fsaiView() const
+    {
+        // the structure of the preconditioner's matrix, the factor's values
+        if constexpr (std::is_same_v<P, double>)
+        {
+            McgsMatrix<double> G = A64_.view(n_);
+            G.values = fsai64_.get();
+            G.diag = fsaiDiag64_.get();
+            G.invDiag = nullptr;
+            return G;
+        }
+        else
+        {
+            McgsMatrix<float> G = A32_.view(n_);
+            G.values = fsai32_.get();
+            G.diag = fsaiDiag32_.get();
+            G.invDiag = nullptr;
+            return G;
+        }
+    }
+
+    void McgsSolver::setPreconditioner(int kind)
+    {
+        if (kind != 0 && kind != 1)
+        {
+            throw std::invalid_argument("preconditioner must be 0 (symmetric multicolour GS) or 1 (factorized approximate inverse)");
+        }
+        if (kind == 1)
+        {
+            const bool pcg = opt_.method == SolverMethod::Pcg;
+            if (!pcg || opt_.restart || numSystems_ != 1 || opt_.numCh > 2)
+            {
+                throw std::invalid_argument("the factorized approximate inverse preconditions pcg / pcg_mixed (without "
+                                            "mixedRestart) on one system with 1 or 2 right-hand sides");
+            }
+            const bool p64 = opt_.precision == SolverPrecision::F64;
+            if ((p64 && !fsai64_.get()) || (!p64 && !fsai32_.get()))
+            {
+                // the colour-major matrix from the device (its FP64 copy where there is one)
+                std::vector<int> rowStart, rowSplit, column;
+                std::vector<double> values, diag;
+                if (A64_.values.get())
+                {
+                    rowStart = A64_.rowStart.download();
+                    rowSplit = A64_.rowSplit.download();
+                    column = A64_.column.download();
+                    values = A64_.values.download();
+                    diag = A64_.diag.download();
+                }
+                else
+                {
+                    rowStart = A32_.rowStart.download();
+                    rowSplit = A32_.rowSplit.download();
+                    column = A32_.column.download();
+                    const std::vector<float> v32 = A32_.values.download(), d32 = A32_.diag.download();
+                    values.assign(v32.begin(), v32.end());
+                    diag.assign(d32.begin(), d32.end());
+                }
+                std::vector<double> g, gd;
+                buildFsai(n_, rowStart, rowSplit, column, values, diag, g, gd);
+                if (column.empty())
+                {
+                    g.assign(1, 0.0); // a diagonal matrix: keep a valid pointer
+                }
+                if (p64)
+                {
+                    fsai64_.upload(g);
+                    fsaiDiag64_.upload(gd);
+                }
+                else
+                {
+                    fsai32_.upload(std::vector<float>(g.begin(), g.end()));
+                    fsaiDiag32_.upload(std::vector<float>(gd.begin(), gd.end()));
+                }
+            }
+            const size_t bytes = static_cast<size_t>(n_) * opt_.numCh * (p64 ? 8 : 4);
+            if (fsaiT_.size() < bytes)
+            {
+                fsaiT_.resize(bytes);
+            }
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        precond_ = kind;
+        destroyGraphs();
     }
 
     template <typename T, typename P>
@@ -1270,11 +1389,10 @@ namespace nsfem
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
                        pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                              color0End_, zeroGuess ? 1 : 0);
+                                                                              zColour0End(), zeroGuess ? 1 : 0);
                        pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, gate, totals);
                    });
-        sgsSweeps<P>(s, v.M, st, v.rP, v.z);
-        launchRho<T, P>(s, v, st, totals, false);
+        launchPrecond<T, P>(s, v, st, totals, false);
         NSFEM_HIP(hipGetLastError());
         return iterate<T, P>(s, v, st, maxIters, sync, usePrediction);
     }
@@ -1754,11 +1872,10 @@ namespace nsfem
         dispatchLC(lanes_, nch, [&]<int L, int C>()
                    {
                        pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                              color0End_, zeroGuess ? 1 : 0);
+                                                                              zColour0End(), zeroGuess ? 1 : 0);
                        pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, nullptr, totals);
                    });
-        sgsSweeps<P>(s, v.M, st, v.rP, v.z);
-        launchRho<T, P>(s, v, st, totals, false);
+        launchPrecond<T, P>(s, v, st, totals, false);
         for (int i = 0; i < budget; i++)
         {
             pcgIteration<T, P>(s, v, st);
@@ -1988,7 +2105,7 @@ namespace nsfem
                 {
                     dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                                {
-                                   pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, nbRho_,
+                                   pcgDirectionKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.z, v.p, v.q, fz, rhoParts(),
                                                                                             nbUpd_);
                                });
                 };
@@ -2039,6 +2156,36 @@ namespace nsfem
                                {
                                    pcgRhoKernel<P, T, L, C><<<nbRho_, kBlock, 0, s>>>(v.M, bmRho_, st, v.rP, v.r, v.z, color0End_,
                                                                                   color0BlocksRho_, v.q, fix);
+                               });
+                };
+            }
+            else if (which == "fsai")
+            {
+                // the factorized approximate inverse's two launches (G r over
+                // the lower parts, G^T t over the higher parts with r.z): every
+                // off-diagonal entry once over both, row bounds and the
+                // factor's diagonal in each, r, t, t, z in P and r in T once
+                // (gathered columns read once)
+                if (precond_ != 1)
+                {
+                    throw std::invalid_argument("fsai: the solver's preconditioner is not the approximate inverse");
+                }
+                double full = 0;
+                for (const ColorStats & c : colorStats_)
+                {
+                    full += c.fullNnz;
+                }
+                launches = 2;
+                bytes = (full * (tsP + 4.0) + 2.0 * ((n + 1) * 4.0 + n * 4.0 + n * tsP) + nch * n * (4.0 * tsP + tsT)) / 2.0;
+                launch = [=, this]()
+                {
+                    dispatchLC(directionLanes(), nch, [&]<int L, int C>()
+                               {
+                                   if constexpr (!kWide<C>)
+                                   {
+                                       fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
+                                       fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
+                                   }
                                });
                 };
             }
@@ -2174,8 +2321,8 @@ namespace nsfem
         {
             *launchesPerIteration = launches;
         }
-        // "sweep" launches several kernels per repetition
-        const double perRep = which == "sweep" ? launches : 1.0;
+        // "sweep" and "fsai" launch several kernels per repetition
+        const double perRep = which == "sweep" || which == "fsai" ? launches : 1.0;
         return static_cast<double>(ms) / (reps * perRep);
     }
 

@@ -115,6 +115,10 @@ namespace nsfem
         P * z;
         T * p;
         T * q;
+        // the factorized approximate inverse (values null unless it is the
+        // preconditioner) and its intermediate vector t = G r
+        McgsMatrix<P> G;
+        P * t;
     };
 
     class McgsSolver
@@ -216,9 +220,17 @@ namespace nsfem
         void setFusedUpdateSweep(int on);
         bool mergedUpdate() const
         {
-            return mergeMode_ && fused() && flat() && opt_.numCh <= 2 && numColors_ >= 2 &&
+            return mergeMode_ && fused() && flat() && opt_.numCh <= 2 && numColors_ >= 2 && precond_ == 0 &&
                    bmUpd_.rowsPerBlock * sweepLanes_ <= 2 * kBlock;
         }
+        // PCG preconditioner: 0 one symmetric multicolour GS sweep (the
+        // reference's); 1 the factorized sparse approximate inverse
+        // G^T G ~ A^-1 on the lower-colour pattern plus the diagonal (built on
+        // the host when first chosen; pcg methods without mixedRestart, one
+        // system, 1-2 channels): two launches per application instead of
+        // 2 colours - 2 dependent sweeps.
+        void setPreconditioner(int kind);
+        int preconditioner() const { return precond_; }
         int regionedDirection() const { return regionDir_; }
         // Fewest rows per workgroup of the default and autotuned shapes.
         int minRowsPerBlock() const;
@@ -287,6 +299,18 @@ namespace nsfem
         void launchRho(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate);
         template <typename T, typename P>
         void launchUpdateSweep(hipStream_t s, const PcgOperands<T, P> & v, SolveState st);
+        // z = preconditioner(rP) with the rho = r.z partial sums (and the
+        // rho finalize without fused reductions)
+        template <typename T, typename P>
+        void launchPrecond(hipStream_t s, const PcgOperands<T, P> & v, SolveState st, double * totals, bool withUpdate);
+        // colour-0 rows whose z the init and update kernels set to r / d
+        // (the start of the forward sweep; none for the approximate inverse)
+        int zColour0End() const { return precond_ == 0 ? color0End_ : 0; }
+        // blocks whose rho partials the next direction kernel sums (the
+        // approximate inverse runs on the direction kernel's map)
+        int rhoParts() const { return precond_ == 0 ? nbRho_ : nbDir_; }
+        template <typename P>
+        McgsMatrix<P> fsaiView() const;
         void runMixedRestart(hipStream_t s);
         template <typename P>
         void runBlockPcg(hipStream_t s, const McgsMatrix<P> & M, P * rP, P * z, bool zeroGuess);
@@ -379,6 +403,11 @@ namespace nsfem
         int lastOuter_ = 0;
         bool lastNonFinite_ = false;
         hipStream_t stream_;
+
+        int precond_ = 0;
+        DeviceArray<double> fsai64_, fsaiDiag64_;
+        DeviceArray<float> fsai32_, fsaiDiag32_;
+        DeviceArray<char> fsaiT_;
 
         std::unique_ptr<AmgPcg> amg_;
         ChunkGraph graph64_, graph32_, graphMixed_;
