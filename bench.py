@@ -601,9 +601,12 @@ def main():
     ap.add_argument("--tau", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=20241103)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--paths", default="fp64,mixed,fp64_mcgs,mixed_mcgs",
-                    help="comma list of fp64, mixed (pressure preconditioner of --pressure-solver) and fp64_mcgs, "
-                         "mixed_mcgs (the one-level multicolour GS preconditioner)")
+    ap.add_argument("--paths", default="fp64,mixed,fp64_fast,fp64_mcgs,mixed_mcgs",
+                    help="comma list of fp64 (the plain path: the reference's velocity solves and starts), mixed (the "
+                         "headline path: FP32 preconditioners, --velocity-precond, --velocity-start, --pressure-start), "
+                         "fp64_fast (the mixed path's algorithm in FP64), fp64_mcgs, mixed_mcgs (plain, with the "
+                         "one-level multicolour GS pressure preconditioner); fp64 and mixed use the pressure "
+                         "preconditioner of --pressure-solver")
     ap.add_argument("--shape", default="0,0,0,0",
                     help="launch shape velocityLanes,velocityRowsPerBlock,pressureLanes,pressureRowsPerBlock (0 = default), "
                          "optionally followed by the same four for the colour sweeps and then by the velocity and "
@@ -693,7 +696,7 @@ def main():
     dir_lanes = shape[14:16] if len(shape) >= 16 else None
     fused_modes = shape[16:18] if len(shape) == 18 else None
 
-    def make_chorin(cfg):
+    def make_chorin(cfg, fast=True):
         ch = fem.Chorin(ctx, cfg)
         if fused_modes is not None:
             for which, v in zip(("velocity", "pressure"), fused_modes):
@@ -706,7 +709,7 @@ def main():
             for which, v in zip(("velocity", "pressure"), dir_lanes):
                 ch.set_solver_option("lanesDirection", v, which)
         apply_amg_smoother(args, ch, cfg)
-        if args.velocity_precond == "fsai":
+        if fast and args.velocity_precond == "fsai":
             ch.set_solver_option("preconditioner", 1, "velocity")
         return ch
 
@@ -719,15 +722,21 @@ def main():
     sg = 1 if args.step_graph else 0
     ps = fem.PRESSURE_START_EXTRAPOLATE if args.pressure_start == "extrapolate" else fem.PRESSURE_START_LAST
     vs = velocity_start_code(args.velocity_start)
-    def config(method, one_level=False):
+    def config(method, one_level=False, fast=False):
+        # the plain paths keep the reference's velocity solves (zero start, the
+        # GS sweep) and pressure start, so that the FP64 one matches the
+        # reference to 1e-10; the fast ones take the options
         return fem.ChorinConfig(*solvers(method, one_level), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP,
-                                stepGraph=sg, pressureStart=ps, velocityStart=vs)
+                                stepGraph=sg, pressureStart=ps if fast else fem.PRESSURE_START_LAST,
+                                velocityStart=vs if fast else fem.VELOCITY_START_ZERO)
 
-    # fp64 / mixed: the pressure preconditioner of --pressure-solver; the
-    # *_mcgs paths: one symmetric multicolour GS sweep (the one-level pcg)
-    configs = {"fp64": config("pcg"), "mixed": config("pcg_mixed"), "fp64_mcgs": config("pcg", True),
-               "mixed_mcgs": config("pcg_mixed", True)}
-    amg_paths = {"fp64", "mixed"} if args.pressure_solver == "amg" else set()
+    # fp64 / mixed / fp64_fast: the pressure preconditioner of
+    # --pressure-solver; the *_mcgs paths: one symmetric multicolour GS sweep
+    # (the one-level pcg)
+    configs = {"fp64": config("pcg"), "mixed": config("pcg_mixed", fast=True), "fp64_fast": config("pcg", fast=True),
+               "fp64_mcgs": config("pcg", True), "mixed_mcgs": config("pcg_mixed", True)}
+    fast_paths = {"mixed", "fp64_fast"}
+    amg_paths = {"fp64", "mixed", "fp64_fast"} if args.pressure_solver == "amg" else set()
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     for p in paths:
         if p not in configs:
@@ -736,10 +745,10 @@ def main():
         out = {"mode": "kernels-only", "config": {"workload": "chorin_step_channel_500k", "nodes": args.nodes},
                "kernels": {}}
         for name in paths:
-            ch = make_chorin(configs[name])
+            ch = make_chorin(configs[name], name in fast_paths)
             ch.upload_velocity(u0)
             out["kernels"][name] = kernel_table(ch, None, args.kernel_reps, amg_pressure=name in amg_paths,
-                                    velocity_fsai=args.velocity_precond == "fsai")
+                                    velocity_fsai=name in fast_paths and args.velocity_precond == "fsai")
             out[f"{name}_shape"] = ch.shape()
             del ch
         print(json.dumps(out))
@@ -750,7 +759,8 @@ def main():
     for name in paths:
         cfg = configs[name]
         # the library's guard clamps the requested tau to 1 / lambda_max(M^-1 nu K)
-        ch, r, first, last = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune, make_chorin)
+        ch, r, first, last = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune,
+                                      lambda c, f=name in fast_paths: make_chorin(c, f))
         ti = r["time_step"]
         log(f"{name}: tau requested {ti['tauRequested']:g}, used {ti['tauUsed']:.4g} "
             f"(lambda_max(M^-1 nu K) = {ti['lambdaNu']:.4g})")
@@ -758,7 +768,8 @@ def main():
         r["velocity_solver"] = cfg.velocitySolver.method
         # the per-kernel table for the two headline paths (the comparators are timed whole)
         kernels[name] = {} if (args.no_kernels or name not in ("fp64", "mixed")) else \
-            kernel_table(ch, r, amg_pressure=name in amg_paths, velocity_fsai=args.velocity_precond == "fsai")
+            kernel_table(ch, r, amg_pressure=name in amg_paths,
+                         velocity_fsai=name in fast_paths and args.velocity_precond == "fsai")
         results[name], firsts[name], lasts[name] = r, first, last
         del ch
 
@@ -795,6 +806,9 @@ def main():
     if "fp64" in results and "mixed" in results:
         errors["mixed_vs_fp64_after_steps"] = {"velocity": rel(lasts["mixed"][0], lasts["fp64"][0]),
                                                "pressure": rel(lasts["mixed"][1], lasts["fp64"][1])}
+    if "fp64_fast" in results and "mixed" in results:
+        errors["mixed_vs_fp64_fast_after_steps"] = {"velocity": rel(lasts["mixed"][0], lasts["fp64_fast"][0]),
+                                                    "pressure": rel(lasts["mixed"][1], lasts["fp64_fast"][1])}
     if "fp64" in results and "fp64_mcgs" in results:
         errors["fp64_vs_fp64_mcgs_after_steps"] = {"velocity": rel(lasts["fp64"][0], lasts["fp64_mcgs"][0]),
                                                    "pressure": rel(lasts["fp64"][1], lasts["fp64_mcgs"][1])}
@@ -853,19 +867,23 @@ def main():
                    "reynolds": 100, "peak_inflow": cond.peakVelocity, "viscosity": cond.viscosity,
                    "tau_requested": args.tau, "tau_used": tau, "lambda_max": lam, "solver_rtol": target,
                    "mixed_scheme": "FP64 Krylov vectors, SpMV and reductions; FP32 preconditioners",
-                   "preconditioners": {"velocity": "one symmetric multicolour GS sweep",
+                   "preconditioners": {"velocity": {"fast paths (mixed, fp64_fast)": "factorized sparse approximate inverse"
+                                                    if args.velocity_precond == "fsai" else "one symmetric multicolour GS sweep",
+                                                    "plain paths (fp64, fp64_mcgs, mixed_mcgs)": "one symmetric multicolour GS sweep"},
                                        "pressure": ("smoothed-aggregation V-cycle, multicolour GS smoothing on the finest level, Chebyshev on the coarser"
                                                     if args.pressure_solver == "amg" else "one symmetric multicolour GS sweep")},
-                   "pressure_start": args.pressure_start,
-                   "velocity_start": args.velocity_start,
+                   "starts": {"fast paths": {"velocity": args.velocity_start, "pressure": args.pressure_start},
+                              "plain paths": {"velocity": "zero", "pressure": "last"}},
                    "pressure_solver": args.pressure_solver,
-                   "velocity_preconditioner": args.velocity_precond,
                    "amg_chebyshev_from_degree_fraction": args.amg_chebyshev or "1,2,0.2 (library default)",
                    "seed": args.seed},
         "path": head,
         "fp64": r64,
         "mixed": results.get("mixed"),
         "mixed_over_fp64": (results["mixed"]["steps_per_s"] / r64["steps_per_s"]) if (r64 and "mixed" in results) else None,
+        "fp64_fast": results.get("fp64_fast"),
+        "mixed_over_fp64_fast": (results["mixed"]["steps_per_s"] / results["fp64_fast"]["steps_per_s"])
+        if ("mixed" in results and "fp64_fast" in results) else None,
         "errors": errors,
         "roofline": roofline,
         "roofline_llc_exceeding": roofline_llc,

@@ -5,7 +5,8 @@
 // src/nsConfig.cpp:8-45, utils ConfigParser key = value with '#' comments).
 // Frames are written as linalg .bin vectors (velocity_<j>.bin = [ux; uy],
 // pressure_<j>.bin) instead of rendered images.  Extra keys: hip.dtype
-// (f64 | f32), hip.innerTol (pcg_mixed).
+// (f64 | f32), hip.innerTol (pcg_mixed), hip.timeStepPolicy, hip.pressureStart,
+// hip.velocityStart, hip.velocityPreconditioner (gs | fsai), hip.autotune.
 #include <cmath>
 #include <cstdio>
 #include <filesystem>
@@ -152,6 +153,18 @@ int main(int argc, char ** argv)
         }
         nsfem_chorin * ch = nullptr;
         cu::check(nsfem_chorin_create(ctx, &cc, &ch));
+        // the velocity solves' preconditioner (pcg methods): the reference's
+        // symmetric GS sweep or the factorized approximate inverse
+        const std::string vprec = cfg.str("hip.velocityPreconditioner", "gs");
+        if (vprec == "fsai")
+        {
+            cu::check(nsfem_chorin_set_solver_option(ch, 0, "preconditioner", 1.0));
+        }
+        else if (vprec != "gs")
+        {
+            std::cerr << "hip.velocityPreconditioner must be gs or fsai\n";
+            return 1;
+        }
         // launch shapes tuned once before the first step (0: the defaults)
         const int tuneReps = static_cast<int>(cfg.num("hip.autotune", 10));
         if (tuneReps > 0)

@@ -310,7 +310,10 @@ namespace nsfem
         {
             return;
         }
-        const int g = elementwiseGrid(total);
+        // one element per thread (each is a dependent index load then a
+        // gather or scatter; a capped grid's second round would wait for the
+        // first): the full grid
+        const int g = static_cast<int>(std::min<long>((total + 255) / 256, 1L << 30));
         const int inv = inverse ? 1 : 0;
         if (srcDt == F64 && dstDt == F64)
         {
