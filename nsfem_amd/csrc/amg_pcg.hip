@@ -783,7 +783,7 @@ namespace nsfem
         }
         // Gauss-Seidel on the finest level, two Chebyshev steps over
         // [0.2, 1] x lambda_max on the coarser ones (measured at 500k rows:
-        // 52 -> 29 launches per V-cycle, 18.3 -> 15.2 CG iterations)
+        // 52 -> 24 launches per V-cycle, 18.3 -> 15.2 CG iterations)
         setSmoother(1, 2, 0.2);
         setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
