@@ -123,6 +123,8 @@ def ensemble_chorin(args, ctx, cfg, B, scales):
     from nsfem_amd import fem
     ch = fem.Chorin(ctx, cfg, instances=B, inflow_scale=scales)
     apply_amg_smoother(args, ch, cfg)
+    if args.velocity_precond == "fsai":
+        ch.set_solver_option("preconditioner", 1, "velocity")
     return ch
 
 

@@ -1721,10 +1721,11 @@ namespace nsfem
     // two).  Two launches replace the multicolour sweeps: t = G r over the
     // lower parts, then z = G^T t over the higher parts with the partial
     // sums of rho = r.z, stored as the rho kernel stores them.  Same block
-    // map as the rho kernel.  1-2 channels (lanes split a row).
+    // map as the direction kernel.  1-2 channels split a row over lanes;
+    // wide tiles run one lane per channel (below).
     template <typename P, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) fsaiLowerKernel(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ r,
-                                                              P * __restrict__ t)
+    __device__ __forceinline__ void fsaiLowerNarrow(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ r,
+                                                    P * __restrict__ t)
     {
         const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
         const int done = st.flags[0];
@@ -1770,8 +1771,8 @@ namespace nsfem
     }
 
     template <typename P, typename T, int LANES, int NCH>
-    __global__ void __launch_bounds__(kBlock) fsaiUpperKernel(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ t,
-                                                              P * __restrict__ z, const T * __restrict__ r)
+    __device__ __forceinline__ void fsaiUpperNarrow(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ t,
+                                                    P * __restrict__ z, const T * __restrict__ r)
     {
         __shared__ double sm[NCH * kWaves];
         const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
@@ -1829,6 +1830,119 @@ namespace nsfem
         }
         blockSum<NCH>(part, sm);
         storePartial<NCH>(part, st.partials, range.w);
+    }
+
+    // Wide tiles (4-32 channels): one lane per channel walks the whole
+    // (lower or higher) part of its row, as the other wide kernels.
+    template <typename P, int NCH>
+    __device__ __forceinline__ void fsaiLowerWide(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ r,
+                                                  P * __restrict__ t)
+    {
+        const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
+        const int done = st.flags[0];
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        int row = range.x + static_cast<int>(threadIdx.x) / NCH;
+        bool valid = row < range.y;
+        int j0 = valid ? G.rowStart[row] : 0;
+        int j1 = valid ? G.rowSplit[row] : 0;
+        const bool act = systemActive<NCH>(st, range.z);
+        if (done | !act | (j1 < j0))
+        {
+            return;
+        }
+        for (int base = range.x; base < range.y; base += kBlock / NCH)
+        {
+            if (base != range.x)
+            {
+                row = base + static_cast<int>(threadIdx.x) / NCH;
+                valid = row < range.y;
+                j0 = valid ? G.rowStart[row] : 0;
+                j1 = valid ? G.rowSplit[row] : 0;
+            }
+            const int rs = valid ? row : base;
+            const P gd = G.diag[rs];
+            const P rv = r[static_cast<long>(rs) * NCH + c];
+            const P acc = rowDotWide<P, NCH>(G, j0, j1, r, c);
+            if (valid)
+            {
+                t[static_cast<long>(row) * NCH + c] = fmaT(gd, rv, acc);
+            }
+        }
+    }
+
+    template <typename P, typename T, int NCH>
+    __device__ __forceinline__ void fsaiUpperWide(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ t,
+                                                  P * __restrict__ z, const T * __restrict__ r)
+    {
+        __shared__ double sm[NCH * kWaves];
+        const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
+        const int done = st.flags[0];
+        const int c = static_cast<int>(threadIdx.x) % NCH;
+        int row = range.x + static_cast<int>(threadIdx.x) / NCH;
+        bool valid = row < range.y;
+        int j0 = valid ? G.rowSplit[row] : 0;
+        int j1 = valid ? G.rowStart[row + 1] : 0;
+        const bool act = systemActive<NCH>(st, range.z);
+        if (done || j1 < j0)
+        {
+            return;
+        }
+        double part = 0;
+        if (act)
+        {
+            for (int base = range.x; base < range.y; base += kBlock / NCH)
+            {
+                if (base != range.x)
+                {
+                    row = base + static_cast<int>(threadIdx.x) / NCH;
+                    valid = row < range.y;
+                    j0 = valid ? G.rowSplit[row] : 0;
+                    j1 = valid ? G.rowStart[row + 1] : 0;
+                }
+                const int rs = valid ? row : base;
+                const long is = static_cast<long>(rs) * NCH + c;
+                const P gd = G.diag[rs];
+                const P tv = t[is];
+                const T rv = r[is];
+                const P acc = rowDotWide<P, NCH>(G, j0, j1, t, c);
+                if (valid)
+                {
+                    const P zi = fmaT(gd, tv, acc);
+                    z[static_cast<long>(row) * NCH + c] = zi;
+                    part = fma(static_cast<double>(rv), static_cast<double>(zi), part);
+                }
+            }
+        }
+        const double total = channelBlockSum<NCH>(part, sm);
+        storeChannelPartial<NCH, NCH>(total, st.partials, range.w, 0);
+    }
+
+    template <typename P, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) fsaiLowerKernel(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ r,
+                                                              P * __restrict__ t)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            fsaiLowerWide<P, NCH>(G, bm, st, r, t);
+        }
+        else
+        {
+            fsaiLowerNarrow<P, LANES, NCH>(G, bm, st, r, t);
+        }
+    }
+
+    template <typename P, typename T, int LANES, int NCH>
+    __global__ void __launch_bounds__(kBlock) fsaiUpperKernel(McgsMatrix<P> G, BlockMap bm, SolveState st, const P * __restrict__ t,
+                                                              P * __restrict__ z, const T * __restrict__ r)
+    {
+        if constexpr (kWide<NCH>)
+        {
+            fsaiUpperWide<P, T, NCH>(G, bm, st, t, z, r);
+        }
+        else
+        {
+            fsaiUpperNarrow<P, T, LANES, NCH>(G, bm, st, t, z, r);
+        }
     }
 
     template <typename T, typename P, int LANES, int NCH>

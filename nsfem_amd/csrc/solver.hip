@@ -1280,14 +1280,11 @@ namespace nsfem
         const bool fin = !fused();
         dispatchLC(directionLanes(), opt_.numCh, [&]<int L, int C>()
                    {
-                       if constexpr (!kWide<C>)
+                       fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
+                       fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
+                       if (fin)
                        {
-                           fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
-                           fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
-                           if (fin)
-                           {
-                               pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
-                           }
+                           pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
                        }
                    });
     }
@@ -1323,10 +1320,10 @@ namespace nsfem
         if (kind == 1)
         {
             const bool pcg = opt_.method == SolverMethod::Pcg;
-            if (!pcg || opt_.restart || numSystems_ != 1 || opt_.numCh > 2)
+            if (!pcg || opt_.restart || numSystems_ != 1)
             {
                 throw std::invalid_argument("the factorized approximate inverse preconditions pcg / pcg_mixed (without "
-                                            "mixedRestart) on one system with 1 or 2 right-hand sides");
+                                            "mixedRestart) on one system");
             }
             const bool p64 = opt_.precision == SolverPrecision::F64;
             if ((p64 && !fsai64_.get()) || (!p64 && !fsai32_.get()))
@@ -2181,11 +2178,8 @@ namespace nsfem
                 {
                     dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                                {
-                                   if constexpr (!kWide<C>)
-                                   {
-                                       fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
-                                       fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
-                                   }
+                                   fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
+                                   fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
                                });
                 };
             }

@@ -227,8 +227,8 @@ namespace nsfem
         // reference's); 1 the factorized sparse approximate inverse
         // G^T G ~ A^-1 on the lower-colour pattern plus the diagonal (built on
         // the host when first chosen; pcg methods without mixedRestart, one
-        // system, 1-2 channels): two launches per application instead of
-        // 2 colours - 2 dependent sweeps.
+        // system, any channel count): two launches per application instead
+        // of 2 colours - 2 dependent sweeps.
         void setPreconditioner(int kind);
         int preconditioner() const { return precond_; }
         int regionedDirection() const { return regionDir_; }

@@ -162,3 +162,26 @@ def test_ensemble_rejects_bad_shapes():
         fem.Chorin(_ctx(), cfg, instances=2, inflow_scale=[1.0])
     with pytest.raises(NsfemError, match="finite"):
         fem.Chorin(_ctx(), cfg, instances=2, inflow_scale=[1.0, float("nan")])
+
+
+@pytest.mark.parametrize("velocity,pressure,B,target,bounds", [("pcg", "amg_pcg", 4, 1e-13, (1e-10, 1e-10)),
+                                                              ("pcg_mixed", "amg_pcg_mixed", 8, 1e-10, (1e-8, 1e-7))])
+def test_ensemble_with_fsai_velocity_matches_oracle(oracle, velocity, pressure, B, target, bounds):
+    """The velocity solves of an ensemble (2B columns, one lane per column)
+    preconditioned by the factorized approximate inverse: every instance
+    against its own oracle run."""
+    from nsfem_amd import fem
+    tau = 1e-3
+    ch = fem.Chorin(_ctx(), _cfg((velocity, 2000, target, 1), (pressure, 500, target, 1)), instances=B,
+                    inflow_scale=SCALES[:B])
+    ch.set_solver_option("preconditioner", 1, "velocity")
+    ch.set_time_step(tau)
+    orcs = _oracles(oracle, SCALES[:B], tau)
+    for _ in range(3):
+        st = ch.step()
+        assert st["relTent"] <= target and st["relFinal"] <= target
+        outs = [orc.step() for orc in orcs]
+    u, p = _split(ch, *ch.download())
+    for b, (uo, po) in enumerate(outs):
+        eu, ep = _rel(u[b], uo), _rel(p[b], po)
+        assert eu <= bounds[0] and ep <= bounds[1], (b, eu, ep)
