@@ -341,9 +341,15 @@ def batch_leg(args, count, step_graph, with_kernels):
     for name, method in (("fp64", "pcg"), ("mixed", "pcg_mixed")):
         cfg = fem.ChorinConfig(fem.SolverConfig(method, 5000, target, 1), fem.SolverConfig(method, 50000, target, 1), dtype=F64,
                                timeStepPolicy=fem.TAU_CLAMP, stepGraph=1 if step_graph else 0)
+        def make(c):
+            ch = fem.Chorin(ctx, c)
+            if args.batch_precond == "fsai":
+                ch.set_solver_option("preconditioner", 1, "both")
+            return ch
         # the guard clamps tau to the batch's limit (the tightest instance's)
-        ch, r, _, _ = run_path(f"batch{count}_{name}", ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune)
-        r["kernels"] = kernel_table(ch, r, args.kernel_reps) if with_kernels else {}
+        ch, r, _, _ = run_path(f"batch{count}_{name}", ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune,
+                               make)
+        r["kernels"] = kernel_table(ch, r, args.kernel_reps, velocity_fsai=args.batch_precond == "fsai") if with_kernels else {}
         del ch
         r["instance_steps_per_s"] = r["steps_per_s"] * count
         r["pressure_dof_steps_per_s"] = r["steps_per_s"] * ctx.numPressureNodes
@@ -353,6 +359,8 @@ def batch_leg(args, count, step_graph, with_kernels):
             "tau_requested": args.tau, "tau_used": res["mixed"]["time_step"]["tauUsed"],
             "lambda_max_nu": res["mixed"]["time_step"]["lambdaNu"], "solver_rtol": target, "seed": args.seed,
             "solvers": "pcg / pcg_mixed (per-instance convergence in one block-diagonal solve)",
+            "preconditioner": "factorized sparse approximate inverse (block-diagonal)" if args.batch_precond == "fsai"
+            else "one symmetric multicolour GS sweep",
             "instances_detail": info, "fp64": res["fp64"], "mixed": res["mixed"],
             "mixed_over_fp64": res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], "setup_s": setup_s,
             "_ctx": ctx}
@@ -646,6 +654,9 @@ def main():
     ap.add_argument("--velocity-precond", default="fsai", choices=["sgs", "fsai"],
                     help="velocity solves' preconditioner (single flows): one symmetric multicolour GS sweep (as the "
                          "reference) or the factorized sparse approximate inverse (default)")
+    ap.add_argument("--batch-precond", default="fsai", choices=["sgs", "fsai"],
+                    help="the batches' (C4) velocity and pressure preconditioner: one symmetric multicolour GS sweep or "
+                         "the factorized sparse approximate inverse (default)")
     ap.add_argument("--amg-chebyshev", default="",
                     help="AMG pressure solves: FROM[,DEGREE[,FRACTION]] = the options amgChebyshevFrom, "
                          "amgChebyshevDegree, amgChebyshevFraction (default: the library's)")

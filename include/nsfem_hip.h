@@ -209,8 +209,8 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  update workgroups hold at most two sweep slabs of rows
  *                  (rows per update workgroup x sweep lanes <= 512); 0: two
  *                  launches.
- *   "preconditioner" pcg / pcg_mixed on one system (without
- *                  mixedRestart; any channel count): 0 (default) one symmetric
+ *   "preconditioner" pcg / pcg_mixed (without mixedRestart; any
+ *                  channel count, batches too): 0 (default) one symmetric
  *                  multicolour GS sweep, the reference's; 1 the factorized
  *                  sparse approximate inverse z = G^T G r (Kolotilina and
  *                  Yeremin), G lower triangular on each row's lower-colour

@@ -1320,10 +1320,10 @@ namespace nsfem
         if (kind == 1)
         {
             const bool pcg = opt_.method == SolverMethod::Pcg;
-            if (!pcg || opt_.restart || numSystems_ != 1)
+            if (!pcg || opt_.restart)
             {
                 throw std::invalid_argument("the factorized approximate inverse preconditions pcg / pcg_mixed (without "
-                                            "mixedRestart) on one system");
+                                            "mixedRestart)");
             }
             const bool p64 = opt_.precision == SolverPrecision::F64;
             if ((p64 && !fsai64_.get()) || (!p64 && !fsai32_.get()))

@@ -226,9 +226,10 @@ namespace nsfem
         // PCG preconditioner: 0 one symmetric multicolour GS sweep (the
         // reference's); 1 the factorized sparse approximate inverse
         // G^T G ~ A^-1 on the lower-colour pattern plus the diagonal (built on
-        // the host when first chosen; pcg methods without mixedRestart, one
-        // system, any channel count): two launches per application instead
-        // of 2 colours - 2 dependent sweeps.
+        // the host when first chosen; pcg methods without mixedRestart, any
+        // channel count; a block-diagonal batch gets a block-diagonal factor,
+        // since no row has a neighbour in another system): two launches per
+        // application instead of 2 colours - 2 dependent sweeps.
         void setPreconditioner(int kind);
         int preconditioner() const { return precond_; }
         int regionedDirection() const { return regionDir_; }
