@@ -461,6 +461,37 @@ namespace nsfem
         return bm.blocks[lb];
     }
 
+    // The permutation of permuteKernel over the rows of a block map: block lb
+    // moves rows blockRange(bm, lb).  On the region-major direction map the
+    // workgroups of one XCD take stretch k of every colour, whose rows come
+    // from one stretch of the original order, so the gathered (forward) or
+    // scattered (inverse) side stays in that XCD's L2 instead of every
+    // colour sweeping the whole vector.
+    template <typename S, typename D, int NCH>
+    __global__ void __launch_bounds__(kBlock) permuteMappedKernel(BlockMap bm, int n, const int * __restrict__ perm,
+                                                                  const S * __restrict__ src, D * __restrict__ dst, int inverse)
+    {
+        const int4 range = blockRange(bm, logicalBlock(blockIdx.x, gridDim.x));
+        for (int row = range.x + static_cast<int>(threadIdx.x); row < range.y; row += kBlock)
+        {
+            const long p = perm[row];
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+            {
+                const long i = static_cast<long>(row) * NCH + c;
+                const long o = static_cast<long>(c) * n + p;
+                if (inverse)
+                {
+                    dst[o] = static_cast<D>(src[i]);
+                }
+                else
+                {
+                    dst[i] = static_cast<D>(src[o]);
+                }
+            }
+        }
+    }
+
     // Block lb of a colour sweep launch: rows [rowBegin, rowEnd) of the colour
     // (flat map) or table entry blockOffset + lb.
     __device__ __forceinline__ int4 sweepRange(const BlockMap & bm, int blockOffset, int lb, int rowBegin, int rowEnd)

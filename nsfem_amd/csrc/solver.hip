@@ -1065,8 +1065,8 @@ namespace nsfem
         fillInt(K, 1, st.active, s);
         fillInt(K + 4, 0, st.iters, s); // iters + flags
         fill(F64, K, -1.0, st.mse, s);
-        permute(dt, dt, n_, nch, perm_.get(), ioRhs_.get(), b, false, s);
-        permute(dt, dt, n_, nch, perm_.get(), ioSol_.get(), x, false, s);
+        permuteRows(dt, ioRhs_.get(), b, false, s);
+        permuteRows(dt, ioSol_.get(), x, false, s);
 
         const McgsMatrix<T> Av = A.view(n_);
         int iter = 0;
@@ -1101,7 +1101,7 @@ namespace nsfem
             }
         }
         readStats(s, inner_, true, opt_.maxIters);
-        permute(dt, dt, n_, nch, perm_.get(), x, ioSol_.get(), true, s);
+        permuteRows(dt, x, ioSol_.get(), true, s);
         NSFEM_HIP(hipStreamSynchronize(s));
     }
 
@@ -1383,11 +1383,12 @@ namespace nsfem
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
         fillInt(4, 0, st.flags, s);
-        dispatchLC(lanes_, nch, [&]<int L, int C>()
+        // r = b - A x is the direction kernel's product: its map and lanes
+        dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                    {
-                       pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                              zColour0End(), zeroGuess ? 1 : 0);
-                       pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, gate, totals);
+                       pcgInitKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                          zColour0End(), zeroGuess ? 1 : 0);
+                       pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bmDir_, st, gate, totals);
                    });
         launchPrecond<T, P>(s, v, st, totals, false);
         NSFEM_HIP(hipGetLastError());
@@ -1496,8 +1497,8 @@ namespace nsfem
         const int nch = opt_.numCh;
         const int K = numSystems_ * nch;
         const long nv = static_cast<long>(n_) * nch;
-        permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
-        permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+        permuteRows(F64, ioRhs_.get(), b64_.get(), false, s);
+        permuteRows(F64, ioSol_.get(), x64_.get(), false, s);
         SolveState outer = outer_.view(K, opt_.target * opt_.target);
         SolveState inner = inner_.view(K, opt_.innerTol * opt_.innerTol);
         const PcgOperands<double, double> v64 = operands<double, double>();
@@ -1549,7 +1550,7 @@ namespace nsfem
         {
             projectMean<double>(s, x64_.get());
         }
-        permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+        permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
         NSFEM_HIP(hipStreamSynchronize(s));
     }
 
@@ -1735,10 +1736,10 @@ namespace nsfem
         lastOuter_ = 0;
         if (opt_.method == SolverMethod::AmgPcg)
         {
-            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permuteRows(F64, ioRhs_.get(), b64_.get(), false, s);
             if (!zeroInitial)
             {
-                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+                permuteRows(F64, ioSol_.get(), x64_.get(), false, s);
             }
             const AmgPcg::Result r = amg_->solve(b64_.get(), x64_.get(), zeroInitial, opt_.target, opt_.maxIters, useGraphs_);
             for (int c = 0; c < nch; c++)
@@ -1752,16 +1753,16 @@ namespace nsfem
             {
                 projectMean<double>(s, x64_.get());
             }
-            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
             NSFEM_HIP(hipStreamSynchronize(s));
             return;
         }
         if (opt_.method == SolverMethod::BlockPcg)
         {
-            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permuteRows(F64, ioRhs_.get(), b64_.get(), false, s);
             if (!zeroInitial)
             {
-                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+                permuteRows(F64, ioSol_.get(), x64_.get(), false, s);
             }
             if (opt_.precision == SolverPrecision::F64)
             {
@@ -1775,7 +1776,7 @@ namespace nsfem
             {
                 projectMean<double>(s, x64_.get());
             }
-            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
             NSFEM_HIP(hipStreamSynchronize(s));
             return;
         }
@@ -1805,10 +1806,10 @@ namespace nsfem
         SolveState st = inner_.view(K, tol2);
         if (opt_.precision == SolverPrecision::F32)
         {
-            permute(F32, F32, n_, nch, perm_.get(), ioRhs_.get(), b32_.get(), false, s);
+            permuteRows(F32, ioRhs_.get(), b32_.get(), false, s);
             if (!pcgZero)
             {
-                permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
+                permuteRows(F32, ioSol_.get(), x32_.get(), false, s);
             }
             runPcg<float, float>(s, operands<float, float>(), st, pcgZero, nullptr, opt_.maxIters, true, true);
             readStats(s, inner_, false, 0);
@@ -1816,16 +1817,16 @@ namespace nsfem
             {
                 projectMean<float>(s, x32_.get());
             }
-            permute(F32, F32, n_, nch, perm_.get(), x32_.get(), ioSol_.get(), true, s);
+            permuteRows(F32, x32_.get(), ioSol_.get(), true, s);
         }
         else
         {
             // FP64 Krylov vectors and products; the preconditioner in FP64
             // (plain) or FP32 (mixed)
-            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permuteRows(F64, ioRhs_.get(), b64_.get(), false, s);
             if (!pcgZero)
             {
-                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+                permuteRows(F64, ioSol_.get(), x64_.get(), false, s);
             }
             if (opt_.precision == SolverPrecision::F64)
             {
@@ -1840,9 +1841,42 @@ namespace nsfem
             {
                 projectMean<double>(s, x64_.get());
             }
-            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
         }
         NSFEM_HIP(hipStreamSynchronize(s));
+    }
+
+    void McgsSolver::permuteRows(int dt, const void * src, void * dst, bool inverse, hipStream_t s)
+    {
+        const int nch = opt_.numCh;
+        if (!bmDir_.rowTable || nch > 2 || n_ <= 0)
+        {
+            permute(dt, dt, n_, nch, perm_.get(), src, dst, inverse, s);
+            return;
+        }
+        const int inv = inverse ? 1 : 0;
+        auto go = [&]<typename T>()
+        {
+            const T * sp = static_cast<const T *>(src);
+            T * dp = static_cast<T *>(dst);
+            if (nch == 1)
+            {
+                permuteMappedKernel<T, T, 1><<<nbDir_, kBlock, 0, s>>>(bmDir_, n_, perm_.get(), sp, dp, inv);
+            }
+            else
+            {
+                permuteMappedKernel<T, T, 2><<<nbDir_, kBlock, 0, s>>>(bmDir_, n_, perm_.get(), sp, dp, inv);
+            }
+        };
+        if (dt == F64)
+        {
+            go.template operator()<double>();
+        }
+        else
+        {
+            go.template operator()<float>();
+        }
+        NSFEM_HIP(hipGetLastError());
     }
 
     template <typename T>
@@ -1866,11 +1900,11 @@ namespace nsfem
         const int nch = opt_.numCh;
         double * totals = inner_.totals.get();
         fillInt(4, 0, st.flags, s);
-        dispatchLC(lanes_, nch, [&]<int L, int C>()
+        dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                    {
-                       pcgInitKernel<T, P, L, C><<<numBlocks_, kBlock, 0, s>>>(v.A, bm_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
-                                                                              zColour0End(), zeroGuess ? 1 : 0);
-                       pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, nullptr, totals);
+                       pcgInitKernel<T, P, L, C><<<nbDir_, kBlock, 0, s>>>(v.A, bmDir_, st, v.b, v.x, v.r, v.rP, v.z, v.M.invDiag,
+                                                                          zColour0End(), zeroGuess ? 1 : 0);
+                       pcgInitFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bmDir_, st, nullptr, totals);
                    });
         launchPrecond<T, P>(s, v, st, totals, false);
         for (int i = 0; i < budget; i++)
@@ -1894,13 +1928,13 @@ namespace nsfem
                 throw std::invalid_argument("captured solve: the iteration budget must be positive");
             }
             const int nch = opt_.numCh;
-            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permuteRows(F64, ioRhs_.get(), b64_.get(), false, s);
             if (!zeroInitial)
             {
-                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+                permuteRows(F64, ioSol_.get(), x64_.get(), false, s);
             }
             amg_->enqueueSolve(s, b64_.get(), x64_.get(), zeroInitial, opt_.target, std::min(budget, opt_.maxIters));
-            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
             return;
         }
         if (opt_.method != SolverMethod::Pcg || (opt_.precision == SolverPrecision::Mixed && opt_.restart))
@@ -1917,24 +1951,24 @@ namespace nsfem
         SolveState st = inner_.view(K, opt_.target * opt_.target);
         if (opt_.precision == SolverPrecision::F32)
         {
-            permute(F32, F32, n_, nch, perm_.get(), ioRhs_.get(), b32_.get(), false, s);
+            permuteRows(F32, ioRhs_.get(), b32_.get(), false, s);
             if (!zeroInitial)
             {
-                permute(F32, F32, n_, nch, perm_.get(), ioSol_.get(), x32_.get(), false, s);
+                permuteRows(F32, ioSol_.get(), x32_.get(), false, s);
             }
             enqueuePcg<float, float>(s, operands<float, float>(), st, zeroInitial, budget);
             if (opt_.projectMean)
             {
                 projectMean<float>(s, x32_.get());
             }
-            permute(F32, F32, n_, nch, perm_.get(), x32_.get(), ioSol_.get(), true, s);
+            permuteRows(F32, x32_.get(), ioSol_.get(), true, s);
         }
         else
         {
-            permute(F64, F64, n_, nch, perm_.get(), ioRhs_.get(), b64_.get(), false, s);
+            permuteRows(F64, ioRhs_.get(), b64_.get(), false, s);
             if (!zeroInitial)
             {
-                permute(F64, F64, n_, nch, perm_.get(), ioSol_.get(), x64_.get(), false, s);
+                permuteRows(F64, ioSol_.get(), x64_.get(), false, s);
             }
             if (opt_.precision == SolverPrecision::F64)
             {
@@ -1948,7 +1982,7 @@ namespace nsfem
             {
                 projectMean<double>(s, x64_.get());
             }
-            permute(F64, F64, n_, nch, perm_.get(), x64_.get(), ioSol_.get(), true, s);
+            permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
         }
     }
 

@@ -322,6 +322,11 @@ namespace nsfem
         template <typename T>
         void projectMean(hipStream_t s, T * v);
         bool pollDone(hipStream_t s, DeviceState & st);
+        // The io vectors (original order, channel-major) to or from the
+        // solver's colour-major rows (inverse), same precision dt on both
+        // sides: over the region-major direction map when there is one (1-2
+        // channels), else one element per thread.
+        void permuteRows(int dt, const void * src, void * dst, bool inverse, hipStream_t s);
 
         SolverOptions opt_;
         int n_ = 0;
