@@ -622,6 +622,12 @@ def main():
                          "optionally followed by the same four for the colour sweeps and then by the velocity and "
                          "pressure direction,update,rho rows, the velocity and pressure direction lanes and "
                          "whether each fuses its reductions (18 numbers, as the JSON records them)")
+    ap.add_argument("--fast-shape", default="2,256,2,512,512,4096,1024,2,1",
+                    help="velocity launch shape of the fast paths (mixed, fp64_fast): lanes, rows per block, sweep "
+                         "lanes, sweep rows, direction / update / rho rows, direction lanes, fused reductions; the "
+                         "measured best on MI355X (the autotuner's one-iteration objective misses a 10 %% per-iteration "
+                         "difference inside the solve); 'auto' autotunes them like the other paths; an explicit "
+                         "--shape takes precedence")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--step-graph", action="store_true",
                     help="each step as one captured graph with device-side convergence (after a first regular step)")
@@ -709,6 +715,12 @@ def main():
     dir_lanes = shape[14:16] if len(shape) >= 16 else None
     fused_modes = shape[16:18] if len(shape) == 18 else None
 
+    fast_shape = None
+    if args.fast_shape != "auto" and args.shape == "0,0,0,0":
+        fast_shape = [int(v) for v in args.fast_shape.split(",")]
+        if len(fast_shape) != 9:
+            raise SystemExit("--fast-shape takes 9 numbers or 'auto'")
+
     def make_chorin(cfg, fast=True):
         ch = fem.Chorin(ctx, cfg)
         if fused_modes is not None:
@@ -721,14 +733,18 @@ def main():
         if dir_lanes is not None:
             for which, v in zip(("velocity", "pressure"), dir_lanes):
                 ch.set_solver_option("lanesDirection", v, which)
+        if fast and fast_shape is not None:
+            for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho", "lanesDirection", "fusedReductions"), fast_shape[4:]):
+                ch.set_solver_option(key, v, "velocity")
         apply_amg_smoother(args, ch, cfg)
         if fast and args.velocity_precond == "fsai":
             ch.set_solver_option("preconditioner", 1, "velocity")
         return ch
 
-    def solvers(method, one_level=False):
+    def solvers(method, one_level=False, fast=False):
         pressure = method if (one_level or args.pressure_solver == "mcgs") else method.replace("pcg", "amg_pcg")
-        return (fem.SolverConfig(method, 5000, target, 1, vl, vr, vsl, vsr),
+        v = fast_shape[:4] if fast and fast_shape is not None else (vl, vr, vsl, vsr)
+        return (fem.SolverConfig(method, 5000, target, 1, *v),
                 fem.SolverConfig(pressure, 50000, target, 1, pl, pr, psl, psr))
 
     eager = 1 if args.eager else 0
@@ -739,7 +755,7 @@ def main():
         # the plain paths keep the reference's velocity solves (zero start, the
         # GS sweep) and pressure start, so that the FP64 one matches the
         # reference to 1e-10; the fast ones take the options
-        return fem.ChorinConfig(*solvers(method, one_level), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP,
+        return fem.ChorinConfig(*solvers(method, one_level, fast), dtype=F64, eagerLaunches=eager, timeStepPolicy=fem.TAU_CLAMP,
                                 stepGraph=sg, pressureStart=ps if fast else fem.PRESSURE_START_LAST,
                                 velocityStart=vs if fast else fem.VELOCITY_START_ZERO)
 
@@ -772,7 +788,10 @@ def main():
     for name in paths:
         cfg = configs[name]
         # the library's guard clamps the requested tau to 1 / lambda_max(M^-1 nu K)
-        ch, r, first, last = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, not args.no_autotune,
+        # the fast paths' velocity shape is fixed (--fast-shape) and their
+        # pressure solver (AMG) has no launch shape to tune
+        tune = not args.no_autotune and not (name in fast_paths and fast_shape is not None and name in amg_paths)
+        ch, r, first, last = run_path(name, ctx, cfg, args.tau, u0, args.warmup, args.steps, tune,
                                       lambda c, f=name in fast_paths: make_chorin(c, f))
         ti = r["time_step"]
         log(f"{name}: tau requested {ti['tauRequested']:g}, used {ti['tauUsed']:.4g} "
@@ -887,6 +906,10 @@ def main():
                                                     if args.pressure_solver == "amg" else "one symmetric multicolour GS sweep")},
                    "starts": {"fast paths": {"velocity": args.velocity_start, "pressure": args.pressure_start},
                               "plain paths": {"velocity": "zero", "pressure": "last"}},
+                   "launch_shapes": {"fast paths velocity": ("fixed " + args.fast_shape) if fast_shape is not None
+                                     else ("--shape" if args.shape != "0,0,0,0" else "autotuned"),
+                                     "other paths": "--shape" if args.shape != "0,0,0,0" else
+                                     ("defaults" if args.no_autotune else "autotuned")},
                    "pressure_solver": args.pressure_solver,
                    "amg_chebyshev_from_degree_fraction": args.amg_chebyshev or "1,2,0.2 (library default)",
                    "seed": args.seed},

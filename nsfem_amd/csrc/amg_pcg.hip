@@ -872,11 +872,6 @@ namespace nsfem
             (void)hipGraphExecDestroy(iterExec_);
             iterExec_ = nullptr;
         }
-        if (chunkExec_)
-        {
-            (void)hipGraphExecDestroy(chunkExec_);
-            chunkExec_ = nullptr;
-        }
     }
 
     template <typename F>
@@ -1303,40 +1298,6 @@ namespace nsfem
             destroyGraphs();
         }
         typed([&]<typename P, int NCH>() { enqueueInit<P, NCH>(s, b, x, zeroGuess, tol2); });
-        auto capture = [&](int iterations, hipGraphExec_t & exec)
-        {
-            hipGraph_t g = nullptr;
-            NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try
-            {
-                typed([&]<typename P, int NCH>()
-                      {
-                          for (int i = 0; i < iterations; i++)
-                          {
-                              enqueueIteration<P, NCH>(s, tol2);
-                          }
-                      });
-            }
-            catch (...)
-            {
-                hipGraph_t gg = nullptr;
-                (void)hipStreamEndCapture(s, &gg);
-                if (gg)
-                {
-                    (void)hipGraphDestroy(gg);
-                }
-                throw;
-            }
-            NSFEM_HIP(hipStreamEndCapture(s, &g));
-            const hipError_t rc = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            NSFEM_HIP(rc);
-        };
-        if (useGraphs && graphTol2_ != tol2)
-        {
-            destroyGraphs();
-            graphTol2_ = tol2;
-        }
         auto iterate = [&]()
         {
             if (!useGraphs)
@@ -1344,9 +1305,30 @@ namespace nsfem
                 typed([&]<typename P, int NCH>() { enqueueIteration<P, NCH>(s, tol2); });
                 return;
             }
-            if (!iterExec_)
+            if (!iterExec_ || graphTol2_ != tol2)
             {
-                capture(1, iterExec_);
+                destroyGraphs();
+                hipGraph_t g = nullptr;
+                NSFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try
+                {
+                    typed([&]<typename P, int NCH>() { enqueueIteration<P, NCH>(s, tol2); });
+                }
+                catch (...)
+                {
+                    hipGraph_t gg = nullptr;
+                    (void)hipStreamEndCapture(s, &gg);
+                    if (gg)
+                    {
+                        (void)hipGraphDestroy(gg);
+                    }
+                    throw;
+                }
+                NSFEM_HIP(hipStreamEndCapture(s, &g));
+                const hipError_t rc = hipGraphInstantiate(&iterExec_, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                NSFEM_HIP(rc);
+                graphTol2_ = tol2;
             }
             NSFEM_HIP(hipGraphLaunch(iterExec_, s));
         };
@@ -1354,21 +1336,9 @@ namespace nsfem
         // iterations: after a solve that took many more, the no-op
         // iterations of a long burst would cost more than the polls it
         // saves), then one iteration at a time with the stop flag polled one
-        // iteration behind; with graphs the burst goes kChunk iterations
-        // per launch
+        // iteration behind
         int launched = 0;
         const int burst = std::max(0, std::min({maxIters, predicted_ - 2, 64}));
-        if (useGraphs && burst >= kChunk)
-        {
-            if (!chunkExec_)
-            {
-                capture(kChunk, chunkExec_);
-            }
-            for (; launched + kChunk <= burst; launched += kChunk)
-            {
-                NSFEM_HIP(hipGraphLaunch(chunkExec_, s));
-            }
-        }
         for (; launched < burst; launched++)
         {
             iterate();

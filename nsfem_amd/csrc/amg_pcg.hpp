@@ -160,10 +160,6 @@ namespace nsfem
 
         double * xDevice_ = nullptr; // the solution vector the captured iteration updates
         hipGraphExec_t iterExec_ = nullptr;
-        // kChunk iterations in one graph, for the unpolled burst: one graph
-        // launch (a few µs of idle GPU each) per kChunk iterations
-        static constexpr int kChunk = 4;
-        hipGraphExec_t chunkExec_ = nullptr;
         double graphTol2_ = -1;
         int predicted_ = 0;
     };

@@ -1439,7 +1439,12 @@ namespace nsfem
         // the next chunk is queued the host waits for the previous chunk's
         // copy, which has finished by the time the current chunk runs, so the
         // GPU never drains and at most one chunk runs past convergence (its
-        // kernels exit at once).
+        // kernels exit at once).  Each copy idles the GPU for some 18 µs
+        // (the copy and the barriers around it), so the chunks up to one
+        // before the last solve's count less two go without (at most 64
+        // iterations: after a solve that took many more, no-op iterations
+        // would cost more than the polls they save).
+        const int quiet = usePrediction ? std::min(lastSolveIters_ - 2 - chunk, 64) : 0;
         int launched = 0;
         int slot = 0;
         bool pending[2] = {false, false};
@@ -1459,7 +1464,7 @@ namespace nsfem
             }
             NSFEM_HIP(hipGetLastError());
             launched += k;
-            if (!sync)
+            if (!sync || launched <= quiet)
             {
                 continue;
             }
@@ -1812,12 +1817,19 @@ namespace nsfem
                 permuteRows(F32, ioSol_.get(), x32_.get(), false, s);
             }
             runPcg<float, float>(s, operands<float, float>(), st, pcgZero, nullptr, opt_.maxIters, true, true);
+            if (!opt_.projectMean)
+            {
+                // queued before the statistics' synchronisation: one host
+                // round trip fewer with the GPU idle
+                permuteRows(F32, x32_.get(), ioSol_.get(), true, s);
+            }
             readStats(s, inner_, false, 0);
+            lastSolveIters_ = *std::max_element(lastIters_.begin(), lastIters_.end());
             if (opt_.projectMean)
             {
                 projectMean<float>(s, x32_.get());
+                permuteRows(F32, x32_.get(), ioSol_.get(), true, s);
             }
-            permuteRows(F32, x32_.get(), ioSol_.get(), true, s);
         }
         else
         {
@@ -1836,12 +1848,19 @@ namespace nsfem
             {
                 runPcg<double, float>(s, operands<double, float>(), st, pcgZero, nullptr, opt_.maxIters, true, true);
             }
+            if (!opt_.projectMean)
+            {
+                // queued before the statistics' synchronisation: one host
+                // round trip fewer with the GPU idle
+                permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
+            }
             readStats(s, inner_, false, 0);
+            lastSolveIters_ = *std::max_element(lastIters_.begin(), lastIters_.end());
             if (opt_.projectMean)
             {
                 projectMean<double>(s, x64_.get());
+                permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
             }
-            permuteRows(F64, x64_.get(), ioSol_.get(), true, s);
         }
         NSFEM_HIP(hipStreamSynchronize(s));
     }

@@ -419,5 +419,8 @@ namespace nsfem
         ChunkGraph graph64_, graph32_, graphMixed_;
         bool useGraphs_ = true;
         int predicted64_ = 0, predicted32_ = 0, predictedMixed_ = 0;
+        // iterations the last pcg solve took (its slowest channel): the
+        // next solve polls the stop flag only from a chunk before that count
+        int lastSolveIters_ = 0;
     };
 } // namespace nsfem
