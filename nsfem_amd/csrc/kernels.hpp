@@ -492,6 +492,29 @@ namespace nsfem
         }
     }
 
+    // The inverse direction of the permutation as a gather: original row r
+    // takes colour-major row iperm[r], dst[c*n + r] = src[iperm[r]*NCH + c]
+    // (iperm the inverse of perm).  The writes are whole lines in order, and
+    // the contiguous range of original rows one XCD takes (logicalBlock)
+    // reads from the matching stretch of each colour; scattering the same
+    // permutation's 8-byte writes takes twice as long.
+    template <typename S, typename D, int NCH>
+    __global__ void __launch_bounds__(kBlock) unpermuteGatherKernel(int n, const int * __restrict__ iperm,
+                                                                    const S * __restrict__ src, D * __restrict__ dst)
+    {
+        const long r = static_cast<long>(logicalBlock(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+        if (r >= n)
+        {
+            return;
+        }
+        const long q = static_cast<long>(iperm[r]) * NCH;
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+        {
+            dst[static_cast<long>(c) * n + r] = static_cast<D>(src[q + c]);
+        }
+    }
+
     // Block lb of a colour sweep launch: rows [rowBegin, rowEnd) of the colour
     // (flat map) or table entry blockOffset + lb.
     __device__ __forceinline__ int4 sweepRange(const BlockMap & bm, int blockOffset, int lb, int rowBegin, int rowEnd)

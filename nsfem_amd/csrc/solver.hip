@@ -434,6 +434,14 @@ namespace nsfem
         }
 
         perm_.upload(perm);
+        {
+            std::vector<int> iperm(n_);
+            for (int i = 0; i < n_; i++)
+            {
+                iperm[perm[i]] = i;
+            }
+            iperm_.upload(iperm);
+        }
         systemRows_.upload(sysRows_);
         if (opt_.method == SolverMethod::AmgPcg)
         {
@@ -1868,7 +1876,7 @@ namespace nsfem
     void McgsSolver::permuteRows(int dt, const void * src, void * dst, bool inverse, hipStream_t s)
     {
         const int nch = opt_.numCh;
-        if (!bmDir_.rowTable || nch > 2 || n_ <= 0)
+        if (nch > 2 || n_ <= 0 || !(inverse || bmDir_.rowTable))
         {
             permute(dt, dt, n_, nch, perm_.get(), src, dst, inverse, s);
             return;
@@ -1878,7 +1886,19 @@ namespace nsfem
         {
             const T * sp = static_cast<const T *>(src);
             T * dp = static_cast<T *>(dst);
-            if (nch == 1)
+            if (inverse)
+            {
+                const int g = (n_ + kBlock - 1) / kBlock;
+                if (nch == 1)
+                {
+                    unpermuteGatherKernel<T, T, 1><<<g, kBlock, 0, s>>>(n_, iperm_.get(), sp, dp);
+                }
+                else
+                {
+                    unpermuteGatherKernel<T, T, 2><<<g, kBlock, 0, s>>>(n_, iperm_.get(), sp, dp);
+                }
+            }
+            else if (nch == 1)
             {
                 permuteMappedKernel<T, T, 1><<<nbDir_, kBlock, 0, s>>>(bmDir_, n_, perm_.get(), sp, dp, inv);
             }

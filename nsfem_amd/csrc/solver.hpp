@@ -324,8 +324,9 @@ namespace nsfem
         bool pollDone(hipStream_t s, DeviceState & st);
         // The io vectors (original order, channel-major) to or from the
         // solver's colour-major rows (inverse), same precision dt on both
-        // sides: over the region-major direction map when there is one (1-2
-        // channels), else one element per thread.
+        // sides, 1-2 channels: into colour order over the region-major
+        // direction map when there is one, out of it as a gather in original
+        // order; otherwise one element per thread.
         void permuteRows(int dt, const void * src, void * dst, bool inverse, hipStream_t s);
 
         SolverOptions opt_;
@@ -375,6 +376,7 @@ namespace nsfem
         void destroyGraphs();
 
         DeviceArray<int> perm_;
+        DeviceArray<int> iperm_; // the inverse: original row -> colour-major row
         DeviceArray<int4> blocks_;
         DeviceArray<int> sysBlockStart_, sysBlocks_, systemRows_;
         BlockMap bm_{};
