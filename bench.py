@@ -624,7 +624,8 @@ def main():
                          "whether each fuses its reductions (18 numbers, as the JSON records them)")
     ap.add_argument("--fast-shape", default="2,256,2,512,512,4096,1024,2,1",
                     help="velocity launch shape of the fast paths (mixed, fp64_fast): lanes, rows per block, sweep "
-                         "lanes, sweep rows, direction / update / rho rows, direction lanes, fused reductions; the "
+                         "lanes, sweep rows, direction / update / rho rows, direction lanes, fused reductions, and "
+                         "optionally the approximate inverse's rows (0: the direction kernel's map); the "
                          "measured best on MI355X (the autotuner's one-iteration objective misses a 10 %% per-iteration "
                          "difference inside the solve); 'auto' autotunes them like the other paths; an explicit "
                          "--shape takes precedence")
@@ -718,8 +719,8 @@ def main():
     fast_shape = None
     if args.fast_shape != "auto" and args.shape == "0,0,0,0":
         fast_shape = [int(v) for v in args.fast_shape.split(",")]
-        if len(fast_shape) != 9:
-            raise SystemExit("--fast-shape takes 9 numbers or 'auto'")
+        if len(fast_shape) not in (9, 10):
+            raise SystemExit("--fast-shape takes 9 or 10 numbers or 'auto'")
 
     def make_chorin(cfg, fast=True):
         ch = fem.Chorin(ctx, cfg)
@@ -734,7 +735,8 @@ def main():
             for which, v in zip(("velocity", "pressure"), dir_lanes):
                 ch.set_solver_option("lanesDirection", v, which)
         if fast and fast_shape is not None:
-            for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho", "lanesDirection", "fusedReductions"), fast_shape[4:]):
+            for key, v in zip(("rowsDirection", "rowsUpdate", "rowsRho", "lanesDirection", "fusedReductions",
+                               "rowsPreconditioner"), fast_shape[4:]):
                 ch.set_solver_option(key, v, "velocity")
         apply_amg_smoother(args, ch, cfg)
         if fast and args.velocity_precond == "fsai":

@@ -219,6 +219,11 @@ int nsfem_solver_set_mse_check_interval(nsfem_solver * s, int interval);
  *                  chosen: two parallel launches per application instead of
  *                  2 colours - 2 dependent sweeps (the P2 mass matrix: 12
  *                  iterations where GS takes 13).
+ *   "rowsPreconditioner" fused reductions on one system, preconditioner
+ *                  1: rows per workgroup of the approximate inverse's two
+ *                  launches on a region-major map of their own (a positive
+ *                  multiple of 256); 0 (default) shares the direction
+ *                  kernel's map.
  *   "amgChebyshevFrom", "amgChebyshevDegree", "amgChebyshevFraction"
  *                  amg_pcg: the V-cycle levels from this one on (but the
  *                  dense coarsest) are smoothed by this many steps of the

@@ -138,6 +138,12 @@ namespace
                 amg->setSmoother(amg->chebyshevFrom(), amg->chebyshevDegree(), value);
             }
         }
+        else if (k == "rowsPreconditioner")
+        {
+            if (!(value >= 0 && value <= 1 << 20) || value != static_cast<int>(value))
+                throw std::invalid_argument("rowsPreconditioner must be an integer in [0, 2^20]");
+            solver.setPreconditionerRows(static_cast<int>(value));
+        }
         else if (k == "rowsDirection" || k == "rowsUpdate" || k == "rowsRho")
         {
             if (!(value >= 0 && value <= 1 << 20)) throw std::invalid_argument(k + " must be in [0, 2^20]");

@@ -742,6 +742,18 @@ namespace nsfem
         destroyGraphs();
     }
 
+    void McgsSolver::setPreconditionerRows(int rows)
+    {
+        if (rows < 0 || rows % kBlock != 0)
+        {
+            throw std::invalid_argument("rowsPreconditioner must be 0 or a positive multiple of 256");
+        }
+        NSFEM_HIP(hipStreamSynchronize(stream_));
+        rowsPrecond_ = rows;
+        setupKindMaps();
+        destroyGraphs();
+    }
+
     void McgsSolver::setDirectionLanes(int lanes)
     {
         if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
@@ -807,15 +819,12 @@ namespace nsfem
                 maxBlocks = std::max(maxBlocks, nb);
                 return m;
             };
-            bmDir_ = flatMap(rowsDir_, nbDir_, 1024);
-            bmUpd_ = flatMap(rowsUpd_, nbUpd_, 1024);
-            bmRho_ = flatMap(rowsRho_, nbRho_, 4096);
-            if (regionDir_ && n_ > 0)
+            // stretch k of every colour, then stretch k + 1: the XCD that
+            // runs a contiguous range of blocks (logicalBlock) gathers from
+            // one stretch of the locality order
+            auto regionMap = [&](BlockMap & m, int & nb, DeviceArray<int2> & table)
             {
-                // stretch k of every colour, then stretch k + 1: the XCD
-                // that runs a contiguous range of blocks (logicalBlock)
-                // gathers from one stretch of the locality order
-                const int rpb = bmDir_.rowsPerBlock;
+                const int rpb = m.rowsPerBlock;
                 std::vector<int2> t;
                 for (int k = 0; k < 8; k++)
                 {
@@ -828,18 +837,39 @@ namespace nsfem
                         }
                     }
                 }
-                dirTable_.upload(t);
-                nbDir_ = static_cast<int>(t.size());
-                bmDir_.numBlocks = nbDir_;
-                bmDir_.rowTable = dirTable_.get();
-                maxBlocks = std::max(maxBlocks, nbDir_);
+                table.upload(t);
+                nb = static_cast<int>(t.size());
+                m.numBlocks = nb;
+                m.rowTable = table.get();
+                maxBlocks = std::max(maxBlocks, nb);
+            };
+            bmDir_ = flatMap(rowsDir_, nbDir_, 1024);
+            bmUpd_ = flatMap(rowsUpd_, nbUpd_, 1024);
+            bmRho_ = flatMap(rowsRho_, nbRho_, 4096);
+            const bool ownPrecondMap = rowsPrecond_ > 0 && rowsPrecond_ != bmDir_.rowsPerBlock;
+            if (ownPrecondMap)
+            {
+                bmPre_ = flatMap(rowsPrecond_, nbPre_, 1024);
+            }
+            if (regionDir_ && n_ > 0)
+            {
+                regionMap(bmDir_, nbDir_, dirTable_);
+                if (ownPrecondMap)
+                {
+                    regionMap(bmPre_, nbPre_, preTable_);
+                }
+            }
+            if (!ownPrecondMap)
+            {
+                bmPre_ = bmDir_;
+                nbPre_ = nbDir_;
             }
             color0BlocksRho_ = (color0End_ + bmRho_.rowsPerBlock - 1) / bmRho_.rowsPerBlock;
         }
         else
         {
-            bmDir_ = bmUpd_ = bmRho_ = bm_;
-            nbDir_ = nbUpd_ = nbRho_ = numBlocks_;
+            bmDir_ = bmUpd_ = bmRho_ = bmPre_ = bm_;
+            nbDir_ = nbUpd_ = nbRho_ = nbPre_ = numBlocks_;
             color0BlocksRho_ = color0Blocks_;
         }
         // reduction scratch sized by the largest block count
@@ -1281,15 +1311,16 @@ namespace nsfem
             launchRho<T, P>(s, v, st, totals, withUpdate);
             return;
         }
-        // on the direction kernel's map (region-major: the workgroups of one
-        // XCD take the same stretch of every colour and gather from about an
-        // eighth of the vector); the next direction kernel reads nbDir_ rho
+        // on a region-major map (the workgroups of one XCD take the same
+        // stretch of every colour and gather from about an eighth of the
+        // vector): the direction kernel's, or one with rows of its own
+        // (rowsPreconditioner); the next direction kernel reads nbPre_ rho
         // partials
         const bool fin = !fused();
         dispatchLC(directionLanes(), opt_.numCh, [&]<int L, int C>()
                    {
-                       fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
-                       fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
+                       fsaiLowerKernel<P, L, C><<<nbPre_, kBlock, 0, s>>>(v.G, bmPre_, st, v.rP, v.t);
+                       fsaiUpperKernel<P, T, L, C><<<nbPre_, kBlock, 0, s>>>(v.G, bmPre_, st, v.t, v.z, v.r);
                        if (fin)
                        {
                            pcgRhoFinalizeKernel<C><<<1, kFinBlock, 0, s>>>(bm_, st, totals, withUpdate ? 1 : 0);
@@ -2251,8 +2282,8 @@ namespace nsfem
                 {
                     dispatchLC(directionLanes(), nch, [&]<int L, int C>()
                                {
-                                   fsaiLowerKernel<P, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.rP, v.t);
-                                   fsaiUpperKernel<P, T, L, C><<<nbDir_, kBlock, 0, s>>>(v.G, bmDir_, st, v.t, v.z, v.r);
+                                   fsaiLowerKernel<P, L, C><<<nbPre_, kBlock, 0, s>>>(v.G, bmPre_, st, v.rP, v.t);
+                                   fsaiUpperKernel<P, T, L, C><<<nbPre_, kBlock, 0, s>>>(v.G, bmPre_, st, v.t, v.z, v.r);
                                });
                 };
             }

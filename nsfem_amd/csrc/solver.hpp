@@ -200,6 +200,11 @@ namespace nsfem
         // partials are redone by every block of the next kernel, so fewer,
         // larger blocks pay there.  reshape resets them.
         void setKindRows(int direction, int update, int rho);
+        // Rows per workgroup of the approximate inverse's two launches
+        // (preconditioner = 1; fused reductions on one system): 0 (default)
+        // shares the direction kernel's map, else a positive multiple of 256.
+        void setPreconditionerRows(int rows);
+        int preconditionerRows() const { return rowsPrecond_; }
         int kindRows(int kind) const; // 0 direction, 1 update, 2 rho
         // Lanes per row of the direction kernel alone (0: the main shape's;
         // 1 and 2 channels).  reshape resets it.
@@ -309,7 +314,7 @@ namespace nsfem
         int zColour0End() const { return precond_ == 0 ? color0End_ : 0; }
         // blocks whose rho partials the next direction kernel sums (the
         // approximate inverse runs on the direction kernel's map)
-        int rhoParts() const { return precond_ == 0 ? nbRho_ : nbDir_; }
+        int rhoParts() const { return precond_ == 0 ? nbRho_ : nbPre_; }
         template <typename P>
         McgsMatrix<P> fsaiView() const;
         void runMixedRestart(hipStream_t s);
@@ -365,6 +370,12 @@ namespace nsfem
         int rowsDir_ = 0, rowsUpd_ = 0, rowsRho_ = 0, lanesDir_ = 0;
         BlockMap bmDir_{}, bmUpd_{}, bmRho_{};
         int nbDir_ = 0, nbUpd_ = 0, nbRho_ = 0, color0BlocksRho_ = 0;
+        // the approximate inverse's map: the direction map, or (rowsPrecond_
+        // > 0, fused reductions on one system) a region-major map of its own
+        int rowsPrecond_ = 0;
+        BlockMap bmPre_{};
+        int nbPre_ = 0;
+        DeviceArray<int2> preTable_;
         void setupKindMaps();
         // the colour sweeps' workgroup map
         int sweepLanes_ = 8, sweepRows_ = 0;

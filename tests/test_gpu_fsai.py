@@ -189,6 +189,42 @@ def test_captured_steps_with_fsai_velocity():
     assert st["capturedStep"] == 0 and captured.step_graph_info()["rollbacks"] == before + 1
 
 
+def test_fsai_on_a_map_of_its_own(channel_20k):
+    """rowsPreconditioner gives the approximate inverse's two launches a
+    region-major map of their own (here 256-row workgroups against the
+    direction kernel's 1,024): the same operator, so the same solution to the
+    target and the same iteration count (the r.z partial sums only regroup);
+    counts that are not multiples of 256 are rejected."""
+    from nsfem_amd import cu
+    from nsfem_amd._lib import NsfemError
+    M, _ = channel_20k
+    n = M.shape[0]
+    rng = np.random.default_rng(47)
+    b = np.concatenate([M @ rng.uniform(-1, 1, n) for _ in range(2)])
+    out = []
+    for rows_pre in (0, 256):
+        s = cu.createSolver("pcg_mixed", 2, M, 2000, 1e-10, 1)
+        s.setOption("fusedReductions", 1)
+        s.setOption("rowsDirection", 1024)
+        s.setOption("preconditioner", 1)
+        s.setOption("rowsPreconditioner", rows_pre)
+        s.getRhs().upload(b)
+        s.getSol().memsetZero()
+        s.solve()
+        out.append((s.getLastIterations(), s.getSol().download()))
+        with pytest.raises(NsfemError, match="multiple of 256"):
+            s.setOption("rowsPreconditioner", 100)
+    (it0, x0), (it1, x1) = out
+    assert it0 == it1
+    lu = spla.splu(sp.csc_matrix(M))
+    for c in range(2):
+        xd = lu.solve(b[c * n:(c + 1) * n])
+        for x in (x0, x1):
+            xc = x[c * n:(c + 1) * n]
+            assert np.linalg.norm(b[c * n:(c + 1) * n] - M @ xc) / np.linalg.norm(b[c * n:(c + 1) * n]) <= 1.01e-10
+            assert np.max(np.abs(xc - xd)) <= 1e-7 * np.max(np.abs(xd))
+
+
 def test_fsai_rejects_what_it_does_not_cover(channel_20k):
     from nsfem_amd import cu
     from nsfem_amd._lib import NsfemError
