@@ -877,7 +877,8 @@ def main():
                     # cache-resident between launches, so the HBM fraction understates what bounds it
                     "working_set": "exceeds the 256 MB Infinity Cache" if k["bytes"] > LLC_BYTES else "Infinity-Cache resident",
                     "traffic_source": (os.path.relpath(TRAFFIC_FILE, ROOT) + " (counter passes of the same kernel and "
-                                       "workload, committed; not measured in this run)") if t else None,
+                                       "workload at the launch shape that file names, committed; not measured in "
+                                       "this run)") if t else None,
                     "achievable_copy": copy, "frac_of_achievable": k["GBps"] / copy}
 
         roofline = line(dom_name)
