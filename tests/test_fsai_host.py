@@ -114,3 +114,74 @@ def test_fsai_rejects_an_indefinite_matrix(driver, tmp_path):
     rs, spl, col, val, dg = [0, 1, 2], [0, 2], [1, 0], [2.0, 2.0], np.array([1.0, 1.0])
     rc, err = _run(driver, tmp_path, 2, rs, spl, col, val, dg)
     assert rc == 3 and "not positive definite" in err
+
+
+def _greedy_colours(A):
+    n = A.shape[0]
+    color = -np.ones(n, int)
+    for i in range(n):
+        used = set(color[A.indices[A.indptr[i]:A.indptr[i + 1]]])
+        c = 0
+        while c in used:
+            c += 1
+        color[i] = c
+    return color
+
+
+def _random_spd(n, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=6 / n, random_state=seed + 1, format="csr")
+    A = abs(A + A.T)
+    A = A + sp.diags(np.asarray(A.sum(axis=1)).ravel() + rng.uniform(0.5, 1.5, n))
+    return sp.csr_matrix(A)
+
+
+def test_fsai_factor_of_a_block_diagonal_batch_is_per_block(driver, tmp_path):
+    """A block-diagonal batch in the solver's batch order (colour-major, each
+    colour cut into one chunk per system): the factor is block-diagonal and
+    each block equals, bit for bit, the factor of that system alone in its
+    own colour-major order (a row's lower-colour neighbours never leave its
+    system and keep their relative order, so every local system is the
+    same)."""
+    systems = [_random_spd(n, 20 + k) for k, n in enumerate((150, 90, 220))]
+    colours = [_greedy_colours(A) for A in systems]
+    # each system alone, colour-major
+    alone = []
+    for A, col in zip(systems, colours):
+        perm = np.argsort(col, kind="stable")
+        Mp = sp.csr_matrix(A[perm][:, perm])
+        Mp.sort_indices()
+        rs, spl, cidx, val, dg = _split(Mp)
+        g, gd = _run(driver, tmp_path, Mp.shape[0], rs, spl, cidx, val, dg)
+        alone.append((perm, Mp, rs, spl, cidx, g, gd))
+    # the batch: colour c of system 0, colour c of system 1, ..., then colour c + 1
+    offsets = np.cumsum([0] + [A.shape[0] for A in systems])
+    N = offsets[-1]
+    where = {}  # (system, local colour-major row) -> batch row
+    at = 0
+    for c in range(max(col.max() for col in colours) + 1):
+        for s, (perm, *_rest) in enumerate(alone):
+            for li in np.flatnonzero(colours[s][perm] == c):
+                where[(s, li)] = at
+                at += 1
+    assert at == N
+    rows, cols, vals = [], [], []
+    for s, (perm, Mp, *_rest) in enumerate(alone):
+        C = Mp.tocoo()
+        rows += [where[(s, i)] for i in C.row]
+        cols += [where[(s, j)] for j in C.col]
+        vals += list(C.data)
+    B = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    B.sort_indices()
+    rs, spl, cidx, val, dg = _split(B)
+    g, gd = _run(driver, tmp_path, N, rs, spl, cidx, val, dg)
+    for s, (perm, Mp, ars, aspl, acol, ag, agd) in enumerate(alone):
+        back = {where[(s, i)]: i for i in range(Mp.shape[0])}
+        for i in range(Mp.shape[0]):
+            bi = where[(s, i)]
+            assert gd[bi] == agd[i]
+            # the batch row's entries, in order, map to the lone row's
+            bcols = [back[c] for c in cidx[rs[bi]:rs[bi + 1]]]
+            assert bcols == list(acol[ars[i]:ars[i + 1]])
+            assert spl[bi] - rs[bi] == aspl[i] - ars[i]
+            assert np.array_equal(g[rs[bi]:rs[bi + 1]], ag[ars[i]:ars[i + 1]])
