@@ -626,8 +626,9 @@ def main():
                     help="velocity launch shape of the fast paths (mixed, fp64_fast): lanes, rows per block, sweep "
                          "lanes, sweep rows, direction / update / rho rows, direction lanes, fused reductions, and "
                          "optionally the approximate inverse's rows (0: the direction kernel's map); the "
-                         "measured best on MI355X (the autotuner's one-iteration objective misses a 10 %% per-iteration "
-                         "difference inside the solve); 'auto' autotunes them like the other paths; an explicit "
+                         "measured best on MI355X (the autotuner's direction-row candidates are multiples of the base "
+                         "rows it picks first, and from a 1,024-row base miss these 512-row workgroups, 10 %% per "
+                         "iteration); 'auto' autotunes them like the other paths; an explicit "
                          "--shape takes precedence")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing table")
     ap.add_argument("--step-graph", action="store_true",
